@@ -1,0 +1,171 @@
+"""Benchmark: cell-steps/s of the LCS per-step hot path on one MI355X.
+
+A step is LCS.timestep! + LCS.evolve! (CFL and particle dt, two RK stages of
+ghost fill + particles + RHS/update + HSMAC, linear-power forcing) over the
+whole box, with all state resident in HBM.  Default workload: one 1024^3 FP32
+box, linear-power P = 0.5, Re = 1333 (Re_lambda ~ 200 at u' = 1, eps = P),
+6.4e7 uniform particles with St = 1, 5 warm-up + 20 timed steps.
+
+Prints one JSON line.  ``roofline`` describes the kernel with the largest total
+time in the timed region, timed live with HIP events on the box stream;
+``cpu_baseline`` times the CPU oracle (the C restatement of the reference
+path) on a bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "lcs.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "cell-steps/s at 1024³ (%HBM peak); ensemble box-steps/s + p99 vs serial"
+
+
+def workload(name, n_override=None, particles_override=None):
+    from lcs_mi355x import config as C
+    if name == "c3":
+        n, npart, dtype, Re, forcing, st = 1024, 64_000_000, "f32", 1333.3, C.ConstantPowerLF(0.5), (1.0, 1.0)
+        desc = "single periodic box 1024^3 FP32, linear-power P=0.5, Re=1333, 6.4e7 particles St=1"
+    elif name == "c2":
+        n, npart, dtype, Re, forcing, st = 256, 2_000_000, "f32", 360.0, C.EnergyPreserveRCF(0.649, 1, 2, 8), (0.1, 10.0)
+        desc = "single periodic box 256^3 FP32, rc-energy (coarse 8), Re=360, 2e6 particles St log-U[0.1,10]"
+    elif name == "c1":
+        n, npart, dtype, Re, forcing, st = 64, 10_000, "f64", 53.3, C.ConstantPowerLF(0.5), (1.0, 1.0)
+        desc = "single periodic box 64^3 FP64, linear-power P=0.5, Re=53.3, 1e4 particles St=1"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    if n_override:
+        npart = int(round(npart * (n_override / n) ** 3))
+        n = n_override
+        desc += f" (sampled at {n}^3 with {npart} particles)"
+    if particles_override is not None:
+        npart = particles_override
+    part = ()
+    if npart:
+        part = (C.ParticleConfig(C.Population(npart), C.ParticleParams(0.0, 843.0), 4, "linear",
+                                 C.ParticleInit(seed=7, velocity="zero-slip", size="stokes", st_min=st[0],
+                                                st_max=st[1], eps=0.5)),)
+    cfg = C.Config(grid=C.Grid((n, n, n), 3), timestep=C.TimeStep(0.3, 0.01), simulate=C.SimulateConfig(0),
+                   flow=C.FlowConfig(C.FlowParams(1.0, 1.0, 1.0 / Re), C.SpectrumInit(1, 4.0, 1.0),
+                                     C.HSMAC(True, True, "red-black-fast", 1.7, 1e-3, 6, 300), forcing),
+                   particles=part, dtype=dtype)
+    return cfg, desc
+
+
+def cpu_baseline(name, n, steps):
+    """Time the CPU oracle on a bounded sample (n^3 box, same parameters)."""
+    import numpy as np
+    import lcs_mi355x as L
+    from oracle.driver import OracleSim, ParticleSet
+    cfg, desc = workload(name, n_override=n)
+    box = L.build(cfg)  # the device only prepares the initial state
+    dt = np.float32 if cfg.dtype == "f32" else np.float64
+    sets = []
+    for q, prof in enumerate(box.profiles):
+        nv = prof["nvalid"]
+        sets.append(ParticleSet([box.download_prop(q, k)[:nv] for k in ("x", "y", "z")],
+                                [box.download_prop(q, k)[:nv] for k in ("u", "v", "w")],
+                                box.download_prop(q, "diams")[:nv], box.download_prop(q, "ids")[:nv],
+                                prof["ntotal"]))
+    sim = OracleSim(cfg, dt, [box.download(k) for k in ("U", "V", "W")], box.download("P"), sets)
+    box.close()
+    sim.timestep()
+    sim.evolve()  # warm-up (first-touch, thread pool)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.timestep()
+        sim.evolve()
+    el = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": n ** 3 * steps / el, "unit": "cell-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} timed steps (after 1 warm-up) of the same pipeline at {n}^3 ({desc}); "
+                      f"{el:.1f} s of CPU time"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3"])
+    ap.add_argument("--size", type=int, default=None, help="override the grid size (testing only)")
+    ap.add_argument("--cpu-size", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the ensemble and single-box "
+              f"paths shard by box across GPUs, which is not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import lcs_mi355x as L
+    cfg, desc = workload(args.workload, n_override=args.size)
+    t0 = time.perf_counter()
+    box = L.build(cfg)
+    init_s = time.perf_counter() - t0
+    print(f"# built {desc} in {init_s:.1f} s, {box.nbytes() / 2**30:.1f} GiB on the device", file=sys.stderr,
+          flush=True)
+    scfg = L.step_config(cfg)
+    for s in range(args.warmup):
+        log = L.step(box, scfg)
+        print(f"# warmup {s}: itr {log['flow']['couple1']['itr']}/{log['flow']['couple2']['itr']} "
+              f"dt {log['dt']:.3e}", file=sys.stderr, flush=True)
+    box.sync()
+    box.profile(True)
+    sweeps = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        log = L.step(box, scfg)
+        sweeps.append((log["flow"]["couple1"]["itr"] + 1, log["flow"]["couple2"]["itr"] + 1))
+    box.sync()
+    el = time.perf_counter() - t0
+    prof = box.profile_read()
+    box.profile(False)
+    n3 = math.prod(cfg.grid.dims)
+    value = n3 * args.steps / el
+    esize = 4 if cfg.dtype == "f32" else 8
+    s_mean = sum(a + b for a, b in sweeps) / len(sweeps)
+    b_cell = esize * (29 + 8 * s_mean)  # algorithmic bytes per cell-step
+    dom = max(prof.items(), key=lambda kv: kv[1][0])
+    name, (ms, launches, byts) = dom
+    achieved = byts / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            t = json.load(fh)
+        if t.get("workload") == args.workload and t.get("size") == cfg.grid.dims[0]:
+            traffic = t.get("bytes_per_launch")
+    kernels = {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1],
+                   "algo_GBs": (v[2] / (v[0] * 1e-3) / 1e9) if v[0] > 0 else None} for k, v in prof.items()}
+    for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_launch"] * kv[1]["launches"]):
+        print(f"# {k:18s} {v['launches']:6d} launches  {v['ms_per_launch']:.3f} ms/launch  "
+              f"{(v['algo_GBs'] or 0):8.1f} GB/s algorithmic", file=sys.stderr, flush=True)
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": cfg.dtype.replace("f", "f"), "data": "synthetic (seeded spectrum init)",
+        "config": {"workload": desc, "grid": list(cfg.grid.dims), "particles": sum(p.population.valid for p in
+                                                                                     cfg.particles),
+                   "hsmac_sweeps_per_step": s_mean},
+        "step_hbm": {"bytes_per_cell_step": b_cell, "achieved_GBs": value * b_cell / 1e9,
+                     "frac": value * b_cell / 1e9 / PEAK_HBM_GBS},
+        "roofline": {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "ms_per_launch": ms / max(launches, 1), "launches": launches},
+        "kernels": kernels,
+    }
+    box.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_size, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,254 @@
+/*
+ * lcs_mi355x.h -- C ABI of the MI355X (gfx950) backend for the LCS.jl
+ * per-time-step hot path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int return codes.
+ * Every function returns LCS_OK (0) or a negative error class and sets a
+ * thread-local message readable with lcs_last_error().
+ *
+ * Layout contract (the reference's, src/flows/buffer.jl:52-55 and
+ * lib/offsets/src/Offsets.jl:23-43): every flow field is a column-major
+ * (x fastest) array of (nx+2h)*(ny+2h)*(nz+2h) elements whose domain index
+ * (i,j,k), 1-based, lives at 0-based offset (i-1+h) + sx*((j-1+h) + sy*(k-1+h))
+ * with sx = nx+2h, sy = ny+2h.  dU/dt fields are interior-only nx*ny*nz.
+ * Particle properties are SoA vectors of ntotal elements (ids int64, the rest
+ * in the box's float type).
+ *
+ * Threading: one HIP stream per box.  Calls that return host scalars
+ * synchronise that stream; all others are asynchronous.  Handles are
+ * thread-compatible, not thread-safe.
+ */
+#ifndef LCS_MI355X_H
+#define LCS_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error classes ---------------------------------------------------- */
+#define LCS_OK 0
+#define LCS_ERR_INVALID -1  /* bad argument / unsupported configuration */
+#define LCS_ERR_HIP -2      /* HIP runtime failure */
+#define LCS_ERR_CAPACITY -3 /* capacity or memory budget exceeded */
+#define LCS_ERR_STATE -4    /* call out of order (e.g. no particles) */
+
+/* ---- enums ------------------------------------------------------------ */
+#define LCS_F64 0 /* the reference's float type (src/LCS.jl:20) */
+#define LCS_F32 1
+
+/* field ids (lcs_field_upload/download, lcs_fill_ghost masks) */
+#define LCS_FIELD_U 0  /* Us[1]  */
+#define LCS_FIELD_V 1  /* Us[2]  */
+#define LCS_FIELD_W 2  /* Us[3]  */
+#define LCS_FIELD_U2 3 /* Us2[1] */
+#define LCS_FIELD_V2 4 /* Us2[2] */
+#define LCS_FIELD_W2 5 /* Us2[3] */
+#define LCS_FIELD_P 6
+#define LCS_FIELD_DU 7 /* dUdts[1], interior-only */
+#define LCS_FIELD_DV 8
+#define LCS_FIELD_DW 9
+
+/* particle property ids, in the order of Tuple(::Properties)
+ * (src/particles/buffer.jl:32-35); N_PROPERTIES = 17 */
+#define LCS_PROP_ID 0 /* int64 */
+#define LCS_PROP_DIAM 1
+#define LCS_PROP_X 2
+#define LCS_PROP_Y 3
+#define LCS_PROP_Z 4
+#define LCS_PROP_X2 5
+#define LCS_PROP_Y2 6
+#define LCS_PROP_Z2 7
+#define LCS_PROP_U 8
+#define LCS_PROP_V 9
+#define LCS_PROP_W 10
+#define LCS_PROP_U2 11
+#define LCS_PROP_V2 12
+#define LCS_PROP_W2 13
+#define LCS_PROP_DUDT 14
+#define LCS_PROP_DVDT 15
+#define LCS_PROP_DWDT 16
+#define LCS_N_PROPERTIES 17
+
+#define LCS_COLORING_RED_BLACK_FAST 0 /* lib/parallel/src/foraxes.jl:93-107 */
+#define LCS_COLORING_RED_BLACK 1      /* lib/parallel/src/foraxes.jl:82-88 */
+
+#define LCS_DRAG_LINEAR 0    /* src/particles/config.jl:124-128 */
+#define LCS_DRAG_NONLINEAR 1 /* src/particles/config.jl:135-139 */
+
+#define LCS_FORCING_NONE 0        /* NoForcing, src/flows/config.jl:113-115 */
+#define LCS_FORCING_LINEAR_POWER 1 /* ConstantPowerLF, src/flows/config.jl:132-136 */
+#define LCS_FORCING_RC_ENERGY 2    /* EnergyPreserveRCF, src/flows/config.jl:162-175 */
+
+typedef struct lcs_ctx lcs_ctx;
+typedef struct lcs_box lcs_box;
+
+/* Grid + FlowParams + float type of one periodic box
+ * (src/config.jl:56-60 Grid; src/flows/config.jl:11-18 FlowParams). */
+typedef struct {
+    int nx, ny, nz;
+    int halo; /* grid.halo_size; flow phases need >= 3 */
+    int dtype; /* LCS_F64 or LCS_F32 */
+    double U0_m_s, L0_m, NU_m2_s;
+} lcs_box_desc;
+
+/* HSMAC couple parameters (src/flows/config.jl:88-96) */
+typedef struct {
+    int time_blocking;
+    int overlap;
+    int coloring; /* LCS_COLORING_* */
+    double omega;
+    double epsp;
+    int itrmin;
+    int itrmax;
+} lcs_hsmac_params;
+
+/* One particle profile (src/particles/config.jl:11-41,51-56,272-281) */
+typedef struct {
+    int64_t nvalid; /* population.valid on one process */
+    int64_t ntotal; /* ceil(nvalid*(1+invalid_ratio)), src/particles/config.jl:22 */
+    double gravity;
+    double density_ratio;
+    int drag;       /* LCS_DRAG_* */
+    int grid_ratio; /* cell.grid_ratio */
+} lcs_particle_desc;
+
+/* Forcing selection for lcs_step (src/flows/config.jl:105-175) */
+typedef struct {
+    int kind; /* LCS_FORCING_* */
+    double power;       /* linear-power */
+    double energy;      /* rc-energy */
+    int kmin, kmax;     /* rc-energy */
+    int coarse_size;    /* rc-energy */
+} lcs_forcing;
+
+/* Everything timestep! + evolve! need beyond the box (src/config.jl:36-54 TimeStep) */
+typedef struct {
+    double cfl;
+    double dtmax;
+    lcs_hsmac_params hsmac;
+    lcs_forcing forcing;
+} lcs_step_config;
+
+/* The log of one step: the reference returns it as a NamedTuple
+ * (src/evolve.jl:127-145) plus the state update of src/evolve.jl:10-13. */
+typedef struct {
+    int64_t step;
+    double t;
+    double dt;
+    int itr1, itr2;   /* couple1.itr, couple2.itr */
+    double div1, div2; /* couple1.div, couple2.div */
+    double p;          /* forcing.p */
+    double e10;        /* forcing.e10 (rc-energy) */
+} lcs_step_log;
+
+/* ---- context / box ------------------------------------------------------ */
+/* replaces Topologies.device! (src/topologies.jl:1-3) for one device */
+int lcs_ctx_create(int device, lcs_ctx **out);
+int lcs_ctx_destroy(lcs_ctx *ctx);
+
+/* replaces LCS.Buffer / Flows.FlowBuffer (src/buffer.jl:27-34,
+ * src/flows/buffer.jl:115-121): allocates Us, Us2, P in HBM */
+int lcs_box_create(lcs_ctx *ctx, const lcs_box_desc *desc, lcs_box **out);
+int lcs_box_destroy(lcs_box *box);
+/* bytes of device memory the box holds now */
+int lcs_box_bytes(const lcs_box *box, size_t *bytes);
+
+/* host <-> device copies in the reference layout (blocking) */
+int lcs_field_upload(lcs_box *box, int field, const void *host, size_t bytes);
+int lcs_field_download(lcs_box *box, int field, void *host, size_t bytes);
+
+/* ---- flow phases ---------------------------------------------------------- */
+/* Topologies.synchalo! single-process (src/topologies.jl:36-45,
+ * lib/topologies/src/halo.jl:138-159): field_mask bit f selects LCS_FIELD_f */
+int lcs_fill_ghost(lcs_box *box, unsigned field_mask, int width, int full);
+
+/* Flows.timestep (src/flows/timestep.jl:6-65): FaceHalo ghost fill of Us,
+ * 4th-order cell-centred |u| maxima, dt.  umax3 may be NULL. */
+int lcs_flow_timestep(lcs_box *box, double cfl, double dtmax, double *dt, double *umax3);
+
+/* Flows.rhs! (src/flows/rhs.jl:9-81): stage 1 reads Us, stage 2 reads Us2;
+ * writes dU/dV/dW */
+int lcs_rhs(lcs_box *box, int stage);
+/* Flows.update! (src/flows/update.jl:12-54): stage 1 Us2 = Us + dt*dUdt,
+ * stage 2 Us = Us + dt*dUdt */
+int lcs_update(lcs_box *box, int stage, double dt_stage);
+/* rhs! followed by update! in one pass, without storing dU/dt */
+int lcs_rhs_update(lcs_box *box, int stage, double dt_stage);
+
+/* Flows.couple!(::HSMAC) (src/flows/couples/couple.jl:13-17,
+ * src/flows/couples/hsmac.jl:52-201): stage 1 acts on Us2, stage 2 on Us */
+int lcs_hsmac(lcs_box *box, int stage, double dt_stage, const lcs_hsmac_params *params, int *itr,
+              double *div);
+
+/* forcing!(::ConstantPowerLF) (src/flows/forcings/lf.jl:6-192) on Us */
+int lcs_force_linear(lcs_box *box, double power, double dt, double *p);
+/* forcing!(::EnergyPreserveRCF) (src/flows/forcings/rcf.jl:1-191) on Us;
+ * box mean, spectrum rescaling and sparse inverse DFT all run on the device */
+int lcs_force_rcf(lcs_box *box, double energy, int kmin, int kmax, int coarse_size, double dt,
+                  double *p);
+/* box-mean filter only (Flows.boxmean_l!, ext/AMDGPUExt.jl:18-41): coarse3
+ * receives 3*c^3 values (U then V then W), column-major, as doubles */
+int lcs_boxmean(lcs_box *box, int coarse_size, double *coarse3);
+
+/* ---- particles ------------------------------------------------------------ */
+/* Particles.ParticleBuffer (src/particles/buffer.jl:124-135) for one profile;
+ * profile index returned in *profile (0-based, in call order) */
+int lcs_particles_add(lcs_box *box, const lcs_particle_desc *desc, int *profile);
+int lcs_particles_upload(lcs_box *box, int profile, int prop, const void *host, size_t bytes);
+int lcs_particles_download(lcs_box *box, int profile, int prop, void *host, size_t bytes);
+/* Particles.boundary! (src/particles/boundary.jl:33-120), single process */
+int lcs_particles_boundary(lcs_box *box, int profile, int stage);
+/* Particles.motion! (src/particles/motion.jl:15-84) */
+int lcs_particles_motion(lcs_box *box, int profile, int stage);
+/* Particles.update! (src/particles/update.jl:18-65) */
+int lcs_particles_update(lcs_box *box, int profile, int stage, double dt_stage);
+/* Particles.timestep (src/particles/timestep.jl:15-25) */
+int lcs_particles_timestep(lcs_box *box, int profile, double dtmax, double *dt);
+/* Particles.makeindex! (src/particles/makeindex.jl:9-67): stage 1 bins xss,
+ * stage 2 bins xss2; permutes all 17 properties; stable sort */
+int lcs_particles_makeindex(lcs_box *box, int profile, int stage);
+/* the CellIndexBuffer after makeindex (src/particles/buffer.jl:45-75):
+ * hashes (nvalid), starts/stops ((nc+2)^3, 1-based); any pointer may be NULL */
+int lcs_particles_cellindex(lcs_box *box, int profile, int64_t *hashes, int64_t *starts,
+                            int64_t *stops, int64_t *ncells);
+
+/* ---- initial conditions (builder-defined; no counterpart in the reference) */
+/* seeded solenoidal field with E(k) ~ k^4 exp(-2 (k/k0)^2), scaled to u' */
+int lcs_init_spectrum(lcs_box *box, uint64_t seed, double k0, double uprime);
+/* uniform positions, log-uniform Stokes numbers in [st_min, st_max] with
+ * St = taup/taueta, taueta = sqrt(1/(eps*Re)); zero_slip != 0 sets the particle
+ * velocity to the interpolated fluid velocity (needs filled ghosts), else rest */
+int lcs_init_particles(lcs_box *box, int profile, uint64_t seed, double st_min, double st_max,
+                       double eps, int zero_slip);
+
+/* ---- whole step ------------------------------------------------------------ */
+/* LCS.timestep! + LCS.evolve!(::FlowParticleMode / ::FlowMode)
+ * (src/timestep.jl:152-173, src/evolve.jl:15-146).  All phases are enqueued
+ * on the box stream; only the log is read back.  log may be NULL, in which
+ * case the call does not synchronise. */
+int lcs_step(lcs_box *box, const lcs_step_config *cfg, lcs_step_log *log);
+/* the most recent step's log (synchronises) */
+int lcs_step_log_read(lcs_box *box, lcs_step_log *log);
+
+/* ---- measurement hook (no counterpart in the reference) ---------------------
+ * enable != 0 starts recording HIP events on the box stream around every
+ * kernel launch, keyed by kernel name, together with each launch's
+ * algorithmic HBM bytes; enable == 0 stops and clears.  lcs_profile_read
+ * returns entry `index` (0-based; LCS_ERR_INVALID past the end): name, total
+ * milliseconds, launch count and total algorithmic bytes (synchronises). */
+int lcs_profile(lcs_box *box, int enable);
+int lcs_profile_read(lcs_box *box, int index, char *name, size_t name_len, double *total_ms,
+                     int64_t *launches, double *bytes);
+
+int lcs_sync(lcs_box *box);
+const char *lcs_last_error(void);
+/* library version string */
+const char *lcs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LCS_MI355X_H */

@@ -1,0 +1,667 @@
+// C ABI of the MI355X backend (include/lcs_mi355x.h): handles, memory,
+// per-phase entry points mirroring the reference's positional methods, and
+// the whole-step driver (LCS.timestep! + LCS.evolve!).
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "lcs_internal.h"
+
+namespace lcs {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string &msg) { g_error = msg; }
+int fail(int code, const std::string &msg)
+{
+    set_error(msg);
+    return code;
+}
+
+__global__ void k_set_double(double *dst, double v) { *dst = v; }
+
+__global__ void k_advance_state(DevScalars *S)
+{
+    // evolve!(state): src/evolve.jl:10-13
+    S->step += 1;
+    S->t += S->dt;
+}
+
+ProfScope::ProfScope(Box &box, const char *name, double bytes) : b(box)
+{
+    if (!b.prof.on) return;
+    auto &E = b.prof.entries;
+    for (size_t q = 0; q < E.size(); q++)
+        if (E[q].name == name) idx = (int)q;
+    if (idx < 0) {
+        E.emplace_back();
+        E.back().name = name;
+        idx = (int)E.size() - 1;
+    }
+    auto take = [&]() {
+        hipEvent_t e = nullptr;
+        if (!b.prof.pool.empty()) {
+            e = b.prof.pool.back();
+            b.prof.pool.pop_back();
+        } else if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+        }
+        return e;
+    };
+    e0 = take();
+    e1 = take();
+    if (e0) (void)hipEventRecord(e0, b.stream);
+    E[idx].bytes += bytes;
+    E[idx].launches += 1;
+}
+
+ProfScope::~ProfScope()
+{
+    if (idx < 0) return;
+    if (e1) (void)hipEventRecord(e1, b.stream);
+    if (e0 && e1) b.prof.entries[idx].pending.emplace_back(e0, e1);
+}
+
+static void prof_collect(Box &b)
+{
+    for (auto &e : b.prof.entries) {
+        for (auto &pr : e.pending) {
+            float ms = 0;
+            if (hipEventSynchronize(pr.second) == hipSuccess && hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess)
+                e.ms += ms;
+            b.prof.pool.push_back(pr.first);
+            b.prof.pool.push_back(pr.second);
+        }
+        e.pending.clear();
+    }
+}
+
+static int set_phase_dt(Box &b, double dt)
+{
+    hipLaunchKernelGGL(k_set_double, dim3(1), dim3(1), 0, b.stream, &b.S->dt_phase, dt);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+static int read_scalars(Box &b)
+{
+    LCS_HIP(hipMemcpyAsync(b.S_host, b.S, sizeof(DevScalars), hipMemcpyDeviceToHost, b.stream));
+    LCS_HIP(hipStreamSynchronize(b.stream));
+    return LCS_OK;
+}
+
+static int device_alloc(Box &b, void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(LCS_ERR_CAPACITY, std::string("hipMalloc of ") + std::to_string(bytes) +
+                                          " bytes failed: " + hipGetErrorString(e));
+    }
+    b.bytes += bytes;
+    return LCS_OK;
+}
+
+static int check_box(const lcs_box *box)
+{
+    if (!box) return fail(LCS_ERR_INVALID, "null box");
+    return LCS_OK;
+}
+
+static int check_stage(int stage)
+{
+    if (stage != 1 && stage != 2) return fail(LCS_ERR_INVALID, "stage must be 1 or 2");
+    return LCS_OK;
+}
+
+static Box *B(lcs_box *b) { return reinterpret_cast<Box *>(b); }
+static const Box *B(const lcs_box *b) { return reinterpret_cast<const Box *>(b); }
+
+static int ensure_dudt(Box &b)
+{
+    for (int q = LCS_FIELD_DU; q <= LCS_FIELD_DW; q++)
+        if (!b.f[q]) {
+            int rc = device_alloc(b, &b.f[q], (size_t)b.g.cells() * b.esize);
+            if (rc) return rc;
+            LCS_HIP(hipMemsetAsync(b.f[q], 0, (size_t)b.g.cells() * b.esize, b.stream));
+        }
+    return LCS_OK;
+}
+
+static int profile_of(Box &b, int profile, ParticleProfile **pp)
+{
+    if (profile < 0 || profile >= (int)b.profiles.size()) return fail(LCS_ERR_STATE, "no such particle profile");
+    *pp = &b.profiles[profile];
+    return LCS_OK;
+}
+
+}  // namespace lcs
+
+using namespace lcs;
+
+extern "C" {
+
+const char *lcs_last_error(void) { return g_error.c_str(); }
+const char *lcs_version(void) { return "lcs-mi355x 0.1 (gfx950)"; }
+
+int lcs_ctx_create(int device, lcs_ctx **out)
+{
+    if (!out) return fail(LCS_ERR_INVALID, "null output pointer");
+    int n = 0;
+    LCS_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(LCS_ERR_INVALID, "no such device");
+    LCS_HIP(hipSetDevice(device));
+    Ctx *c = new Ctx;
+    c->device = device;
+    *out = reinterpret_cast<lcs_ctx *>(c);
+    return LCS_OK;
+}
+
+int lcs_ctx_destroy(lcs_ctx *ctx)
+{
+    delete reinterpret_cast<Ctx *>(ctx);
+    return LCS_OK;
+}
+
+int lcs_box_create(lcs_ctx *ctx, const lcs_box_desc *d, lcs_box **out)
+{
+    if (!ctx || !d || !out) return fail(LCS_ERR_INVALID, "null argument");
+    if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(LCS_ERR_INVALID, "grid dims must be positive");
+    if (d->halo < 1) return fail(LCS_ERR_INVALID, "halo_size must be >= 1");
+    if (d->dtype != LCS_F64 && d->dtype != LCS_F32) return fail(LCS_ERR_INVALID, "dtype must be LCS_F64 or LCS_F32");
+    if (!(d->NU_m2_s > 0) || !(d->U0_m_s > 0) || !(d->L0_m > 0))
+        return fail(LCS_ERR_INVALID, "flow params must be positive");
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    LCS_HIP(hipSetDevice(c->device));
+    Box *b = new Box;
+    b->ctx = c;
+    b->desc = *d;
+    b->g = make_geom(d->nx, d->ny, d->nz, d->halo);
+    b->esize = d->dtype == LCS_F64 ? 8 : 4;
+    int rc = LCS_OK;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return fail(LCS_ERR_HIP, "hipStreamCreate failed");
+    }
+    size_t fb = (size_t)b->g.len * b->esize;
+    for (int q = 0; q <= LCS_FIELD_P && rc == LCS_OK; q++) {
+        rc = device_alloc(*b, &b->f[q], fb);
+        if (rc == LCS_OK && hipMemsetAsync(b->f[q], 0, fb, b->stream) != hipSuccess)
+            rc = fail(LCS_ERR_HIP, "hipMemset failed");
+    }
+    if (rc == LCS_OK) rc = device_alloc(*b, (void **)&b->S, sizeof(DevScalars));
+    if (rc == LCS_OK) rc = device_alloc(*b, (void **)&b->part, sizeof(double) * kPartSlots);
+    if (rc == LCS_OK && hipHostMalloc((void **)&b->S_host, sizeof(DevScalars), 0) != hipSuccess)
+        rc = fail(LCS_ERR_HIP, "hipHostMalloc failed");
+    if (rc == LCS_OK && hipMemsetAsync(b->S, 0, sizeof(DevScalars), b->stream) != hipSuccess)
+        rc = fail(LCS_ERR_HIP, "hipMemset failed");
+    if (rc == LCS_OK && hipStreamSynchronize(b->stream) != hipSuccess) rc = fail(LCS_ERR_HIP, "sync failed");
+    if (rc != LCS_OK) {
+        std::string msg = g_error;
+        lcs_box_destroy(reinterpret_cast<lcs_box *>(b));
+        set_error(msg);
+        return rc;
+    }
+    *out = reinterpret_cast<lcs_box *>(b);
+    return LCS_OK;
+}
+
+int lcs_box_destroy(lcs_box *box)
+{
+    if (!box) return LCS_OK;
+    Box *b = B(box);
+    (void)hipSetDevice(b->ctx->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (void *p : b->f)
+        if (p) (void)hipFree(p);
+    for (auto &pp : b->profiles) {
+        for (void *p : pp.props)
+            if (p) (void)hipFree(p);
+        void *extra[] = {pp.hashes, pp.perm, pp.starts, pp.stops, pp.scratch, pp.sort_tmp};
+        for (void *p : extra)
+            if (p) (void)hipFree(p);
+    }
+    if (b->S) (void)hipFree(b->S);
+    if (b->part) (void)hipFree(b->part);
+    if (b->rcf) (void)hipFree(b->rcf);
+    if (b->S_host) (void)hipHostFree(b->S_host);
+    prof_collect(*b);
+    for (hipEvent_t e : b->prof.pool) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return LCS_OK;
+}
+
+int lcs_box_bytes(const lcs_box *box, size_t *bytes)
+{
+    if (check_box(box) || !bytes) return fail(LCS_ERR_INVALID, "null argument");
+    *bytes = B(box)->bytes;
+    return LCS_OK;
+}
+
+static int field_xfer(lcs_box *box, int field, void *host, size_t bytes, bool up)
+{
+    if (check_box(box)) return LCS_ERR_INVALID;
+    Box &b = *B(box);
+    if (field < 0 || field > LCS_FIELD_DW) return fail(LCS_ERR_INVALID, "no such field");
+    if (!host) return fail(LCS_ERR_INVALID, "null host pointer");
+    size_t need = (size_t)(field >= LCS_FIELD_DU ? b.g.cells() : b.g.len) * b.esize;
+    if (bytes != need)
+        return fail(LCS_ERR_INVALID, "field size mismatch: expected " + std::to_string(need) + " bytes, got " +
+                                         std::to_string(bytes));
+    if (field >= LCS_FIELD_DU) {
+        int rc = ensure_dudt(b);
+        if (rc) return rc;
+    }
+    LCS_HIP(hipSetDevice(b.ctx->device));
+    if (up)
+        LCS_HIP(hipMemcpyAsync(b.f[field], host, bytes, hipMemcpyHostToDevice, b.stream));
+    else
+        LCS_HIP(hipMemcpyAsync(host, b.f[field], bytes, hipMemcpyDeviceToHost, b.stream));
+    LCS_HIP(hipStreamSynchronize(b.stream));
+    return LCS_OK;
+}
+
+int lcs_field_upload(lcs_box *box, int field, const void *host, size_t bytes)
+{
+    return field_xfer(box, field, const_cast<void *>(host), bytes, true);
+}
+int lcs_field_download(lcs_box *box, int field, void *host, size_t bytes)
+{
+    return field_xfer(box, field, host, bytes, false);
+}
+
+int lcs_fill_ghost(lcs_box *box, unsigned mask, int width, int full)
+{
+    if (check_box(box)) return LCS_ERR_INVALID;
+    Box &b = *B(box);
+    void *arr[4];
+    int n = 0;
+    for (int q = 0; q <= LCS_FIELD_P; q++)
+        if (mask & (1u << q)) {
+            if (n == 4) {
+                int rc = launch_ghost_fill(b, arr, n, width, full, nullptr);
+                if (rc) return rc;
+                n = 0;
+            }
+            arr[n++] = b.f[q];
+        }
+    if (mask >> (LCS_FIELD_P + 1)) return fail(LCS_ERR_INVALID, "ghost fill applies to halo fields only");
+    if (n) return launch_ghost_fill(b, arr, n, width, full, nullptr);
+    return LCS_OK;
+}
+
+int lcs_flow_timestep(lcs_box *box, double cfl, double dtmax, double *dt, double *umax3)
+{
+    if (check_box(box)) return LCS_ERR_INVALID;
+    Box &b = *B(box);
+    int rc = launch_cfl(b, cfl, dtmax);
+    if (rc || (rc = read_scalars(b))) return rc;
+    if (dt) *dt = b.S_host->dt_flow;
+    if (umax3)
+        for (int a = 0; a < 3; a++) umax3[a] = b.S_host->umax[a];
+    return LCS_OK;
+}
+
+int lcs_rhs(lcs_box *box, int stage)
+{
+    if (check_box(box) || check_stage(stage)) return LCS_ERR_INVALID;
+    Box &b = *B(box);
+    if (b.g.h < 3) return fail(LCS_ERR_INVALID, "the RHS needs halo_size >= 3");
+    int rc = ensure_dudt(b);
+    if (rc) return rc;
+    void *out[3] = {b.f[LCS_FIELD_DU], b.f[LCS_FIELD_DV], b.f[LCS_FIELD_DW]};
+    void *base[3] = {nullptr, nullptr, nullptr};
+    return launch_rhs(b, stage, false, out, base, DtRef{&b.S->dt_phase, 1.0});
+}
+
+int lcs_update(lcs_box *box, int stage, double dt_stage)
+{
+    if (check_box(box) || check_stage(stage)) return LCS_ERR_INVALID;
+    Box &b = *B(box);
+    int rc = ensure_dudt(b);
+    if (rc || (rc = set_phase_dt(b, dt_stage))) return rc;
+    void *const *out = stage == 1 ? &b.f[LCS_FIELD_U2] : &b.f[LCS_FIELD_U];
+    void *const *base = &b.f[LCS_FIELD_U];
+    return launch_update(b, out, base, DtRef{&b.S->dt_phase, 1.0});
+}
+
+int lcs_rhs_update(lcs_box *box, int stage, double dt_stage)
+{
+    if (check_box(box) || check_stage(stage)) return LCS_ERR_INVALID;
+    Box &b = *B(box);
+    if (b.g.h < 3) return fail(LCS_ERR_INVALID, "the RHS needs halo_size >= 3");
+    int rc = set_phase_dt(b, dt_stage);
+    if (rc) return rc;
+    void *const *out = stage == 1 ? &b.f[LCS_FIELD_U2] : &b.f[LCS_FIELD_U];
+    return launch_rhs(b, stage, true, out, &b.f[LCS_FIELD_U], DtRef{&b.S->dt_phase, 1.0});
+}
+
+int lcs_hsmac(lcs_box *box, int stage, double dt_stage, const lcs_hsmac_params *prm, int *itr, double *div)
+{
+    if (check_box(box) || check_stage(stage)) return LCS_ERR_INVALID;
+    if (!prm) return fail(LCS_ERR_INVALID, "null params");
+    Box &b = *B(box);
+    int rc = set_phase_dt(b, dt_stage);
+    if (rc) return rc;
+    void *const *Us = stage == 1 ? &b.f[LCS_FIELD_U2] : &b.f[LCS_FIELD_U];
+    if ((rc = launch_hsmac(b, Us, DtRef{&b.S->dt_phase, 1.0}, *prm, 0))) return rc;
+    if ((rc = read_scalars(b))) return rc;
+    if (itr) *itr = b.S_host->itr[0];
+    if (div) *div = b.S_host->div[0];
+    return LCS_OK;
+}
+
+int lcs_force_linear(lcs_box *box, double power, double dt, double *p)
+{
+    if (check_box(box)) return LCS_ERR_INVALID;
+    Box &b = *B(box);
+    int rc = set_phase_dt(b, dt);
+    if (rc || (rc = launch_force_linear(b, power, DtRef{&b.S->dt_phase, 1.0})) || (rc = read_scalars(b))) return rc;
+    if (p) *p = b.S_host->p;
+    return LCS_OK;
+}
+
+int lcs_force_rcf(lcs_box *box, double energy, int kmin, int kmax, int c, double dt, double *p)
+{
+    if (check_box(box)) return LCS_ERR_INVALID;
+    Box &b = *B(box);
+    int rc = set_phase_dt(b, dt);
+    if (rc || (rc = launch_force_rcf(b, energy, kmin, kmax, c, DtRef{&b.S->dt_phase, 1.0})) ||
+        (rc = read_scalars(b)))
+        return rc;
+    if (p) *p = b.S_host->p;
+    return LCS_OK;
+}
+
+int lcs_boxmean(lcs_box *box, int c, double *coarse3)
+{
+    if (check_box(box) || !coarse3) return fail(LCS_ERR_INVALID, "null argument");
+    Box &b = *B(box);
+    double *dev = nullptr;
+    size_t bytes = sizeof(double) * 3 * (size_t)c * c * c;
+    LCS_HIP(hipMalloc(&dev, bytes));
+    int rc = launch_boxmean(b, c, dev);
+    if (rc == LCS_OK) {
+        hipError_t e = hipMemcpyAsync(coarse3, dev, bytes, hipMemcpyDeviceToHost, b.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b.stream);
+        if (e != hipSuccess) rc = fail(LCS_ERR_HIP, hipGetErrorString(e));
+    }
+    (void)hipFree(dev);
+    return rc;
+}
+
+// ---- particles ----------------------------------------------------------------
+int lcs_particles_add(lcs_box *box, const lcs_particle_desc *d, int *profile)
+{
+    if (check_box(box) || !d) return fail(LCS_ERR_INVALID, "null argument");
+    Box &b = *B(box);
+    if ((int)b.profiles.size() >= kMaxProfiles) return fail(LCS_ERR_CAPACITY, "at most 8 particle profiles");
+    if (d->nvalid < 0 || d->ntotal < d->nvalid) return fail(LCS_ERR_INVALID, "need 0 <= nvalid <= ntotal");
+    if (d->ntotal >= (int64_t(1) << 31)) return fail(LCS_ERR_CAPACITY, "at most 2^31-1 particles per profile");
+    if (d->grid_ratio < 1) return fail(LCS_ERR_INVALID, "grid_ratio must be >= 1");
+    if (d->drag != LCS_DRAG_LINEAR && d->drag != LCS_DRAG_NONLINEAR) return fail(LCS_ERR_INVALID, "unknown drag");
+    ParticleProfile pp;
+    memset(pp.props, 0, sizeof(pp.props));
+    pp.desc = *d;
+    pp.nc[0] = b.g.nx / d->grid_ratio;
+    pp.nc[1] = b.g.ny / d->grid_ratio;
+    pp.nc[2] = b.g.nz / d->grid_ratio;
+    long ncell = (long)(pp.nc[0] + 2) * (pp.nc[1] + 2) * (pp.nc[2] + 2);
+    if (ncell > 4294967295L) return fail(LCS_ERR_INVALID, "cell count exceeds typemax(UInt32)");
+    size_t nt = (size_t)std::max<int64_t>(1, d->ntotal);
+    int rc = LCS_OK;
+    for (int q = 0; q < LCS_N_PROPERTIES && rc == LCS_OK; q++) {
+        size_t es = q == LCS_PROP_ID ? 8 : b.esize;
+        rc = device_alloc(b, &pp.props[q], es * nt);
+        if (rc == LCS_OK && q != LCS_PROP_ID && hipMemsetAsync(pp.props[q], 0, es * nt, b.stream) != hipSuccess)
+            rc = fail(LCS_ERR_HIP, "hipMemset failed");
+    }
+    if (rc == LCS_OK) rc = device_alloc(b, &pp.hashes, 8 * nt);
+    if (rc == LCS_OK) rc = device_alloc(b, &pp.perm, 4 * nt);
+    if (rc == LCS_OK) rc = device_alloc(b, &pp.starts, 8 * (size_t)ncell);
+    if (rc == LCS_OK) rc = device_alloc(b, &pp.stops, 8 * (size_t)ncell);
+    if (rc == LCS_OK) rc = device_alloc(b, &pp.scratch, 8 * nt);
+    if (rc == LCS_OK) {
+        // ids = INVALID (src/particles/buffer.jl:19-20)
+        std::vector<int64_t> inval(nt, kInvalidId);
+        hipError_t e = hipMemcpyAsync(pp.props[LCS_PROP_ID], inval.data(), 8 * nt, hipMemcpyHostToDevice, b.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b.stream);
+        if (e != hipSuccess) rc = fail(LCS_ERR_HIP, hipGetErrorString(e));
+    }
+    if (rc != LCS_OK) {
+        for (void *p : pp.props)
+            if (p) (void)hipFree(p);
+        void *extra[] = {pp.hashes, pp.perm, pp.starts, pp.stops, pp.scratch};
+        for (void *p : extra)
+            if (p) (void)hipFree(p);
+        return rc;
+    }
+    b.profiles.push_back(pp);
+    if (profile) *profile = (int)b.profiles.size() - 1;
+    return LCS_OK;
+}
+
+static int prop_xfer(lcs_box *box, int profile, int prop, void *host, size_t bytes, bool up)
+{
+    if (check_box(box) || !host) return fail(LCS_ERR_INVALID, "null argument");
+    Box &b = *B(box);
+    ParticleProfile *pp;
+    int rc = profile_of(b, profile, &pp);
+    if (rc) return rc;
+    if (prop < 0 || prop >= LCS_N_PROPERTIES) return fail(LCS_ERR_INVALID, "no such property");
+    size_t es = prop == LCS_PROP_ID ? 8 : b.esize;
+    size_t need = es * (size_t)pp->desc.ntotal;
+    if (bytes != need)
+        return fail(LCS_ERR_INVALID, "property size mismatch: expected " + std::to_string(need) + " bytes");
+    if (need == 0) return LCS_OK;
+    if (up)
+        LCS_HIP(hipMemcpyAsync(pp->props[prop], host, bytes, hipMemcpyHostToDevice, b.stream));
+    else
+        LCS_HIP(hipMemcpyAsync(host, pp->props[prop], bytes, hipMemcpyDeviceToHost, b.stream));
+    LCS_HIP(hipStreamSynchronize(b.stream));
+    return LCS_OK;
+}
+
+int lcs_particles_upload(lcs_box *box, int profile, int prop, const void *host, size_t bytes)
+{
+    return prop_xfer(box, profile, prop, const_cast<void *>(host), bytes, true);
+}
+int lcs_particles_download(lcs_box *box, int profile, int prop, void *host, size_t bytes)
+{
+    return prop_xfer(box, profile, prop, host, bytes, false);
+}
+
+#define WITH_PROFILE                                            \
+    if (check_box(box)) return LCS_ERR_INVALID;                 \
+    Box &b = *B(box);                                           \
+    ParticleProfile *pp;                                        \
+    int rc = profile_of(b, profile, &pp);                       \
+    if (rc) return rc;
+
+int lcs_particles_boundary(lcs_box *box, int profile, int stage)
+{
+    WITH_PROFILE
+    if (check_stage(stage)) return LCS_ERR_INVALID;
+    return launch_p_boundary(b, *pp, stage);
+}
+
+int lcs_particles_motion(lcs_box *box, int profile, int stage)
+{
+    WITH_PROFILE
+    if (check_stage(stage)) return LCS_ERR_INVALID;
+    return launch_p_motion(b, *pp, stage);
+}
+
+int lcs_particles_update(lcs_box *box, int profile, int stage, double dt_stage)
+{
+    WITH_PROFILE
+    if (check_stage(stage)) return LCS_ERR_INVALID;
+    if ((rc = set_phase_dt(b, dt_stage))) return rc;
+    return launch_p_update(b, *pp, stage, DtRef{&b.S->dt_phase, 1.0});
+}
+
+int lcs_particles_timestep(lcs_box *box, int profile, double dtmax, double *dt)
+{
+    WITH_PROFILE
+    if ((rc = launch_p_timestep(b, *pp, dtmax, &b.S->dtp[profile])) || (rc = read_scalars(b))) return rc;
+    if (dt) *dt = b.S_host->dtp[profile];
+    return LCS_OK;
+}
+
+int lcs_particles_makeindex(lcs_box *box, int profile, int stage)
+{
+    WITH_PROFILE
+    if (check_stage(stage)) return LCS_ERR_INVALID;
+    return launch_p_makeindex(b, *pp, stage);
+}
+
+int lcs_particles_cellindex(lcs_box *box, int profile, int64_t *hashes, int64_t *starts, int64_t *stops,
+                            int64_t *ncells)
+{
+    WITH_PROFILE
+    if (!pp->indexed) return fail(LCS_ERR_STATE, "makeindex has not run for this profile");
+    long nc = (long)(pp->nc[0] + 2) * (pp->nc[1] + 2) * (pp->nc[2] + 2);
+    if (ncells) *ncells = nc;
+    if (hashes && pp->desc.nvalid)
+        LCS_HIP(hipMemcpyAsync(hashes, pp->hashes, 8 * (size_t)pp->desc.nvalid, hipMemcpyDeviceToHost, b.stream));
+    if (starts) LCS_HIP(hipMemcpyAsync(starts, pp->starts, 8 * (size_t)nc, hipMemcpyDeviceToHost, b.stream));
+    if (stops) LCS_HIP(hipMemcpyAsync(stops, pp->stops, 8 * (size_t)nc, hipMemcpyDeviceToHost, b.stream));
+    LCS_HIP(hipStreamSynchronize(b.stream));
+    return LCS_OK;
+}
+
+int lcs_init_spectrum(lcs_box *box, uint64_t seed, double k0, double uprime)
+{
+    if (check_box(box)) return LCS_ERR_INVALID;
+    if (!(k0 > 0) || !(uprime > 0)) return fail(LCS_ERR_INVALID, "k0 and uprime must be positive");
+    return launch_init_spectrum(*B(box), seed, k0, uprime);
+}
+
+int lcs_init_particles(lcs_box *box, int profile, uint64_t seed, double st_min, double st_max, double eps,
+                       int zero_slip)
+{
+    WITH_PROFILE
+    if (!(st_min > 0) || !(st_max >= st_min) || !(eps > 0))
+        return fail(LCS_ERR_INVALID, "need 0 < st_min <= st_max and eps > 0");
+    rc = launch_p_init(b, *pp, seed, st_min, st_max, eps, zero_slip);
+    if (rc) return rc;
+    LCS_HIP(hipStreamSynchronize(b.stream));
+    return LCS_OK;
+}
+
+// ---- whole step -------------------------------------------------------------
+static void fill_log(const DevScalars &s, lcs_step_log *log)
+{
+    log->step = s.step;
+    log->t = s.t;
+    log->dt = s.dt;
+    log->itr1 = s.itr[0];
+    log->itr2 = s.itr[1];
+    log->div1 = s.div[0];
+    log->div2 = s.div[1];
+    log->p = s.p;
+    log->e10 = s.e10;
+}
+
+int lcs_step(lcs_box *box, const lcs_step_config *cfg, lcs_step_log *log)
+{
+    if (check_box(box) || !cfg) return fail(LCS_ERR_INVALID, "null argument");
+    Box &b = *B(box);
+    if (b.g.h < 3) return fail(LCS_ERR_INVALID, "a flow step needs halo_size >= 3");
+    int rc;
+    // timestep!: src/timestep.jl:152-173
+    if ((rc = launch_cfl(b, cfg->cfl, cfg->dtmax))) return rc;
+    int np = (int)b.profiles.size();
+    for (int q = 0; q < np; q++)
+        if ((rc = launch_p_timestep(b, b.profiles[q], cfg->dtmax, &b.S->dtp[q]))) return rc;
+    if ((rc = launch_finalize_dt(b, np, cfg->dtmax))) return rc;
+
+    // evolve!: src/evolve.jl:63-146 (FlowMode :15-61 when there are no particles)
+    const DtRef dt1{&b.S->dt, 0.5}, dt2{&b.S->dt, 1.0};
+    void *UsP[4] = {b.f[LCS_FIELD_U], b.f[LCS_FIELD_V], b.f[LCS_FIELD_W], b.f[LCS_FIELD_P]};
+    void *Us2P[4] = {b.f[LCS_FIELD_U2], b.f[LCS_FIELD_V2], b.f[LCS_FIELD_W2], b.f[LCS_FIELD_P]};
+    // stage 1
+    if ((rc = launch_ghost_fill(b, UsP, 4, b.g.h, 1, nullptr))) return rc;
+    for (int q = 0; q < np; q++)
+        if ((rc = launch_p_stage(b, b.profiles[q], 1, dt1))) return rc;
+    if ((rc = launch_rhs(b, 1, true, &b.f[LCS_FIELD_U2], &b.f[LCS_FIELD_U], dt1))) return rc;
+    if ((rc = launch_hsmac(b, &b.f[LCS_FIELD_U2], dt1, cfg->hsmac, 0))) return rc;
+    // stage 2
+    if ((rc = launch_ghost_fill(b, Us2P, 4, b.g.h, 1, nullptr))) return rc;
+    for (int q = 0; q < np; q++)
+        if ((rc = launch_p_stage(b, b.profiles[q], 2, dt2))) return rc;
+    if ((rc = launch_rhs(b, 2, true, &b.f[LCS_FIELD_U], &b.f[LCS_FIELD_U], dt2))) return rc;
+    switch (cfg->forcing.kind) {
+    case LCS_FORCING_NONE:
+        break;
+    case LCS_FORCING_LINEAR_POWER:
+        if ((rc = launch_force_linear(b, cfg->forcing.power, dt2))) return rc;
+        break;
+    case LCS_FORCING_RC_ENERGY:
+        if ((rc = launch_force_rcf(b, cfg->forcing.energy, cfg->forcing.kmin, cfg->forcing.kmax,
+                                   cfg->forcing.coarse_size, dt2)))
+            return rc;
+        break;
+    default:
+        return fail(LCS_ERR_INVALID, "unknown forcing kind");
+    }
+    if ((rc = launch_hsmac(b, &b.f[LCS_FIELD_U], dt2, cfg->hsmac, 1))) return rc;
+    hipLaunchKernelGGL(k_advance_state, dim3(1), dim3(1), 0, b.stream, b.S);
+    LCS_HIP(hipGetLastError());
+    if (log) {
+        if ((rc = read_scalars(b))) return rc;
+        fill_log(*b.S_host, log);
+    }
+    return LCS_OK;
+}
+
+int lcs_step_log_read(lcs_box *box, lcs_step_log *log)
+{
+    if (check_box(box) || !log) return fail(LCS_ERR_INVALID, "null argument");
+    Box &b = *B(box);
+    int rc = read_scalars(b);
+    if (rc) return rc;
+    fill_log(*b.S_host, log);
+    return LCS_OK;
+}
+
+int lcs_profile(lcs_box *box, int enable)
+{
+    if (check_box(box)) return LCS_ERR_INVALID;
+    Box &b = *B(box);
+    LCS_HIP(hipStreamSynchronize(b.stream));
+    prof_collect(b);
+    b.prof.entries.clear();
+    b.prof.on = enable != 0;
+    return LCS_OK;
+}
+
+int lcs_profile_read(lcs_box *box, int index, char *name, size_t name_len, double *total_ms, int64_t *launches,
+                     double *bytes)
+{
+    if (check_box(box)) return LCS_ERR_INVALID;
+    Box &b = *B(box);
+    LCS_HIP(hipStreamSynchronize(b.stream));
+    prof_collect(b);
+    if (index < 0 || index >= (int)b.prof.entries.size()) return fail(LCS_ERR_INVALID, "no such profile entry");
+    const ProfEntry &e = b.prof.entries[index];
+    if (name && name_len) {
+        size_t n = std::min(name_len - 1, e.name.size());
+        memcpy(name, e.name.data(), n);
+        name[n] = 0;
+    }
+    if (total_ms) *total_ms = e.ms;
+    if (launches) *launches = e.launches;
+    if (bytes) *bytes = e.bytes;
+    return LCS_OK;
+}
+
+int lcs_sync(lcs_box *box)
+{
+    if (check_box(box)) return LCS_ERR_INVALID;
+    LCS_HIP(hipStreamSynchronize(B(box)->stream));
+    return LCS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1285 @@
+// Flow kernels of the MI355X backend (gfx950, wave64): periodic ghost fill,
+// staggered 4th-order RHS (+ fused RK update), HSMAC colour passes with a
+// device-side convergence test, CFL reduction, linear-power and rc-energy
+// forcing, and the builder-defined spectrum initialiser.
+//
+// All kernels are templated on the field type T (float or double).  Scalars
+// that later phases need (dt, HSMAC state, forcing means) live in the box's
+// DevScalars block so a step never has to return to the host.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "lcs_internal.h"
+
+namespace lcs {
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+__device__ inline int wrap1(int i, int n)
+{
+    int r = (i - 1) % n;
+    if (r < 0) r += n;
+    return r + 1;
+}
+
+// Sum of a double over a block of blockDim.x threads (multiple of 64),
+// deterministic for a fixed block size.  Result valid in thread 0.
+__device__ inline double block_sum(double v, double *lds)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) lds[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        int nw = (blockDim.x + 63) >> 6;
+        for (int q = 0; q < nw; q++) s += lds[q];
+    }
+    __syncthreads();
+    return s;
+}
+
+__device__ inline double block_max(double v, double *lds)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) lds[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        int nw = (blockDim.x + 63) >> 6;
+        s = lds[0];
+        for (int q = 1; q < nw; q++) s = fmax(s, lds[q]);
+    }
+    __syncthreads();
+    return s;
+}
+
+static int grid_for(long work, int block, int cap = kMaxPartBlocks)
+{
+    long g = (work + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+// ---------------------------------------------------------------------------
+// Stencils of src/diff.jl:1-31 (same operand order; the compiler may contract
+// multiply-adds, which changes results only at rounding level).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline T interp_o2(T a, T b)
+{
+    return T(0.5) * (a + b);
+}
+template <typename T>
+__device__ inline T interp_o4(T a, T b, T c, T d)
+{
+    return T(1.0 / 16.0) * (((-a + T(9) * b) + T(9) * c) - d);
+}
+template <typename T>
+__device__ inline T diff2_o4(T a, T b, T c, T d, T e)
+{
+    return T(1.0 / 12.0) * ((((-a + T(16) * b) - T(30) * c) + T(16) * d) - e);
+}
+template <typename T>
+__device__ inline T sum_flux_o4(T j1a, T j1b, T j3a, T j3b)
+{
+    return T(9.0 / 8.0) * (-j1a + j1b) - T((1.0 / 8.0) * (1.0 / 3.0)) * (-j3a + j3b);
+}
+
+// ---------------------------------------------------------------------------
+// Periodic ghost fill (lib/topologies/src/halo.jl:138-159 single process):
+// A[i] = A[mod1(i, n)] for every ghost cell of width r; FaceHalo (full=0) only
+// cells with exactly one axis outside 1..n.  One thread per ghost cell,
+// enumerated as z-slabs, then y-slabs, then x-slabs; up to four arrays.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_ghost_fill(Geom g, T *a0, T *a1, T *a2, T *a3, int narr, int r, int full,
+                             long cz, long cy, long total, const int *skip)
+{
+    if (skip && *skip) return;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int X = full ? g.nx + 2 * r : g.nx;
+    const int Y = full ? g.ny + 2 * r : g.ny;
+    const int x0 = full ? 1 - r : 1, y0 = full ? 1 - r : 1;
+    const int r2 = 2 * r;
+    int i, j, k;
+    if (t < cz) {
+        int ii = (int)(t % X);
+        long q = t / X;
+        int jj = (int)(q % Y);
+        int kk = (int)(q / Y);
+        i = x0 + ii;
+        j = y0 + jj;
+        k = kk < r ? 1 - r + kk : g.nz + 1 + kk - r;
+    } else if (t < cz + cy) {
+        t -= cz;
+        int ii = (int)(t % X);
+        long q = t / X;
+        int jj = (int)(q % r2);
+        int kk = (int)(q / r2);
+        i = x0 + ii;
+        j = jj < r ? 1 - r + jj : g.ny + 1 + jj - r;
+        k = 1 + kk;
+    } else {
+        t -= cz + cy;
+        int ii = (int)(t % r2);
+        long q = t / r2;
+        int jj = (int)(q % g.ny);
+        int kk = (int)(q / g.ny);
+        i = ii < r ? 1 - r + ii : g.nx + 1 + ii - r;
+        j = 1 + jj;
+        k = 1 + kk;
+    }
+    long d = g.at(i, j, k);
+    long s = g.at(wrap1(i, g.nx), wrap1(j, g.ny), wrap1(k, g.nz));
+    a0[d] = a0[s];
+    if (narr > 1) a1[d] = a1[s];
+    if (narr > 2) a2[d] = a2[s];
+    if (narr > 3) a3[d] = a3[s];
+}
+
+template <typename T>
+static int ghost_fill_t(Box &b, void *const *arrays, int narrays, int r, int full, const int *skip)
+{
+    const Geom &g = b.g;
+    if (r < 1 || r > g.h) return fail(LCS_ERR_INVALID, "ghost width must be in 1..halo_size");
+    if (narrays < 1 || narrays > 4) return fail(LCS_ERR_INVALID, "1..4 arrays per ghost fill");
+    long X = full ? g.nx + 2 * r : g.nx, Y = full ? g.ny + 2 * r : g.ny;
+    long cz = 2L * r * X * Y, cy = 2L * r * g.nz * X, cx = 2L * r * g.ny * g.nz;
+    long total = cz + cy + cx;
+    T *a[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int q = 0; q < narrays; q++) a[q] = (T *)arrays[q];
+    int blocks = (int)((total + 255) / 256);
+    ProfScope ps(b, "ghost_fill", 2.0 * total * narrays * sizeof(T));
+    hipLaunchKernelGGL(k_ghost_fill<T>, dim3(blocks), dim3(256), 0, b.stream, g, a[0], a[1], a[2], a[3],
+                       narrays, r, full, cz, cy, total, skip);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+int launch_ghost_fill(Box &b, void *const *arrays, int narrays, int width, int full, const int *skip)
+{
+    return b.esize == 8 ? ghost_fill_t<double>(b, arrays, narrays, width, full, skip)
+                        : ghost_fill_t<float>(b, arrays, narrays, width, full, skip);
+}
+
+// ---------------------------------------------------------------------------
+// RHS (src/flows/rhs.jl:59-291) fused over the three components.
+// One thread per interior cell; fluxes J1/J3 are recomputed where needed
+// instead of being stored in six scratch arrays.  FUSED writes
+// out = base + dUdt*dt (update!, src/flows/update.jl:255-268), otherwise
+// out = dUdt (interior layout).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RhsCoef {
+    T di[3];
+    T di2[3];
+    T Rei;
+};
+
+template <typename T>
+__device__ inline T flux_at(const T *__restrict__ A, const T *__restrict__ Q, long q, long sc, long sd,
+                            bool three)
+{
+    // J1(q) = interp_o4(A along c at q) * interp_o2(Q[q], Q[q+sd])
+    // J3(q) = interp_o4(A along c at q) * interp_o2(Q[q-sd], Q[q+2sd])
+    T ac = interp_o4(A[q - sc], A[q], A[q + sc], A[q + 2 * sc]);
+    T qq = three ? interp_o2(Q[q - sd], Q[q + 2 * sd]) : interp_o2(Q[q], Q[q + sd]);
+    return ac * qq;
+}
+
+template <typename T, bool FUSED>
+__global__ void __launch_bounds__(256) k_rhs_cell(Geom g, const T *__restrict__ U, const T *__restrict__ V,
+                                                  const T *__restrict__ W, const T *__restrict__ P,
+                                                  T *o0, T *o1, T *o2, const T *b0, const T *b1,
+                                                  const T *b2, RhsCoef<T> c, DtRef dt)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.cells()) return;
+    int i = (int)(t % g.nx) + 1;
+    long q = t / g.nx;
+    int j = (int)(q % g.ny) + 1;
+    int k = (int)(q / g.ny) + 1;
+    const long p = g.at(i, j, k);
+    const long st[3] = {1, g.sx, g.sx * g.sy};
+    const T *Vel[3] = {U, V, W};
+    T *out[3] = {o0, o1, o2};
+    const T *base[3] = {b0, b1, b2};
+    T rdt = T(0);
+    if (FUSED) rdt = T(dt.factor * (*dt.src));
+#pragma unroll
+    for (int comp = 0; comp < 3; comp++) {
+        const T *Q = Vel[comp];
+        const long sc = st[comp];
+        T d2[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            long s = st[d];
+            d2[d] = diff2_o4(Q[p - 2 * s], Q[p - s], Q[p], Q[p + s], Q[p + 2 * s]);
+        }
+        T diff = ((c.di2[0] * d2[0] + c.di2[1] * d2[1]) + c.di2[2] * d2[2]) * c.Rei;
+        T pres = c.di[comp] * (-P[p] + P[p + sc]);
+        T f[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const long sd = st[d];
+            const T *A = Vel[d];
+            T j1a = flux_at(A, Q, p - sd, sc, sd, false);
+            T j1b = flux_at(A, Q, p, sc, sd, false);
+            T j3a = flux_at(A, Q, p - 2 * sd, sc, sd, true);
+            T j3b = flux_at(A, Q, p + sd, sc, sd, true);
+            f[d] = sum_flux_o4(j1a, j1b, j3a, j3b);
+        }
+        T advect = (c.di[0] * f[0] + c.di[1] * f[1]) + c.di[2] * f[2];
+        T dq = (diff - pres) - advect;
+        if (FUSED) {
+            T du = dq * rdt;
+            out[comp][p] = base[comp][p] + du;
+        } else {
+            out[comp][g.interior(i, j, k)] = dq;
+        }
+    }
+}
+
+template <typename T>
+static RhsCoef<T> rhs_coef(const Box &b)
+{
+    RhsCoef<T> c;
+    double d[3] = {spacing(b.g.nx), spacing(b.g.ny), spacing(b.g.nz)};
+    for (int a = 0; a < 3; a++) {
+        c.di[a] = T(1.0 / d[a]);
+        c.di2[a] = c.di[a] * c.di[a];
+    }
+    c.Rei = T(1.0 / b.Re());
+    return c;
+}
+
+template <typename T>
+static int rhs_t(Box &b, int stage_in, bool fused, void *const out[3], void *const base[3], DtRef dt)
+{
+    const Geom &g = b.g;
+    void *const *in = stage_in == 1 ? &b.f[LCS_FIELD_U] : &b.f[LCS_FIELD_U2];
+    RhsCoef<T> c = rhs_coef<T>(b);
+    long cells = g.cells();
+    int blocks = (int)((cells + 255) / 256);
+    // algorithmic bytes: read U,V,W,P (+ the three base fields in stage 2), write 3
+    ProfScope ps(b, fused ? "rhs_update" : "rhs",
+                 (double)cells * sizeof(T) * (fused && stage_in == 2 ? 10 : 7));
+    if (fused)
+        hipLaunchKernelGGL((k_rhs_cell<T, true>), dim3(blocks), dim3(256), 0, b.stream, g, (const T *)in[0],
+                           (const T *)in[1], (const T *)in[2], (const T *)b.f[LCS_FIELD_P], (T *)out[0],
+                           (T *)out[1], (T *)out[2], (const T *)base[0], (const T *)base[1],
+                           (const T *)base[2], c, dt);
+    else
+        hipLaunchKernelGGL((k_rhs_cell<T, false>), dim3(blocks), dim3(256), 0, b.stream, g, (const T *)in[0],
+                           (const T *)in[1], (const T *)in[2], (const T *)b.f[LCS_FIELD_P], (T *)out[0],
+                           (T *)out[1], (T *)out[2], (const T *)nullptr, (const T *)nullptr,
+                           (const T *)nullptr, c, dt);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+int launch_rhs(Box &b, int stage_in, bool fused, void *const out[3], void *const base[3], DtRef dt)
+{
+    return b.esize == 8 ? rhs_t<double>(b, stage_in, fused, out, base, dt)
+                        : rhs_t<float>(b, stage_in, fused, out, base, dt);
+}
+
+// update! alone: out = base + dUdt*dt (src/flows/update.jl:255-268)
+template <typename T>
+__global__ void k_update(Geom g, T *o0, T *o1, T *o2, const T *b0, const T *b1, const T *b2,
+                         const T *__restrict__ d0, const T *__restrict__ d1, const T *__restrict__ d2,
+                         DtRef dt)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.cells()) return;
+    int i = (int)(t % g.nx) + 1;
+    long q = t / g.nx;
+    int j = (int)(q % g.ny) + 1;
+    int k = (int)(q / g.ny) + 1;
+    long p = g.at(i, j, k);
+    T rdt = T(dt.factor * (*dt.src));
+    T du = d0[t] * rdt, dv = d1[t] * rdt, dw = d2[t] * rdt;
+    T u = b0[p], v = b1[p], w = b2[p];
+    o0[p] = u + du;
+    o1[p] = v + dv;
+    o2[p] = w + dw;
+}
+
+int launch_update(Box &b, void *const out[3], void *const base[3], DtRef dt)
+{
+    long cells = b.g.cells();
+    int blocks = (int)((cells + 255) / 256);
+    void *const *d = &b.f[LCS_FIELD_DU];
+    ProfScope ps(b, "update", (double)cells * b.esize * 9);
+    if (b.esize == 8)
+        hipLaunchKernelGGL(k_update<double>, dim3(blocks), dim3(256), 0, b.stream, b.g, (double *)out[0],
+                           (double *)out[1], (double *)out[2], (const double *)base[0],
+                           (const double *)base[1], (const double *)base[2], (const double *)d[0],
+                           (const double *)d[1], (const double *)d[2], dt);
+    else
+        hipLaunchKernelGGL(k_update<float>, dim3(blocks), dim3(256), 0, b.stream, b.g, (float *)out[0],
+                           (float *)out[1], (float *)out[2], (const float *)base[0], (const float *)base[1],
+                           (const float *)base[2], (const float *)d[0], (const float *)d[1],
+                           (const float *)d[2], dt);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// HSMAC (src/flows/couples/hsmac.jl).  One kernel per (region, colour) pass,
+// exactly the reference's pass structure; cells of one colour share no face,
+// so a pass is race-free in place.  Each pass also reduces div^2 over the
+// cells it visits inside (1:n+1)^3 into per-block partials; a one-block check
+// kernel turns them into the RMS divergence and decides convergence on the
+// device (hs_done), after which every later HSMAC kernel of the call returns
+// immediately.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct HsCoef {
+    T dxi, dyi, dzi, dl, omega;
+};
+
+struct Box3 {
+    int lo[3], hi[3];
+};
+
+template <typename T, int COLORING>
+__global__ void __launch_bounds__(256) k_hsmac_pass(Geom g, T *U, T *V, T *W, T *P, Box3 r, int color,
+                                                   HsCoef<T> c, DtRef dt, const DevScalars *S,
+                                                   double *part)
+{
+    __shared__ double lds[4];
+    if (S->hs_done) return;
+    const int a = r.lo[0];
+    const int hw = (r.hi[0] - r.lo[0] + 2) / 2;  // cld(n, 2)
+    const int ny = r.hi[1] - r.lo[1] + 1;
+    const long total = (long)hw * ny * (r.hi[2] - r.lo[2] + 1);
+    const T dti = T(1.0 / (dt.factor * (*dt.src)));
+    const long sy = g.sx, sz = g.sx * g.sy;
+    double acc = 0.0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        int ih = (int)(t % hw);
+        long q = t / hw;
+        int j = r.lo[1] + (int)(q % ny);
+        int k = r.lo[2] + (int)(q / ny);
+        // RedBlackFast index map: lib/parallel/src/foraxes.jl:100-105
+        int i = a + 2 * ih + ((j + k - a + color) & 1);
+        if (i > r.hi[0]) continue;
+        if (COLORING == LCS_COLORING_RED_BLACK) {
+            int s = i + j + k;
+            if (s % 2 != color) continue;  // truncating remainder, foraxes.jl:86
+        }
+        long p = g.at(i, j, k);
+        T dudx = c.dxi * (-U[p - 1] + U[p]);
+        T dvdy = c.dyi * (-V[p - sy] + V[p]);
+        T dwdz = c.dzi * (-W[p - sz] + W[p]);
+        T div = (dudx + dvdy) + dwdz;
+        T dp = (-c.omega * div) * c.dl;
+        P[p] += dp * dti;
+        U[p - 1] -= dp * c.dxi;
+        U[p] += dp * c.dxi;
+        V[p - sy] -= dp * c.dyi;
+        V[p] += dp * c.dyi;
+        W[p - sz] -= dp * c.dzi;
+        W[p] += dp * c.dzi;
+        if (i >= 1 && i <= g.nx + 1 && j >= 1 && j <= g.ny + 1 && k >= 1 && k <= g.nz + 1) {
+            T d2 = div * div;
+            acc += (double)d2;
+        }
+    }
+    double s = block_sum(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+struct CheckArgs {
+    int npass;
+    int cnt[kMaxSweepPasses];
+    double len3, dxmin, epsp;
+    int itrmin;
+};
+
+__global__ void k_hsmac_check(const double *part, CheckArgs a, DevScalars *S)
+{
+    __shared__ double lds[4];
+    if (S->hs_done) return;  // uniform across the block
+    double acc = 0.0;
+    for (int q = 0; q < a.npass; q++) {
+        const double *pp = part + (long)q * kMaxPartBlocks;
+        for (int m = threadIdx.x; m < a.cnt[q]; m += blockDim.x) acc += pp[m];
+    }
+    double s = block_sum(acc, lds);
+    if (threadIdx.x == 0) {
+        double div = sqrt(s / a.len3);
+        int itr = S->hs_itr;
+        S->hs_div = div;
+        if (div * a.dxmin < a.epsp && a.itrmin <= itr)
+            S->hs_done = 1;
+        else
+            S->hs_itr = itr + 1;
+    }
+}
+
+__global__ void k_hsmac_begin(DevScalars *S)
+{
+    S->hs_done = 0;
+    S->hs_itr = 0;
+    S->hs_div = 0.0;
+}
+
+__global__ void k_hsmac_end(DevScalars *S, int slot)
+{
+    if (slot >= 0 && slot < 2) {
+        S->itr[slot] = S->hs_itr;
+        S->div[slot] = S->hs_div;
+    }
+}
+
+template <typename T>
+struct HsmacRun {
+    Box &b;
+    T *U, *V, *W, *P;
+    HsCoef<T> c;
+    DtRef dt;
+    int coloring;
+    CheckArgs ca;
+    int npass = 0;
+
+    int pass(const Box3 &r, int color)
+    {
+        int hw = (r.hi[0] - r.lo[0] + 2) / 2;
+        long total = (long)hw * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
+        if (hw <= 0 || total <= 0) {
+            ca.cnt[npass++] = 0;
+            return LCS_OK;
+        }
+        int blocks = grid_for(total, 256);
+        double *part = b.part + (long)npass * kMaxPartBlocks;
+        // algorithmic: half of the 8 element passes (U,V,W,P read+write) per cell-sweep
+        long cells = (long)(r.hi[0] - r.lo[0] + 1) * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
+        ProfScope ps(b, "hsmac_pass", 4.0 * sizeof(T) * cells);
+        if (coloring == LCS_COLORING_RED_BLACK)
+            hipLaunchKernelGGL((k_hsmac_pass<T, LCS_COLORING_RED_BLACK>), dim3(blocks), dim3(256), 0, b.stream,
+                               b.g, U, V, W, P, r, color, c, dt, b.S, part);
+        else
+            hipLaunchKernelGGL((k_hsmac_pass<T, LCS_COLORING_RED_BLACK_FAST>), dim3(blocks), dim3(256), 0,
+                               b.stream, b.g, U, V, W, P, r, color, c, dt, b.S, part);
+        LCS_HIP(hipGetLastError());
+        ca.cnt[npass++] = blocks;
+        return LCS_OK;
+    }
+    int sweep_box(const Box3 &r)
+    {
+        int rc = pass(r, 0);
+        if (rc) return rc;
+        return pass(r, 1);
+    }
+    // Topologies.faces order z-, z+, y-, y+, x-, x+ (lib/topologies/src/halo.jl:349-366)
+    int sweep_core_faces(const Box3 &d)
+    {
+        const Geom &g = b.g;
+        Box3 core = {{2, 2, 2}, {g.nx, g.ny, g.nz}};
+        int rc = sweep_box(core);
+        if (rc) return rc;
+        Box3 f[6] = {
+            {{d.lo[0], d.lo[1], d.lo[2]}, {d.hi[0], d.hi[1], 1}},
+            {{d.lo[0], d.lo[1], g.nz + 1}, {d.hi[0], d.hi[1], d.hi[2]}},
+            {{d.lo[0], d.lo[1], 2}, {d.hi[0], 1, g.nz}},
+            {{d.lo[0], g.ny + 1, 2}, {d.hi[0], d.hi[1], g.nz}},
+            {{d.lo[0], 2, 2}, {1, g.ny, g.nz}},
+            {{g.nx + 1, 2, 2}, {d.hi[0], g.ny, g.nz}},
+        };
+        for (int q = 0; q < 6; q++) {
+            rc = sweep_box(f[q]);
+            if (rc) return rc;
+        }
+        return LCS_OK;
+    }
+    int check()
+    {
+        ca.npass = npass;
+        ProfScope ps(b, "hsmac_check", 0.0);
+        hipLaunchKernelGGL(k_hsmac_check, dim3(1), dim3(256), 0, b.stream, b.part, ca, b.S);
+        LCS_HIP(hipGetLastError());
+        npass = 0;
+        return LCS_OK;
+    }
+    int fill(int width, bool withP)
+    {
+        void *arr[4] = {U, V, W, P};
+        return launch_ghost_fill(b, arr, withP ? 4 : 3, width, 1, &b.S->hs_done);
+    }
+    int poll(bool &done)
+    {
+        LCS_HIP(hipMemcpyAsync(&b.S_host->hs_done, &b.S->hs_done, sizeof(int), hipMemcpyDeviceToHost, b.stream));
+        LCS_HIP(hipStreamSynchronize(b.stream));
+        done = b.S_host->hs_done != 0;
+        return LCS_OK;
+    }
+};
+
+template <typename T>
+static int hsmac_t(Box &b, void *const Us[3], DtRef dt, const lcs_hsmac_params &prm, int slot)
+{
+    const Geom &g = b.g;
+    if (g.h < 3 && prm.time_blocking) return fail(LCS_ERR_INVALID, "time-blocked HSMAC needs halo_size >= 3");
+    double dx = spacing(g.nx), dy = spacing(g.ny), dz = spacing(g.nz);
+    T dxi = T(1.0 / dx), dyi = T(1.0 / dy), dzi = T(1.0 / dz);
+    HsmacRun<T> run{b, (T *)Us[0], (T *)Us[1], (T *)Us[2], (T *)b.f[LCS_FIELD_P], {}, dt, prm.coloring, {}};
+    run.c.dxi = dxi;
+    run.c.dyi = dyi;
+    run.c.dzi = dzi;
+    run.c.dl = T(1) / (T(2) * ((dxi * dxi + dyi * dyi) + dzi * dzi));
+    run.c.omega = T(prm.omega);
+    run.ca.len3 = (double)(g.nx + 1) * (double)(g.ny + 1) * (double)(g.nz + 1);
+    run.ca.dxmin = std::min(std::min(dx, dy), dz);
+    run.ca.epsp = prm.epsp;
+    run.ca.itrmin = prm.itrmin;
+
+    hipLaunchKernelGGL(k_hsmac_begin, dim3(1), dim3(1), 0, b.stream, b.S);
+    LCS_HIP(hipGetLastError());
+    int rc;
+    bool done = false;
+    const Box3 d3 = {{1, 1, 1}, {g.nx + 1, g.ny + 1, g.nz + 1}};
+    if (!prm.time_blocking) {
+        // _couple_naive!: hsmac.jl:77-129
+        for (int s = 0; s < prm.itrmax && !done; s++) {
+            if ((rc = run.fill(1, !prm.overlap))) return rc;
+            rc = prm.overlap ? run.sweep_core_faces(d3) : run.sweep_box(d3);
+            if (rc) return rc;
+            if ((rc = run.check())) return rc;
+            if (s >= prm.itrmin && (rc = run.poll(done))) return rc;
+        }
+    } else {
+        // _couple_time_blocking!: hsmac.jl:131-201
+        const Box3 d1 = {{-1, -1, -1}, {g.nx + 3, g.ny + 3, g.nz + 3}};
+        const Box3 d2 = {{0, 0, 0}, {g.nx + 2, g.ny + 2, g.nz + 2}};
+        for (int grp = 0; 3 * grp < prm.itrmax && !done; grp++) {
+            if ((rc = run.fill(g.h, false))) return rc;
+            rc = prm.overlap ? run.sweep_core_faces(d1) : run.sweep_box(d1);
+            if (rc) return rc;
+            if ((rc = run.check())) return rc;
+            if ((rc = run.sweep_box(d2))) return rc;
+            if ((rc = run.check())) return rc;
+            if ((rc = run.sweep_box(d3))) return rc;
+            if ((rc = run.check())) return rc;
+            if (3 * grp + 2 >= prm.itrmin && (rc = run.poll(done))) return rc;
+        }
+    }
+    hipLaunchKernelGGL(k_hsmac_end, dim3(1), dim3(1), 0, b.stream, b.S, slot);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+int launch_hsmac(Box &b, void *const Us[3], DtRef dt, const lcs_hsmac_params &prm, int slot)
+{
+    if (prm.coloring != LCS_COLORING_RED_BLACK_FAST && prm.coloring != LCS_COLORING_RED_BLACK)
+        return fail(LCS_ERR_INVALID, "unsupported coloring");
+    if (prm.itrmax < 0 || prm.itrmin < 0) return fail(LCS_ERR_INVALID, "itrmin/itrmax must be >= 0");
+    return b.esize == 8 ? hsmac_t<double>(b, Us, dt, prm, slot) : hsmac_t<float>(b, Us, dt, prm, slot);
+}
+
+// ---------------------------------------------------------------------------
+// CFL time step (src/flows/timestep.jl:18-65): FaceHalo(h) fill, then max of
+// |interp_o4| per axis, then dt on the device.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_cfl(Geom g, const T *__restrict__ U, const T *__restrict__ V,
+                                            const T *__restrict__ W, double *part)
+{
+    __shared__ double lds[4];
+    const long sy = g.sx, sz = g.sx * g.sy;
+    double m[3] = {0.0, 0.0, 0.0};
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < g.cells(); t += (long)gridDim.x * blockDim.x) {
+        int i = (int)(t % g.nx) + 1;
+        long q = t / g.nx;
+        int j = (int)(q % g.ny) + 1;
+        int k = (int)(q / g.ny) + 1;
+        long p = g.at(i, j, k);
+        T uc = interp_o4(U[p - 2], U[p - 1], U[p], U[p + 1]);
+        T vc = interp_o4(V[p - 2 * sy], V[p - sy], V[p], V[p + sy]);
+        T wc = interp_o4(W[p - 2 * sz], W[p - sz], W[p], W[p + sz]);
+        m[0] = fmax(m[0], fabs((double)uc));
+        m[1] = fmax(m[1], fabs((double)vc));
+        m[2] = fmax(m[2], fabs((double)wc));
+    }
+    for (int a = 0; a < 3; a++) {
+        double s = block_max(m[a], lds);
+        if (threadIdx.x == 0) part[(long)a * kMaxPartBlocks + blockIdx.x] = s;
+    }
+}
+
+__global__ void k_cfl_finalize(const double *part, int nblk, double cfl, double dtmax, double dx, double dy,
+                               double dz, double Re, DevScalars *S)
+{
+    __shared__ double lds[4];
+    double mm[3];
+    for (int a = 0; a < 3; a++) {
+        double m = 0.0;
+        for (int q = threadIdx.x; q < nblk; q += blockDim.x) m = fmax(m, part[(long)a * kMaxPartBlocks + q]);
+        mm[a] = block_max(m, lds);
+    }
+    if (threadIdx.x == 0) {
+        double umax = mm[0] + 1e-10, vmax = mm[1] + 1e-10, wmax = mm[2] + 1e-10;
+        S->umax[0] = umax;
+        S->umax[1] = vmax;
+        S->umax[2] = wmax;
+        double dt = dtmax;
+        dt = fmin(dt, cfl * dx / umax);
+        dt = fmin(dt, cfl * dy / vmax);
+        dt = fmin(dt, cfl * dz / wmax);
+        dt = fmin(dt, 0.5 * ((dx / 2) * (dx / 2)) / (1 / Re));
+        S->dt_flow = dt;
+        S->dt = dt;
+    }
+}
+
+int launch_cfl(Box &b, double cfl, double dtmax)
+{
+    void *arr[3] = {b.f[0], b.f[1], b.f[2]};
+    int rc = launch_ghost_fill(b, arr, 3, b.g.h, 0, nullptr);
+    if (rc) return rc;
+    int blocks = grid_for(b.g.cells(), 256);
+    ProfScope ps(b, "cfl", 3.0 * b.esize * b.g.cells());
+    if (b.esize == 8)
+        hipLaunchKernelGGL(k_cfl<double>, dim3(blocks), dim3(256), 0, b.stream, b.g, (const double *)b.f[0],
+                           (const double *)b.f[1], (const double *)b.f[2], b.part);
+    else
+        hipLaunchKernelGGL(k_cfl<float>, dim3(blocks), dim3(256), 0, b.stream, b.g, (const float *)b.f[0],
+                           (const float *)b.f[1], (const float *)b.f[2], b.part);
+    LCS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(256), 0, b.stream, b.part, blocks, cfl, dtmax,
+                       spacing(b.g.nx), spacing(b.g.ny), spacing(b.g.nz), b.Re(), b.S);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Constant-power linear forcing (src/flows/forcings/lf.jl:113-192)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_lf_means(Geom g, const T *__restrict__ U, const T *__restrict__ V,
+                                                 const T *__restrict__ W, double *part)
+{
+    __shared__ double lds[4];
+    const long sy = g.sx, sz = g.sx * g.sy;
+    double s[4] = {0, 0, 0, 0};
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < g.cells(); t += (long)gridDim.x * blockDim.x) {
+        int i = (int)(t % g.nx) + 1;
+        long q = t / g.nx;
+        int j = (int)(q % g.ny) + 1;
+        int k = (int)(q / g.ny) + 1;
+        long p = g.at(i, j, k);
+        T u = interp_o2(U[p - 1], U[p]);
+        T v = interp_o2(V[p - sy], V[p]);
+        T w = interp_o2(W[p - sz], W[p]);
+        T u2 = (u * u + v * v) + w * w;
+        s[0] += (double)u2;
+        s[1] += (double)U[p];
+        s[2] += (double)V[p];
+        s[3] += (double)W[p];
+    }
+    for (int a = 0; a < 4; a++) {
+        double r = block_sum(s[a], lds);
+        if (threadIdx.x == 0) part[(long)a * kMaxPartBlocks + blockIdx.x] = r;
+    }
+}
+
+__global__ void k_lf_finalize(const double *part, int nblk, double len, double power, DevScalars *S)
+{
+    __shared__ double lds[4];
+    double m[4];
+    for (int a = 0; a < 4; a++) {
+        double s = 0.0;
+        for (int q = threadIdx.x; q < nblk; q += blockDim.x) s += part[(long)a * kMaxPartBlocks + q];
+        m[a] = block_sum(s, lds);
+    }
+    if (threadIdx.x == 0) {
+        double u2 = m[0] / len, u = m[1] / len, v = m[2] / len, w = m[3] / len;
+        double e = 0.5 * (((u2 - u * u) - v * v) - w * w);
+        S->lf[0] = u2;
+        S->lf[1] = u;
+        S->lf[2] = v;
+        S->lf[3] = w;
+        S->lf[4] = power / (2 * e);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_lf_apply(Geom g, T *U, T *V, T *W, const DevScalars *S, DtRef dt,
+                                                 double *part)
+{
+    __shared__ double lds[4];
+    const T A = T(S->lf[4]), um = T(S->lf[1]), vm = T(S->lf[2]), wm = T(S->lf[3]);
+    const T rdt = T(dt.factor * (*dt.src));
+    double sp = 0.0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < g.cells(); t += (long)gridDim.x * blockDim.x) {
+        int i = (int)(t % g.nx) + 1;
+        long q = t / g.nx;
+        int j = (int)(q % g.ny) + 1;
+        int k = (int)(q / g.ny) + 1;
+        long p = g.at(i, j, k);
+        T u = U[p], v = V[p], w = W[p];
+        T usrc = A * (u - um), vsrc = A * (v - vm), wsrc = A * (w - wm);
+        T pw = (u * usrc + v * vsrc) + w * wsrc;
+        U[p] = u + usrc * rdt;
+        V[p] = v + vsrc * rdt;
+        W[p] = w + wsrc * rdt;
+        sp += (double)pw;
+    }
+    double r = block_sum(sp, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ void k_mean_to(const double *part, int nblk, double len, double *dst)
+{
+    __shared__ double lds[4];
+    double s = 0.0;
+    for (int q = threadIdx.x; q < nblk; q += blockDim.x) s += part[q];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) *dst = s / len;
+}
+
+int launch_force_linear(Box &b, double power, DtRef dt)
+{
+    void *arr[3] = {b.f[0], b.f[1], b.f[2]};
+    int rc = launch_ghost_fill(b, arr, 3, b.g.h, 1, nullptr);
+    if (rc) return rc;
+    int blocks = grid_for(b.g.cells(), 256);
+    double len = (double)b.g.cells();
+    {
+    ProfScope ps(b, "lf_means", 3.0 * b.esize * len);
+    if (b.esize == 8)
+        hipLaunchKernelGGL(k_lf_means<double>, dim3(blocks), dim3(256), 0, b.stream, b.g, (const double *)b.f[0],
+                           (const double *)b.f[1], (const double *)b.f[2], b.part);
+    else
+        hipLaunchKernelGGL(k_lf_means<float>, dim3(blocks), dim3(256), 0, b.stream, b.g, (const float *)b.f[0],
+                           (const float *)b.f[1], (const float *)b.f[2], b.part);
+    LCS_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_lf_finalize, dim3(1), dim3(256), 0, b.stream, b.part, blocks, len, power, b.S);
+    LCS_HIP(hipGetLastError());
+    {
+    ProfScope ps(b, "lf_apply", 6.0 * b.esize * len);
+    if (b.esize == 8)
+        hipLaunchKernelGGL(k_lf_apply<double>, dim3(blocks), dim3(256), 0, b.stream, b.g, (double *)b.f[0],
+                           (double *)b.f[1], (double *)b.f[2], b.S, dt, b.part);
+    else
+        hipLaunchKernelGGL(k_lf_apply<float>, dim3(blocks), dim3(256), 0, b.stream, b.g, (float *)b.f[0],
+                           (float *)b.f[1], (float *)b.f[2], b.S, dt, b.part);
+    LCS_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_mean_to, dim3(1), dim3(256), 0, b.stream, b.part, blocks, len, &b.S->p);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// rc-energy forcing (src/flows/forcings/rcf.jl, lib/fouriertools)
+// rcf buffer layout (doubles): [0, 3c^3) coarse grids U,V,W;
+// then hat: 6 doubles per crop slot; then crop: 2*3*ncrop.
+// ---------------------------------------------------------------------------
+// Box mean over (n/c)^3 blocks (ext/AMDGPUExt.jl:18-41): one block per coarse
+// cell and component, deterministic tree sum, times the reciprocal weight.
+template <typename T>
+__global__ void __launch_bounds__(256) k_boxmean(Geom g, const T *U, const T *V, const T *W, int c,
+                                                double *coarse)
+{
+    __shared__ double lds[4];
+    int cell = blockIdx.x, comp = blockIdx.y;
+    const T *A = comp == 0 ? U : (comp == 1 ? V : W);
+    int ic = cell % c, jc = (cell / c) % c, kc = cell / (c * c);
+    int kx = g.nx / c, ky = g.ny / c, kz = g.nz / c;
+    long nb = (long)kx * ky * kz;
+    double s = 0.0;
+    for (long t = threadIdx.x; t < nb; t += blockDim.x) {
+        int di = (int)(t % kx);
+        long q = t / kx;
+        int dj = (int)(q % ky), dk = (int)(q / ky);
+        s += (double)A[g.at(ic * kx + di + 1, jc * ky + dj + 1, kc * kz + dk + 1)];
+    }
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) {
+        T w = T(1) / T((double)kx * ky * kz);
+        coarse[(long)comp * c * c * c + cell] = (double)(T(s) * w);
+    }
+}
+
+__host__ __device__ inline int fftidx(int k, int n) { return k >= 0 ? k : n + k; }
+
+// Real DFT (FFTW sign, unnormalised) of each coarse grid at the wavevector
+// whose rfft-array index is (kx, fftindex(ky,c), fftindex(kz,c)); one block
+// per crop slot (kx in 0:kmax, ky,kz in -kmax:kmax).
+__global__ void __launch_bounds__(256) k_rcf_dft(const double *coarse, int c, int kmax, double *hat)
+{
+    __shared__ double lds[4];
+    int nk = 2 * kmax + 1;
+    int slot = blockIdx.x;
+    int kx = slot % (kmax + 1);
+    int ky = (slot / (kmax + 1)) % nk - kmax;
+    int kz = slot / ((kmax + 1) * nk) - kmax;
+    int fx = kx, fy = fftidx(ky, c), fz = fftidx(kz, c);
+    long den = (long)c * c * c;
+    long n3 = den;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (long t = threadIdx.x; t < n3; t += blockDim.x) {
+        int x = (int)(t % c), y = (int)((t / c) % c), z = (int)(t / ((long)c * c));
+        long num = (long)fx * x * c * c + (long)fy * y * c * c + (long)fz * z * c * c;
+        long m = num % den;
+        double sn, cs;
+        sincospi(-2.0 * (double)m / (double)den, &sn, &cs);
+        for (int a = 0; a < 3; a++) {
+            double v = coarse[(long)a * n3 + t];
+            acc[2 * a] += v * cs;
+            acc[2 * a + 1] += v * sn;
+        }
+    }
+    for (int q = 0; q < 6; q++) {
+        double s = block_sum(acc[q], lds);
+        if (threadIdx.x == 0) hat[6L * slot + q] = s;
+    }
+}
+
+__device__ inline int kcount_dev(int k0)
+{
+    int c = 0;
+    double lo = (k0 - 0.5) * (k0 - 0.5), hi = (k0 + 0.5) * (k0 + 0.5);
+    for (int kz = -k0; kz <= k0; kz++)
+        for (int ky = -k0; ky <= k0; ky++)
+            for (int kx = -k0; kx <= k0; kx++) {
+                int r2 = kx * kx + ky * ky + kz * kz;
+                if (lo <= r2 && r2 < hi) c++;
+            }
+    return c;
+}
+
+// enespe + shell rescaling + crop (rcf.jl:49-67, FourierTools.jl:103-157) on
+// one thread: a few hundred operations.  Aliased wavevectors (c < 2kmax+1)
+// share one rfft entry, as in the reference, via rep[].
+__global__ void k_rcf_spectrum(double *hat, int c, int kmin, int kmax, double e10, double *crop, DevScalars *S)
+{
+    if (threadIdx.x != 0) return;
+    int nk = 2 * kmax + 1;
+    int ncrop = (kmax + 1) * nk * nk;
+    auto slot_of = [&](int kx, int ky, int kz) { return kx + (kmax + 1) * ((ky + kmax) + nk * (kz + kmax)); };
+    auto rep_of = [&](int kx, int ky, int kz) {
+        int fy = fftidx(ky, c), fz = fftidx(kz, c);
+        for (int z = -kmax; z <= kmax; z++)
+            for (int y = -kmax; y <= kmax; y++)
+                if (fftidx(y, c) == fy && fftidx(z, c) == fz) return slot_of(kx, y, z);
+        return slot_of(kx, ky, kz);
+    };
+    double norm = (double)((c / 2 + 1) * 2 - 2) * c * c;
+    for (int k0 = kmin; k0 <= kmax; k0++) {
+        double e = 0;
+        double lo = (k0 - 0.5) * (k0 - 0.5), hi = (k0 + 0.5) * (k0 + 0.5);
+        for (int kz = -k0; kz <= k0; kz++)
+            for (int ky = -k0; ky <= k0; ky++)
+                for (int kx = 0; kx <= k0; kx++) {
+                    int r2 = kx * kx + ky * ky + kz * kz;
+                    if (!(lo <= r2 && r2 < hi)) continue;
+                    const double *hq = hat + 6L * rep_of(kx, ky, kz);
+                    double w = (kx == 0) ? 1 : 2;
+                    double s = 0;
+                    for (int a = 0; a < 3; a++) {
+                        double re = hq[2 * a] / norm, im = hq[2 * a + 1] / norm;
+                        double ab = re * re + im * im;
+                        s = (a == 0) ? ab : s + ab;
+                    }
+                    e += w * s;
+                }
+        double en = e / 2;
+        if (k0 - kmin < 8) S->rcf_en[k0 - kmin] = en;
+        double c1 = (double)kcount_dev(1), ck = (double)kcount_dev(k0);
+        double A = k0 + 0.5, B = k0 - 0.5;
+        double en0 = e10 * pow((double)k0, -5.0 / 3.0) * (1.5 * 1.5 * 1.5 - 0.5 * 0.5 * 0.5) /
+                     ((A * A * A) - (B * B * B)) * ck / c1;
+        double r = sqrt(en0 / fmax(en, 1e-8)) - 1.0;
+        for (int kz = -kmax; kz <= kmax; kz++)
+            for (int ky = -kmax; ky <= kmax; ky++)
+                for (int kx = 0; kx <= kmax; kx++) {
+                    int r2 = kx * kx + ky * ky + kz * kz;
+                    if (!(lo <= r2 && r2 < hi)) continue;
+                    double *hq = hat + 6L * rep_of(kx, ky, kz);
+                    for (int a = 0; a < 6; a++) hq[a] *= r;
+                }
+    }
+    for (int kz = -kmax; kz <= kmax; kz++)
+        for (int ky = -kmax; ky <= kmax; ky++)
+            for (int kx = 0; kx <= kmax; kx++) {
+                const double *hq = hat + 6L * rep_of(kx, ky, kz);
+                int o = kx + (kmax + 1) * (fftidx(ky, nk) + nk * fftidx(kz, nk));
+                for (int a = 0; a < 3; a++) {
+                    crop[2 * ((long)a * ncrop + o)] = hq[2 * a] / norm;
+                    crop[2 * ((long)a * ncrop + o) + 1] = hq[2 * a + 1] / norm;
+                }
+            }
+    S->e10 = e10;
+}
+
+struct Cx {
+    double re, im;
+};
+__device__ inline Cx cx_mul(Cx a, Cx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ inline Cx cx_pow(Cx x, int n)
+{
+    if (n == 0) return {1.0, 0.0};
+    if (n < 0) {
+        Cx r = cx_pow(x, -n);
+        return {r.re, -r.im};
+    }
+    Cx r = x;
+    for (int m = 1; m < n; m++) r = cx_mul(x, r);
+    return r;
+}
+
+constexpr int kMaxRcfModes = 128;
+struct RcfModes {
+    int n;
+    int kmin;
+    int kof[8], kn[8];
+    signed char k[kMaxRcfModes][3];
+};
+
+// Sparse inverse DFT + apply (rcf.jl:74-98; sirdft FourierTools.jl:165-208):
+// per cell, cispi phases from the global index, 2*Re(A_hat(k) e^{ik.x}) summed
+// per shell in kset order, U += src, power = u.src/dt.
+template <typename T>
+__global__ void __launch_bounds__(256) k_rcf_apply(Geom g, T *U, T *V, T *W, int c, int kmax, RcfModes md,
+                                                  const double *crop, DtRef dt, double *part)
+{
+    __shared__ double lds[4];
+    const int nk = 2 * kmax + 1, ncrop = (kmax + 1) * nk * nk;
+    const double phi = (double)(g.nx / c) / 2.0;
+    const double nxi = 1.0 / g.nx, nyi = 1.0 / g.ny, nzi = 1.0 / g.nz;
+    const double dti = 1.0 / (dt.factor * (*dt.src));
+    double sp = 0.0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < g.cells(); t += (long)gridDim.x * blockDim.x) {
+        int i = (int)(t % g.nx) + 1;
+        long q = t / g.nx;
+        int j = (int)(q % g.ny) + 1;
+        int k = (int)(q / g.ny) + 1;
+        Cx ek, ej, ei;
+        sincospi(2 * ((double)k - phi) * nzi, &ek.im, &ek.re);
+        sincospi(2 * ((double)j - phi) * nyi, &ej.im, &ej.re);
+        sincospi(2 * ((double)i - phi) * nxi, &ei.im, &ei.re);
+        double src[3] = {0, 0, 0};
+        for (int K = 0; K <= kmax - md.kmin; K++) {
+            double acc[3] = {0, 0, 0};
+            for (int m = md.kof[K]; m < md.kof[K] + md.kn[K]; m++) {
+                int kx = md.k[m][0], ky = md.k[m][1], kz = md.k[m][2];
+                Cx xyz = cx_mul(cx_mul(cx_pow(ei, kx), cx_pow(ej, ky)), cx_pow(ek, kz));
+                int o = kx + (kmax + 1) * (fftidx(ky, nk) + nk * fftidx(kz, nk));
+                for (int a = 0; a < 3; a++) {
+                    double are = crop[2 * ((long)a * ncrop + o)], aim = crop[2 * ((long)a * ncrop + o) + 1];
+                    acc[a] += 2 * (are * xyz.re - aim * xyz.im);
+                }
+            }
+            for (int a = 0; a < 3; a++) src[a] += acc[a];
+        }
+        long p = g.at(i, j, k);
+        T usrc = T(src[0]), vsrc = T(src[1]), wsrc = T(src[2]);
+        T u = U[p], v = V[p], w = W[p];
+        T pw = ((u * usrc + v * vsrc) + w * wsrc) * T(dti);
+        U[p] = u + usrc;
+        V[p] = v + vsrc;
+        W[p] = w + wsrc;
+        sp += (double)pw;
+    }
+    double r = block_sum(sp, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+static int kset_host(int k0, signed char (*out)[3], int cap)
+{
+    int n = 0;
+    double lo = (k0 - 0.5) * (k0 - 0.5), hi = (k0 + 0.5) * (k0 + 0.5);
+    for (int kz = -k0; kz <= k0; kz++)
+        for (int ky = -k0; ky <= k0; ky++)
+            for (int kx = 0; kx <= k0; kx++) {
+                int r2 = kx * kx + ky * ky + kz * kz;
+                if (!(lo <= r2 && r2 < hi)) continue;
+                int v[3] = {kx, ky, kz};
+                int sgn = 0;
+                for (int a = 0; a < 3 && sgn == 0; a++) sgn = v[a] > 0 ? 1 : (v[a] < 0 ? -1 : 0);
+                if (sgn < 0)
+                    for (int a = 0; a < 3; a++) v[a] = -v[a];
+                bool dup = false;
+                for (int m = 0; m < n && !dup; m++) dup = out[m][0] == v[0] && out[m][1] == v[1] && out[m][2] == v[2];
+                if (dup) continue;
+                if (n >= cap) return -1;
+                for (int a = 0; a < 3; a++) out[n][a] = (signed char)v[a];
+                n++;
+            }
+    return n;
+}
+
+static int rcf_buffers(Box &b, int c, int kmax)
+{
+    int nk = 2 * kmax + 1, ncrop = (kmax + 1) * nk * nk;
+    size_t need = sizeof(double) * ((size_t)3 * c * c * c + 6 * (size_t)ncrop + 6 * (size_t)ncrop);
+    if (need > b.rcf_bytes) {
+        if (b.rcf) LCS_HIP(hipFree(b.rcf));
+        b.rcf = nullptr;
+        LCS_HIP(hipMalloc(&b.rcf, need));
+        b.bytes += need - b.rcf_bytes;
+        b.rcf_bytes = need;
+    }
+    return LCS_OK;
+}
+
+int launch_boxmean(Box &b, int c, double *coarse_dev)
+{
+    const Geom &g = b.g;
+    if (c < 1 || g.nx % c || g.ny % c || g.nz % c)
+        return fail(LCS_ERR_INVALID, "coarse_size must divide the grid dims");
+    dim3 grid(c * c * c, 3);
+    ProfScope ps(b, "boxmean", 3.0 * b.esize * g.cells());
+    if (b.esize == 8)
+        hipLaunchKernelGGL(k_boxmean<double>, grid, dim3(256), 0, b.stream, g, (const double *)b.f[0],
+                           (const double *)b.f[1], (const double *)b.f[2], c, coarse_dev);
+    else
+        hipLaunchKernelGGL(k_boxmean<float>, grid, dim3(256), 0, b.stream, g, (const float *)b.f[0],
+                           (const float *)b.f[1], (const float *)b.f[2], c, coarse_dev);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+int launch_force_rcf(Box &b, double energy, int kmin, int kmax, int c, DtRef dt)
+{
+    if (kmin < 1 || kmax < kmin) return fail(LCS_ERR_INVALID, "rc-energy needs 1 <= kmin <= kmax");
+    if (kmax > c / 2) return fail(LCS_ERR_INVALID, "rc-energy needs kmax <= coarse_size/2");
+    if (kmax - kmin + 1 > 8) return fail(LCS_ERR_INVALID, "at most 8 forced shells");
+    int rc = rcf_buffers(b, c, kmax);
+    if (rc) return rc;
+    RcfModes md;
+    memset(&md, 0, sizeof(md));
+    md.kmin = kmin;
+    for (int K = kmin; K <= kmax; K++) {
+        md.kof[K - kmin] = md.n;
+        int n = kset_host(K, md.k + md.n, kMaxRcfModes - md.n);
+        if (n < 0) return fail(LCS_ERR_INVALID, "too many forced modes");
+        md.kn[K - kmin] = n;
+        md.n += n;
+    }
+    int nk = 2 * kmax + 1, ncrop = (kmax + 1) * nk * nk;
+    double *coarse = b.rcf;
+    double *hat = coarse + 3L * c * c * c;
+    double *crop = hat + 6L * ncrop;
+    if ((rc = launch_boxmean(b, c, coarse))) return rc;
+    hipLaunchKernelGGL(k_rcf_dft, dim3(ncrop), dim3(256), 0, b.stream, coarse, c, kmax, hat);
+    LCS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_rcf_spectrum, dim3(1), dim3(64), 0, b.stream, hat, c, kmin, kmax, energy, crop, b.S);
+    LCS_HIP(hipGetLastError());
+    int blocks = grid_for(b.g.cells(), 256);
+    ProfScope ps(b, "rcf_apply", 6.0 * b.esize * b.g.cells());
+    if (b.esize == 8)
+        hipLaunchKernelGGL(k_rcf_apply<double>, dim3(blocks), dim3(256), 0, b.stream, b.g, (double *)b.f[0],
+                           (double *)b.f[1], (double *)b.f[2], c, kmax, md, (const double *)crop, dt, b.part);
+    else
+        hipLaunchKernelGGL(k_rcf_apply<float>, dim3(blocks), dim3(256), 0, b.stream, b.g, (float *)b.f[0],
+                           (float *)b.f[1], (float *)b.f[2], c, kmax, md, (const double *)crop, dt, b.part);
+    LCS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_mean_to, dim3(1), dim3(256), 0, b.stream, b.part, blocks, (double)b.g.cells(), &b.S->p);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Spectrum initialiser (builder-defined input, mirrors the oracle's
+// o_spectrum_modes/o_init_spectrum): separable evaluation of
+//   F_c(x) = sum_m 2 Re(a_{m,c} e^{i k_m . x})
+// on the staggered positions, in three 1-D passes (x, then y, then z).
+// ---------------------------------------------------------------------------
+static int spectrum_modes_host(uint64_t seed, double k0, int K, std::vector<int> &kt, std::vector<double> &at)
+{
+    for (int kz = -K; kz <= K; kz++)
+        for (int ky = -K; ky <= K; ky++)
+            for (int kx = 0; kx <= K; kx++) {
+                int canonical = (kx > 0) || (kx == 0 && ky > 0) || (kx == 0 && ky == 0 && kz > 0);
+                int r2 = kx * kx + ky * ky + kz * kz;
+                if (!canonical || r2 == 0 || r2 > K * K) continue;
+                uint64_t idx = (uint64_t)(kx + (K + 1) * ((ky + K) + (2 * K + 1) * (kz + K)));
+                double a[6];
+                for (int c = 0; c < 6; c++) a[c] = 2.0 * rand_u01(seed, (uint64_t)c, idx) - 1.0;
+                double kk = std::sqrt((double)r2);
+                double amp = kk * std::exp(-(kk / k0) * (kk / k0));
+                double kv[3] = {(double)kx, (double)ky, (double)kz};
+                for (int part = 0; part < 2; part++) {
+                    double d = 0;
+                    for (int c = 0; c < 3; c++) d += kv[c] * a[2 * c + part];
+                    for (int c = 0; c < 3; c++) a[2 * c + part] = amp * (a[2 * c + part] - kv[c] * d / r2);
+                }
+                kt.push_back(kx);
+                kt.push_back(ky);
+                kt.push_back(kz);
+                for (int c = 0; c < 6; c++) at.push_back(a[c]);
+            }
+    return (int)(kt.size() / 3);
+}
+
+// pass 1: G1[(ky,kz)][x] = sum_kx a(kx,ky,kz) e^{i kx x}; table A is dense
+// over (kx in 0..K, ky, kz in -K..K) for one component (re, im).
+__global__ void k_spec_x(const double *A, int K, int nx, double xoff, double dx, double *G1)
+{
+    int nk = 2 * K + 1;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long total = (long)nk * nk * nx;
+    if (t >= total) return;
+    int x = (int)(t % nx);
+    int yz = (int)(t / nx);
+    double xp = ((double)(x + 1) - xoff) * dx;
+    double sr = 0, si = 0;
+    for (int kx = 0; kx <= K; kx++) {
+        const double *a = A + 2L * (kx + (K + 1) * (long)yz);
+        double sn, cs;
+        sincos(kx * xp, &sn, &cs);
+        sr += a[0] * cs - a[1] * sn;
+        si += a[0] * sn + a[1] * cs;
+    }
+    G1[2 * t] = sr;
+    G1[2 * t + 1] = si;
+}
+
+// pass 2: G2[kz][y][x] = sum_ky G1[(ky,kz)][x] e^{i ky y}
+__global__ void k_spec_y(const double *G1, int K, int nx, int ny, double yoff, double dy, double *G2)
+{
+    int nk = 2 * K + 1;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long total = (long)nk * ny * nx;
+    if (t >= total) return;
+    int x = (int)(t % nx);
+    int y = (int)((t / nx) % ny);
+    int kz = (int)(t / ((long)nx * ny));
+    double yp = ((double)(y + 1) - yoff) * dy;
+    double sr = 0, si = 0;
+    for (int ky = -K; ky <= K; ky++) {
+        const double *a = G1 + 2L * (x + (long)nx * ((ky + K) + nk * (long)kz));
+        double sn, cs;
+        sincos(ky * yp, &sn, &cs);
+        sr += a[0] * cs - a[1] * sn;
+        si += a[0] * sn + a[1] * cs;
+    }
+    G2[2 * t] = sr;
+    G2[2 * t + 1] = si;
+}
+
+// pass 3: F(x,y,z) = sum_kz 2 Re(G2[kz][y][x] e^{i kz z}); writes the field
+// interior and accumulates sum F^2 partials.
+template <typename T>
+__global__ void __launch_bounds__(256) k_spec_z(Geom g, const double *G2, int K, double zoff, double dz, T *F,
+                                               double *part)
+{
+    __shared__ double lds[4];
+    double ss = 0;
+    int nk = 2 * K + 1;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < g.cells(); t += (long)gridDim.x * blockDim.x) {
+        int x = (int)(t % g.nx);
+        long q = t / g.nx;
+        int y = (int)(q % g.ny);
+        int z = (int)(q / g.ny);
+        double zp = ((double)(z + 1) - zoff) * dz;
+        double s = 0;
+        for (int kz = -K; kz <= K; kz++) {
+            const double *a = G2 + 2L * (x + (long)g.nx * (y + (long)g.ny * (kz + K)));
+            double sn, cs;
+            sincos(kz * zp, &sn, &cs);
+            s += 2.0 * (a[0] * cs - a[1] * sn);
+        }
+        F[g.at(x + 1, y + 1, z + 1)] = T(s);
+        ss += s * s;
+    }
+    (void)nk;
+    double r = block_sum(ss, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+template <typename T>
+__global__ void k_scale_interior(Geom g, T *U, T *V, T *W, double scale)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.cells()) return;
+    int i = (int)(t % g.nx) + 1;
+    long q = t / g.nx;
+    int j = (int)(q % g.ny) + 1;
+    int k = (int)(q / g.ny) + 1;
+    long p = g.at(i, j, k);
+    U[p] = T((double)U[p] * scale);
+    V[p] = T((double)V[p] * scale);
+    W[p] = T((double)W[p] * scale);
+}
+
+template <typename T>
+static int init_spectrum_t(Box &b, uint64_t seed, double k0, double uprime)
+{
+    const Geom &g = b.g;
+    int nmin = std::min(g.nx, std::min(g.ny, g.nz));
+    int K = (int)std::ceil(3.0 * k0);
+    if (K > nmin / 2 - 1) K = nmin / 2 - 1;
+    if (K < 1) return fail(LCS_ERR_INVALID, "grid too small for the spectrum initialiser");
+    std::vector<int> kt;
+    std::vector<double> at;
+    int nm = spectrum_modes_host(seed, k0, K, kt, at);
+    int nk = 2 * K + 1;
+    size_t dense = (size_t)(K + 1) * nk * nk;
+    std::vector<double> A(2 * dense);
+    int nmax = std::max(g.nx, std::max(g.ny, g.nz));
+    size_t g1n = 2 * (size_t)nk * nk * nmax;
+    size_t g2n = 2 * (size_t)nk * g.ny * g.nx;
+    double *dA = nullptr, *dG1 = nullptr, *dG2 = nullptr;
+    LCS_HIP(hipMalloc(&dA, sizeof(double) * A.size()));
+    LCS_HIP(hipMalloc(&dG1, sizeof(double) * g1n));
+    LCS_HIP(hipMalloc(&dG2, sizeof(double) * g2n));
+    for (int a = 0; a < 4; a++) LCS_HIP(hipMemsetAsync(b.f[LCS_FIELD_U + (a == 3 ? LCS_FIELD_P : a)], 0, g.len * sizeof(T), b.stream));
+    double d[3] = {spacing(g.nx), spacing(g.ny), spacing(g.nz)};
+    int blocks = grid_for(g.cells(), 256);
+    std::vector<double> sums(kMaxPartBlocks);
+    double total = 0.0;
+    for (int comp = 0; comp < 3; comp++) {
+        std::fill(A.begin(), A.end(), 0.0);
+        for (int m = 0; m < nm; m++) {
+            int kx = kt[3 * m], ky = kt[3 * m + 1], kz = kt[3 * m + 2];
+            size_t q = (size_t)kx + (K + 1) * ((size_t)(ky + K) + nk * (size_t)(kz + K));
+            A[2 * q] = at[6 * m + 2 * comp];
+            A[2 * q + 1] = at[6 * m + 2 * comp + 1];
+        }
+        LCS_HIP(hipMemcpyAsync(dA, A.data(), sizeof(double) * A.size(), hipMemcpyHostToDevice, b.stream));
+        // staggering: component c sits on the face at index*d along axis c,
+        // at (index - 1/2)*d along the others
+        double off[3] = {0.5, 0.5, 0.5};
+        off[comp] = 0.0;
+        long t1 = (long)nk * nk * g.nx;
+        hipLaunchKernelGGL(k_spec_x, dim3((t1 + 255) / 256), dim3(256), 0, b.stream, dA, K, g.nx, off[0], d[0], dG1);
+        long t2 = (long)nk * g.ny * g.nx;
+        hipLaunchKernelGGL(k_spec_y, dim3((t2 + 255) / 256), dim3(256), 0, b.stream, dG1, K, g.nx, g.ny, off[1], d[1],
+                           dG2);
+        hipLaunchKernelGGL(k_spec_z<T>, dim3(blocks), dim3(256), 0, b.stream, g, dG2, K, off[2], d[2],
+                           (T *)b.f[LCS_FIELD_U + comp], b.part);
+        LCS_HIP(hipGetLastError());
+        LCS_HIP(hipMemcpyAsync(sums.data(), b.part, sizeof(double) * blocks, hipMemcpyDeviceToHost, b.stream));
+        LCS_HIP(hipStreamSynchronize(b.stream));
+        for (int q = 0; q < blocks; q++) total += sums[q];
+    }
+    double up = std::sqrt(total / (double)g.cells() / 3.0);
+    double scale = uprime / up;
+    hipLaunchKernelGGL(k_scale_interior<T>, dim3((g.cells() + 255) / 256), dim3(256), 0, b.stream, g,
+                       (T *)b.f[0], (T *)b.f[1], (T *)b.f[2], scale);
+    LCS_HIP(hipGetLastError());
+    LCS_HIP(hipStreamSynchronize(b.stream));
+    LCS_HIP(hipFree(dA));
+    LCS_HIP(hipFree(dG1));
+    LCS_HIP(hipFree(dG2));
+    return LCS_OK;
+}
+
+int launch_init_spectrum(Box &b, uint64_t seed, double k0, double uprime)
+{
+    return b.esize == 8 ? init_spectrum_t<double>(b, seed, k0, uprime) : init_spectrum_t<float>(b, seed, k0, uprime);
+}
+
+}  // namespace lcs

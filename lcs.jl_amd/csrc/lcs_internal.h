@@ -1,0 +1,200 @@
+// Internal declarations of the MI355X backend: box state, device scalar
+// block, kernel launchers.  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "lcs_mi355x.h"
+
+namespace lcs {
+
+// 2π and origin: src/LCS.jl:16-17
+constexpr double kDomainLength = 6.283185307179586;
+constexpr double kDomainOrigin = 0.0;
+// src/particles/Particles.jl:18
+constexpr int64_t kInvalidId = (int64_t(1) << 53) - 1;
+
+// Index geometry of one halo array.  Domain index (i,j,k), 1-based, maps to
+// linear offset (i-1+h) + sx*((j-1+h) + sy*(k-1+h)).
+struct Geom {
+    int nx, ny, nz, h;
+    long sx, sy;  // sx = nx+2h, sy = ny+2h
+    long len;     // elements per halo array
+    __host__ __device__ long at(int i, int j, int k) const
+    {
+        return (long)(i + h - 1) + sx * ((long)(j + h - 1) + sy * (long)(k + h - 1));
+    }
+    __host__ __device__ long interior(int i, int j, int k) const
+    {
+        return (long)(i - 1) + (long)nx * ((long)(j - 1) + (long)ny * (long)(k - 1));
+    }
+    __host__ __device__ long cells() const { return (long)nx * ny * nz; }
+};
+
+inline Geom make_geom(int nx, int ny, int nz, int h)
+{
+    Geom g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.h = h;
+    g.sx = nx + 2 * h;
+    g.sy = ny + 2 * h;
+    g.len = g.sx * g.sy * (long)(nz + 2 * h);
+    return g;
+}
+
+// Per-box scalars that stay on the device so a whole step can be enqueued
+// without host round trips.  Written and read only by kernels and by
+// explicit host copies.
+struct DevScalars {
+    double dt;        // state.dt (src/state.jl:15-38)
+    double dt_phase;  // explicit stage dt of a per-phase call
+    double t;
+    int64_t step;
+    double umax[3];
+    double dt_flow;
+    double dt_part;
+    // HSMAC call in flight
+    int hs_done;
+    int hs_itr;
+    double hs_div;
+    // log of the current step
+    int itr[2];
+    double div[2];
+    // forcing
+    double lf[5];  // <u2>, <U>, <V>, <W>, A
+    double p;
+    double e10;
+    double rcf_en[8];
+    double dtp[8];  // particle dt per profile
+};
+constexpr int kMaxProfiles = 8;
+
+// Reduction partial slots (doubles) available to one box.
+constexpr int kMaxPartBlocks = 4096;     // blocks per reducing pass
+constexpr int kMaxSweepPasses = 14;      // core 2 + 6 faces x 2 colours
+constexpr long kPartSlots = (long)kMaxPartBlocks * (kMaxSweepPasses + 4);
+
+struct ParticleProfile {
+    lcs_particle_desc desc;
+    int nc[3];  // cell dims (n / grid_ratio)
+    void *props[LCS_N_PROPERTIES];  // [0] int64 ids, others real
+    void *hashes = nullptr;         // int64 ntotal
+    void *perm = nullptr;           // int32 ntotal
+    void *starts = nullptr;         // int64 (nc+2)^3
+    void *stops = nullptr;
+    void *scratch = nullptr;        // 8 bytes * ntotal (permutation staging)
+    void *sort_tmp = nullptr;       // keys/values ping-pong + histograms
+    size_t sort_tmp_bytes = 0;
+    bool indexed = false;
+};
+
+struct Ctx {
+    int device;
+};
+
+// Live kernel timing: HIP events recorded on the box stream around each
+// launch of a named kernel, plus the launch's algorithmic HBM bytes.
+struct ProfEntry {
+    std::string name;
+    double bytes = 0;
+    int64_t launches = 0;
+    double ms = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+struct Profiler {
+    bool on = false;
+    std::vector<ProfEntry> entries;
+    std::vector<hipEvent_t> pool;
+};
+
+struct Box {
+    Ctx *ctx = nullptr;
+    lcs_box_desc desc;
+    Geom g;
+    int esize;  // 4 or 8
+    hipStream_t stream = nullptr;
+    void *f[10] = {};  // LCS_FIELD_* device pointers (DU..DW lazily allocated)
+    DevScalars *S = nullptr;  // device scalar block
+    DevScalars *S_host = nullptr;  // pinned mirror for readback
+    double *part = nullptr;  // reduction partials (kPartSlots)
+    double *rcf = nullptr;   // coarse grids + cropped spectra
+    size_t rcf_bytes = 0;
+    std::vector<ParticleProfile> profiles;
+    Profiler prof;
+    size_t bytes = 0;
+    double Re() const { return desc.U0_m_s * desc.L0_m / desc.NU_m2_s; }
+};
+
+// ---- error handling --------------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define LCS_HIP(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return ::lcs::fail(LCS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// Records start/stop events around the launches in its scope when the box
+// profiler is on (lcs_profile); a no-op otherwise.
+struct ProfScope {
+    Box &b;
+    int idx = -1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ProfScope(Box &box, const char *name, double bytes);
+    ~ProfScope();
+};
+
+// ---- launchers (lcs_flow.hip) ----------------------------------------------
+// Where a kernel needs the stage dt it reads factor * (*dtsrc) on the device.
+struct DtRef {
+    const double *src;
+    double factor;
+};
+
+int launch_ghost_fill(Box &b, void *const *arrays, int narrays, int width, int full,
+                      const int *skip_flag);
+int launch_rhs(Box &b, int stage_in, bool fused, void *const out[3], void *const base[3], DtRef dt);
+int launch_update(Box &b, void *const out[3], void *const base[3], DtRef dt);
+int launch_cfl(Box &b, double cfl, double dtmax);  // writes S->umax, S->dt_flow
+int launch_hsmac(Box &b, void *const Us[3], DtRef dt, const lcs_hsmac_params &prm, int log_slot);
+int launch_force_linear(Box &b, double power, DtRef dt);  // writes S->lf, S->p
+int launch_boxmean(Box &b, int c, double *coarse_dev);
+int launch_force_rcf(Box &b, double energy, int kmin, int kmax, int c, DtRef dt);
+int launch_init_spectrum(Box &b, uint64_t seed, double k0, double uprime);
+
+// ---- launchers (lcs_particles.hip) -----------------------------------------
+int launch_p_boundary(Box &b, ParticleProfile &pp, int stage);
+int launch_p_motion(Box &b, ParticleProfile &pp, int stage);
+int launch_p_update(Box &b, ParticleProfile &pp, int stage, DtRef dt);
+int launch_p_stage(Box &b, ParticleProfile &pp, int stage, DtRef dt);
+int launch_p_timestep(Box &b, ParticleProfile &pp, double dtmax, double *dtp_dev);
+int launch_p_makeindex(Box &b, ParticleProfile &pp, int stage);
+int launch_p_init(Box &b, ParticleProfile &pp, uint64_t seed, double st_min, double st_max,
+                  double eps, int zero_slip);
+int launch_finalize_dt(Box &b, int nprofiles, double dtmax);
+
+// ---- counter-based RNG shared with the oracle ------------------------------
+__host__ __device__ inline uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__host__ __device__ inline double rand_u01(uint64_t seed, uint64_t stream, uint64_t idx)
+{
+    uint64_t z = mix64(seed ^ mix64(stream ^ mix64(idx)));
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__host__ __device__ inline double spacing(int n) { return kDomainLength / (double)n; }
+
+}  // namespace lcs

@@ -1,0 +1,677 @@
+// Particle kernels of the MI355X backend: periodic wrap, trilinear
+// interpolation + Stokes/Schiller-Naumann drag, RK update (separate phases and
+// a fused per-stage kernel), particle dt, and cell re-binning with a
+// hand-written stable LSD radix sort (wave64 ballot ranking).
+#include <algorithm>
+#include <cmath>
+
+#include "lcs_internal.h"
+
+namespace lcs {
+
+static int pgrid(long n, int block = 256)
+{
+    long g = (n + block - 1) / block;
+    return (int)std::max(1L, g);
+}
+
+// ---------------------------------------------------------------------------
+// apply_periodic_boundary!: src/particles/boundary.jl:81-120
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline void wrap_axis(T &x, T &x2)
+{
+    const T L = T(kDomainLength), O = T(kDomainOrigin), E = T(kDomainOrigin + kDomainLength);
+    T x0 = x, s = x2;
+    if (x0 < O) {
+        x = x0 + L;
+        x2 = s + L;
+    }
+    if (x0 >= E) {
+        x = x0 - L;
+        x2 = s - L;
+    }
+}
+
+template <typename T>
+__global__ void k_p_boundary(long n, T *xs, T *ys, T *zs, T *xs2, T *ys2, T *zs2)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    T x = xs[p], y = ys[p], z = zs[p], x2 = xs2[p], y2 = ys2[p], z2 = zs2[p];
+    wrap_axis(x, x2);
+    wrap_axis(y, y2);
+    wrap_axis(z, z2);
+    xs[p] = x;
+    ys[p] = y;
+    zs[p] = z;
+    xs2[p] = x2;
+    ys2[p] = y2;
+    zs2[p] = z2;
+}
+
+// ---------------------------------------------------------------------------
+// Interpolation (src/particles/interp.jl:62-126)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline T trilinear(const Geom &g, const T *__restrict__ A, int i, int j, int k, T r1, T r2, T r3)
+{
+    long p = g.at(i, j, k);
+    const long sy = g.sx, sz = g.sx * g.sy;
+    T c0 = A[p], c1 = A[p + 1], c2 = A[p + 1 + sy], c3 = A[p + sy];
+    T c4 = A[p + sz], c5 = A[p + 1 + sz], c6 = A[p + 1 + sy + sz], c7 = A[p + sy + sz];
+    const T o = T(1);
+    T s = (((o - r1) * (o - r2)) * (o - r3)) * c0;
+    s = s + ((r1 * (o - r2)) * (o - r3)) * c1;
+    s = s + ((r1 * r2) * (o - r3)) * c2;
+    s = s + (((o - r1) * r2) * (o - r3)) * c3;
+    s = s + (((o - r1) * (o - r2)) * r3) * c4;
+    s = s + ((r1 * (o - r2)) * r3) * c5;
+    s = s + ((r1 * r2) * r3) * c6;
+    s = s + (((o - r1) * r2) * r3) * c7;
+    return s;
+}
+
+// unsafe_floor (src/utils/fast.jl:49-52): Int32(floor(x))
+template <typename T>
+__device__ inline int ufloor(T x)
+{
+    return (int)floor(x);
+}
+
+template <typename T>
+__device__ inline T interp_axis(const Geom &g, const T *__restrict__ A, int axis, T x, T y, T z, T dx, T dy, T dz)
+{
+    const T h5 = T(0.5);
+    // IEEE divisions (never reciprocal multiplies): cell indices must be bit-exact
+    T qx = x / dx, qy = y / dy, qz = z / dz;
+    int i, j, k;
+    T r1, r2, r3;
+    if (axis == 0) {  // xmapping
+        i = ufloor(qx);
+        j = ufloor(qy + h5);
+        k = ufloor(qz + h5);
+        r1 = qx - T(i);
+        r2 = (qy - T(j)) + h5;
+        r3 = (qz - T(k)) + h5;
+    } else if (axis == 1) {  // ymapping
+        j = ufloor(qy);
+        k = ufloor(qz + h5);
+        i = ufloor(qx + h5);
+        r2 = qy - T(j);
+        r3 = (qz - T(k)) + h5;
+        r1 = (qx - T(i)) + h5;
+    } else {  // zmapping
+        k = ufloor(qz);
+        i = ufloor(qx + h5);
+        j = ufloor(qy + h5);
+        r3 = qz - T(k);
+        r2 = (qy - T(j)) + h5;
+        r1 = (qx - T(i)) + h5;
+    }
+    // A position outside the ghost-covered range (the reference leaves this
+    // undefined) yields NaN instead of an out-of-bounds load.
+    if (i < 1 - g.h || i > g.nx + g.h - 2 || j < 1 - g.h || j > g.ny + g.h - 2 || k < 1 - g.h ||
+        k > g.nz + g.h - 2)
+        return T(NAN);
+    return trilinear(g, A, i, j, k, r1, r2, r3);
+}
+
+template <typename T>
+struct MotionCoef {
+    T dx, dy, dz, Re, g, kap;  // taup = kap * d^2 (src/particles/config.jl:41)
+    int drag;
+};
+
+// motion! per particle (src/particles/motion.jl:188-206)
+template <typename T>
+__device__ inline void motion_one(const Geom &g, const T *U, const T *V, const T *W, const MotionCoef<T> &c, T x,
+                                  T y, T z, T up, T vp, T wp, T diam, T &du, T &dv, T &dw)
+{
+    x = x - T(kDomainOrigin);
+    y = y - T(kDomainOrigin);
+    z = z - T(kDomainOrigin);
+    T uf = interp_axis(g, U, 0, x, y, z, c.dx, c.dy, c.dz);
+    T vf = interp_axis(g, V, 1, x, y, z, c.dx, c.dy, c.dz);
+    T wf = interp_axis(g, W, 2, x, y, z, c.dx, c.dy, c.dz);
+    T a = uf - up, b = vf - vp, e = wf - wp;
+    T vrel = sqrt((a * a + b * b) + e * e);
+    T Rep = (diam * vrel) * c.Re;
+    T f = c.drag ? (T(1.0) + T(0.15) * T(pow((double)Rep, 0.687))) : T(1.0);
+    T taup = c.kap * (diam * diam);
+    T taui = T(1) / taup;
+    du = ((f * (uf - up)) * taui) - c.g;
+    dv = (f * (vf - vp)) * taui;
+    dw = (f * (wf - wp)) * taui;
+}
+
+template <typename T>
+__global__ void k_p_motion(Geom g, const T *U, const T *V, const T *W, MotionCoef<T> c, long n, const T *xs,
+                           const T *ys, const T *zs, const T *us, const T *vs, const T *ws, const T *diams,
+                           T *dudts, T *dvdts, T *dwdts)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    T du, dv, dw;
+    motion_one(g, U, V, W, c, xs[p], ys[p], zs[p], us[p], vs[p], ws[p], diams[p], du, dv, dw);
+    dudts[p] = du;
+    dvdts[p] = dv;
+    dwdts[p] = dw;
+}
+
+// update! (src/particles/update.jl:258-272)
+template <typename T>
+__global__ void k_p_update(long n, DtRef dt, T *xs, T *ys, T *zs, const T *xs0, const T *ys0, const T *zs0, T *us,
+                           T *vs, T *ws, const T *us0, const T *vs0, const T *ws0, const T *dudts, const T *dvdts,
+                           const T *dwdts)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    T rdt = T(dt.factor * (*dt.src));
+    T x0 = xs0[p], y0 = ys0[p], z0 = zs0[p];
+    T u0 = us0[p], v0 = vs0[p], w0 = ws0[p];
+    T a = dudts[p], b = dvdts[p], e = dwdts[p];
+    xs[p] = x0 + rdt * u0;
+    ys[p] = y0 + rdt * v0;
+    zs[p] = z0 + rdt * w0;
+    us[p] = u0 + rdt * a;
+    vs[p] = v0 + rdt * b;
+    ws[p] = w0 + rdt * e;
+}
+
+// boundary! + motion! + update! of one RK stage in one pass (src/evolve.jl:83-91,
+// 110-118).  Stage 1: wrap xss (shadow xss2), forces from Us at xss/uss,
+// xss2 = xss + dt uss, uss2 = uss + dt f.  Stage 2: wrap xss2 (shadow xss),
+// forces from Us2 at xss2/uss2, xss = xss + dt uss, uss = uss + dt f.
+template <typename T>
+__global__ void k_p_stage(Geom g, const T *U, const T *V, const T *W, MotionCoef<T> c, long n, int stage,
+                          DtRef dt, T *xs, T *ys, T *zs, T *xs2, T *ys2, T *zs2, T *us, T *vs, T *ws, T *us2,
+                          T *vs2, T *ws2, const T *diams, T *dudts, T *dvdts, T *dwdts)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    T rdt = T(dt.factor * (*dt.src));
+    T x = xs[p], y = ys[p], z = zs[p], x2 = xs2[p], y2 = ys2[p], z2 = zs2[p];
+    T du, dv, dw;
+    if (stage == 1) {
+        wrap_axis(x, x2);
+        wrap_axis(y, y2);
+        wrap_axis(z, z2);
+        T up = us[p], vp = vs[p], wp = ws[p];
+        motion_one(g, U, V, W, c, x, y, z, up, vp, wp, diams[p], du, dv, dw);
+        xs[p] = x;
+        ys[p] = y;
+        zs[p] = z;
+        xs2[p] = x + rdt * up;
+        ys2[p] = y + rdt * vp;
+        zs2[p] = z + rdt * wp;
+        us2[p] = up + rdt * du;
+        vs2[p] = vp + rdt * dv;
+        ws2[p] = wp + rdt * dw;
+    } else {
+        wrap_axis(x2, x);
+        wrap_axis(y2, y);
+        wrap_axis(z2, z);
+        T up = us2[p], vp = vs2[p], wp = ws2[p];
+        motion_one(g, U, V, W, c, x2, y2, z2, up, vp, wp, diams[p], du, dv, dw);
+        xs2[p] = x2;
+        ys2[p] = y2;
+        zs2[p] = z2;
+        T u0 = us[p], v0 = vs[p], w0 = ws[p];
+        xs[p] = x + rdt * u0;
+        ys[p] = y + rdt * v0;
+        zs[p] = z + rdt * w0;
+        us[p] = u0 + rdt * du;
+        vs[p] = v0 + rdt * dv;
+        ws[p] = w0 + rdt * dw;
+    }
+    dudts[p] = du;
+    dvdts[p] = dv;
+    dwdts[p] = dw;
+}
+
+template <typename T>
+static MotionCoef<T> motion_coef(const Box &b, const ParticleProfile &pp)
+{
+    MotionCoef<T> c;
+    c.dx = T(spacing(b.g.nx));
+    c.dy = T(spacing(b.g.ny));
+    c.dz = T(spacing(b.g.nz));
+    c.Re = T(b.Re());
+    c.g = T(pp.desc.gravity);
+    c.kap = T(((1.0 / 18.0) * pp.desc.density_ratio) * b.Re());
+    c.drag = pp.desc.drag;
+    return c;
+}
+
+#define PR(T, q) ((T *)pp.props[q])
+
+template <typename T>
+static int p_boundary_t(Box &b, ParticleProfile &pp, int stage)
+{
+    long n = pp.desc.nvalid;
+    if (n == 0) return LCS_OK;
+    int a = stage == 1 ? LCS_PROP_X : LCS_PROP_X2, s = stage == 1 ? LCS_PROP_X2 : LCS_PROP_X;
+    hipLaunchKernelGGL(k_p_boundary<T>, dim3(pgrid(n)), dim3(256), 0, b.stream, n, PR(T, a), PR(T, a + 1),
+                       PR(T, a + 2), PR(T, s), PR(T, s + 1), PR(T, s + 2));
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+template <typename T>
+static int p_motion_t(Box &b, ParticleProfile &pp, int stage)
+{
+    long n = pp.desc.nvalid;
+    if (n == 0) return LCS_OK;
+    void *const *Us = stage == 1 ? &b.f[LCS_FIELD_U] : &b.f[LCS_FIELD_U2];
+    int x = stage == 1 ? LCS_PROP_X : LCS_PROP_X2, u = stage == 1 ? LCS_PROP_U : LCS_PROP_U2;
+    hipLaunchKernelGGL(k_p_motion<T>, dim3(pgrid(n)), dim3(256), 0, b.stream, b.g, (const T *)Us[0],
+                       (const T *)Us[1], (const T *)Us[2], motion_coef<T>(b, pp), n, PR(T, x), PR(T, x + 1),
+                       PR(T, x + 2), PR(T, u), PR(T, u + 1), PR(T, u + 2), PR(T, LCS_PROP_DIAM),
+                       PR(T, LCS_PROP_DUDT), PR(T, LCS_PROP_DVDT), PR(T, LCS_PROP_DWDT));
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+template <typename T>
+static int p_update_t(Box &b, ParticleProfile &pp, int stage, DtRef dt)
+{
+    long n = pp.desc.nvalid;
+    if (n == 0) return LCS_OK;
+    // xss_update/xss0_update/uss_update/uss0_update: src/particles/update.jl:1-11
+    int x = stage == 1 ? LCS_PROP_X2 : LCS_PROP_X, x0 = LCS_PROP_X;
+    int u = stage == 1 ? LCS_PROP_U2 : LCS_PROP_U, u0 = LCS_PROP_U;
+    hipLaunchKernelGGL(k_p_update<T>, dim3(pgrid(n)), dim3(256), 0, b.stream, n, dt, PR(T, x), PR(T, x + 1),
+                       PR(T, x + 2), PR(T, x0), PR(T, x0 + 1), PR(T, x0 + 2), PR(T, u), PR(T, u + 1),
+                       PR(T, u + 2), PR(T, u0), PR(T, u0 + 1), PR(T, u0 + 2), PR(T, LCS_PROP_DUDT),
+                       PR(T, LCS_PROP_DVDT), PR(T, LCS_PROP_DWDT));
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+template <typename T>
+static int p_stage_t(Box &b, ParticleProfile &pp, int stage, DtRef dt)
+{
+    long n = pp.desc.nvalid;
+    if (n == 0) return LCS_OK;
+    void *const *Us = stage == 1 ? &b.f[LCS_FIELD_U] : &b.f[LCS_FIELD_U2];
+    // algorithmic per particle: stage 1 reads x,u,d writes x*,u* (13), stage 2 19
+    ProfScope ps(b, "particles_stage", (double)n * sizeof(T) * (stage == 1 ? 13 : 19));
+    hipLaunchKernelGGL(k_p_stage<T>, dim3(pgrid(n)), dim3(256), 0, b.stream, b.g, (const T *)Us[0],
+                       (const T *)Us[1], (const T *)Us[2], motion_coef<T>(b, pp), n, stage, dt, PR(T, LCS_PROP_X),
+                       PR(T, LCS_PROP_Y), PR(T, LCS_PROP_Z), PR(T, LCS_PROP_X2), PR(T, LCS_PROP_Y2),
+                       PR(T, LCS_PROP_Z2), PR(T, LCS_PROP_U), PR(T, LCS_PROP_V), PR(T, LCS_PROP_W),
+                       PR(T, LCS_PROP_U2), PR(T, LCS_PROP_V2), PR(T, LCS_PROP_W2), (const T *)PR(T, LCS_PROP_DIAM),
+                       PR(T, LCS_PROP_DUDT), PR(T, LCS_PROP_DVDT), PR(T, LCS_PROP_DWDT));
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+int launch_p_boundary(Box &b, ParticleProfile &pp, int stage)
+{
+    return b.esize == 8 ? p_boundary_t<double>(b, pp, stage) : p_boundary_t<float>(b, pp, stage);
+}
+int launch_p_motion(Box &b, ParticleProfile &pp, int stage)
+{
+    return b.esize == 8 ? p_motion_t<double>(b, pp, stage) : p_motion_t<float>(b, pp, stage);
+}
+int launch_p_update(Box &b, ParticleProfile &pp, int stage, DtRef dt)
+{
+    return b.esize == 8 ? p_update_t<double>(b, pp, stage, dt) : p_update_t<float>(b, pp, stage, dt);
+}
+int launch_p_stage(Box &b, ParticleProfile &pp, int stage, DtRef dt)
+{
+    return b.esize == 8 ? p_stage_t<double>(b, pp, stage, dt) : p_stage_t<float>(b, pp, stage, dt);
+}
+
+// ---------------------------------------------------------------------------
+// Particle dt (src/particles/timestep.jl:15-25): max diameter over all ntotal
+// slots, then min(0.5 taup, dtmax); flow/particle minimum (src/timestep.jl:160-164).
+// ---------------------------------------------------------------------------
+__device__ inline double block_max_d(double v, double *lds)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) lds[w] = v;
+    __syncthreads();
+    double s = -INFINITY;
+    if (threadIdx.x == 0) {
+        int nw = (blockDim.x + 63) >> 6;
+        for (int q = 0; q < nw; q++) s = fmax(s, lds[q]);
+    }
+    __syncthreads();
+    return s;
+}
+
+template <typename T>
+__global__ void k_p_maxdiam(long n, const T *d, double *part)
+{
+    __shared__ double lds[4];
+    double m = -INFINITY;
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x)
+        m = fmax(m, (double)d[p]);
+    m = block_max_d(m, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+__global__ void k_p_dt(const double *part, int nblk, double kap, double dtmax, double *dst)
+{
+    __shared__ double lds[4];
+    double m = -INFINITY;
+    for (int q = threadIdx.x; q < nblk; q += blockDim.x) m = fmax(m, part[q]);
+    m = block_max_d(m, lds);
+    if (threadIdx.x == 0) {
+        double taup = kap * (m * m);
+        *dst = fmin(0.5 * taup, dtmax);
+    }
+}
+
+int launch_p_timestep(Box &b, ParticleProfile &pp, double dtmax, double *dtp_dev)
+{
+    long n = pp.desc.ntotal;
+    int blocks = (int)std::min<long>(kMaxPartBlocks, std::max(1L, (n + 255) / 256));
+    double *part = b.part + 3L * kMaxPartBlocks;  // keep the CFL partials intact
+    ProfScope ps(b, "particles_maxdiam", (double)n * b.esize);
+    if (b.esize == 8)
+        hipLaunchKernelGGL(k_p_maxdiam<double>, dim3(blocks), dim3(256), 0, b.stream, n,
+                           (const double *)pp.props[LCS_PROP_DIAM], part);
+    else
+        hipLaunchKernelGGL(k_p_maxdiam<float>, dim3(blocks), dim3(256), 0, b.stream, n,
+                           (const float *)pp.props[LCS_PROP_DIAM], part);
+    LCS_HIP(hipGetLastError());
+    double kap = ((1.0 / 18.0) * pp.desc.density_ratio) * b.Re();
+    hipLaunchKernelGGL(k_p_dt, dim3(1), dim3(256), 0, b.stream, part, blocks, kap, dtmax, dtp_dev);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+__global__ void k_finalize_dt(DevScalars *S, int np)
+{
+    double dt = S->dt_flow;
+    for (int q = 0; q < np; q++) dt = fmin(dt, S->dtp[q]);
+    S->dt = dt;
+}
+
+int launch_finalize_dt(Box &b, int nprofiles, double)
+{
+    hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(1), 0, b.stream, b.S, nprofiles);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Cell re-binning: makeindex! (src/particles/makeindex.jl:29-67)
+// ---------------------------------------------------------------------------
+// hash = encode(hasher, floor((x - x0)/dxc) + 1) over axes 0:nc+1
+// (src/utils/hasher.jl:82-105); keys are < 2^32 (hasher.jl:74-75).
+template <typename T>
+__global__ void k_hash(long n, const T *xs, const T *ys, const T *zs, T dxc0, T dxc1, T dxc2, int nc0, int nc1,
+                       unsigned *keys, unsigned *vals)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const T *X[3] = {xs, ys, zs};
+    const T dxc[3] = {dxc0, dxc1, dxc2};
+    const int nc[2] = {nc0, nc1};
+    int64_t idx = 1, stride = 1;
+    for (int a = 0; a < 3; a++) {
+        int c = (int)floor((X[a][p] - T(kDomainOrigin)) / dxc[a]);
+        int64_t ci = (int64_t)c + 1;
+        idx += ci * stride;
+        if (a < 2) stride *= (nc[a] + 2);
+    }
+    keys[p] = (unsigned)idx;
+    vals[p] = (unsigned)p;
+}
+
+constexpr int kSortBlock = 256;
+
+__global__ void __launch_bounds__(kSortBlock) k_rs_hist(const unsigned *keys, long n, int shift, long chunk,
+                                                      unsigned *hist, int nblk)
+{
+    __shared__ unsigned cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    long lo = (long)blockIdx.x * chunk, hi = std::min(n, lo + chunk);
+    for (long p = lo + threadIdx.x; p < hi; p += kSortBlock) atomicAdd(&cnt[(keys[p] >> shift) & 255u], 1u);
+    __syncthreads();
+    hist[(long)threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// exclusive scan of m unsigned counts in one block of 1024 threads
+__global__ void __launch_bounds__(1024) k_rs_scan(unsigned *hist, long m)
+{
+    __shared__ unsigned tot[1024];
+    long seg = (m + 1023) / 1024;
+    long lo = threadIdx.x * seg, hi = std::min(m, lo + seg);
+    unsigned s = 0;
+    for (long q = lo; q < hi; q++) s += hist[q];
+    tot[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        unsigned v = threadIdx.x >= (unsigned)o ? tot[threadIdx.x - o] : 0u;
+        __syncthreads();
+        tot[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned run = tot[threadIdx.x] - s;
+    for (long q = lo; q < hi; q++) {
+        unsigned v = hist[q];
+        hist[q] = run;
+        run += v;
+    }
+}
+
+// Stable scatter: items keep their input order within a digit.  Ranks inside
+// a 256-item tile come from 8 ballots per wave (lanes with equal digit) plus
+// per-wave digit counts in LDS.
+__global__ void __launch_bounds__(kSortBlock) k_rs_scatter(const unsigned *kin, const unsigned *vin, unsigned *kout,
+                                                         unsigned *vout, long n, int shift, long chunk,
+                                                         const unsigned *hist, int nblk)
+{
+    __shared__ unsigned run[256];
+    __shared__ unsigned wc[4][256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    run[threadIdx.x] = hist[(long)threadIdx.x * nblk + blockIdx.x];
+    long lo = (long)blockIdx.x * chunk, hi = std::min(n, lo + chunk);
+    const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (long base = lo; base < hi; base += kSortBlock) {
+        for (int q = 0; q < 4; q++) wc[q][threadIdx.x] = 0;
+        __syncthreads();
+        long p = base + threadIdx.x;
+        bool valid = p < hi;
+        unsigned key = valid ? kin[p] : 0u, val = valid ? vin[p] : 0u;
+        unsigned d = (key >> shift) & 255u;
+        unsigned long long m = __ballot(valid);
+        for (int bit = 0; bit < 8; bit++) {
+            unsigned long long bb = __ballot(valid && ((d >> bit) & 1u));
+            m &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
+        unsigned rank = (unsigned)__popcll(m & lt);
+        if (valid && rank == 0) wc[w][d] = (unsigned)__popcll(m);
+        __syncthreads();
+        unsigned pre = 0;
+        for (int q = 0; q < w; q++) pre += wc[q][d];
+        unsigned dst = run[d] + pre + rank;
+        __syncthreads();
+        if (valid) {
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+        unsigned tsum = wc[0][threadIdx.x] + wc[1][threadIdx.x] + wc[2][threadIdx.x] + wc[3][threadIdx.x];
+        run[threadIdx.x] += tsum;
+        __syncthreads();
+    }
+}
+
+template <typename E>
+__global__ void k_permute(long n, long ntotal, const unsigned *perm, const E *src, E *dst)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= ntotal) return;
+    dst[p] = p < n ? src[perm[p]] : src[p];
+}
+
+__global__ void k_store_hashes(long n, const unsigned *keys, int64_t *hashes)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    hashes[p] = (int64_t)keys[p];
+}
+
+// starts[x] = searchsortedfirst(hashes, x), stops[x] = searchsortedlast(hashes, x)
+// for every hash x (src/utils/fast.jl:160-188), 1-based.
+__global__ void k_starts_stops(const int64_t *v, long n, long ncell, int64_t *starts, int64_t *stops)
+{
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int64_t x = c + 1;
+    long lo = 1, len = n;
+    while (len != 0) {
+        long half = len >> 1;
+        long m = lo + half;
+        bool isless = v[m - 1] < x;
+        lo = isless ? m + 1 : lo;
+        len = isless ? len - (half + 1) : half;
+    }
+    starts[c] = lo;
+    long l2 = 0, h2 = n + 1;
+    while (l2 < h2 - 1) {
+        long m = l2 + ((h2 - l2) >> 1);
+        bool isless = x < v[m - 1];
+        h2 = isless ? m : h2;
+        l2 = isless ? l2 : m;
+    }
+    stops[c] = l2;
+}
+
+template <typename T>
+static int makeindex_t(Box &b, ParticleProfile &pp, int stage)
+{
+    long n = pp.desc.nvalid, ntotal = pp.desc.ntotal;
+    long ncell = (long)(pp.nc[0] + 2) * (pp.nc[1] + 2) * (pp.nc[2] + 2);
+    int nblk = (int)std::min<long>(1024, std::max(1L, (n + 4 * kSortBlock - 1) / (4 * kSortBlock)));
+    long chunk = (n + nblk - 1) / nblk;
+    chunk = ((chunk + kSortBlock - 1) / kSortBlock) * kSortBlock;
+    if (chunk == 0) chunk = kSortBlock;
+    size_t need = sizeof(unsigned) * (4 * (size_t)std::max(1L, n) + 256 * (size_t)nblk);
+    if (need > pp.sort_tmp_bytes) {
+        if (pp.sort_tmp) LCS_HIP(hipFree(pp.sort_tmp));
+        pp.sort_tmp = nullptr;
+        LCS_HIP(hipMalloc(&pp.sort_tmp, need));
+        b.bytes += need - pp.sort_tmp_bytes;
+        pp.sort_tmp_bytes = need;
+    }
+    unsigned *kA = (unsigned *)pp.sort_tmp, *vA = kA + std::max(1L, n), *kB = vA + std::max(1L, n),
+             *vB = kB + std::max(1L, n), *hist = vB + std::max(1L, n);
+    int x = stage == 1 ? LCS_PROP_X : LCS_PROP_X2;
+    double dxc[3] = {spacing(b.g.nx) * pp.desc.grid_ratio, spacing(b.g.ny) * pp.desc.grid_ratio,
+                     spacing(b.g.nz) * pp.desc.grid_ratio};
+    if (n > 0) {
+        hipLaunchKernelGGL(k_hash<T>, dim3(pgrid(n)), dim3(256), 0, b.stream, n, (const T *)pp.props[x],
+                           (const T *)pp.props[x + 1], (const T *)pp.props[x + 2], T(dxc[0]), T(dxc[1]), T(dxc[2]),
+                           pp.nc[0], pp.nc[1], kA, vA);
+        LCS_HIP(hipGetLastError());
+        int bits = 0;
+        while (bits < 32 && ((unsigned long long)ncell >> bits) != 0) bits++;
+        for (int shift = 0; shift < bits; shift += 8) {
+            hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(kSortBlock), 0, b.stream, kA, n, shift, chunk, hist, nblk);
+            hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, b.stream, hist, 256L * nblk);
+            hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(kSortBlock), 0, b.stream, kA, vA, kB, vB, n, shift,
+                               chunk, hist, nblk);
+            LCS_HIP(hipGetLastError());
+            std::swap(kA, kB);
+            std::swap(vA, vB);
+        }
+        hipLaunchKernelGGL(k_store_hashes, dim3(pgrid(n)), dim3(256), 0, b.stream, n, kA, (int64_t *)pp.hashes);
+        LCS_HIP(hipGetLastError());
+        // sortbyperm! of all 17 properties (src/utils/sort.jl:133-148): permute
+        // into the staging buffer, then swap buffers
+        for (int q = 0; q < LCS_N_PROPERTIES; q++) {
+            if (q == LCS_PROP_ID)
+                hipLaunchKernelGGL(k_permute<int64_t>, dim3(pgrid(ntotal)), dim3(256), 0, b.stream, n, ntotal, vA,
+                                   (const int64_t *)pp.props[q], (int64_t *)pp.scratch);
+            else
+                hipLaunchKernelGGL(k_permute<T>, dim3(pgrid(ntotal)), dim3(256), 0, b.stream, n, ntotal, vA,
+                                   (const T *)pp.props[q], (T *)pp.scratch);
+            LCS_HIP(hipGetLastError());
+            std::swap(pp.props[q], pp.scratch);
+        }
+        LCS_HIP(hipMemcpyAsync(pp.perm, vA, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, b.stream));
+    }
+    hipLaunchKernelGGL(k_starts_stops, dim3(pgrid(ncell)), dim3(256), 0, b.stream, (const int64_t *)pp.hashes, n,
+                       ncell, (int64_t *)pp.starts, (int64_t *)pp.stops);
+    LCS_HIP(hipGetLastError());
+    pp.indexed = true;
+    return LCS_OK;
+}
+
+int launch_p_makeindex(Box &b, ParticleProfile &pp, int stage)
+{
+    return b.esize == 8 ? makeindex_t<double>(b, pp, stage) : makeindex_t<float>(b, pp, stage);
+}
+
+// ---------------------------------------------------------------------------
+// Builder-defined particle initialiser (mirrors the oracle's o_init_particles)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_p_init(long n, long ntotal, uint64_t seed, double st_min, double st_max, double taueta,
+                         double density_ratio, double Re, T *xs, T *ys, T *zs, T *diams, int64_t *ids)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= ntotal) return;
+    if (p >= n) {
+        ids[p] = kInvalidId;
+        return;
+    }
+    xs[p] = T(kDomainLength * rand_u01(seed, 0, (uint64_t)p));
+    ys[p] = T(kDomainLength * rand_u01(seed, 1, (uint64_t)p));
+    zs[p] = T(kDomainLength * rand_u01(seed, 2, (uint64_t)p));
+    double st = st_min;
+    if (st_max != st_min) st = st_min * pow(st_max / st_min, rand_u01(seed, 3, (uint64_t)p));
+    diams[p] = T(sqrt(18.0 * st * taueta / (density_ratio * Re)));
+    ids[p] = (int64_t)p + 1;
+}
+
+template <typename T>
+__global__ void k_p_fluid(Geom g, const T *U, const T *V, const T *W, long n, const T *xs, const T *ys, const T *zs,
+                          T *us, T *vs, T *ws)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    T dx = T(spacing(g.nx)), dy = T(spacing(g.ny)), dz = T(spacing(g.nz));
+    T x = xs[p] - T(kDomainOrigin), y = ys[p] - T(kDomainOrigin), z = zs[p] - T(kDomainOrigin);
+    us[p] = interp_axis(g, U, 0, x, y, z, dx, dy, dz);
+    vs[p] = interp_axis(g, V, 1, x, y, z, dx, dy, dz);
+    ws[p] = interp_axis(g, W, 2, x, y, z, dx, dy, dz);
+}
+
+template <typename T>
+static int p_init_t(Box &b, ParticleProfile &pp, uint64_t seed, double st_min, double st_max, double eps,
+                    int zero_slip)
+{
+    long n = pp.desc.nvalid, ntotal = pp.desc.ntotal;
+    for (int q = 1; q < LCS_N_PROPERTIES; q++)
+        LCS_HIP(hipMemsetAsync(pp.props[q], 0, sizeof(T) * ntotal, b.stream));
+    double taueta = std::sqrt(1.0 / (eps * b.Re()));
+    hipLaunchKernelGGL(k_p_init<T>, dim3(pgrid(ntotal)), dim3(256), 0, b.stream, n, ntotal, seed, st_min, st_max,
+                       taueta, pp.desc.density_ratio, b.Re(), PR(T, LCS_PROP_X), PR(T, LCS_PROP_Y), PR(T, LCS_PROP_Z),
+                       PR(T, LCS_PROP_DIAM), (int64_t *)pp.props[LCS_PROP_ID]);
+    LCS_HIP(hipGetLastError());
+    if (zero_slip && n > 0) {
+        hipLaunchKernelGGL(k_p_fluid<T>, dim3(pgrid(n)), dim3(256), 0, b.stream, b.g, (const T *)b.f[0],
+                           (const T *)b.f[1], (const T *)b.f[2], n, PR(T, LCS_PROP_X), PR(T, LCS_PROP_Y),
+                           PR(T, LCS_PROP_Z), PR(T, LCS_PROP_U), PR(T, LCS_PROP_V), PR(T, LCS_PROP_W));
+        LCS_HIP(hipGetLastError());
+    }
+    return LCS_OK;
+}
+
+int launch_p_init(Box &b, ParticleProfile &pp, uint64_t seed, double st_min, double st_max, double eps,
+                  int zero_slip)
+{
+    return b.esize == 8 ? p_init_t<double>(b, pp, seed, st_min, st_max, eps, zero_slip)
+                        : p_init_t<float>(b, pp, seed, st_min, st_max, eps, zero_slip);
+}
+
+}  // namespace lcs

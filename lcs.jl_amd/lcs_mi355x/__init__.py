@@ -1,0 +1,21 @@
+"""MI355X (gfx950) backend for the LCS.jl per-time-step hot path.
+
+All computation runs in hand-written HIP kernels inside the in-tree shared
+library ``liblcs_mi355x.so`` (C ABI: include/lcs_mi355x.h).  This package is
+the host-side mirror of the reference's configuration and phase API.
+"""
+from . import config
+from ._lib import LCSArgumentError, LCSError, LIBPATH, load
+from .config import (Config, ConstantPowerLF, EnergyPreserveRCF, FlowParams, Grid, HSMAC, NoForcing,
+                     ParticleConfig, SpectrumInit, TimeStep, from_dict, readconfig)
+from .core import (Box, boundary, boxmean, couple, fill_ghost, flow_timestep, forcing, makeindex, motion,
+                   particle_timestep, particle_update, read_log, rhs, rhs_update, step, step_config, update)
+from .simulate import build, simulate
+
+__all__ = [
+    "Box", "Config", "ConstantPowerLF", "EnergyPreserveRCF", "FlowParams", "Grid", "HSMAC", "LCSArgumentError",
+    "LCSError", "LIBPATH", "NoForcing", "ParticleConfig", "SpectrumInit", "TimeStep", "boundary", "boxmean",
+    "build", "config", "couple", "fill_ghost", "flow_timestep", "forcing", "from_dict", "load", "makeindex",
+    "motion", "particle_timestep", "particle_update", "read_log", "readconfig", "rhs", "rhs_update", "simulate",
+    "step", "step_config", "update",
+]

@@ -1,0 +1,292 @@
+"""Host-side mirror of the reference's buffer and phase API over the C ABI.
+
+``Box`` plays ``LCS.Buffer`` (src/buffer.jl:13-34): it owns one periodic box's
+fields (Us, Us2, P) and particle profiles in HBM.  The module-level functions
+carry the reference's phase names and argument meaning:
+
+=====================  ==========================================================
+``fill_ghost``         ``Topologies.synchalo!`` single process (src/topologies.jl:36-45)
+``flow_timestep``      ``Flows.timestep`` (src/flows/timestep.jl:6-65)
+``rhs``                ``Flows.rhs!`` (src/flows/rhs.jl:9-81)
+``update``             ``Flows.update!`` (src/flows/update.jl:12-54)
+``couple``             ``Flows.couple!`` with ``HSMAC`` (src/flows/couples/hsmac.jl)
+``forcing``            ``Flows.forcing!`` (src/flows/forcings/{lf,rcf}.jl)
+``boundary``/``motion``/``particle_update``/``particle_timestep``/``makeindex``
+                       ``Particles.*`` (src/particles/*.jl)
+``timestep``/``evolve`` ``LCS.timestep!`` + ``LCS.evolve!`` (src/timestep.jl, src/evolve.jl)
+=====================  ==========================================================
+
+Errors raise ``LCSArgumentError`` (a ``ValueError``) for invalid arguments, as
+the reference raises ``ArgumentError``, and ``LCSError`` otherwise.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import FIELDS, PROP_ID, check
+from .config import Config, ConstantPowerLF, EnergyPreserveRCF, HSMAC, NoForcing
+
+_ctx_cache = {}
+
+
+def context(device=0):
+    """One lcs_ctx per device, created on first use."""
+    if device not in _ctx_cache:
+        L = _lib.load()
+        h = C.c_void_p()
+        check(L.lcs_ctx_create(device, C.byref(h)))
+        _ctx_cache[device] = h
+    return _ctx_cache[device]
+
+
+def _np_dtype(dtype):
+    d = np.dtype({"f64": np.float64, "f32": np.float32}.get(dtype, dtype))
+    if d not in (np.float64, np.float32):
+        raise ValueError("dtype must be float64 or float32")
+    return d
+
+
+class Box:
+    """One periodic box resident in HBM (LCS.Buffer for one process)."""
+
+    def __init__(self, dims, halo_size=3, params=None, dtype=np.float64, device=0):
+        from .config import FlowParams
+        self.L = _lib.load()
+        self.dims = tuple(int(n) for n in dims)
+        self.h = int(halo_size)
+        self.dtype = _np_dtype(dtype)
+        self.params = params or FlowParams(1.0, 1.0, 1.0)
+        desc = _lib.BoxDesc(*self.dims, self.h, _lib.LCS_F64 if self.dtype == np.float64 else _lib.LCS_F32,
+                            self.params.U0_m_s, self.params.L0_m, self.params.NU_m2_s)
+        h = C.c_void_p()
+        check(self.L.lcs_box_create(context(device), C.byref(desc), C.byref(h)))
+        self.handle = h
+        self.profiles = []
+
+    # -- memory -------------------------------------------------------------
+    @property
+    def shape(self):
+        return tuple(n + 2 * self.h for n in self.dims)
+
+    def nbytes(self):
+        v = C.c_size_t()
+        check(self.L.lcs_box_bytes(self.handle, C.byref(v)))
+        return v.value
+
+    def upload(self, name, A):
+        fid = FIELDS[name]
+        shape = self.dims if fid >= FIELDS["dU"] else self.shape
+        A = np.asfortranarray(A, dtype=self.dtype)
+        if A.shape != shape:
+            raise ValueError(f"{name}: expected shape {shape}, got {A.shape}")
+        check(self.L.lcs_field_upload(self.handle, fid, A.ctypes.data, A.nbytes))
+
+    def download(self, name):
+        fid = FIELDS[name]
+        shape = self.dims if fid >= FIELDS["dU"] else self.shape
+        A = np.empty(shape, dtype=self.dtype, order="F")
+        check(self.L.lcs_field_download(self.handle, fid, A.ctypes.data, A.nbytes))
+        return A
+
+    def interior(self, name):
+        A = self.download(name)
+        h = self.h
+        return A[h:-h, h:-h, h:-h] if h else A
+
+    def sync(self):
+        check(self.L.lcs_sync(self.handle))
+
+    def profile(self, enable=True):
+        """Start (or stop and clear) live per-kernel timing on the box stream."""
+        check(self.L.lcs_profile(self.handle, int(bool(enable))))
+
+    def profile_read(self):
+        """{kernel: (total_ms, launches, algorithmic_bytes)} since profile(True)."""
+        out = {}
+        i = 0
+        buf = C.create_string_buffer(64)
+        while True:
+            ms = C.c_double()
+            n = C.c_int64()
+            by = C.c_double()
+            rc = self.L.lcs_profile_read(self.handle, i, buf, 64, C.byref(ms), C.byref(n), C.byref(by))
+            if rc != 0:
+                break
+            out[buf.value.decode()] = (ms.value, n.value, by.value)
+            i += 1
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.L.lcs_box_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- particles ---------------------------------------------------------
+    def add_particles(self, nvalid, ntotal=None, gravity=0.0, density_ratio=843.0, drag="linear", grid_ratio=4):
+        ntotal = nvalid if ntotal is None else ntotal
+        d = _lib.ParticleDesc(int(nvalid), int(ntotal), float(gravity), float(density_ratio), _lib.DRAG[drag],
+                              int(grid_ratio))
+        idx = C.c_int()
+        check(self.L.lcs_particles_add(self.handle, C.byref(d), C.byref(idx)))
+        self.profiles.append(dict(nvalid=int(nvalid), ntotal=int(ntotal), grid_ratio=int(grid_ratio)))
+        return idx.value
+
+    def _pdt(self, prop):
+        return np.int64 if prop == "ids" else self.dtype
+
+    def upload_prop(self, profile, prop, a):
+        nt = self.profiles[profile]["ntotal"]
+        a = np.ascontiguousarray(a, dtype=self._pdt(prop))
+        if a.shape != (nt,):
+            raise ValueError(f"{prop}: expected shape ({nt},), got {a.shape}")
+        check(self.L.lcs_particles_upload(self.handle, profile, PROP_ID[prop], a.ctypes.data, a.nbytes))
+
+    def download_prop(self, profile, prop):
+        nt = self.profiles[profile]["ntotal"]
+        a = np.empty(nt, dtype=self._pdt(prop))
+        check(self.L.lcs_particles_download(self.handle, profile, PROP_ID[prop], a.ctypes.data, a.nbytes))
+        return a
+
+    def cellindex(self, profile):
+        nv = self.profiles[profile]["nvalid"]
+        n = C.c_int64()
+        check(self.L.lcs_particles_cellindex(self.handle, profile, None, None, None, C.byref(n)))
+        hashes = np.zeros(max(nv, 1), np.int64)
+        starts = np.zeros(n.value, np.int64)
+        stops = np.zeros(n.value, np.int64)
+        check(self.L.lcs_particles_cellindex(self.handle, profile, hashes.ctypes.data, starts.ctypes.data,
+                                             stops.ctypes.data, None))
+        return hashes[:nv], starts, stops
+
+    # -- initial conditions (builder-defined) ------------------------------------
+    def init_spectrum(self, seed, k0=4.0, uprime=1.0):
+        check(self.L.lcs_init_spectrum(self.handle, int(seed), float(k0), float(uprime)))
+
+    def init_particles(self, profile, seed, st_min=1.0, st_max=1.0, eps=0.5, zero_slip=True):
+        check(self.L.lcs_init_particles(self.handle, profile, int(seed), float(st_min), float(st_max), float(eps),
+                                        int(bool(zero_slip))))
+
+
+# ---------------------------------------------------------------------------
+# Phase functions
+# ---------------------------------------------------------------------------
+def fill_ghost(box, names, width, full=True):
+    mask = 0
+    for n in names:
+        mask |= 1 << FIELDS[n]
+    check(box.L.lcs_fill_ghost(box.handle, mask, int(width), int(bool(full))))
+
+
+def flow_timestep(box, cfl, dtmax):
+    dt = C.c_double()
+    um = (C.c_double * 3)()
+    check(box.L.lcs_flow_timestep(box.handle, float(cfl), float(dtmax), C.byref(dt), um))
+    return dt.value, tuple(um)
+
+
+def rhs(box, stage):
+    check(box.L.lcs_rhs(box.handle, int(stage)))
+
+
+def update(box, stage, dt_stage):
+    check(box.L.lcs_update(box.handle, int(stage), float(dt_stage)))
+
+
+def rhs_update(box, stage, dt_stage):
+    check(box.L.lcs_rhs_update(box.handle, int(stage), float(dt_stage)))
+
+
+def _hsmac_params(c):
+    return _lib.HsmacParams(int(c.time_blocking), int(c.overlap), c.coloring_id, float(c.omega), float(c.epsp),
+                            int(c.itrmin), int(c.itrmax))
+
+
+def couple(box, hsmac: HSMAC, stage, dt_stage):
+    """Returns the reference's (; itr, div)."""
+    p = _hsmac_params(hsmac)
+    itr = C.c_int()
+    div = C.c_double()
+    check(box.L.lcs_hsmac(box.handle, int(stage), float(dt_stage), C.byref(p), C.byref(itr), C.byref(div)))
+    return {"itr": itr.value, "div": div.value}
+
+
+def forcing(box, f, dt):
+    p = C.c_double()
+    if isinstance(f, ConstantPowerLF):
+        check(box.L.lcs_force_linear(box.handle, float(f.power), float(dt), C.byref(p)))
+        return {"p": p.value}
+    if isinstance(f, EnergyPreserveRCF):
+        check(box.L.lcs_force_rcf(box.handle, float(f.energy), int(f.kmin), int(f.kmax), int(f.coarse_size),
+                                  float(dt), C.byref(p)))
+        return {"p": p.value, "e10": f.energy}
+    if isinstance(f, NoForcing):
+        return {}
+    raise ValueError(f"unknown forcing {f!r}")
+
+
+def boxmean(box, coarse_size):
+    c = int(coarse_size)
+    out = np.zeros(3 * c ** 3)
+    check(box.L.lcs_boxmean(box.handle, c, out.ctypes.data))
+    return [out[a * c ** 3:(a + 1) * c ** 3].reshape((c, c, c), order="F") for a in range(3)]
+
+
+def boundary(box, profile, stage):
+    check(box.L.lcs_particles_boundary(box.handle, profile, int(stage)))
+
+
+def motion(box, profile, stage):
+    check(box.L.lcs_particles_motion(box.handle, profile, int(stage)))
+
+
+def particle_update(box, profile, stage, dt_stage):
+    check(box.L.lcs_particles_update(box.handle, profile, int(stage), float(dt_stage)))
+
+
+def particle_timestep(box, profile, dtmax):
+    dt = C.c_double()
+    check(box.L.lcs_particles_timestep(box.handle, profile, float(dtmax), C.byref(dt)))
+    return dt.value
+
+
+def makeindex(box, profile, stage=1):
+    check(box.L.lcs_particles_makeindex(box.handle, profile, int(stage)))
+
+
+# ---------------------------------------------------------------------------
+# Whole step
+# ---------------------------------------------------------------------------
+def step_config(config: Config):
+    f = config.flow.forcing
+    fk = _lib.Forcing(_lib.FORCING[f.kind], getattr(f, "power", 0.0), getattr(f, "energy", 0.0),
+                      getattr(f, "kmin", 0), getattr(f, "kmax", 0), getattr(f, "coarse_size", 0))
+    return _lib.StepConfig(config.timestep.cfl, config.timestep.dtmax, _hsmac_params(config.flow.couple), fk)
+
+
+def step(box, scfg, want_log=True):
+    """LCS.timestep! + LCS.evolve! on the device; returns the step log."""
+    log = _lib.StepLog()
+    check(box.L.lcs_step(box.handle, C.byref(scfg), C.byref(log) if want_log else None))
+    if not want_log:
+        return None
+    return log_dict(log)
+
+
+def read_log(box):
+    log = _lib.StepLog()
+    check(box.L.lcs_step_log_read(box.handle, C.byref(log)))
+    return log_dict(log)
+
+
+def log_dict(log):
+    return {"step": log.step, "t": log.t, "dt": log.dt,
+            "flow": {"couple1": {"itr": log.itr1, "div": log.div1},
+                     "forcing": {"p": log.p, "e10": log.e10},
+                     "couple2": {"itr": log.itr2, "div": log.div2}}}

@@ -1,0 +1,317 @@
+"""ctypes bindings for the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg may
+import this module; it is the checker, never the product path.  The C
+restatement lives in ``lcs_oracle.c`` (built by ``make -C oracle``); this module
+only marshals numpy arrays and composes the phases in the order the reference
+calls them (``src/evolve.jl:63-146``, ``src/timestep.jl:160-173``).
+
+Arrays use the reference layout: Fortran order (x fastest) with halo ``h`` on
+every side, shape ``(nx+2h, ny+2h, nz+2h)``; dU/dt arrays are interior-only.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_BUILD = os.path.join(_HERE, "_build")
+
+DOMAIN_LENGTH = 6.283185307179586
+INVALID = 2**53 - 1  # src/particles/Particles.jl:18
+
+
+def build():
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_libs = {}
+
+
+def lib(dtype):
+    dtype = np.dtype(dtype)
+    key = "f64" if dtype == np.float64 else "f32"
+    if key not in _libs:
+        path = os.path.join(_BUILD, f"liboracle_{key}.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        _declare(L)
+        _libs[key] = L
+    return _libs[key]
+
+
+P = C.c_void_p
+I = C.c_int
+D = C.c_double
+Lg = C.c_long
+U64 = C.c_uint64
+
+
+def _declare(L):
+    sig = {
+        "o_ghost_fill": (None, [P, I, I, I, I, I, I]),
+        "o_rhs": (None, [I, I, I, I, D, P, P, P, P, P, P, P]),
+        "o_update": (None, [I, I, I, I, D, P, P, P, P, P, P, P, P, P]),
+        "o_hsmac": (None, [I, I, I, I, D, I, I, I, D, D, I, I, P, P, P, P, P, P]),
+        "o_flow_timestep": (D, [I, I, I, I, D, D, D, P, P, P, P]),
+        "o_force_linear": (D, [I, I, I, I, D, D, P, P, P, P]),
+        "o_boxmean": (None, [I, I, I, I, I, I, I, P, P]),
+        "o_colour_mask": (None, [P, P, I, I, P]),
+        "o_hsmac_pass": (None, [I, I, I, I, P, P, I, I, D, D, P, P, P, P, P]),
+        "o_kcount": (I, [I]),
+        "o_kset": (I, [I, P, I]),
+        "o_enespe0": (D, [I, D]),
+        "o_rcf_spectrum": (I, [I, I, I, D, P, P, P, P, P]),
+        "o_rcf_apply": (D, [I, I, I, I, I, I, I, D, P, P, P, P]),
+        "o_p_boundary": (None, [Lg, P, P, P, P, P, P]),
+        "o_p_fluid_velocity": (None, [I, I, I, I, P, P, P, Lg, P, P, P, P, P, P]),
+        "o_p_motion": (None, [I, I, I, I, D, D, D, I, P, P, P, Lg, P, P, P, P, P, P, P, P, P, P]),
+        "o_p_update": (None, [Lg, D] + [P] * 15),
+        "o_p_timestep": (D, [Lg, P, D, D, D]),
+        "o_hash": (None, [I, I, I, I, Lg, P, P, P, P]),
+        "o_makeindex": (None, [I, I, I, I, Lg, P, P, P, P, P, I, P, P, P, P]),
+        "o_searchsortedfirst": (Lg, [P, Lg, C.c_int64]),
+        "o_searchsortedlast": (Lg, [P, Lg, C.c_int64]),
+        "o_rand_u01": (D, [U64, U64, U64]),
+        "o_spectrum_kcut": (I, [I, D]),
+        "o_spectrum_modes": (I, [U64, D, I, P, P, I]),
+        "o_init_spectrum": (None, [I, I, I, I, U64, D, D, P, P, P, P]),
+        "o_init_particles": (None, [U64, Lg, D, D, D, D, D, P, P, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["F_CONTIGUOUS"] or a.ndim == 1, "oracle arrays must be Fortran-ordered"
+    return a.ctypes.data
+
+
+def dims_of(A, h):
+    return tuple(int(s) - 2 * h for s in A.shape)
+
+
+# --------------------------------------------------------------------------
+# Flow phases
+# --------------------------------------------------------------------------
+def halo_zeros(dims, h, dtype):
+    return np.zeros(tuple(n + 2 * h for n in dims), dtype=dtype, order="F")
+
+
+def ghost_fill(A, h, width, full):
+    nx, ny, nz = dims_of(A, h)
+    lib(A.dtype).o_ghost_fill(_p(A), nx, ny, nz, h, width, int(full))
+
+
+def rhs(U, V, W, P_, h, Re):
+    """Flows.rhs!: src/flows/rhs.jl:59-81. Returns interior dU, dV, dW."""
+    nx, ny, nz = dims_of(U, h)
+    out = [np.zeros((nx, ny, nz), dtype=U.dtype, order="F") for _ in range(3)]
+    lib(U.dtype).o_rhs(nx, ny, nz, h, Re, _p(U), _p(V), _p(W), _p(P_), *[_p(a) for a in out])
+    return out
+
+
+def update(Us, Us0, dUs, h, dt):
+    """Flows.update!: src/flows/update.jl:241-272 (in place on Us)."""
+    nx, ny, nz = dims_of(Us[0], h)
+    lib(Us[0].dtype).o_update(nx, ny, nz, h, dt, *[_p(a) for a in Us], *[_p(a) for a in Us0], *[_p(a) for a in dUs])
+
+
+def hsmac(U, V, W, P_, h, dt, time_blocking=True, overlap=True, coloring=0, omega=1.7, epsp=1e-3, itrmin=6, itrmax=300):
+    """couple!(::HSMAC): src/flows/couples/hsmac.jl:52-201 (in place). Returns (itr, div)."""
+    nx, ny, nz = dims_of(U, h)
+    itr = C.c_int(0)
+    div = C.c_double(0)
+    lib(U.dtype).o_hsmac(nx, ny, nz, h, dt, int(time_blocking), int(overlap), int(coloring), omega, epsp, itrmin, itrmax,
+                         _p(U), _p(V), _p(W), _p(P_), C.addressof(itr), C.addressof(div))
+    return itr.value, div.value
+
+
+def colour_mask(lo, hi, color, coloring=0):
+    lo = np.asarray(lo, dtype=np.int32)
+    hi = np.asarray(hi, dtype=np.int32)
+    shape = tuple(int(b - a + 1) for a, b in zip(lo, hi))
+    m = np.zeros(shape, dtype=np.int32, order="F")
+    lib(np.float64).o_colour_mask(_p(lo), _p(hi), color, coloring, _p(m))
+    return m
+
+
+def hsmac_pass(U, V, W, P_, Div2, h, lo, hi, color, dt, omega=1.7, coloring=0):
+    """One colour pass of hsmac_cellwise! (hsmac.jl:1-35) over the box [lo, hi]."""
+    nx, ny, nz = dims_of(U, h)
+    lo = np.asarray(lo, dtype=np.int32)
+    hi = np.asarray(hi, dtype=np.int32)
+    lib(U.dtype).o_hsmac_pass(nx, ny, nz, h, _p(lo), _p(hi), color, coloring, dt, omega, _p(U), _p(V), _p(W),
+                              _p(P_), _p(Div2))
+
+
+def flow_timestep(U, V, W, h, cfl, dtmax, Re):
+    """Flows.timestep: src/flows/timestep.jl:18-65 (fills face ghosts in place)."""
+    nx, ny, nz = dims_of(U, h)
+    m = np.zeros(3)
+    dt = lib(U.dtype).o_flow_timestep(nx, ny, nz, h, cfl, dtmax, Re, _p(U), _p(V), _p(W), _p(m))
+    return dt, m
+
+
+def force_linear(U, V, W, h, power, dt):
+    """forcing!(::ConstantPowerLF): src/flows/forcings/lf.jl:113-192. Returns p."""
+    nx, ny, nz = dims_of(U, h)
+    m = np.zeros(5)
+    p = lib(U.dtype).o_force_linear(nx, ny, nz, h, power, dt, _p(U), _p(V), _p(W), _p(m))
+    return p, m
+
+
+def boxmean(U, h, coarse):
+    nx, ny, nz = dims_of(U, h)
+    cx, cy, cz = coarse
+    out = np.zeros((cx, cy, cz), dtype=U.dtype, order="F")
+    lib(U.dtype).o_boxmean(nx, ny, nz, h, cx, cy, cz, _p(U), _p(out))
+    return out
+
+
+def kcount(k0):
+    return lib(np.float64).o_kcount(k0)
+
+
+def kset(k0):
+    cap = 4 * (2 * k0 + 1) ** 3
+    buf = np.zeros(3 * cap, dtype=np.int32)
+    n = lib(np.float64).o_kset(k0, _p(buf), cap)
+    return [tuple(int(v) for v in buf[3 * m:3 * m + 3]) for m in range(n)]
+
+
+def enespe0(k0, e10):
+    return lib(np.float64).o_enespe0(k0, e10)
+
+
+def rcf_spectrum(coarse3, kmin, kmax, e10):
+    """Spectral part of forcing!(::EnergyPreserveRCF): rcf.jl:43-67."""
+    c = coarse3[0].shape[0]
+    nk = 2 * kmax + 1
+    ncrop = (kmax + 1) * nk * nk
+    crop = np.zeros(2 * 3 * ncrop)
+    en = np.zeros(kmax - kmin + 1)
+    dt = coarse3[0].dtype
+    keep = [np.asfortranarray(a, dtype=dt) for a in coarse3]
+    rc = lib(dt).o_rcf_spectrum(c, kmin, kmax, e10, *[_p(a) for a in keep], _p(crop), _p(en))
+    if rc != 0:
+        raise ValueError("kmax must satisfy 1 <= kmin <= kmax <= coarse_size/2")
+    return crop, en
+
+
+def rcf_apply(U, V, W, h, coarse_size, kmin, kmax, dt, crop):
+    nx, ny, nz = dims_of(U, h)
+    return lib(U.dtype).o_rcf_apply(nx, ny, nz, h, coarse_size, kmin, kmax, dt, _p(crop), _p(U), _p(V), _p(W))
+
+
+def force_rcf(U, V, W, h, energy, kmin, kmax, coarse_size, dt):
+    """forcing!(::EnergyPreserveRCF): src/flows/forcings/rcf.jl:21-100. Returns (p, e10)."""
+    coarse = [boxmean(A, h, (coarse_size,) * 3) for A in (U, V, W)]
+    crop, _ = rcf_spectrum(coarse, kmin, kmax, energy)
+    p = rcf_apply(U, V, W, h, coarse_size, kmin, kmax, dt, crop)
+    return p, energy
+
+
+# --------------------------------------------------------------------------
+# Particle phases (SoA vectors of length >= n)
+# --------------------------------------------------------------------------
+def p_boundary(xss, xss2, n):
+    lib(xss[0].dtype).o_p_boundary(n, *[_p(a) for a in xss], *[_p(a) for a in xss2])
+
+
+def p_fluid_velocity(Us, h, xss, n):
+    U = Us[0]
+    nx, ny, nz = dims_of(U, h)
+    out = [np.zeros(n, dtype=U.dtype) for _ in range(3)]
+    lib(U.dtype).o_p_fluid_velocity(nx, ny, nz, h, *[_p(a) for a in Us], n, *[_p(a) for a in xss], *[_p(a) for a in out])
+    return out
+
+
+def p_motion(Us, h, Re, gravity, density_ratio, drag, xss, uss, diams, dudtss, n):
+    U = Us[0]
+    nx, ny, nz = dims_of(U, h)
+    lib(U.dtype).o_p_motion(nx, ny, nz, h, Re, gravity, density_ratio, int(drag), *[_p(a) for a in Us], n,
+                            *[_p(a) for a in xss], *[_p(a) for a in uss], _p(diams), *[_p(a) for a in dudtss])
+
+
+def p_update(xss, xss0, uss, uss0, dudtss, n, dt):
+    lib(xss[0].dtype).o_p_update(n, dt, *[_p(a) for a in xss], *[_p(a) for a in xss0], *[_p(a) for a in uss],
+                                 *[_p(a) for a in uss0], *[_p(a) for a in dudtss])
+
+
+def p_timestep(diams, Re, density_ratio, dtmax):
+    return lib(diams.dtype).o_p_timestep(len(diams), _p(diams), Re, density_ratio, dtmax)
+
+
+def hash_cells(dims, grid_ratio, xss, n):
+    out = np.zeros(n, dtype=np.int64)
+    lib(xss[0].dtype).o_hash(*dims, grid_ratio, n, *[_p(a) for a in xss], _p(out))
+    return out
+
+
+def makeindex(dims, grid_ratio, xss, ids, props, n):
+    """makeindex!: src/particles/makeindex.jl:29-67 (in place on xss, ids, props).
+
+    ``props`` are the other real SoA properties permuted together with xss.
+    Returns (hashes, perm, starts, stops); perm/starts/stops are 1-based."""
+    nc = [d // grid_ratio + 2 for d in dims]
+    ncell = nc[0] * nc[1] * nc[2]
+    hashes = np.zeros(max(n, 1), dtype=np.int64)
+    perm = np.zeros(max(n, 1), dtype=np.int64)
+    starts = np.zeros(ncell, dtype=np.int64)
+    stops = np.zeros(ncell, dtype=np.int64)
+    allp = list(props)
+    arr = (C.c_void_p * max(len(allp), 1))(*[_p(a) for a in allp])
+    lib(xss[0].dtype).o_makeindex(*dims, grid_ratio, n, *[_p(a) for a in xss], _p(ids), arr, len(allp),
+                                  _p(hashes), _p(perm), _p(starts), _p(stops))
+    return hashes[:n], perm[:n], starts, stops
+
+
+def searchsortedfirst(v, x):
+    v = np.ascontiguousarray(v, dtype=np.int64)
+    return lib(np.float64).o_searchsortedfirst(_p(v) if len(v) else None, len(v), int(x))
+
+
+def searchsortedlast(v, x):
+    v = np.ascontiguousarray(v, dtype=np.int64)
+    return lib(np.float64).o_searchsortedlast(_p(v) if len(v) else None, len(v), int(x))
+
+
+# --------------------------------------------------------------------------
+# Builder-defined initial conditions
+# --------------------------------------------------------------------------
+def rand_u01(seed, stream, idx):
+    return lib(np.float64).o_rand_u01(seed, stream, idx)
+
+
+def spectrum_modes(seed, k0, K):
+    cap = (K + 1) * (2 * K + 1) ** 2
+    kt = np.zeros(3 * cap, dtype=np.int32)
+    at = np.zeros(6 * cap)
+    n = lib(np.float64).o_spectrum_modes(seed, k0, K, _p(kt), _p(at), cap)
+    return kt[:3 * n].reshape(n, 3), at[:6 * n].reshape(n, 6)
+
+
+def spectrum_kcut(nmin, k0):
+    return lib(np.float64).o_spectrum_kcut(nmin, k0)
+
+
+def init_spectrum(dims, h, seed, k0, uprime, dtype):
+    Us = [halo_zeros(dims, h, dtype) for _ in range(4)]
+    lib(dtype).o_init_spectrum(*dims, h, seed, k0, uprime, *[_p(a) for a in Us])
+    return Us[:3], Us[3]
+
+
+def init_particles(seed, n, st_min, st_max, eps, Re, density_ratio, dtype):
+    xs = [np.zeros(n, dtype=dtype) for _ in range(3)]
+    diams = np.zeros(n, dtype=dtype)
+    ids = np.zeros(n, dtype=np.int64)
+    lib(dtype).o_init_particles(seed, n, st_min, st_max, eps, Re, density_ratio, *[_p(a) for a in xs], _p(diams), _p(ids))
+    return xs, diams, ids

@@ -1,0 +1,490 @@
+"""Pins the CPU oracle's flow phases against the reference's known-answer tests
+and against closed forms the mathematics fixes (no GPU needed).
+
+Each test names what it pins.  The reference's golden JLD2 blobs are Git-LFS
+pointers that are absent from the checkout, so the pins are: the reference's
+own KATs (colour masks, box mean, kset/kcount/enespe, sirdft vs irfft), and
+closed forms chosen so that a dropped term, a wrong sign or a shifted index
+fails (exactness on linear fields, discrete Fourier symbols, invariants).
+"""
+import numpy as np
+import pytest
+
+from conftest import interior, rand_halo, rel_l2
+from oracle import oracle as O
+
+TWO_PI = O.DOMAIN_LENGTH
+
+
+def field_from(fn, dims, h, stagger_axis, dtype=np.float64):
+    """Evaluate fn(x, y, z) on the staggered positions of a halo array:
+    face position index*d along stagger_axis, (index-1/2)*d along the others
+    (src/flows/inits/idealflow.jl:23-41).  No periodic wrap: ghosts get the
+    analytic value at their own position."""
+    h3 = [np.arange(1 - h, n + h + 1, dtype=np.float64) for n in dims]
+    d = [TWO_PI / n for n in dims]
+    pos = []
+    for a in range(3):
+        off = 0.0 if a == stagger_axis else 0.5
+        pos.append((h3[a] - off) * d[a])
+    X, Y, Z = np.meshgrid(*pos, indexing="ij")
+    return np.asfortranarray(fn(X, Y, Z).astype(dtype))
+
+
+def centre_field(fn, dims, h, dtype=np.float64):
+    return field_from(fn, dims, h, stagger_axis=-1, dtype=dtype)
+
+
+# ----------------------------------------------------------------------------
+# ghost fill: lib/topologies/test/test_mpi.jl:36-53 restated as A[i] = A[mod1(i,n)]
+# ----------------------------------------------------------------------------
+@pytest.mark.parametrize("full", [True, False])
+@pytest.mark.parametrize("width", [1, 2, 3])
+def test_ghost_fill_closed_form(rng, full, width):
+    dims, h = (5, 6, 7), 3
+    A = rand_halo(rng, dims, h)
+    B = A.copy(order="F")
+    O.ghost_fill(B, h, width, full)
+    idx = [np.arange(1 - h, n + h + 1) for n in dims]
+    for a, (I, n) in enumerate(zip(idx, dims)):
+        pass
+    I, J, K = np.meshgrid(*idx, indexing="ij")
+    out = [(I < 1) | (I > dims[0]), (J < 1) | (J > dims[1]), (K < 1) | (K > dims[2])]
+    within = (np.abs(I - np.clip(I, 1, dims[0])) <= width) & (np.abs(J - np.clip(J, 1, dims[1])) <= width) & \
+             (np.abs(K - np.clip(K, 1, dims[2])) <= width)
+    nout = out[0].astype(int) + out[1] + out[2]
+    sel = within & (nout >= 1) & ((nout == 1) if not full else True)
+    src = A[(np.mod(I - 1, dims[0])) + h, (np.mod(J - 1, dims[1])) + h, (np.mod(K - 1, dims[2])) + h]
+    expect = np.where(sel, src, A)
+    assert np.array_equal(B, expect)
+
+
+# ----------------------------------------------------------------------------
+# RHS: exactness on linear fields (4th-order interpolation/flux combos are
+# exact for the quadratic fluxes of linear velocities) on a non-periodic array
+# ----------------------------------------------------------------------------
+def test_rhs_self_advection_linear_field():
+    dims, h, Re = (8, 8, 8), 3, 7.0
+    a, b = 0.3, 0.7
+    U = field_from(lambda x, y, z: a + b * x, dims, h, 0)
+    Z = np.zeros_like(U)
+    dU, dV, dW = O.rhs(U, Z.copy(order="F"), Z.copy(order="F"), Z.copy(order="F"), h, Re)
+    # d(U^2)/dx = 2 U b at the U faces; diffusion of a linear field is 0
+    expect = -2.0 * b * interior(U, h)
+    assert np.allclose(dU, expect, rtol=1e-12, atol=1e-12)
+    assert np.all(dV == 0) and np.all(dW == 0)
+
+
+def test_rhs_cross_advection_linear_fields():
+    dims, h, Re = (8, 8, 8), 3, 3.0
+    b, c = 0.5, -1.3
+    U = field_from(lambda x, y, z: b * y, dims, h, 0)
+    V = field_from(lambda x, y, z: c * x, dims, h, 1)
+    Z = np.zeros_like(U)
+    dU, dV, dW = O.rhs(U, V, Z.copy(order="F"), Z.copy(order="F"), h, Re)
+    d = TWO_PI / 8
+    i = np.arange(1, 9)
+    xU = i[:, None, None] * d * np.ones((1, 8, 8))
+    yV = i[None, :, None] * d * np.ones((8, 1, 8))
+    # dU/dt = -d(VU)/dy = -b c x at the U face; dV/dt = -d(UV)/dx = -b c y at the V face
+    assert np.allclose(dU, -b * c * xU, rtol=1e-12, atol=1e-12)
+    assert np.allclose(dV, -b * c * yV, rtol=1e-12, atol=1e-12)
+    assert np.allclose(dW, 0, atol=1e-13)
+
+
+def test_rhs_pressure_gradient_closed_form(rng):
+    dims, h = (6, 7, 8), 3
+    Z = np.zeros(tuple(n + 2 * h for n in dims), order="F")
+    P = rand_halo(rng, dims, h)
+    dU, dV, dW = O.rhs(Z.copy(order="F"), Z.copy(order="F"), Z.copy(order="F"), P, h, 1.0)
+    di = [n / TWO_PI for n in dims]
+    Pi = P[h:-h, h:-h, h:-h]
+    assert np.allclose(dU, -di[0] * (P[h + 1:-h + 1, h:-h, h:-h] - Pi), rtol=1e-14, atol=1e-14)
+    assert np.allclose(dV, -di[1] * (P[h:-h, h + 1:-h + 1, h:-h] - Pi), rtol=1e-14, atol=1e-14)
+    assert np.allclose(dW, -di[2] * (P[h:-h, h:-h, h + 1:-h + 1] - Pi), rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("m", [1, 2, 3])
+def test_rhs_diffusion_symbol(m):
+    """Shear U = cos(m y): advection vanishes identically; the 4th-order
+    Laplacian has the symbol (-2 cos 2θ + 32 cos θ - 30)/12 per dy^2."""
+    dims, h, Re = (8, 16, 8), 3, 5.0
+    U = field_from(lambda x, y, z: np.cos(m * y), dims, h, 0)
+    Z = np.zeros_like(U)
+    dU, dV, dW = O.rhs(U, Z.copy(order="F"), Z.copy(order="F"), Z.copy(order="F"), h, Re)
+    dy = TWO_PI / 16
+    th = m * dy
+    sym = (-2 * np.cos(2 * th) + 32 * np.cos(th) - 30) / 12 / dy ** 2
+    assert np.allclose(dU, sym / Re * interior(U, h), rtol=1e-12, atol=1e-12)
+    assert np.allclose(dV, 0, atol=1e-13) and np.allclose(dW, 0, atol=1e-13)
+
+
+def test_rhs_flux_form_conserves_momentum(rng):
+    dims, h = (8, 8, 8), 3
+    Us = [rand_halo(rng, dims, h) for _ in range(4)]
+    for A in Us:
+        O.ghost_fill(A, h, h, True)
+    dUs = O.rhs(*Us, h, 11.0)
+    for d in dUs:
+        assert abs(d.mean()) < 1e-12 * np.abs(d).max()
+
+
+def test_update_closed_form(rng):
+    dims, h = (5, 5, 5), 3
+    Us0 = [rand_halo(rng, dims, h) for _ in range(3)]
+    Us = [np.zeros_like(a) for a in Us0]
+    dUs = [np.asfortranarray(rng.random(dims)) for _ in range(3)]
+    O.update(Us, Us0, dUs, h, 0.01)
+    for a, a0, d in zip(Us, Us0, dUs):
+        assert np.array_equal(interior(a, h), interior(a0, h) + d * 0.01)
+
+
+def test_rhs_f32_matches_f64(rng):
+    dims, h = (8, 8, 8), 3
+    Us = [rand_halo(rng, dims, h) for _ in range(4)]
+    d64 = O.rhs(*Us, h, 11.0)
+    d32 = O.rhs(*[a.astype(np.float32, order="F") for a in Us], h, 11.0)
+    for a, b in zip(d32, d64):
+        assert rel_l2(a, b) < 1e-5
+
+
+# ----------------------------------------------------------------------------
+# colouring: lib/parallel/test/runtests.jl:73-113 (5^3 arrays, axes 2:4)
+# ----------------------------------------------------------------------------
+CHECKER = np.array([[[1, 0, 1], [0, 1, 0], [1, 0, 1]],
+                    [[0, 1, 0], [1, 0, 1], [0, 1, 0]],
+                    [[1, 0, 1], [0, 1, 0], [1, 0, 1]]])
+
+
+@pytest.mark.parametrize("coloring", [0, 1])
+def test_colour_mask_reference_kat(coloring):
+    m0 = O.colour_mask((2, 2, 2), (4, 4, 4), 0, coloring)
+    m1 = O.colour_mask((2, 2, 2), (4, 4, 4), 1, coloring)
+    assert np.array_equal(m0, CHECKER)
+    assert np.array_equal(m1, 1 - CHECKER)
+
+
+def test_colour_masks_partition_domain_with_negative_indices():
+    lo, hi = (-1, -1, -1), (4, 5, 3)
+    m0 = O.colour_mask(lo, hi, 0, 0)
+    m1 = O.colour_mask(lo, hi, 1, 0)
+    assert np.all(m0 + m1 == 1)
+    # red-black (truncating %) never visits cells with a negative odd index sum
+    r0 = O.colour_mask(lo, hi, 0, 1)
+    r1 = O.colour_mask(lo, hi, 1, 1)
+    I, J, K = np.meshgrid(*[np.arange(a, b + 1) for a, b in zip(lo, hi)], indexing="ij")
+    s = I + J + K
+    assert np.array_equal(r0 + r1 == 0, (s < 0) & (s % 2 == 1))
+
+
+# ----------------------------------------------------------------------------
+# HSMAC
+# ----------------------------------------------------------------------------
+def divergence(U, V, W, h, cells):
+    nx = U.shape[0] - 2 * h
+    ny = U.shape[1] - 2 * h
+    nz = U.shape[2] - 2 * h
+    dxi, dyi, dzi = nx / TWO_PI, ny / TWO_PI, nz / TWO_PI
+    out = []
+    for (i, j, k) in cells:
+        a, b, c = i - 1 + h, j - 1 + h, k - 1 + h
+        out.append(dxi * (U[a, b, c] - U[a - 1, b, c]) + dyi * (V[a, b, c] - V[a, b - 1, c]) +
+                   dzi * (W[a, b, c] - W[a, b, c - 1]))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.7])
+def test_hsmac_single_pass_relaxes_visited_cells(rng, omega):
+    """After one colour pass every visited cell's divergence is (1-ω) times its
+    value before (its faces are shared only with cells of the other colour);
+    P grows by dp/dt with dp = -ω div dl; Div2 holds div^2."""
+    dims, h, dt = (6, 6, 6), 3, 0.01
+    U, V, W, P = [rand_halo(rng, dims, h) for _ in range(4)]
+    Div2 = np.zeros_like(U)
+    lo, hi = (1, 1, 1), (6, 6, 6)
+    cells = [(i, j, k) for i in range(1, 7) for j in range(1, 7) for k in range(1, 7) if (i + j + k) % 2 == 0]
+    before = divergence(U, V, W, h, cells)
+    P0 = P.copy(order="F")
+    O.hsmac_pass(U, V, W, P, Div2, h, lo, hi, 0, dt, omega)
+    after = divergence(U, V, W, h, cells)
+    assert np.allclose(after, (1 - omega) * before, rtol=1e-10, atol=1e-10)
+    dxi = 6 / TWO_PI
+    dl = 1 / (2 * 3 * dxi ** 2)
+    for (i, j, k), d in zip(cells, before):
+        p = (i - 1 + h, j - 1 + h, k - 1 + h)
+        assert np.isclose(P[p] - P0[p], -omega * d * dl / dt, rtol=1e-10)
+        assert np.isclose(Div2[p], d * d, rtol=1e-12)
+
+
+def test_hsmac_core_pass_conserves_means(rng):
+    """A colour pass over the core (2:n)^3 touches only interior faces, each
+    cell adding +dp and -dp to its two faces per axis: the interior means of
+    U, V, W are unchanged (pins the signs of the face corrections)."""
+    dims, h = (8, 8, 8), 3
+    U, V, W, P = [rand_halo(rng, dims, h) for _ in range(4)]
+    Div2 = np.zeros_like(U)
+    m0 = [interior(a, h).sum() for a in (U, V, W)]
+    for c in (0, 1):
+        O.hsmac_pass(U, V, W, P, Div2, h, (2, 2, 2), (8, 8, 8), c, 0.01)
+    for a, m in zip((U, V, W), m0):
+        assert abs(interior(a, h).sum() - m) < 1e-11
+
+
+def test_hsmac_naive_sweeps_reduce_divergence(rng):
+    dims, h = (8, 8, 8), 3
+    U, V, W, P = [rand_halo(rng, dims, h) for _ in range(4)]
+    divs = []
+    for it in range(6):
+        itr, div = O.hsmac(U, V, W, P, h, 0.01, time_blocking=False, overlap=False, itrmin=0, itrmax=1,
+                           epsp=0.0)
+        divs.append(div)
+    assert all(b < a for a, b in zip(divs, divs[1:]))
+
+
+def _replay(U, V, W, P, h, dt, tb, ov, itrmin, itrmax, epsp=1e-3, omega=1.7):
+    """Sequential replay of couple!(::HSMAC) with single colour passes, written
+    from hsmac.jl:77-201, faces() (halo.jl:349-366) and foraxes_colored
+    (foraxes.jl:73-77)."""
+    nx, ny, nz = [s - 2 * h for s in U.shape]
+    Div2 = np.zeros_like(U)
+    dxmin = min(TWO_PI / nx, TWO_PI / ny, TWO_PI / nz)
+
+    def sweep(lo, hi):
+        for c in (0, 1):
+            O.hsmac_pass(U, V, W, P, Div2, h, lo, hi, c, dt, omega)
+
+    def core_faces(dlo, dhi):
+        clo, chi = (2, 2, 2), (nx, ny, nz)
+        sweep(clo, chi)
+        faces = [((dlo[0], dlo[1], dlo[2]), (dhi[0], dhi[1], clo[2] - 1)),
+                 ((dlo[0], dlo[1], chi[2] + 1), (dhi[0], dhi[1], dhi[2])),
+                 ((dlo[0], dlo[1], clo[2]), (dhi[0], clo[1] - 1, chi[2])),
+                 ((dlo[0], chi[1] + 1, clo[2]), (dhi[0], dhi[1], chi[2])),
+                 ((dlo[0], clo[1], clo[2]), (clo[0] - 1, chi[1], chi[2])),
+                 ((chi[0] + 1, clo[1], clo[2]), (dhi[0], chi[1], chi[2]))]
+        for lo, hi in faces:
+            sweep(lo, hi)
+
+    def rms():
+        return np.sqrt(Div2[h:h + nx + 1, h:h + ny + 1, h:h + nz + 1].sum() / ((nx + 1) * (ny + 1) * (nz + 1)))
+
+    d3 = ((1, 1, 1), (nx + 1, ny + 1, nz + 1))
+    itr, div = 0, 0.0
+    if not tb:
+        while itr < itrmax:
+            for A in ((U, V, W) if ov else (U, V, W, P)):
+                O.ghost_fill(A, h, 1, True)
+            core_faces(*d3) if ov else sweep(*d3)
+            div = rms()
+            if div * dxmin < epsp and itrmin <= itr:
+                break
+            itr += 1
+        return itr, div
+    d1 = ((-1, -1, -1), (nx + 3, ny + 3, nz + 3))
+    d2 = ((0, 0, 0), (nx + 2, ny + 2, nz + 2))
+    while itr < itrmax:
+        for s, d in enumerate((d1, d2, d3)):
+            if s == 0:
+                for A in (U, V, W):
+                    O.ghost_fill(A, h, h, True)
+                core_faces(*d) if ov else sweep(*d)
+            else:
+                sweep(*d)
+            div = rms()
+            if div * dxmin < epsp and itrmin <= itr:
+                return itr, div
+            itr += 1
+    return itr, div
+
+
+@pytest.mark.parametrize("tb", [True, False])
+@pytest.mark.parametrize("ov", [True, False])
+@pytest.mark.parametrize("itrmin,itrmax,epsp", [(6, 6, 1e-3), (2, 300, 50.0), (0, 7, 1e9)])
+def test_hsmac_matches_sequential_pass_replay(rng, tb, ov, itrmin, itrmax, epsp):
+    dims, h = (6, 7, 8), 3
+    base = [rand_halo(rng, dims, h) for _ in range(4)]
+    A = [b.copy(order="F") for b in base]
+    B = [b.copy(order="F") for b in base]
+    r1 = O.hsmac(*A, h, 0.01, tb, ov, 0, 1.7, epsp, itrmin, itrmax)
+    r2 = _replay(*B, h, 0.01, tb, ov, itrmin, itrmax, epsp)
+    assert r1[0] == r2[0]
+    assert np.isclose(r1[1], r2[1], rtol=1e-12)
+    for a, b in zip(A, B):
+        assert np.array_equal(a, b)
+
+
+def test_hsmac_itr_semantics():
+    """itr counts the sweeps before the converged one (hsmac.jl:122-125): a
+    converged call performs itr+1 sweeps; an unconverged naive call returns
+    itrmax; the time-blocked loop finishes whole groups of three."""
+    rng = np.random.default_rng(3)
+    dims, h = (6, 6, 6), 3
+    for tb, itrmax, expect in [(False, 5, 5), (True, 5, 6), (True, 7, 9)]:
+        A = [rand_halo(rng, dims, h) for _ in range(4)]
+        itr, _ = O.hsmac(*A, h, 0.01, tb, False, 0, 1.7, 0.0, 0, itrmax)
+        assert itr == expect
+    A = [rand_halo(rng, dims, h) for _ in range(4)]
+    itr, _ = O.hsmac(*A, h, 0.01, True, True, 0, 1.7, 1e9, 4, 300)
+    assert itr == 4
+
+
+# ----------------------------------------------------------------------------
+# CFL (src/flows/timestep.jl:56-64): uniform field closed form
+# ----------------------------------------------------------------------------
+def test_cfl_uniform_closed_form():
+    dims, h = (8, 16, 32), 3
+    u0, v0, w0 = 0.8, -2.5, 0.1
+    U, V, W = [np.full(tuple(n + 2 * h for n in dims), c, order="F") for c in (u0, v0, w0)]
+    cfl, dtmax, Re = 0.3, 1.0, 400.0
+    dt, m = O.flow_timestep(U, V, W, h, cfl, dtmax, Re)
+    d = [TWO_PI / n for n in dims]
+    cands = [dtmax, cfl * d[0] / (abs(u0) + 1e-10), cfl * d[1] / (abs(v0) + 1e-10),
+             cfl * d[2] / (abs(w0) + 1e-10), 0.5 * (d[0] / 2) ** 2 * Re]
+    assert np.isclose(dt, min(cands), rtol=1e-15)
+    assert np.allclose(m, [abs(u0) + 1e-10, abs(v0) + 1e-10, abs(w0) + 1e-10], rtol=1e-15)
+
+
+# ----------------------------------------------------------------------------
+# linear forcing (src/flows/forcings/lf.jl): invariants
+# ----------------------------------------------------------------------------
+def test_linear_forcing_invariants(rng):
+    dims, h, power, dt = (8, 8, 8), 3, 0.5, 0.01
+    Us = [rand_halo(rng, dims, h) for _ in range(3)]
+    before = [interior(a, h).copy() for a in Us]
+    p, m = O.force_linear(*Us, h, power, dt)
+    A = m[4]
+    means = m[1:4]
+    for a, b, mu in zip(Us, before, means):
+        assert np.isclose(interior(a, h).mean(), mu, rtol=1e-12)  # mean preserved
+        assert np.allclose(interior(a, h) - mu, (b - mu) * (1 + A * dt), rtol=1e-12, atol=1e-14)
+        assert np.isclose(b.mean(), mu, rtol=1e-14)
+    # p = A * sum_c (<u_c^2> - <u_c>^2) with face values
+    expect_p = A * sum((b * b).mean() - b.mean() * b.mean() for b in before)
+    assert np.isclose(p, expect_p, rtol=1e-12)
+    # A = P0 / (2e), e from cell-centred interpolated |u|^2
+    e = 0.5 * (m[0] - sum(mu * mu for mu in means))
+    assert np.isclose(A, power / (2 * e), rtol=1e-14)
+
+
+# ----------------------------------------------------------------------------
+# rc-energy forcing: FourierTools KATs (lib/fouriertools/test/runtests.jl:62-127)
+# and the box-mean KAT (test/test_flows.jl:122-136)
+# ----------------------------------------------------------------------------
+def test_kcount_kset_reference_kats():
+    assert O.kcount(1) == 18
+    assert O.kcount(2) == 62
+    assert set(O.kset(1)) == {(0, 1, 1), (0, 0, 1), (1, 0, -1), (0, 1, -1), (0, 1, 0), (1, -1, 0), (1, 0, 0),
+                              (1, 1, 0), (1, 0, 1)}
+    # kset is the canonical half of the shell: every shell vector or its negative, once
+    for k0 in (1, 2, 3):
+        ks = O.kset(k0)
+        assert 2 * len(ks) == O.kcount(k0)
+
+
+def test_boxmean_block_constant_kat():
+    dims, coarse, h = (8, 8, 8), (2, 2, 2), 3
+    A = np.zeros(tuple(n + 2 * h for n in dims), order="F")
+    ref = np.arange(1, 9, dtype=np.float64).reshape(coarse, order="F")
+    A[h:-h, h:-h, h:-h] = np.kron(ref, np.ones((4, 4, 4)))
+    assert np.array_equal(O.boxmean(A, h, coarse), ref)
+
+
+def test_enespe_flat_spectrum_kat():
+    """A delta at the origin has a flat normalised spectrum (every coefficient 1),
+    so enespe(k0) = kcount(k0) * 3 / 2 as in the reference's KAT."""
+    c = 8
+    delta = np.zeros((c, c, c), order="F")
+    delta[0, 0, 0] = c ** 3
+    _, en = O.rcf_spectrum([delta, delta, delta], 1, 2, 1.0)
+    assert np.isclose(en[0], O.kcount(1) * 3 / 2, rtol=1e-12)
+    assert np.isclose(en[1], O.kcount(2) * 3 / 2, rtol=1e-12)
+
+
+def test_sirdft_matches_inverse_fft_of_shell_filtered_spectrum():
+    """lib/fouriertools/test/runtests.jl:99-127: the sparse inverse DFT over
+    shells 2..3 equals irfft of the shell-filtered spectrum.  The apply kernel
+    samples at cispi(2(i - φ)/n); with n = 8 and coarse 4, φ = 1."""
+    rng = np.random.default_rng(5)
+    n, kmin, kmax = 8, 2, 3
+    A = rng.random((n, n, n))
+    Ahat = np.fft.rfftn(A, axes=(2, 1, 0))  # halves axis 0 (x), FFTW sign
+    nk = 2 * kmax + 1
+    ncrop = (kmax + 1) * nk * nk
+    crop = np.zeros(2 * 3 * ncrop)
+    filt = np.zeros_like(Ahat)
+    for kz in range(-kmax, kmax + 1):
+        for ky in range(-kmax, kmax + 1):
+            for kx in range(0, kmax + 1):
+                o = kx + (kmax + 1) * ((ky % nk) + nk * (kz % nk))
+                v = Ahat[kx, ky % n, kz % n] / n ** 3
+                for a in range(3):
+                    crop[2 * (a * ncrop + o)] = v.real
+                    crop[2 * (a * ncrop + o) + 1] = v.imag
+                if kmin - 0.5 <= np.sqrt(kx * kx + ky * ky + kz * kz) < kmax + 0.5:
+                    filt[kx, ky % n, kz % n] = Ahat[kx, ky % n, kz % n]
+    expect = np.fft.irfftn(filt, s=(n, n, n), axes=(2, 1, 0))
+    h = 3
+    Us = [np.zeros((n + 2 * h,) * 3, order="F") for _ in range(3)]
+    O.rcf_apply(*Us, h, 4, kmin, kmax, 0.01, crop)
+    for U in Us:
+        assert np.allclose(interior(U, h), expect, rtol=1e-10, atol=1e-12)
+
+
+def test_enespe0_closed_form():
+    for k0 in (1, 2, 3):
+        e = O.enespe0(k0, 0.649)
+        expect = 0.649 * k0 ** (-5 / 3) * (1.5 ** 3 - 0.5 ** 3) / ((k0 + .5) ** 3 - (k0 - .5) ** 3) * \
+            O.kcount(k0) / O.kcount(1)
+        assert np.isclose(e, expect, rtol=1e-14)
+
+
+def test_rcf_rescales_shell_energy_to_target(rng):
+    """After the shell rescaling, the coarse-grid spectrum carries the target
+    shell energy en0 in every forced shell: the forcing spectrum is r*A_hat, so
+    the forced field's shell energy is r^2 en, with r = sqrt(en0/en) - 1 ...
+    i.e. (A + r A) has energy en0."""
+    c = 8
+    coarse = [np.asfortranarray(rng.standard_normal((c, c, c))) for _ in range(3)]
+    crop, en = O.rcf_spectrum(coarse, 1, 2, 0.649)
+    nk = 5
+    ncrop = 3 * nk * nk
+    for k0, e in zip((1, 2), en):
+        en0 = O.enespe0(k0, 0.649)
+        r = np.sqrt(en0 / max(e, 1e-8)) - 1
+        # recompute the shell energy of the cropped (scaled) spectrum, half-space weights
+        s = 0.0
+        for kz in range(-k0, k0 + 1):
+            for ky in range(-k0, k0 + 1):
+                for kx in range(0, k0 + 1):
+                    r2 = kx * kx + ky * ky + kz * kz
+                    if (k0 - .5) ** 2 <= r2 < (k0 + .5) ** 2:
+                        o = kx + 3 * ((ky % nk) + nk * (kz % nk))
+                        w = 1 if kx == 0 else 2
+                        s += w * sum(crop[2 * (a * ncrop + o)] ** 2 + crop[2 * (a * ncrop + o) + 1] ** 2
+                                     for a in range(3))
+        assert np.isclose(s / 2, r * r * e, rtol=1e-10)
+        assert np.isclose((1 + r) ** 2 * e, en0, rtol=1e-10)
+
+
+def test_rcf_rejects_kmax_beyond_nyquist():
+    c = 4
+    z = [np.zeros((c, c, c), order="F")] * 3
+    with pytest.raises(ValueError):
+        O.rcf_spectrum(z, 1, 3, 1.0)
+
+
+# ----------------------------------------------------------------------------
+# builder-defined spectrum initialiser
+# ----------------------------------------------------------------------------
+def test_spectrum_init_uprime_and_divergence():
+    dims, h = (32, 32, 32), 3
+    (U, V, W), P = O.init_spectrum(dims, h, 11, 4.0, 1.0, np.float64)
+    up = np.sqrt((interior(U, h) ** 2 + interior(V, h) ** 2 + interior(W, h) ** 2).mean() / 3)
+    assert np.isclose(up, 1.0, rtol=1e-12)
+    assert np.all(P == 0)
+    for A in (U, V, W):
+        O.ghost_fill(A, h, h, True)
+    cells = [(i, j, k) for i in range(1, 33, 3) for j in range(1, 33, 5) for k in range(1, 33, 7)]
+    div = divergence(U, V, W, h, cells)
+    gradscale = np.abs(np.diff(interior(U, h), axis=0)).mean() * 32 / TWO_PI
+    assert np.sqrt((div ** 2).mean()) < 0.05 * gradscale  # solenoidal up to O(k^2 dx^2)
