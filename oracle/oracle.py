@@ -259,7 +259,8 @@ def hash_cells(dims, grid_ratio, xss, n):
 def makeindex(dims, grid_ratio, xss, ids, props, n):
     """makeindex!: src/particles/makeindex.jl:29-67 (in place on xss, ids, props).
 
-    ``props`` are the other real SoA properties permuted together with xss.
+    ``props`` are the other real SoA properties; they are permuted together with
+    xss (which are permuted too).
     Returns (hashes, perm, starts, stops); perm/starts/stops are 1-based."""
     nc = [d // grid_ratio + 2 for d in dims]
     ncell = nc[0] * nc[1] * nc[2]
@@ -267,7 +268,9 @@ def makeindex(dims, grid_ratio, xss, ids, props, n):
     perm = np.zeros(max(n, 1), dtype=np.int64)
     starts = np.zeros(ncell, dtype=np.int64)
     stops = np.zeros(ncell, dtype=np.int64)
-    allp = list(props)
+    # all real properties are permuted (src/particles/buffer.jl:39-43), the
+    # hashing positions included exactly once
+    allp = list(xss) + [a for a in props if all(a is not x for x in xss)]
     arr = (C.c_void_p * max(len(allp), 1))(*[_p(a) for a in allp])
     lib(xss[0].dtype).o_makeindex(*dims, grid_ratio, n, *[_p(a) for a in xss], _p(ids), arr, len(allp),
                                   _p(hashes), _p(perm), _p(starts), _p(stops))
