@@ -214,6 +214,8 @@ int lcs_box_destroy(lcs_box *box)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (void *p : b->f)
         if (p) (void)hipFree(p);
+    for (void *p : b->spare)
+        if (p) (void)hipFree(p);
     for (auto &pp : b->profiles) {
         for (void *p : pp.props)
             if (p) (void)hipFree(p);
@@ -344,7 +346,7 @@ int lcs_hsmac(lcs_box *box, int stage, double dt_stage, const lcs_hsmac_params *
     Box &b = *B(box);
     int rc = set_phase_dt(b, dt_stage);
     if (rc) return rc;
-    void *const *Us = stage == 1 ? &b.f[LCS_FIELD_U2] : &b.f[LCS_FIELD_U];
+    void **Us = stage == 1 ? &b.f[LCS_FIELD_U2] : &b.f[LCS_FIELD_U];
     if ((rc = launch_hsmac(b, Us, DtRef{&b.S->dt_phase, 1.0}, *prm, 0))) return rc;
     if ((rc = read_scalars(b))) return rc;
     if (itr) *itr = b.S_host->itr[0];
@@ -579,8 +581,8 @@ int lcs_step(lcs_box *box, const lcs_step_config *cfg, lcs_step_log *log)
 
     // evolve!: src/evolve.jl:63-146 (FlowMode :15-61 when there are no particles)
     const DtRef dt1{&b.S->dt, 0.5}, dt2{&b.S->dt, 1.0};
+    // HSMAC may swap field pointers with its ping-pong partners: read b.f at use
     void *UsP[4] = {b.f[LCS_FIELD_U], b.f[LCS_FIELD_V], b.f[LCS_FIELD_W], b.f[LCS_FIELD_P]};
-    void *Us2P[4] = {b.f[LCS_FIELD_U2], b.f[LCS_FIELD_V2], b.f[LCS_FIELD_W2], b.f[LCS_FIELD_P]};
     // stage 1
     if ((rc = launch_ghost_fill(b, UsP, 4, b.g.h, 1, nullptr))) return rc;
     for (int q = 0; q < np; q++)
@@ -588,6 +590,7 @@ int lcs_step(lcs_box *box, const lcs_step_config *cfg, lcs_step_log *log)
     if ((rc = launch_rhs(b, 1, true, &b.f[LCS_FIELD_U2], &b.f[LCS_FIELD_U], dt1))) return rc;
     if ((rc = launch_hsmac(b, &b.f[LCS_FIELD_U2], dt1, cfg->hsmac, 0))) return rc;
     // stage 2
+    void *Us2P[4] = {b.f[LCS_FIELD_U2], b.f[LCS_FIELD_V2], b.f[LCS_FIELD_W2], b.f[LCS_FIELD_P]};
     if ((rc = launch_ghost_fill(b, Us2P, 4, b.g.h, 1, nullptr))) return rc;
     for (int q = 0; q < np; q++)
         if ((rc = launch_p_stage(b, b.profiles[q], 2, dt2))) return rc;

@@ -11,6 +11,7 @@
 #include <cstring>
 
 #include "lcs_internal.h"
+#include "lcs_rhs_tile.h"
 
 namespace lcs {
 
@@ -24,17 +25,20 @@ __device__ inline int wrap1(int i, int n)
     return r + 1;
 }
 
-// Sum of a double over a block of blockDim.x threads (multiple of 64),
-// deterministic for a fixed block size.  Result valid in thread 0.
+// Sum / max of a double over a whole block (any shape, wave64), deterministic
+// for a fixed block shape.  The result is valid in linear thread 0.
+__device__ inline int lin_tid() { return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z); }
+__device__ inline int block_threads() { return blockDim.x * blockDim.y * blockDim.z; }
+
 __device__ inline double block_sum(double v, double *lds)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int t = lin_tid(), w = t >> 6, l = t & 63;
     if (l == 0) lds[w] = v;
     __syncthreads();
     double s = 0.0;
-    if (threadIdx.x == 0) {
-        int nw = (blockDim.x + 63) >> 6;
+    if (t == 0) {
+        int nw = (block_threads() + 63) >> 6;
         for (int q = 0; q < nw; q++) s += lds[q];
     }
     __syncthreads();
@@ -44,12 +48,12 @@ __device__ inline double block_sum(double v, double *lds)
 __device__ inline double block_max(double v, double *lds)
 {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
-    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int t = lin_tid(), w = t >> 6, l = t & 63;
     if (l == 0) lds[w] = v;
     __syncthreads();
     double s = 0.0;
-    if (threadIdx.x == 0) {
-        int nw = (blockDim.x + 63) >> 6;
+    if (t == 0) {
+        int nw = (block_threads() + 63) >> 6;
         s = lds[0];
         for (int q = 1; q < nw; q++) s = fmax(s, lds[q]);
     }
@@ -57,7 +61,7 @@ __device__ inline double block_max(double v, double *lds)
     return s;
 }
 
-static int grid_for(long work, int block, int cap = kMaxPartBlocks)
+static int grid_for(long work, int block, int cap = 4096)
 {
     long g = (work + block - 1) / block;
     if (g < 1) g = 1;
@@ -97,10 +101,19 @@ __device__ inline T sum_flux_o4(T j1a, T j1b, T j3a, T j3b)
 // enumerated as z-slabs, then y-slabs, then x-slabs; up to four arrays.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_ghost_fill(Geom g, T *a0, T *a1, T *a2, T *a3, int narr, int r, int full,
-                             long cz, long cy, long total, const int *skip)
+struct Ptr4 {
+    T *p[4];
+};
+
+// Inside an HSMAC call (S != nullptr) the fill is skipped once the call has
+// converged and acts on the spare buffer set B when the ping-pong parity says
+// the current state lives there.
+template <typename T>
+__global__ void k_ghost_fill(Geom g, Ptr4<T> A, Ptr4<T> Bs, int narr, int r, int full, long cz, long cy,
+                             long total, const DevScalars *S)
 {
-    if (skip && *skip) return;
+    if (S && S->hs_done) return;
+    const bool alt = S && S->hs_par;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
     const int X = full ? g.nx + 2 * r : g.nx;
@@ -137,14 +150,15 @@ __global__ void k_ghost_fill(Geom g, T *a0, T *a1, T *a2, T *a3, int narr, int r
     }
     long d = g.at(i, j, k);
     long s = g.at(wrap1(i, g.nx), wrap1(j, g.ny), wrap1(k, g.nz));
-    a0[d] = a0[s];
-    if (narr > 1) a1[d] = a1[s];
-    if (narr > 2) a2[d] = a2[s];
-    if (narr > 3) a3[d] = a3[s];
+    for (int a = 0; a < narr; a++) {
+        T *q = alt ? Bs.p[a] : A.p[a];
+        q[d] = q[s];
+    }
 }
 
 template <typename T>
-static int ghost_fill_t(Box &b, void *const *arrays, int narrays, int r, int full, const int *skip)
+static int ghost_fill_t(Box &b, void *const *arrays, void *const *alt, int narrays, int r, int full,
+                        const DevScalars *S)
 {
     const Geom &g = b.g;
     if (r < 1 || r > g.h) return fail(LCS_ERR_INVALID, "ghost width must be in 1..halo_size");
@@ -152,28 +166,30 @@ static int ghost_fill_t(Box &b, void *const *arrays, int narrays, int r, int ful
     long X = full ? g.nx + 2 * r : g.nx, Y = full ? g.ny + 2 * r : g.ny;
     long cz = 2L * r * X * Y, cy = 2L * r * g.nz * X, cx = 2L * r * g.ny * g.nz;
     long total = cz + cy + cx;
-    T *a[4] = {nullptr, nullptr, nullptr, nullptr};
-    for (int q = 0; q < narrays; q++) a[q] = (T *)arrays[q];
+    Ptr4<T> A = {{nullptr, nullptr, nullptr, nullptr}}, Bs = A;
+    for (int q = 0; q < narrays; q++) {
+        A.p[q] = (T *)arrays[q];
+        Bs.p[q] = alt ? (T *)alt[q] : (T *)arrays[q];
+    }
     int blocks = (int)((total + 255) / 256);
     ProfScope ps(b, "ghost_fill", 2.0 * total * narrays * sizeof(T));
-    hipLaunchKernelGGL(k_ghost_fill<T>, dim3(blocks), dim3(256), 0, b.stream, g, a[0], a[1], a[2], a[3],
-                       narrays, r, full, cz, cy, total, skip);
+    hipLaunchKernelGGL(k_ghost_fill<T>, dim3(blocks), dim3(256), 0, b.stream, g, A, Bs, narrays, r, full, cz, cy,
+                       total, S);
     LCS_HIP(hipGetLastError());
     return LCS_OK;
 }
 
-int launch_ghost_fill(Box &b, void *const *arrays, int narrays, int width, int full, const int *skip)
+int launch_ghost_fill(Box &b, void *const *arrays, int narrays, int width, int full, const int *)
 {
-    return b.esize == 8 ? ghost_fill_t<double>(b, arrays, narrays, width, full, skip)
-                        : ghost_fill_t<float>(b, arrays, narrays, width, full, skip);
+    return b.esize == 8 ? ghost_fill_t<double>(b, arrays, nullptr, narrays, width, full, nullptr)
+                        : ghost_fill_t<float>(b, arrays, nullptr, narrays, width, full, nullptr);
 }
 
 // ---------------------------------------------------------------------------
-// RHS (src/flows/rhs.jl:59-291) fused over the three components.
-// One thread per interior cell; fluxes J1/J3 are recomputed where needed
-// instead of being stored in six scratch arrays.  FUSED writes
-// out = base + dUdt*dt (update!, src/flows/update.jl:255-268), otherwise
-// out = dUdt (interior layout).
+// RHS (src/flows/rhs.jl:59-291) fused over the three components and tiled
+// through LDS (lcs_rhs_tile.h): each flux J1/J3 is formed once instead of being
+// stored in six scratch arrays.  FUSED writes out = base + dUdt*dt (update!,
+// src/flows/update.jl:255-268), otherwise out = dUdt (interior layout).
 // ---------------------------------------------------------------------------
 template <typename T>
 struct RhsCoef {
@@ -181,70 +197,6 @@ struct RhsCoef {
     T di2[3];
     T Rei;
 };
-
-template <typename T>
-__device__ inline T flux_at(const T *__restrict__ A, const T *__restrict__ Q, long q, long sc, long sd,
-                            bool three)
-{
-    // J1(q) = interp_o4(A along c at q) * interp_o2(Q[q], Q[q+sd])
-    // J3(q) = interp_o4(A along c at q) * interp_o2(Q[q-sd], Q[q+2sd])
-    T ac = interp_o4(A[q - sc], A[q], A[q + sc], A[q + 2 * sc]);
-    T qq = three ? interp_o2(Q[q - sd], Q[q + 2 * sd]) : interp_o2(Q[q], Q[q + sd]);
-    return ac * qq;
-}
-
-template <typename T, bool FUSED>
-__global__ void __launch_bounds__(256) k_rhs_cell(Geom g, const T *__restrict__ U, const T *__restrict__ V,
-                                                  const T *__restrict__ W, const T *__restrict__ P,
-                                                  T *o0, T *o1, T *o2, const T *b0, const T *b1,
-                                                  const T *b2, RhsCoef<T> c, DtRef dt)
-{
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.cells()) return;
-    int i = (int)(t % g.nx) + 1;
-    long q = t / g.nx;
-    int j = (int)(q % g.ny) + 1;
-    int k = (int)(q / g.ny) + 1;
-    const long p = g.at(i, j, k);
-    const long st[3] = {1, g.sx, g.sx * g.sy};
-    const T *Vel[3] = {U, V, W};
-    T *out[3] = {o0, o1, o2};
-    const T *base[3] = {b0, b1, b2};
-    T rdt = T(0);
-    if (FUSED) rdt = T(dt.factor * (*dt.src));
-#pragma unroll
-    for (int comp = 0; comp < 3; comp++) {
-        const T *Q = Vel[comp];
-        const long sc = st[comp];
-        T d2[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            long s = st[d];
-            d2[d] = diff2_o4(Q[p - 2 * s], Q[p - s], Q[p], Q[p + s], Q[p + 2 * s]);
-        }
-        T diff = ((c.di2[0] * d2[0] + c.di2[1] * d2[1]) + c.di2[2] * d2[2]) * c.Rei;
-        T pres = c.di[comp] * (-P[p] + P[p + sc]);
-        T f[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const long sd = st[d];
-            const T *A = Vel[d];
-            T j1a = flux_at(A, Q, p - sd, sc, sd, false);
-            T j1b = flux_at(A, Q, p, sc, sd, false);
-            T j3a = flux_at(A, Q, p - 2 * sd, sc, sd, true);
-            T j3b = flux_at(A, Q, p + sd, sc, sd, true);
-            f[d] = sum_flux_o4(j1a, j1b, j3a, j3b);
-        }
-        T advect = (c.di[0] * f[0] + c.di[1] * f[1]) + c.di[2] * f[2];
-        T dq = (diff - pres) - advect;
-        if (FUSED) {
-            T du = dq * rdt;
-            out[comp][p] = base[comp][p] + du;
-        } else {
-            out[comp][g.interior(i, j, k)] = dq;
-        }
-    }
-}
 
 template <typename T>
 static RhsCoef<T> rhs_coef(const Box &b)
@@ -259,29 +211,98 @@ static RhsCoef<T> rhs_coef(const Box &b)
     return c;
 }
 
+// Each workgroup marches a TX x TY column up a chunk of zc planes.
+template <typename T, int TX, int TY, bool FUSED>
+__global__ void __launch_bounds__(TX *TY) k_rhs_tile(RhsArgs<T> a, DtRef dt)
+{
+    using Tile = RhsTile<T, TX, TY>;
+    extern __shared__ __align__(16) unsigned char rhs_lds_raw[];
+    T *lds = reinterpret_cast<T *>(rhs_lds_raw);
+    typename Tile::Thread th;
+    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    const int X0 = blockIdx.x * TX + 1, Y0 = blockIdx.y * TY + 1, z0 = blockIdx.z * a.zc + 1;
+    const int z1 = min(z0 + a.zc - 1, a.g.nz);
+    const T rdt = FUSED ? T(dt.factor * (*dt.src)) : T(0);
+    for (int q = z0 - 3; q <= z0 + 3; q++) {
+        Tile::fetch(a, X0, Y0, q, tid, th);
+        Tile::store(lds, q, tid, th);
+    }
+    __syncthreads();
+    Tile::warmup(lds, z0, tx, ty, th);
+    for (int z = z0; z <= z1; z++) {
+        if (z < z1) Tile::fetch(a, X0, Y0, z + 4, tid, th);
+        Tile::phase_fluxes(lds, z, tid);
+        __syncthreads();
+        Tile::template phase_output<FUSED>(a, lds, X0, Y0, z, tx, ty, rdt, th);
+        if (z < z1) Tile::store(lds, z + 4, tid, th);
+        __syncthreads();
+    }
+}
+
+// Tile shapes: FP32 32x8 (60 KB of LDS, two workgroups per CU), FP64 32x4 (81 KB).
+template <typename T>
+struct RhsShape;
+template <>
+struct RhsShape<float> {
+    static constexpr int TX = 32, TY = 8;
+};
+template <>
+struct RhsShape<double> {
+    static constexpr int TX = 32, TY = 4;
+};
+
+template <typename T, bool FUSED>
+static int rhs_tile_launch(Box &b, const RhsArgs<T> &a0, DtRef dt)
+{
+    constexpr int TX = RhsShape<T>::TX, TY = RhsShape<T>::TY;
+    using Tile = RhsTile<T, TX, TY>;
+    static_assert(Tile::LDS_ELEMS * sizeof(T) <= 160 * 1024, "tile exceeds the LDS of a CU");
+    const Geom &g = a0.g;
+    if (g.h < 3) return fail(LCS_ERR_INVALID, "the fourth-order RHS needs halo_size >= 3");
+    RhsArgs<T> a = a0;
+    const int gx = (g.nx + TX - 1) / TX, gy = (g.ny + TY - 1) / TY;
+    // z-chunks: enough workgroups to cover the chip several times over, but
+    // chunks of at least 8 planes (each chunk re-reads 6 halo planes)
+    const long cols = (long)gx * gy;
+    int gz = (int)std::max(1L, std::min((long)(g.nz + 7) / 8, (2048 + cols - 1) / cols));
+    a.zc = (g.nz + gz - 1) / gz;
+    gz = (g.nz + a.zc - 1) / a.zc;
+    const size_t lds = Tile::LDS_ELEMS * sizeof(T);
+    static bool attr_set = false;  // per template instance; the attribute is a property of the function
+    if (!attr_set) {
+        LCS_HIP(hipFuncSetAttribute((const void *)k_rhs_tile<T, TX, TY, FUSED>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_rhs_tile<T, TX, TY, FUSED>), dim3(gx, gy, gz), dim3(TX * TY), lds, b.stream, a, dt);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
 template <typename T>
 static int rhs_t(Box &b, int stage_in, bool fused, void *const out[3], void *const base[3], DtRef dt)
 {
     const Geom &g = b.g;
     void *const *in = stage_in == 1 ? &b.f[LCS_FIELD_U] : &b.f[LCS_FIELD_U2];
     RhsCoef<T> c = rhs_coef<T>(b);
-    long cells = g.cells();
-    int blocks = (int)((cells + 255) / 256);
+    RhsArgs<T> a;
+    a.g = g;
+    a.U = (const T *)in[0];
+    a.V = (const T *)in[1];
+    a.W = (const T *)in[2];
+    a.P = (const T *)b.f[LCS_FIELD_P];
+    for (int q = 0; q < 3; q++) {
+        a.out[q] = (T *)out[q];
+        a.base[q] = fused ? (const T *)base[q] : nullptr;
+        a.di[q] = c.di[q];
+        a.di2[q] = c.di2[q];
+    }
+    a.Rei = c.Rei;
+    a.zc = g.nz;
     // algorithmic bytes: read U,V,W,P (+ the three base fields in stage 2), write 3
     ProfScope ps(b, fused ? "rhs_update" : "rhs",
-                 (double)cells * sizeof(T) * (fused && stage_in == 2 ? 10 : 7));
-    if (fused)
-        hipLaunchKernelGGL((k_rhs_cell<T, true>), dim3(blocks), dim3(256), 0, b.stream, g, (const T *)in[0],
-                           (const T *)in[1], (const T *)in[2], (const T *)b.f[LCS_FIELD_P], (T *)out[0],
-                           (T *)out[1], (T *)out[2], (const T *)base[0], (const T *)base[1],
-                           (const T *)base[2], c, dt);
-    else
-        hipLaunchKernelGGL((k_rhs_cell<T, false>), dim3(blocks), dim3(256), 0, b.stream, g, (const T *)in[0],
-                           (const T *)in[1], (const T *)in[2], (const T *)b.f[LCS_FIELD_P], (T *)out[0],
-                           (T *)out[1], (T *)out[2], (const T *)nullptr, (const T *)nullptr,
-                           (const T *)nullptr, c, dt);
-    LCS_HIP(hipGetLastError());
-    return LCS_OK;
+                 (double)g.cells() * sizeof(T) * (fused && stage_in == 2 ? 10 : 7));
+    return fused ? rhs_tile_launch<T, true>(b, a, dt) : rhs_tile_launch<T, false>(b, a, dt);
 }
 
 int launch_rhs(Box &b, int stage_in, bool fused, void *const out[3], void *const base[3], DtRef dt)
@@ -332,13 +353,21 @@ int launch_update(Box &b, void *const out[3], void *const base[3], DtRef dt)
 }
 
 // ---------------------------------------------------------------------------
-// HSMAC (src/flows/couples/hsmac.jl).  One kernel per (region, colour) pass,
-// exactly the reference's pass structure; cells of one colour share no face,
-// so a pass is race-free in place.  Each pass also reduces div^2 over the
-// cells it visits inside (1:n+1)^3 into per-block partials; a one-block check
-// kernel turns them into the RMS divergence and decides convergence on the
-// device (hs_done), after which every later HSMAC kernel of the call returns
-// immediately.
+// HSMAC (src/flows/couples/hsmac.jl).
+//
+// A sweep over a box R is the reference's colour-0 pass followed by its
+// colour-1 pass (foraxes.jl:73-77).  Full boxes run as ONE fused kernel that
+// reads the current (U,V,W,P) and writes the result to the other buffer set
+// (ping-pong): red corrections depend only on the sweep's input, black ones on
+// the input plus the red corrections of their six neighbours, so each face's
+// final value -- input + red correction + black correction, in that order --
+// is computed by exactly one thread.  The thin face slabs of the overlap
+// ordering run as in-place single-colour passes (cells of one colour share no
+// face).  Every kernel also reduces div^2 over its cells inside (1:n+1)^3 into
+// per-block partials; a one-block check kernel forms the RMS divergence and
+// decides convergence on the device (hs_done), after which every later HSMAC
+// kernel of the call returns immediately.  The device flag hs_par says which
+// buffer set holds the current state; it flips only when a fused sweep ran.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct HsCoef {
@@ -349,13 +378,44 @@ struct Box3 {
     int lo[3], hi[3];
 };
 
+template <typename T>
+struct BufSets {
+    T *a[4];  // (U, V, W, P) as the call started
+    T *b[4];  // spare partners
+};
+
+__device__ inline bool in_box(const Box3 &r, int x, int y, int z)
+{
+    return x >= r.lo[0] && x <= r.hi[0] && y >= r.lo[1] && y <= r.hi[1] && z >= r.lo[2] && z <= r.hi[2];
+}
+
+// red-black (truncating %) never visits cells with a negative odd index sum
+template <int COLORING>
+__device__ inline bool visited(const Box3 &r, int x, int y, int z)
+{
+    if (!in_box(r, x, y, z)) return false;
+    if (COLORING == LCS_COLORING_RED_BLACK) {
+        int s = x + y + z;
+        if (s < 0 && (s & 1)) return false;
+    }
+    return true;
+}
+
+__device__ inline bool in_d3(const Geom &g, int x, int y, int z)
+{
+    return x >= 1 && x <= g.nx + 1 && y >= 1 && y <= g.ny + 1 && z >= 1 && z <= g.nz + 1;
+}
+
+// One colour pass over a (thin) box, in place on the current buffer set.
 template <typename T, int COLORING>
-__global__ void __launch_bounds__(256) k_hsmac_pass(Geom g, T *U, T *V, T *W, T *P, Box3 r, int color,
-                                                   HsCoef<T> c, DtRef dt, const DevScalars *S,
-                                                   double *part)
+__global__ void __launch_bounds__(256) k_hsmac_pass(Geom g, BufSets<T> bs, Box3 r, int color, HsCoef<T> c,
+                                                   DtRef dt, const DevScalars *S, double *part)
 {
     __shared__ double lds[4];
     if (S->hs_done) return;
+    const bool alt = S->hs_par != 0;
+    T *U = alt ? bs.b[0] : bs.a[0], *V = alt ? bs.b[1] : bs.a[1];
+    T *W = alt ? bs.b[2] : bs.a[2], *P = alt ? bs.b[3] : bs.a[3];
     const int a = r.lo[0];
     const int hw = (r.hi[0] - r.lo[0] + 2) / 2;  // cld(n, 2)
     const int ny = r.hi[1] - r.lo[1] + 1;
@@ -371,10 +431,7 @@ __global__ void __launch_bounds__(256) k_hsmac_pass(Geom g, T *U, T *V, T *W, T 
         // RedBlackFast index map: lib/parallel/src/foraxes.jl:100-105
         int i = a + 2 * ih + ((j + k - a + color) & 1);
         if (i > r.hi[0]) continue;
-        if (COLORING == LCS_COLORING_RED_BLACK) {
-            int s = i + j + k;
-            if (s % 2 != color) continue;  // truncating remainder, foraxes.jl:86
-        }
+        if (!visited<COLORING>(r, i, j, k)) continue;
         long p = g.at(i, j, k);
         T dudx = c.dxi * (-U[p - 1] + U[p]);
         T dvdy = c.dyi * (-V[p - sy] + V[p]);
@@ -388,13 +445,166 @@ __global__ void __launch_bounds__(256) k_hsmac_pass(Geom g, T *U, T *V, T *W, T 
         V[p] += dp * c.dyi;
         W[p - sz] -= dp * c.dzi;
         W[p] += dp * c.dzi;
-        if (i >= 1 && i <= g.nx + 1 && j >= 1 && j <= g.ny + 1 && k >= 1 && k <= g.nz + 1) {
-            T d2 = div * div;
-            acc += (double)d2;
-        }
+        if (in_d3(g, i, j, k)) acc += (double)(div * div);
     }
     double s = block_sum(acc, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Fused red+black sweep over R, current -> other buffer set, covering the
+// whole array so untouched elements are carried over.  A workgroup owns a
+// 64 x TY column of cells (TY = 4*RY) and a chunk of planes and marches up in
+// z with rings of LDS planes: at step q it stages plane q of U,V,W, computes
+// red corrections on plane q, black corrections on plane q-1 and writes plane
+// q-2.  Each owned cell (x,y,z) writes P(x,y,z) and the faces U(x), V(y),
+// W(z) that sit on its upper side.
+template <typename T, int COLORING, int RY>
+__global__ void __launch_bounds__(256) k_hsmac_fused(Geom g, BufSets<T> bs, Box3 R, HsCoef<T> c, DtRef dt,
+                                                    DevScalars *S, double *part, int zc)
+{
+    constexpr int TX = 64, TY = 4 * RY;
+    constexpr int UX = TX + 4, UY = TY + 3;  // U:  x in [X0-2, X0+TX+1], y in [Y0-1, Y0+TY+1]
+    constexpr int VX = TX + 3, VY = TY + 4;  // V:  x in [X0-1, X0+TX+1], y in [Y0-2, Y0+TY+1]
+    constexpr int WX = TX + 3, WY = TY + 3;  // W and red dp: x in [X0-1, X0+TX+1], y in [Y0-1, Y0+TY+1]
+    constexpr int BX = TX + 1, BY = TY + 1;  // black dp: x in [X0, X0+TX], y in [Y0, Y0+TY]
+    __shared__ T sU[3][UY][UX];
+    __shared__ T sV[3][VY][VX];
+    __shared__ T sW[3][WY][WX];
+    __shared__ T sR[3][WY][WX];
+    __shared__ T sB[2][BY][BX];
+    __shared__ double lds[4];
+    if (S->hs_done) return;
+    const int par = S->hs_par;
+    const T *Us = par ? bs.b[0] : bs.a[0], *Vs = par ? bs.b[1] : bs.a[1];
+    const T *Ws = par ? bs.b[2] : bs.a[2], *Ps = par ? bs.b[3] : bs.a[3];
+    T *Ud = par ? bs.a[0] : bs.b[0], *Vd = par ? bs.a[1] : bs.b[1];
+    T *Wd = par ? bs.a[2] : bs.b[2], *Pd = par ? bs.a[3] : bs.b[3];
+    const int tid = threadIdx.x + 64 * threadIdx.y;
+    const int X0 = 1 - g.h + TX * blockIdx.x;
+    const int Y0 = 1 - g.h + TY * blockIdx.y;
+    const int z0 = 1 - g.h + zc * blockIdx.z;
+    const int zend = g.nz + g.h;  // last array plane
+    const int z1 = min(z0 + zc, zend + 1);
+    const T dti = T(1.0 / (dt.factor * (*dt.src)));
+    auto inA = [&](int x, int y, int z) {
+        return x >= 1 - g.h && x <= g.nx + g.h && y >= 1 - g.h && y <= g.ny + g.h && z >= 1 - g.h && z <= zend;
+    };
+    auto slot = [](int q) { return ((q % 3) + 3) % 3; };
+    auto bslot = [](int q) { return q & 1; };
+    auto load_plane = [&](int q, bool uv) {
+        const int s = slot(q);
+        if (uv) {
+            for (int e = tid; e < UX * UY; e += 256) {
+                int lx = e % UX, ly = e / UX, x = X0 - 2 + lx, y = Y0 - 1 + ly;
+                sU[s][ly][lx] = inA(x, y, q) ? Us[g.at(x, y, q)] : T(0);
+            }
+            for (int e = tid; e < VX * VY; e += 256) {
+                int lx = e % VX, ly = e / VX, x = X0 - 1 + lx, y = Y0 - 2 + ly;
+                sV[s][ly][lx] = inA(x, y, q) ? Vs[g.at(x, y, q)] : T(0);
+            }
+        }
+        for (int e = tid; e < WX * WY; e += 256) {
+            int lx = e % WX, ly = e / WX, x = X0 - 1 + lx, y = Y0 - 1 + ly;
+            sW[s][ly][lx] = inA(x, y, q) ? Ws[g.at(x, y, q)] : T(0);
+        }
+    };
+    auto owned = [&](int x, int y, int z) {
+        return x >= X0 && x < X0 + TX && y >= Y0 && y < Y0 + TY && z >= z0 && z < z1;
+    };
+    double acc = 0.0;
+    load_plane(z0 - 2, false);
+    for (int q = z0 - 1; q <= z1 + 1; q++) {
+        load_plane(q, true);
+        __syncthreads();
+        // red corrections on plane q (colour 0 pass of the reference)
+        {
+            const int s = slot(q), sm = slot(q - 1);
+            for (int e = tid; e < WX * WY; e += 256) {
+                int lx = e % WX, ly = e / WX, x = X0 - 1 + lx, y = Y0 - 1 + ly;
+                T dp = T(0);
+                if (((x + y + q) & 1) == 0 && visited<COLORING>(R, x, y, q)) {
+                    T dudx = c.dxi * (-sU[s][ly][lx] + sU[s][ly][lx + 1]);
+                    T dvdy = c.dyi * (-sV[s][ly][lx] + sV[s][ly + 1][lx]);
+                    T dwdz = c.dzi * (-sW[sm][ly][lx] + sW[s][ly][lx]);
+                    T div = (dudx + dvdy) + dwdz;
+                    dp = (-c.omega * div) * c.dl;
+                    if (owned(x, y, q) && in_d3(g, x, y, q)) acc += (double)(div * div);
+                }
+                sR[s][ly][lx] = dp;
+            }
+        }
+        __syncthreads();
+        // black corrections on plane p = q-1 (colour 1 pass), on faces already
+        // corrected by their red neighbours
+        const int p = q - 1;
+        if (p >= z0) {
+            const int sp = slot(p), spm = slot(p - 1), sq = slot(q), sb = bslot(p);
+            for (int e = tid; e < BX * BY; e += 256) {
+                int bx = e % BX, by = e / BX, x = X0 + bx, y = Y0 + by;
+                int lx = bx + 1, ly = by + 1;
+                T dp = T(0);
+                if (((x + y + p) & 1) == 1 && visited<COLORING>(R, x, y, p)) {
+                    T Ur = sU[sp][ly][lx + 1] - sR[sp][ly][lx + 1] * c.dxi;
+                    T Ul = sU[sp][ly][lx] + sR[sp][ly][lx - 1] * c.dxi;
+                    T Vr = sV[sp][ly + 1][lx] - sR[sp][ly + 1][lx] * c.dyi;
+                    T Vl = sV[sp][ly][lx] + sR[sp][ly - 1][lx] * c.dyi;
+                    T Wt = sW[sp][ly][lx] - sR[sq][ly][lx] * c.dzi;
+                    T Wb = sW[spm][ly][lx] + sR[spm][ly][lx] * c.dzi;
+                    T dudx = c.dxi * (-Ul + Ur);
+                    T dvdy = c.dyi * (-Vl + Vr);
+                    T dwdz = c.dzi * (-Wb + Wt);
+                    T div = (dudx + dvdy) + dwdz;
+                    dp = (-c.omega * div) * c.dl;
+                    if (owned(x, y, p) && in_d3(g, x, y, p)) acc += (double)(div * div);
+                }
+                sB[sb][by][bx] = dp;
+            }
+        }
+        __syncthreads();
+        // write plane o = q-2
+        const int o = q - 2;
+        if (o >= z0 && o < z1) {
+            const int so = slot(o), sop = slot(o + 1), bo = bslot(o), bop = bslot(o + 1);
+            for (int r = 0; r < RY; r++) {
+                int tx = threadIdx.x, ty = threadIdx.y + 4 * r;
+                int x = X0 + tx, y = Y0 + ty;
+                if (!inA(x, y, o)) continue;
+                int lx = tx + 1, ly = ty + 1;
+                bool red = ((x + y + o) & 1) == 0;
+                // correction of this cell and of its +x, +y, +z neighbours (0 outside R)
+                T d0 = red ? sR[so][ly][lx] : sB[bo][ty][tx];
+                T dx1 = red ? sB[bo][ty][tx + 1] : sR[so][ly][lx + 1];
+                T dy1 = red ? sB[bo][ty + 1][tx] : sR[so][ly + 1][lx];
+                T dz1 = red ? sB[bop][ty][tx] : sR[sop][ly][lx];
+                long gi = g.at(x, y, o);
+                T pv = Ps[gi];
+                if (visited<COLORING>(R, x, y, o)) pv += d0 * dti;
+                Pd[gi] = pv;
+                T u = sU[so][ly][lx + 1], v = sV[so][ly + 1][lx], w = sW[so][ly][lx];
+                if (red) {
+                    u += d0 * c.dxi;
+                    u -= dx1 * c.dxi;
+                    v += d0 * c.dyi;
+                    v -= dy1 * c.dyi;
+                    w += d0 * c.dzi;
+                    w -= dz1 * c.dzi;
+                } else {
+                    u -= dx1 * c.dxi;
+                    u += d0 * c.dxi;
+                    v -= dy1 * c.dyi;
+                    v += d0 * c.dyi;
+                    w -= dz1 * c.dzi;
+                    w += d0 * c.dzi;
+                }
+                Ud[gi] = u;
+                Vd[gi] = v;
+                Wd[gi] = w;
+            }
+        }
+        __syncthreads();
+    }
+    double sum = block_sum(acc, lds);
+    if (tid == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = sum;
 }
 
 struct CheckArgs {
@@ -430,6 +640,12 @@ __global__ void k_hsmac_begin(DevScalars *S)
     S->hs_done = 0;
     S->hs_itr = 0;
     S->hs_div = 0.0;
+    S->hs_par = 0;
+}
+
+__global__ void k_hsmac_flip(DevScalars *S)
+{
+    if (!S->hs_done) S->hs_par ^= 1;
 }
 
 __global__ void k_hsmac_end(DevScalars *S, int slot)
@@ -443,7 +659,7 @@ __global__ void k_hsmac_end(DevScalars *S, int slot)
 template <typename T>
 struct HsmacRun {
     Box &b;
-    T *U, *V, *W, *P;
+    BufSets<T> bs;
     HsCoef<T> c;
     DtRef dt;
     int coloring;
@@ -458,33 +674,63 @@ struct HsmacRun {
             ca.cnt[npass++] = 0;
             return LCS_OK;
         }
-        int blocks = grid_for(total, 256);
+        int blocks = grid_for(total, 256, 4096);
         double *part = b.part + (long)npass * kMaxPartBlocks;
         // algorithmic: half of the 8 element passes (U,V,W,P read+write) per cell-sweep
         long cells = (long)(r.hi[0] - r.lo[0] + 1) * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
         ProfScope ps(b, "hsmac_pass", 4.0 * sizeof(T) * cells);
         if (coloring == LCS_COLORING_RED_BLACK)
             hipLaunchKernelGGL((k_hsmac_pass<T, LCS_COLORING_RED_BLACK>), dim3(blocks), dim3(256), 0, b.stream,
-                               b.g, U, V, W, P, r, color, c, dt, b.S, part);
+                               b.g, bs, r, color, c, dt, b.S, part);
         else
             hipLaunchKernelGGL((k_hsmac_pass<T, LCS_COLORING_RED_BLACK_FAST>), dim3(blocks), dim3(256), 0,
-                               b.stream, b.g, U, V, W, P, r, color, c, dt, b.S, part);
+                               b.stream, b.g, bs, r, color, c, dt, b.S, part);
         LCS_HIP(hipGetLastError());
         ca.cnt[npass++] = blocks;
         return LCS_OK;
     }
-    int sweep_box(const Box3 &r)
+    int sweep_box_passes(const Box3 &r)
     {
         int rc = pass(r, 0);
         if (rc) return rc;
         return pass(r, 1);
+    }
+    // one fused red+black sweep over a full box, then the parity flip
+    int fused(const Box3 &r)
+    {
+        constexpr int RY = sizeof(T) == 8 ? 1 : 3;
+        constexpr int TY = 4 * RY;
+        const Geom &g = b.g;
+        int gx = (g.nx + 2 * g.h + 63) / 64, gy = (g.ny + 2 * g.h + TY - 1) / TY;
+        int zplanes = g.nz + 2 * g.h;
+        long cols = (long)gx * gy;
+        int nzc = (int)std::max(1L, std::min<long>((8192 + cols - 1) / cols, zplanes / 16));
+        while (cols * nzc > kMaxPartBlocks) nzc--;
+        if (nzc < 1) return fail(LCS_ERR_INVALID, "grid too large for the fused HSMAC sweep");
+        int zc = (zplanes + nzc - 1) / nzc;
+        nzc = (zplanes + zc - 1) / zc;
+        double *part = b.part + (long)npass * kMaxPartBlocks;
+        long cells = (long)(r.hi[0] - r.lo[0] + 1) * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
+        ProfScope ps(b, "hsmac_sweep", 8.0 * sizeof(T) * cells);
+        dim3 grid(gx, gy, nzc), block(64, 4);
+        if (coloring == LCS_COLORING_RED_BLACK)
+            hipLaunchKernelGGL((k_hsmac_fused<T, LCS_COLORING_RED_BLACK, RY>), grid, block, 0, b.stream, g, bs, r,
+                               c, dt, b.S, part, zc);
+        else
+            hipLaunchKernelGGL((k_hsmac_fused<T, LCS_COLORING_RED_BLACK_FAST, RY>), grid, block, 0, b.stream, g,
+                               bs, r, c, dt, b.S, part, zc);
+        LCS_HIP(hipGetLastError());
+        ca.cnt[npass++] = (int)(cols * nzc);
+        hipLaunchKernelGGL(k_hsmac_flip, dim3(1), dim3(1), 0, b.stream, b.S);
+        LCS_HIP(hipGetLastError());
+        return LCS_OK;
     }
     // Topologies.faces order z-, z+, y-, y+, x-, x+ (lib/topologies/src/halo.jl:349-366)
     int sweep_core_faces(const Box3 &d)
     {
         const Geom &g = b.g;
         Box3 core = {{2, 2, 2}, {g.nx, g.ny, g.nz}};
-        int rc = sweep_box(core);
+        int rc = fused(core);
         if (rc) return rc;
         Box3 f[6] = {
             {{d.lo[0], d.lo[1], d.lo[2]}, {d.hi[0], d.hi[1], 1}},
@@ -495,7 +741,7 @@ struct HsmacRun {
             {{g.nx + 1, 2, 2}, {d.hi[0], g.ny, g.nz}},
         };
         for (int q = 0; q < 6; q++) {
-            rc = sweep_box(f[q]);
+            rc = sweep_box_passes(f[q]);
             if (rc) return rc;
         }
         return LCS_OK;
@@ -511,12 +757,13 @@ struct HsmacRun {
     }
     int fill(int width, bool withP)
     {
-        void *arr[4] = {U, V, W, P};
-        return launch_ghost_fill(b, arr, withP ? 4 : 3, width, 1, &b.S->hs_done);
+        if (b.esize == 8)
+            return ghost_fill_t<double>(b, (void *const *)bs.a, (void *const *)bs.b, withP ? 4 : 3, width, 1, b.S);
+        return ghost_fill_t<float>(b, (void *const *)bs.a, (void *const *)bs.b, withP ? 4 : 3, width, 1, b.S);
     }
     int poll(bool &done)
     {
-        LCS_HIP(hipMemcpyAsync(&b.S_host->hs_done, &b.S->hs_done, sizeof(int), hipMemcpyDeviceToHost, b.stream));
+        LCS_HIP(hipMemcpyAsync(b.S_host, b.S, sizeof(DevScalars), hipMemcpyDeviceToHost, b.stream));
         LCS_HIP(hipStreamSynchronize(b.stream));
         done = b.S_host->hs_done != 0;
         return LCS_OK;
@@ -524,13 +771,23 @@ struct HsmacRun {
 };
 
 template <typename T>
-static int hsmac_t(Box &b, void *const Us[3], DtRef dt, const lcs_hsmac_params &prm, int slot)
+static int hsmac_t(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int slot)
 {
     const Geom &g = b.g;
     if (g.h < 3 && prm.time_blocking) return fail(LCS_ERR_INVALID, "time-blocked HSMAC needs halo_size >= 3");
+    for (int q = 0; q < 4; q++)
+        if (!b.spare[q]) {
+            LCS_HIP(hipMalloc(&b.spare[q], (size_t)g.len * sizeof(T)));
+            b.bytes += (size_t)g.len * sizeof(T);
+        }
     double dx = spacing(g.nx), dy = spacing(g.ny), dz = spacing(g.nz);
     T dxi = T(1.0 / dx), dyi = T(1.0 / dy), dzi = T(1.0 / dz);
-    HsmacRun<T> run{b, (T *)Us[0], (T *)Us[1], (T *)Us[2], (T *)b.f[LCS_FIELD_P], {}, dt, prm.coloring, {}};
+    BufSets<T> bs;
+    void **Pp = &b.f[LCS_FIELD_P];
+    for (int q = 0; q < 3; q++) bs.a[q] = (T *)Us[q];
+    bs.a[3] = (T *)*Pp;
+    for (int q = 0; q < 4; q++) bs.b[q] = (T *)b.spare[q];
+    HsmacRun<T> run{b, bs, {}, dt, prm.coloring, {}};
     run.c.dxi = dxi;
     run.c.dyi = dyi;
     run.c.dzi = dzi;
@@ -544,16 +801,20 @@ static int hsmac_t(Box &b, void *const Us[3], DtRef dt, const lcs_hsmac_params &
     hipLaunchKernelGGL(k_hsmac_begin, dim3(1), dim3(1), 0, b.stream, b.S);
     LCS_HIP(hipGetLastError());
     int rc;
-    bool done = false;
+    bool done = false, polled_last = false;
     const Box3 d3 = {{1, 1, 1}, {g.nx + 1, g.ny + 1, g.nz + 1}};
     if (!prm.time_blocking) {
         // _couple_naive!: hsmac.jl:77-129
         for (int s = 0; s < prm.itrmax && !done; s++) {
             if ((rc = run.fill(1, !prm.overlap))) return rc;
-            rc = prm.overlap ? run.sweep_core_faces(d3) : run.sweep_box(d3);
+            rc = prm.overlap ? run.sweep_core_faces(d3) : run.fused(d3);
             if (rc) return rc;
             if ((rc = run.check())) return rc;
-            if (s >= prm.itrmin && (rc = run.poll(done))) return rc;
+            polled_last = false;
+            if (s >= prm.itrmin) {
+                if ((rc = run.poll(done))) return rc;
+                polled_last = true;
+            }
         }
     } else {
         // _couple_time_blocking!: hsmac.jl:131-201
@@ -561,22 +822,32 @@ static int hsmac_t(Box &b, void *const Us[3], DtRef dt, const lcs_hsmac_params &
         const Box3 d2 = {{0, 0, 0}, {g.nx + 2, g.ny + 2, g.nz + 2}};
         for (int grp = 0; 3 * grp < prm.itrmax && !done; grp++) {
             if ((rc = run.fill(g.h, false))) return rc;
-            rc = prm.overlap ? run.sweep_core_faces(d1) : run.sweep_box(d1);
+            rc = prm.overlap ? run.sweep_core_faces(d1) : run.fused(d1);
             if (rc) return rc;
             if ((rc = run.check())) return rc;
-            if ((rc = run.sweep_box(d2))) return rc;
+            if ((rc = run.fused(d2))) return rc;
             if ((rc = run.check())) return rc;
-            if ((rc = run.sweep_box(d3))) return rc;
+            if ((rc = run.fused(d3))) return rc;
             if ((rc = run.check())) return rc;
-            if (3 * grp + 2 >= prm.itrmin && (rc = run.poll(done))) return rc;
+            polled_last = false;
+            if (3 * grp + 2 >= prm.itrmin) {
+                if ((rc = run.poll(done))) return rc;
+                polled_last = true;
+            }
         }
     }
     hipLaunchKernelGGL(k_hsmac_end, dim3(1), dim3(1), 0, b.stream, b.S, slot);
     LCS_HIP(hipGetLastError());
+    if (!polled_last && (rc = run.poll(done))) return rc;
+    if (b.S_host->hs_par) {
+        // the result lives in the spare set: swap it in
+        for (int q = 0; q < 3; q++) std::swap(Us[q], b.spare[q]);
+        std::swap(*Pp, b.spare[3]);
+    }
     return LCS_OK;
 }
 
-int launch_hsmac(Box &b, void *const Us[3], DtRef dt, const lcs_hsmac_params &prm, int slot)
+int launch_hsmac(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int slot)
 {
     if (prm.coloring != LCS_COLORING_RED_BLACK_FAST && prm.coloring != LCS_COLORING_RED_BLACK)
         return fail(LCS_ERR_INVALID, "unsupported coloring");

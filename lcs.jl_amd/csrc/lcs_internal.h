@@ -62,6 +62,8 @@ struct DevScalars {
     // HSMAC call in flight
     int hs_done;
     int hs_itr;
+    int hs_par;  // 1 when the current HSMAC state lives in the spare buffers
+    int hs_pad;
     double hs_div;
     // log of the current step
     int itr[2];
@@ -76,7 +78,7 @@ struct DevScalars {
 constexpr int kMaxProfiles = 8;
 
 // Reduction partial slots (doubles) available to one box.
-constexpr int kMaxPartBlocks = 4096;     // blocks per reducing pass
+constexpr int kMaxPartBlocks = 16384;    // blocks per reducing pass
 constexpr int kMaxSweepPasses = 14;      // core 2 + 6 faces x 2 colours
 constexpr long kPartSlots = (long)kMaxPartBlocks * (kMaxSweepPasses + 4);
 
@@ -120,6 +122,7 @@ struct Box {
     int esize;  // 4 or 8
     hipStream_t stream = nullptr;
     void *f[10] = {};  // LCS_FIELD_* device pointers (DU..DW lazily allocated)
+    void *spare[4] = {};  // ping-pong partners of (U,V,W,P) for fused HSMAC sweeps
     DevScalars *S = nullptr;  // device scalar block
     DevScalars *S_host = nullptr;  // pinned mirror for readback
     double *part = nullptr;  // reduction partials (kPartSlots)
@@ -164,7 +167,9 @@ int launch_ghost_fill(Box &b, void *const *arrays, int narrays, int width, int f
 int launch_rhs(Box &b, int stage_in, bool fused, void *const out[3], void *const base[3], DtRef dt);
 int launch_update(Box &b, void *const out[3], void *const base[3], DtRef dt);
 int launch_cfl(Box &b, double cfl, double dtmax);  // writes S->umax, S->dt_flow
-int launch_hsmac(Box &b, void *const Us[3], DtRef dt, const lcs_hsmac_params &prm, int log_slot);
+// Us points at the box's three velocity pointers of the stage; when the
+// result ends up in the spare buffers these pointers (and P) are swapped.
+int launch_hsmac(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int log_slot);
 int launch_force_linear(Box &b, double power, DtRef dt);  // writes S->lf, S->p
 int launch_boxmean(Box &b, int c, double *coarse_dev);
 int launch_force_rcf(Box &b, double energy, int kmin, int kmax, int c, DtRef dt);

@@ -136,6 +136,26 @@ def test_hsmac_reference_case(rng, dtype, tb, ov):
 
 @pytest.mark.parametrize("tb,ov", HSMAC_CASES)
 @pytest.mark.parametrize("coloring", ["red-black-fast", "red-black"])
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_hsmac_multi_tile_non_cubic(rng, tb, ov, coloring, dtype):
+    """Several x tiles, y tiles and z chunks of the fused sweep kernel, odd
+    extents and both colourings (red-black's truncating % skips cells)."""
+    dims, h, dt = (70, 21, 40), 3, 0.02
+    L = lcs()
+    A = [rand_halo(rng, dims, h, dtype) for _ in range(4)]
+    box = make_box(dims, h, dtype)
+    upload_all(box, {"U": A[0], "V": A[1], "W": A[2], "P": A[3]})
+    hs = L.HSMAC(tb, ov, coloring, 1.7, 1e-3, 5, 5)
+    res = L.couple(box, hs, 2, dt)
+    itr, div = O.hsmac(*A, h, dt, tb, ov, 0 if coloring == "red-black-fast" else 1, 1.7, 1e-3, 5, 5)
+    assert res["itr"] == itr
+    assert abs(res["div"] - div) <= tol(dtype) * 10 * div
+    for name, e in zip(("U", "V", "W", "P"), A):
+        assert rel_l2(interior(box.download(name), h), interior(e, h)) < tol(dtype)
+
+
+@pytest.mark.parametrize("tb,ov", HSMAC_CASES)
+@pytest.mark.parametrize("coloring", ["red-black-fast", "red-black"])
 def test_hsmac_converging_smooth_field(tb, ov, coloring):
     """Convergence path (itrmin 6, itrmax 300, epsp 1e-3): same exit sweep."""
     dims, h, dt = (24, 24, 24), 3, 0.005
