@@ -11,6 +11,7 @@
 #include <cstring>
 
 #include "lcs_internal.h"
+#include "lcs_hsmac_tile.h"
 #include "lcs_rhs_tile.h"
 
 namespace lcs {
@@ -370,48 +371,17 @@ int launch_update(Box &b, void *const out[3], void *const base[3], DtRef dt)
 // buffer set holds the current state; it flips only when a fused sweep ran.
 // ---------------------------------------------------------------------------
 template <typename T>
-struct HsCoef {
-    T dxi, dyi, dzi, dl, omega;
-};
-
-struct Box3 {
-    int lo[3], hi[3];
-};
-
-template <typename T>
 struct BufSets {
     T *a[4];  // (U, V, W, P) as the call started
     T *b[4];  // spare partners
 };
-
-__device__ inline bool in_box(const Box3 &r, int x, int y, int z)
-{
-    return x >= r.lo[0] && x <= r.hi[0] && y >= r.lo[1] && y <= r.hi[1] && z >= r.lo[2] && z <= r.hi[2];
-}
-
-// red-black (truncating %) never visits cells with a negative odd index sum
-template <int COLORING>
-__device__ inline bool visited(const Box3 &r, int x, int y, int z)
-{
-    if (!in_box(r, x, y, z)) return false;
-    if (COLORING == LCS_COLORING_RED_BLACK) {
-        int s = x + y + z;
-        if (s < 0 && (s & 1)) return false;
-    }
-    return true;
-}
-
-__device__ inline bool in_d3(const Geom &g, int x, int y, int z)
-{
-    return x >= 1 && x <= g.nx + 1 && y >= 1 && y <= g.ny + 1 && z >= 1 && z <= g.nz + 1;
-}
 
 // One colour pass over a (thin) box, in place on the current buffer set.
 template <typename T, int COLORING>
 __global__ void __launch_bounds__(256) k_hsmac_pass(Geom g, BufSets<T> bs, Box3 r, int color, HsCoef<T> c,
                                                    DtRef dt, const DevScalars *S, double *part)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[16];
     if (S->hs_done) return;
     const bool alt = S->hs_par != 0;
     T *U = alt ? bs.b[0] : bs.a[0], *V = alt ? bs.b[1] : bs.a[1];
@@ -451,160 +421,52 @@ __global__ void __launch_bounds__(256) k_hsmac_pass(Geom g, BufSets<T> bs, Box3 
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// Fused red+black sweep over R, current -> other buffer set, covering the
-// whole array so untouched elements are carried over.  A workgroup owns a
-// 64 x TY column of cells (TY = 4*RY) and a chunk of planes and marches up in
-// z with rings of LDS planes: at step q it stages plane q of U,V,W, computes
-// red corrections on plane q, black corrections on plane q-1 and writes plane
-// q-2.  Each owned cell (x,y,z) writes P(x,y,z) and the faces U(x), V(y),
-// W(z) that sit on its upper side.
-template <typename T, int COLORING, int RY>
-__global__ void __launch_bounds__(256) k_hsmac_fused(Geom g, BufSets<T> bs, Box3 R, HsCoef<T> c, DtRef dt,
-                                                    DevScalars *S, double *part, int zc)
+// Fused red+black sweep over R, current -> other buffer set (the phase
+// functions and their schedule are in lcs_hsmac_tile.h).
+template <typename T, int COLORING, int TY>
+__global__ void __launch_bounds__(64 * TY) k_hsmac_sweep(Geom g, BufSets<T> bs, Box3 R, HsCoef<T> c, DtRef dt,
+                                                        DevScalars *S, double *part, int zc)
 {
-    constexpr int TX = 64, TY = 4 * RY;
-    constexpr int UX = TX + 4, UY = TY + 3;  // U:  x in [X0-2, X0+TX+1], y in [Y0-1, Y0+TY+1]
-    constexpr int VX = TX + 3, VY = TY + 4;  // V:  x in [X0-1, X0+TX+1], y in [Y0-2, Y0+TY+1]
-    constexpr int WX = TX + 3, WY = TY + 3;  // W and red dp: x in [X0-1, X0+TX+1], y in [Y0-1, Y0+TY+1]
-    constexpr int BX = TX + 1, BY = TY + 1;  // black dp: x in [X0, X0+TX], y in [Y0, Y0+TY]
-    __shared__ T sU[3][UY][UX];
-    __shared__ T sV[3][VY][VX];
-    __shared__ T sW[3][WY][WX];
-    __shared__ T sR[3][WY][WX];
-    __shared__ T sB[2][BY][BX];
-    __shared__ double lds[4];
+    using Tile = HsTile<T, COLORING, TY>;
+    extern __shared__ __align__(16) unsigned char hs_lds_raw[];
+    __shared__ double red_lds[16];
+    T *lds = reinterpret_cast<T *>(hs_lds_raw);
     if (S->hs_done) return;
-    const int par = S->hs_par;
-    const T *Us = par ? bs.b[0] : bs.a[0], *Vs = par ? bs.b[1] : bs.a[1];
-    const T *Ws = par ? bs.b[2] : bs.a[2], *Ps = par ? bs.b[3] : bs.a[3];
-    T *Ud = par ? bs.a[0] : bs.b[0], *Vd = par ? bs.a[1] : bs.b[1];
-    T *Wd = par ? bs.a[2] : bs.b[2], *Pd = par ? bs.a[3] : bs.b[3];
-    const int tid = threadIdx.x + 64 * threadIdx.y;
-    const int X0 = 1 - g.h + TX * blockIdx.x;
-    const int Y0 = 1 - g.h + TY * blockIdx.y;
-    const int z0 = 1 - g.h + zc * blockIdx.z;
-    const int zend = g.nz + g.h;  // last array plane
-    const int z1 = min(z0 + zc, zend + 1);
-    const T dti = T(1.0 / (dt.factor * (*dt.src)));
-    auto inA = [&](int x, int y, int z) {
-        return x >= 1 - g.h && x <= g.nx + g.h && y >= 1 - g.h && y <= g.ny + g.h && z >= 1 - g.h && z <= zend;
-    };
-    auto slot = [](int q) { return ((q % 3) + 3) % 3; };
-    auto bslot = [](int q) { return q & 1; };
-    auto load_plane = [&](int q, bool uv) {
-        const int s = slot(q);
-        if (uv) {
-            for (int e = tid; e < UX * UY; e += 256) {
-                int lx = e % UX, ly = e / UX, x = X0 - 2 + lx, y = Y0 - 1 + ly;
-                sU[s][ly][lx] = inA(x, y, q) ? Us[g.at(x, y, q)] : T(0);
-            }
-            for (int e = tid; e < VX * VY; e += 256) {
-                int lx = e % VX, ly = e / VX, x = X0 - 1 + lx, y = Y0 - 2 + ly;
-                sV[s][ly][lx] = inA(x, y, q) ? Vs[g.at(x, y, q)] : T(0);
-            }
-        }
-        for (int e = tid; e < WX * WY; e += 256) {
-            int lx = e % WX, ly = e / WX, x = X0 - 1 + lx, y = Y0 - 1 + ly;
-            sW[s][ly][lx] = inA(x, y, q) ? Ws[g.at(x, y, q)] : T(0);
-        }
-    };
-    auto owned = [&](int x, int y, int z) {
-        return x >= X0 && x < X0 + TX && y >= Y0 && y < Y0 + TY && z >= z0 && z < z1;
-    };
-    double acc = 0.0;
-    load_plane(z0 - 2, false);
-    for (int q = z0 - 1; q <= z1 + 1; q++) {
-        load_plane(q, true);
+    const bool par = S->hs_par != 0;
+    HsArgs<T> a;
+    a.g = g;
+    a.Us = par ? bs.b[0] : bs.a[0];
+    a.Vs = par ? bs.b[1] : bs.a[1];
+    a.Ws = par ? bs.b[2] : bs.a[2];
+    a.Ps = par ? bs.b[3] : bs.a[3];
+    a.Ud = par ? bs.a[0] : bs.b[0];
+    a.Vd = par ? bs.a[1] : bs.b[1];
+    a.Wd = par ? bs.a[2] : bs.b[2];
+    a.Pd = par ? bs.a[3] : bs.b[3];
+    a.R = R;
+    a.c = c;
+    a.dti = T(1.0 / (dt.factor * (*dt.src)));
+    a.zc = zc;
+    typename Tile::Thread th;
+    Tile::init(a, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x, th);
+    const int z0 = th.z0, z1 = th.z1;
+    Tile::fetch(a, z0 - 2, th);
+    Tile::store(lds, z0 - 2, th);
+    Tile::fetch(a, z0 - 1, th);
+    Tile::store(lds, z0 - 1, th);
+    __syncthreads();
+    for (int q = z0 - 1; q <= z1 + 3; q++) {
+        if (q + 1 <= z1 + 2) Tile::fetch(a, q + 1, th);
+        if (q - 3 >= z0) Tile::fetch_p(a, q - 3, th);
+        if (q <= z1 + 2) Tile::red(a, lds, q, th);
         __syncthreads();
-        // red corrections on plane q (colour 0 pass of the reference)
-        {
-            const int s = slot(q), sm = slot(q - 1);
-            for (int e = tid; e < WX * WY; e += 256) {
-                int lx = e % WX, ly = e / WX, x = X0 - 1 + lx, y = Y0 - 1 + ly;
-                T dp = T(0);
-                if (((x + y + q) & 1) == 0 && visited<COLORING>(R, x, y, q)) {
-                    T dudx = c.dxi * (-sU[s][ly][lx] + sU[s][ly][lx + 1]);
-                    T dvdy = c.dyi * (-sV[s][ly][lx] + sV[s][ly + 1][lx]);
-                    T dwdz = c.dzi * (-sW[sm][ly][lx] + sW[s][ly][lx]);
-                    T div = (dudx + dvdy) + dwdz;
-                    dp = (-c.omega * div) * c.dl;
-                    if (owned(x, y, q) && in_d3(g, x, y, q)) acc += (double)(div * div);
-                }
-                sR[s][ly][lx] = dp;
-            }
-        }
-        __syncthreads();
-        // black corrections on plane p = q-1 (colour 1 pass), on faces already
-        // corrected by their red neighbours
-        const int p = q - 1;
-        if (p >= z0) {
-            const int sp = slot(p), spm = slot(p - 1), sq = slot(q), sb = bslot(p);
-            for (int e = tid; e < BX * BY; e += 256) {
-                int bx = e % BX, by = e / BX, x = X0 + bx, y = Y0 + by;
-                int lx = bx + 1, ly = by + 1;
-                T dp = T(0);
-                if (((x + y + p) & 1) == 1 && visited<COLORING>(R, x, y, p)) {
-                    T Ur = sU[sp][ly][lx + 1] - sR[sp][ly][lx + 1] * c.dxi;
-                    T Ul = sU[sp][ly][lx] + sR[sp][ly][lx - 1] * c.dxi;
-                    T Vr = sV[sp][ly + 1][lx] - sR[sp][ly + 1][lx] * c.dyi;
-                    T Vl = sV[sp][ly][lx] + sR[sp][ly - 1][lx] * c.dyi;
-                    T Wt = sW[sp][ly][lx] - sR[sq][ly][lx] * c.dzi;
-                    T Wb = sW[spm][ly][lx] + sR[spm][ly][lx] * c.dzi;
-                    T dudx = c.dxi * (-Ul + Ur);
-                    T dvdy = c.dyi * (-Vl + Vr);
-                    T dwdz = c.dzi * (-Wb + Wt);
-                    T div = (dudx + dvdy) + dwdz;
-                    dp = (-c.omega * div) * c.dl;
-                    if (owned(x, y, p) && in_d3(g, x, y, p)) acc += (double)(div * div);
-                }
-                sB[sb][by][bx] = dp;
-            }
-        }
-        __syncthreads();
-        // write plane o = q-2
-        const int o = q - 2;
-        if (o >= z0 && o < z1) {
-            const int so = slot(o), sop = slot(o + 1), bo = bslot(o), bop = bslot(o + 1);
-            for (int r = 0; r < RY; r++) {
-                int tx = threadIdx.x, ty = threadIdx.y + 4 * r;
-                int x = X0 + tx, y = Y0 + ty;
-                if (!inA(x, y, o)) continue;
-                int lx = tx + 1, ly = ty + 1;
-                bool red = ((x + y + o) & 1) == 0;
-                // correction of this cell and of its +x, +y, +z neighbours (0 outside R)
-                T d0 = red ? sR[so][ly][lx] : sB[bo][ty][tx];
-                T dx1 = red ? sB[bo][ty][tx + 1] : sR[so][ly][lx + 1];
-                T dy1 = red ? sB[bo][ty + 1][tx] : sR[so][ly + 1][lx];
-                T dz1 = red ? sB[bop][ty][tx] : sR[sop][ly][lx];
-                long gi = g.at(x, y, o);
-                T pv = Ps[gi];
-                if (visited<COLORING>(R, x, y, o)) pv += d0 * dti;
-                Pd[gi] = pv;
-                T u = sU[so][ly][lx + 1], v = sV[so][ly + 1][lx], w = sW[so][ly][lx];
-                if (red) {
-                    u += d0 * c.dxi;
-                    u -= dx1 * c.dxi;
-                    v += d0 * c.dyi;
-                    v -= dy1 * c.dyi;
-                    w += d0 * c.dzi;
-                    w -= dz1 * c.dzi;
-                } else {
-                    u -= dx1 * c.dxi;
-                    u += d0 * c.dxi;
-                    v -= dy1 * c.dyi;
-                    v += d0 * c.dyi;
-                    w -= dz1 * c.dzi;
-                    w += d0 * c.dzi;
-                }
-                Ud[gi] = u;
-                Vd[gi] = v;
-                Wd[gi] = w;
-            }
-        }
+        if (q - 1 >= z0 && q - 1 <= z1 + 1) Tile::black(a, lds, q - 1, th);
+        if (q - 3 >= z0) Tile::output(a, lds, q - 3, th);
+        if (q + 1 <= z1 + 2) Tile::store(lds, q + 1, th);
         __syncthreads();
     }
-    double sum = block_sum(acc, lds);
-    if (tid == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = sum;
+    const double s = block_sum(th.acc, red_lds);
+    if (threadIdx.x == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
 }
 
 struct CheckArgs {
@@ -616,7 +478,7 @@ struct CheckArgs {
 
 __global__ void k_hsmac_check(const double *part, CheckArgs a, DevScalars *S)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[16];
     if (S->hs_done) return;  // uniform across the block
     double acc = 0.0;
     for (int q = 0; q < a.npass; q++) {
@@ -696,34 +558,45 @@ struct HsmacRun {
         return pass(r, 1);
     }
     // one fused red+black sweep over a full box, then the parity flip
-    int fused(const Box3 &r)
+    template <int COLORING, int TY>
+    int fused_launch(const Box3 &r)
     {
-        constexpr int RY = sizeof(T) == 8 ? 1 : 3;
-        constexpr int TY = 4 * RY;
+        using Tile = HsTile<T, COLORING, TY>;
+        static_assert(Tile::LDS_ELEMS * sizeof(T) <= 160 * 1024, "tile exceeds the LDS of a CU");
         const Geom &g = b.g;
-        int gx = (g.nx + 2 * g.h + 63) / 64, gy = (g.ny + 2 * g.h + TY - 1) / TY;
-        int zplanes = g.nz + 2 * g.h;
-        long cols = (long)gx * gy;
-        int nzc = (int)std::max(1L, std::min<long>((8192 + cols - 1) / cols, zplanes / 16));
-        while (cols * nzc > kMaxPartBlocks) nzc--;
-        if (nzc < 1) return fail(LCS_ERR_INVALID, "grid too large for the fused HSMAC sweep");
-        int zc = (zplanes + nzc - 1) / nzc;
+        const int gx = (g.nx + 2 * g.h + Tile::TX - 1) / Tile::TX, gy = (g.ny + 2 * g.h + TY - 1) / TY;
+        const int zplanes = g.nz + 2 * g.h;
+        const long cols = (long)gx * gy;
+        // z-chunks: about four workgroups per CU slot, chunks of at least 16
+        // planes (each chunk re-reads five halo planes)
+        int nzc = (int)std::max(1L, std::min<long>((4096 + cols - 1) / cols, zplanes / 16));
+        while (nzc > 1 && cols * nzc > kMaxPartBlocks) nzc--;
+        if (cols * nzc > kMaxPartBlocks) return fail(LCS_ERR_INVALID, "grid too large for the fused HSMAC sweep");
+        const int zc = (zplanes + nzc - 1) / nzc;
         nzc = (zplanes + zc - 1) / zc;
+        const size_t lds = Tile::LDS_ELEMS * sizeof(T);
+        static bool attr_set = false;  // per template instance
+        if (!attr_set) {
+            LCS_HIP(hipFuncSetAttribute((const void *)k_hsmac_sweep<T, COLORING, TY>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            attr_set = true;
+        }
         double *part = b.part + (long)npass * kMaxPartBlocks;
         long cells = (long)(r.hi[0] - r.lo[0] + 1) * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
         ProfScope ps(b, "hsmac_sweep", 8.0 * sizeof(T) * cells);
-        dim3 grid(gx, gy, nzc), block(64, 4);
-        if (coloring == LCS_COLORING_RED_BLACK)
-            hipLaunchKernelGGL((k_hsmac_fused<T, LCS_COLORING_RED_BLACK, RY>), grid, block, 0, b.stream, g, bs, r,
-                               c, dt, b.S, part, zc);
-        else
-            hipLaunchKernelGGL((k_hsmac_fused<T, LCS_COLORING_RED_BLACK_FAST, RY>), grid, block, 0, b.stream, g,
-                               bs, r, c, dt, b.S, part, zc);
+        hipLaunchKernelGGL((k_hsmac_sweep<T, COLORING, TY>), dim3(gx, gy, nzc), dim3(Tile::NT), lds, b.stream, g,
+                           bs, r, c, dt, b.S, part, zc);
         LCS_HIP(hipGetLastError());
         ca.cnt[npass++] = (int)(cols * nzc);
         hipLaunchKernelGGL(k_hsmac_flip, dim3(1), dim3(1), 0, b.stream, b.S);
         LCS_HIP(hipGetLastError());
         return LCS_OK;
+    }
+    int fused(const Box3 &r)
+    {
+        constexpr int TY = sizeof(T) == 8 ? 4 : 8;
+        return coloring == LCS_COLORING_RED_BLACK ? fused_launch<LCS_COLORING_RED_BLACK, TY>(r)
+                                                  : fused_launch<LCS_COLORING_RED_BLACK_FAST, TY>(r);
     }
     // Topologies.faces order z-, z+, y-, y+, x-, x+ (lib/topologies/src/halo.jl:349-366)
     int sweep_core_faces(const Box3 &d)
@@ -863,7 +736,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_cfl(Geom g, const T *__restrict__ U, const T *__restrict__ V,
                                             const T *__restrict__ W, double *part)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[16];
     const long sy = g.sx, sz = g.sx * g.sy;
     double m[3] = {0.0, 0.0, 0.0};
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < g.cells(); t += (long)gridDim.x * blockDim.x) {
@@ -888,7 +761,7 @@ __global__ void __launch_bounds__(256) k_cfl(Geom g, const T *__restrict__ U, co
 __global__ void k_cfl_finalize(const double *part, int nblk, double cfl, double dtmax, double dx, double dy,
                                double dz, double Re, DevScalars *S)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[16];
     double mm[3];
     for (int a = 0; a < 3; a++) {
         double m = 0.0;
@@ -937,7 +810,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_lf_means(Geom g, const T *__restrict__ U, const T *__restrict__ V,
                                                  const T *__restrict__ W, double *part)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[16];
     const long sy = g.sx, sz = g.sx * g.sy;
     double s[4] = {0, 0, 0, 0};
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < g.cells(); t += (long)gridDim.x * blockDim.x) {
@@ -963,7 +836,7 @@ __global__ void __launch_bounds__(256) k_lf_means(Geom g, const T *__restrict__ 
 
 __global__ void k_lf_finalize(const double *part, int nblk, double len, double power, DevScalars *S)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[16];
     double m[4];
     for (int a = 0; a < 4; a++) {
         double s = 0.0;
@@ -985,7 +858,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_lf_apply(Geom g, T *U, T *V, T *W, const DevScalars *S, DtRef dt,
                                                  double *part)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[16];
     const T A = T(S->lf[4]), um = T(S->lf[1]), vm = T(S->lf[2]), wm = T(S->lf[3]);
     const T rdt = T(dt.factor * (*dt.src));
     double sp = 0.0;
@@ -1009,7 +882,7 @@ __global__ void __launch_bounds__(256) k_lf_apply(Geom g, T *U, T *V, T *W, cons
 
 __global__ void k_mean_to(const double *part, int nblk, double len, double *dst)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[16];
     double s = 0.0;
     for (int q = threadIdx.x; q < nblk; q += blockDim.x) s += part[q];
     s = block_sum(s, lds);
@@ -1061,7 +934,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_boxmean(Geom g, const T *U, const T *V, const T *W, int c,
                                                 double *coarse)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[16];
     int cell = blockIdx.x, comp = blockIdx.y;
     const T *A = comp == 0 ? U : (comp == 1 ? V : W);
     int ic = cell % c, jc = (cell / c) % c, kc = cell / (c * c);
@@ -1088,7 +961,7 @@ __host__ __device__ inline int fftidx(int k, int n) { return k >= 0 ? k : n + k;
 // per crop slot (kx in 0:kmax, ky,kz in -kmax:kmax).
 __global__ void __launch_bounds__(256) k_rcf_dft(const double *coarse, int c, int kmax, double *hat)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[16];
     int nk = 2 * kmax + 1;
     int slot = blockIdx.x;
     int kx = slot % (kmax + 1);
@@ -1224,7 +1097,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_rcf_apply(Geom g, T *U, T *V, T *W, int c, int kmax, RcfModes md,
                                                   const double *crop, DtRef dt, double *part)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[16];
     const int nk = 2 * kmax + 1, ncrop = (kmax + 1) * nk * nk;
     const double phi = (double)(g.nx / c) / 2.0;
     const double nxi = 1.0 / g.nx, nyi = 1.0 / g.ny, nzi = 1.0 / g.nz;
@@ -1446,7 +1319,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_spec_z(Geom g, const double *G2, int K, double zoff, double dz, T *F,
                                                double *part)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[16];
     double ss = 0;
     int nk = 2 * K + 1;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < g.cells(); t += (long)gridDim.x * blockDim.x) {

@@ -346,7 +346,7 @@ __device__ inline double block_max_d(double v, double *lds)
 template <typename T>
 __global__ void k_p_maxdiam(long n, const T *d, double *part)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[16];
     double m = -INFINITY;
     for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x)
         m = fmax(m, (double)d[p]);
@@ -356,7 +356,7 @@ __global__ void k_p_maxdiam(long n, const T *d, double *part)
 
 __global__ void k_p_dt(const double *part, int nblk, double kap, double dtmax, double *dst)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[16];
     double m = -INFINITY;
     for (int q = threadIdx.x; q < nblk; q += blockDim.x) m = fmax(m, part[q]);
     m = block_max_d(m, lds);

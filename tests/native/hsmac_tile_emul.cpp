@@ -1,0 +1,70 @@
+// Host execution of the fused HSMAC sweep kernel's phase functions
+// (lcs.jl_amd/csrc/lcs_hsmac_tile.h): every workgroup of the launch grid is
+// run thread by thread, phase by phase, in the kernel's barrier order, so a
+// test can compare the kernel's own code with the oracle's two colour passes.
+#include <vector>
+
+#include "lcs_hsmac_tile.h"
+
+using namespace lcs;
+
+template <typename T, int COLORING, int TY>
+static double run(int nx, int ny, int nz, int h, const int *lo, const int *hi, double dt, double omega,
+                  void *const *src, void *const *dst, int zc)
+{
+    using Tile = HsTile<T, COLORING, TY>;
+    HsArgs<T> a;
+    a.g = make_geom(nx, ny, nz, h);
+    a.Us = (const T *)src[0];
+    a.Vs = (const T *)src[1];
+    a.Ws = (const T *)src[2];
+    a.Ps = (const T *)src[3];
+    a.Ud = (T *)dst[0];
+    a.Vd = (T *)dst[1];
+    a.Wd = (T *)dst[2];
+    a.Pd = (T *)dst[3];
+    for (int q = 0; q < 3; q++) {
+        a.R.lo[q] = lo[q];
+        a.R.hi[q] = hi[q];
+    }
+    const T dxi = T(1.0 / spacing(nx)), dyi = T(1.0 / spacing(ny)), dzi = T(1.0 / spacing(nz));
+    a.c.dxi = dxi;
+    a.c.dyi = dyi;
+    a.c.dzi = dzi;
+    a.c.dl = T(1) / (T(2) * ((dxi * dxi + dyi * dyi) + dzi * dzi));
+    a.c.omega = T(omega);
+    a.dti = T(1.0 / dt);
+    a.zc = zc;
+    std::vector<T> lds(Tile::LDS_ELEMS);
+    std::vector<typename Tile::Thread> th(Tile::NT);
+    const int gx = (nx + 2 * h + Tile::TX - 1) / Tile::TX, gy = (ny + 2 * h + TY - 1) / TY;
+    const int gz = (nz + 2 * h + zc - 1) / zc;
+    double s = 0.0;
+    for (int bz = 0; bz < gz; bz++)
+        for (int by = 0; by < gy; by++)
+            for (int bx = 0; bx < gx; bx++) s += Tile::emulate_block(a, lds.data(), th.data(), bx, by, bz);
+    return s;
+}
+
+extern "C" int hsmac_sweep_emul(int ty, int esize, int coloring, int nx, int ny, int nz, int h, const int *lo,
+                                const int *hi, double dt, double omega, void *const *src, void *const *dst, int zc,
+                                double *div2)
+{
+#define RUN(T, C, Y) *div2 = run<T, C, Y>(nx, ny, nz, h, lo, hi, dt, omega, src, dst, zc)
+#define BY_TY(T, C)               \
+    switch (ty) {                 \
+    case 4: RUN(T, C, 4); break;  \
+    case 8: RUN(T, C, 8); break;  \
+    default: return -1;           \
+    }
+    if (esize == 4) {
+        if (coloring == LCS_COLORING_RED_BLACK) { BY_TY(float, LCS_COLORING_RED_BLACK) }
+        else { BY_TY(float, LCS_COLORING_RED_BLACK_FAST) }
+    } else {
+        if (coloring == LCS_COLORING_RED_BLACK) { BY_TY(double, LCS_COLORING_RED_BLACK) }
+        else { BY_TY(double, LCS_COLORING_RED_BLACK_FAST) }
+    }
+#undef BY_TY
+#undef RUN
+    return 0;
+}

@@ -1,0 +1,74 @@
+"""The fused HSMAC sweep kernel's own phase functions, executed on the host.
+
+``tests/native/hsmac_tile_emul.cpp`` runs every workgroup of the launch grid
+of ``k_hsmac_sweep`` thread by thread and phase by phase, in the kernel's
+barrier order, over the same ``HsTile`` code the kernel compiles
+(lcs_hsmac_tile.h).  One fused sweep over a box R must equal the oracle's
+colour-0 pass followed by its colour-1 pass over R (hsmac.jl:1-35,
+lib/parallel/src/foraxes.jl:73-107) bit for bit -- every element of U, V, W,
+P, ghosts included -- and its div^2 sum over (1:n+1)^3 must equal the
+oracle's.  Boxes: the three time-blocking domains d1, d2, d3 and the overlap
+core (2:n)^3; ragged tiles in x and y, several z-chunkings, both colourings,
+both element types.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def emul(tmp_path_factory):
+    if not os.path.isdir("/opt/rocm/include/hip"):
+        pytest.skip("HIP headers not installed")
+    out = tmp_path_factory.mktemp("hsmac_tile") / "libhsmac_tile_emul.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-shared", "-fPIC", "-D__HIP_PLATFORM_AMD__",
+                    "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include"),
+                    "-I" + os.path.join(ROOT, "lcs.jl_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "native", "hsmac_tile_emul.cpp"), "-o", str(out)], check=True)
+    return C.CDLL(str(out))
+
+
+def _box(name, n):
+    n = np.asarray(n)
+    lo, hi = {"d1": (-np.ones(3), n + 3), "d2": (np.zeros(3), n + 2), "d3": (np.ones(3), n + 1),
+              "core": (2 * np.ones(3), n)}[name]
+    return np.asarray(lo, dtype=np.int32), np.asarray(hi, dtype=np.int32)
+
+
+CASES = [((8, 8, 8), 4, 14, "d1"), ((16, 12, 10), 4, 5, "d2"), ((70, 9, 7), 8, 3, "d3"), ((17, 5, 9), 8, 4, "core"),
+         ((20, 13, 11), 4, 1, "d1"), ((66, 20, 6), 8, 12, "d3"), ((5, 3, 4), 4, 2, "d1")]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("coloring", [0, 1])
+@pytest.mark.parametrize("dims,ty,zc,box", CASES)
+def test_fused_sweep_equals_two_colour_passes(emul, dims, ty, zc, box, coloring, dtype):
+    h, dt, omega = 3, 0.01, 1.7
+    rng = np.random.default_rng(abs(hash((dims, ty, zc, box, coloring))) % 2**32)
+    src = [np.asfortranarray(rng.standard_normal(tuple(n + 2 * h for n in dims)).astype(dtype)) for _ in range(4)]
+    lo, hi = _box(box, dims)
+    ref = [a.copy(order="F") for a in src]
+    div2 = np.zeros_like(ref[0])
+    O.hsmac_pass(*ref, div2, h, lo, hi, 0, dt, omega, coloring)
+    O.hsmac_pass(*ref, div2, h, lo, hi, 1, dt, omega, coloring)
+    nx, ny, nz = dims
+    s_ref = div2[h:h + nx + 1, h:h + ny + 1, h:h + nz + 1].astype(np.float64).sum()
+
+    dst = [np.full_like(a, np.nan) for a in src]
+    sp = (C.c_void_p * 4)(*[a.ctypes.data for a in src])
+    dp = (C.c_void_p * 4)(*[a.ctypes.data for a in dst])
+    s = C.c_double(0)
+    rc = emul.hsmac_sweep_emul(ty, np.dtype(dtype).itemsize, coloring, nx, ny, nz, h,
+                               lo.ctypes.data_as(C.c_void_p), hi.ctypes.data_as(C.c_void_p), C.c_double(dt),
+                               C.c_double(omega), sp, dp, zc, C.byref(s))
+    assert rc == 0
+    for name, a, b in zip("UVWP", dst, ref):
+        np.testing.assert_array_equal(a, b, err_msg=name)
+    assert s.value == pytest.approx(s_ref, rel=1e-12)
