@@ -174,6 +174,11 @@ int lcs_box_create(lcs_ctx *ctx, const lcs_box_desc *d, lcs_box **out)
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     LCS_HIP(hipSetDevice(c->device));
     Box *b = new Box;
+    {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && cus > 0)
+            b->cus = cus;
+    }
     b->ctx = c;
     b->desc = *d;
     b->g = make_geom(d->nx, d->ny, d->nz, d->halo);

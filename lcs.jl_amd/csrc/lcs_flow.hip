@@ -457,12 +457,16 @@ __global__ void __launch_bounds__(64 * TY) k_hsmac_sweep(Geom g, BufSets<T> bs, 
     __syncthreads();
     for (int q = z0 - 1; q <= z1 + 3; q++) {
         if (q + 1 <= z1 + 2) Tile::fetch(a, q + 1, th);
-        if (q - 3 >= z0) Tile::fetch_p(a, q - 3, th);
+        if (q - 2 >= z0 && q - 2 <= z1) Tile::fetch_p(a, q - 2, th);
         if (q <= z1 + 2) Tile::red(a, lds, q, th);
         __syncthreads();
         if (q - 1 >= z0 && q - 1 <= z1 + 1) Tile::black(a, lds, q - 1, th);
         if (q - 3 >= z0) Tile::output(a, lds, q - 3, th);
-        if (q + 1 <= z1 + 2) Tile::store(lds, q + 1, th);
+        if (q + 1 <= z1 + 2)
+            Tile::store(lds, q + 1, th);
+        else
+            Tile::shift(th);
+        th.pP = th.pPn;
         __syncthreads();
     }
     const double s = block_sum(th.acc, red_lds);
@@ -567,20 +571,33 @@ struct HsmacRun {
         const int gx = (g.nx + 2 * g.h + Tile::TX - 1) / Tile::TX, gy = (g.ny + 2 * g.h + TY - 1) / TY;
         const int zplanes = g.nz + 2 * g.h;
         const long cols = (long)gx * gy;
-        // z-chunks: about four workgroups per CU slot, chunks of at least 16
-        // planes (each chunk re-reads five halo planes)
-        int nzc = (int)std::max(1L, std::min<long>((4096 + cols - 1) / cols, zplanes / 16));
-        while (nzc > 1 && cols * nzc > kMaxPartBlocks) nzc--;
+        const size_t lds = Tile::LDS_ELEMS * sizeof(T);
+        static int per_cu = 0;  // resident workgroups per CU, per template instance
+        if (!per_cu) {
+            LCS_HIP(hipFuncSetAttribute((const void *)k_hsmac_sweep<T, COLORING, TY>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            LCS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_hsmac_sweep<T, COLORING, TY>, Tile::NT,
+                                                                 lds));
+            if (per_cu < 1) per_cu = 1;
+        }
+        // z-chunks: a chunk re-reads five halo planes and the last round of
+        // workgroups may be partial; pick the count with the fewest
+        // plane-steps on the busiest CU slot
+        const long slots = (long)b.cus * per_cu;
+        int nzc = 1;
+        double best = 1e300;
+        for (int n = 1; n <= 64 && n <= zplanes; n++) {
+            if (cols * n > kMaxPartBlocks) break;
+            const long zc_n = (zplanes + n - 1) / n;
+            const double cost = (double)((cols * n + slots - 1) / slots) * (double)(zc_n + 5);
+            if (cost < best * 0.999) {
+                best = cost;
+                nzc = n;
+            }
+        }
         if (cols * nzc > kMaxPartBlocks) return fail(LCS_ERR_INVALID, "grid too large for the fused HSMAC sweep");
         const int zc = (zplanes + nzc - 1) / nzc;
         nzc = (zplanes + zc - 1) / zc;
-        const size_t lds = Tile::LDS_ELEMS * sizeof(T);
-        static bool attr_set = false;  // per template instance
-        if (!attr_set) {
-            LCS_HIP(hipFuncSetAttribute((const void *)k_hsmac_sweep<T, COLORING, TY>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            attr_set = true;
-        }
         double *part = b.part + (long)npass * kMaxPartBlocks;
         long cells = (long)(r.hi[0] - r.lo[0] + 1) * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
         ProfScope ps(b, "hsmac_sweep", 8.0 * sizeof(T) * cells);
@@ -594,7 +611,7 @@ struct HsmacRun {
     }
     int fused(const Box3 &r)
     {
-        constexpr int TY = sizeof(T) == 8 ? 4 : 8;
+        constexpr int TY = 8;
         return coloring == LCS_COLORING_RED_BLACK ? fused_launch<LCS_COLORING_RED_BLACK, TY>(r)
                                                   : fused_launch<LCS_COLORING_RED_BLACK_FAST, TY>(r);
     }

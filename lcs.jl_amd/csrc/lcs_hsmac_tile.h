@@ -10,16 +10,19 @@
 //
 // A workgroup owns a 64 x TY column of cells and a chunk of planes [z0, z1]
 // and marches up in z.  At step q:
-//   phase A  red dp on plane q over x in [X0-1, X1+2], y in [Y0-1, Y1+2]
-//            (the red neighbours of the black cells below);
+//   phase A  fetch plane q+1 into registers; red dp on plane q over x in
+//            [X0-1, X1+2], y in [Y0-1, Y1+2] (the red neighbours of the black
+//            cells below);
 //   phase B  black dp on plane q-1 over x in [X0, X1+1], y in [Y0, Y1+1];
-//            final U, V, W, P of plane q-3 for the owned cells; the
-//            prefetched plane q+1 goes into the ring slot of plane q-4.
-// LDS holds U, V, W for planes q-3..q+1 (ring of 5) over x in [X0-2, X1+2],
-// y in [Y0-2, Y1+2], red dp for q-3..q (ring of 4), black dp for q-3..q-1
-// (ring of 3).  Two barriers per plane.  Every cell (x, y, z) owned by the
-// workgroup writes P and its upper faces U(x), V(y), W(z); the grid covers
-// the whole array, so elements outside R are carried over to the other set.
+//            final U, V, W, P of plane q-3 for the owned cells; plane q+1
+//            goes into LDS.
+// LDS holds U and V for planes q-1..q+1, W for q-2..q+1 over x in
+// [X0-2, X1+2], y in [Y0-2, Y1+2], and one dp plane per z for q-3..q in
+// which red cells hold red dp and black cells black dp.  A thread keeps its
+// own cell's U, V, W for planes q-3..q in registers (the only reader of plane
+// q-3).  Two barriers per plane.  Every cell (x, y, z) owned by the workgroup
+// writes P and its upper faces U(x), V(y), W(z); the grid covers the whole
+// array, so elements outside R are carried over to the other set.
 //
 // All of it is __host__ __device__ so a host harness can run the phases
 // thread by thread against the oracle.
@@ -77,38 +80,47 @@ template <typename T, int COLORING, int TY_>
 struct HsTile {
     static constexpr int TX = 64, TY = TY_, NT = TX * TY;
     static constexpr int LX = TX + 4, LY = TY + 4, PLANE = LX * LY;  // fields: x in [X0-2, X1+2]
-    static constexpr int RXN = TX + 3, RYN = TY + 3, RPL = RXN * RYN;  // red dp: x in [X0-1, X1+2]
-    static constexpr int BXN = TX + 1, BYN = TY + 1, BPL = BXN * BYN;  // black dp: x in [X0, X1+1]
-    static constexpr int NF = 5, NRS = 4, NBS = 3;                     // ring depths
-    static constexpr int OFF_R = 3 * NF * PLANE, OFF_B = OFF_R + NRS * RPL;
-    static constexpr int LDS_ELEMS = OFF_B + NBS * BPL;
-    static constexpr int NLD = (PLANE + NT - 1) / NT;
-    static constexpr int RPR = (RXN + 1) / 2, BPR = (BXN + 1) / 2;  // cells of one colour per row
-    static_assert(RPR * RYN <= NT && BPR * BYN <= NT, "one red and one black slot per thread");
+    static constexpr int DXN = TX + 3, DYN = TY + 3, DPL = DXN * DYN;  // dp: x in [X0-1, X1+2]
+    static constexpr int OFF_V = 3 * PLANE, OFF_W = 6 * PLANE, OFF_D = 10 * PLANE;
+    static constexpr int LDS_ELEMS = OFF_D + 4 * DPL;
+    static constexpr int NHALO = 4 * LX + 4 * TY;                   // ring of width 2 around the owned cells
+    static constexpr int NHS = (NHALO + NT - 1) / NT;              // halo elements per thread
+    static constexpr int RPR = (DXN + 1) / 2, BPR = (TX + 2) / 2;  // cells of one colour per row
+    static constexpr int BYN = TY + 1;
+    static_assert(RPR * DYN <= NT && BPR * BYN <= NT, "one red and one black slot per thread");
 
     struct Thread {
         int tid, X0, Y0, z0, z1;
-        int ld_lds[NLD];
-        long ld_off[NLD];  // in-plane offset, -1 outside the array
+        int h_lds[NHS];   // LDS offsets of this thread's halo elements, -1 if none
+        long h_off[NHS];  // their in-plane offsets, -1 outside the array
+        long o_off;     // in-plane offset of the owned cell, -1 outside the array
         int r_row, r_k, b_row, b_k;
-        long o_off;  // in-plane offset of the owned cell, -1 outside the array
-        T pf[3][NLD];
-        T pP;
+        T pf[3][1 + NHS];  // fetched plane: own cell, halo elements
+        T own[3][4];    // own cell's U, V, W at planes q-3..q
+        T pP, pPn;      // P of the owned cell at planes q-3 and q-2
         double acc;
     };
 
-    __host__ __device__ static int m5(int q) { return ((q % 5) + 5) % 5; }
-    __host__ __device__ static int m4(int q) { return q & 3; }
     __host__ __device__ static int m3(int q) { return ((q % 3) + 3) % 3; }
-    __host__ __device__ static T &F(T *lds, int f, int q, int ly, int lx)
+    __host__ __device__ static int m4(int q) { return q & 3; }
+    __host__ __device__ static T &Uf(T *lds, int q, int ly, int lx) { return lds[m3(q) * PLANE + ly * LX + lx]; }
+    __host__ __device__ static T &Vf(T *lds, int q, int ly, int lx)
     {
-        return lds[(f * NF + m5(q)) * PLANE + ly * LX + lx];
+        return lds[OFF_V + m3(q) * PLANE + ly * LX + lx];
     }
-    __host__ __device__ static T &Rd(T *lds, int q, int ry, int rx) { return lds[OFF_R + m4(q) * RPL + ry * RXN + rx]; }
-    __host__ __device__ static T &Bd(T *lds, int q, int by, int bx) { return lds[OFF_B + m3(q) * BPL + by * BXN + bx]; }
+    __host__ __device__ static T &Wf(T *lds, int q, int ly, int lx)
+    {
+        return lds[OFF_W + m4(q) * PLANE + ly * LX + lx];
+    }
+    __host__ __device__ static T &D(T *lds, int q, int dy, int dx) { return lds[OFF_D + m4(q) * DPL + dy * DXN + dx]; }
 
     __host__ __device__ static long plane_base(const Geom &g, int q) { return (long)(q - 1 + g.h) * g.sx * g.sy; }
     __host__ __device__ static bool plane_in(const Geom &g, int q) { return q >= 1 - g.h && q <= g.nz + g.h; }
+    __host__ __device__ static long in_plane(const Geom &g, int x, int y)
+    {
+        const bool ok = x >= 1 - g.h && x <= g.nx + g.h && y >= 1 - g.h && y <= g.ny + g.h;
+        return ok ? (long)(x - 1 + g.h) + g.sx * (long)(y - 1 + g.h) : -1;
+    }
 
     __host__ __device__ static void init(const HsArgs<T> &a, int bx, int by, int bz, int tid, Thread &th)
     {
@@ -118,20 +130,32 @@ struct HsTile {
         th.Y0 = 1 - g.h + TY * by;
         th.z0 = 1 - g.h + a.zc * bz;
         th.z1 = th.z0 + a.zc - 1 < g.nz + g.h ? th.z0 + a.zc - 1 : g.nz + g.h;
-        for (int r = 0; r < NLD; r++) {
-            const int e = tid + r * NT;
-            th.ld_lds[r] = e < PLANE ? e : -1;
-            const int x = th.X0 - 2 + e % LX, y = th.Y0 - 2 + e / LX;
-            const bool ok = e < PLANE && x >= 1 - g.h && x <= g.nx + g.h && y >= 1 - g.h && y <= g.ny + g.h;
-            th.ld_off[r] = ok ? (long)(x - 1 + g.h) + g.sx * (long)(y - 1 + g.h) : -1;
+        th.o_off = in_plane(g, th.X0 + tid % TX, th.Y0 + tid / TX);
+        for (int s = 0; s < NHS; s++) {
+            const int e = tid + s * NT;
+            th.h_lds[s] = -1;
+            th.h_off[s] = -1;
+            if (e >= NHALO) continue;
+            int lx, ly;
+            if (e < 4 * LX) {  // rows 0, 1, TY+2, TY+3
+                const int r = e / LX;
+                ly = r < 2 ? r : TY + r;
+                lx = e % LX;
+            } else {  // columns 0, 1, TX+2, TX+3 of rows 2..TY+1
+                const int f = e - 4 * LX, c = f % 4;
+                ly = 2 + f / 4;
+                lx = c < 2 ? c : TX + c;
+            }
+            th.h_lds[s] = ly * LX + lx;
+            th.h_off[s] = in_plane(g, th.X0 - 2 + lx, th.Y0 - 2 + ly);
         }
         th.r_row = tid / RPR;
         th.r_k = tid % RPR;
         th.b_row = tid / BPR;
         th.b_k = tid % BPR;
-        const int x = th.X0 + tid % TX, y = th.Y0 + tid / TX;
-        th.o_off = (x <= g.nx + g.h && y <= g.ny + g.h) ? (long)(x - 1 + g.h) + g.sx * (long)(y - 1 + g.h) : -1;
-        th.pP = T(0);
+        for (int f = 0; f < 3; f++)
+            for (int s = 0; s < 4; s++) th.own[f][s] = T(0);
+        th.pP = th.pPn = T(0);
         th.acc = 0.0;
     }
 
@@ -139,27 +163,47 @@ struct HsTile {
     {
         const bool zin = plane_in(a.g, q);
         const long base = plane_base(a.g, q);
-        for (int r = 0; r < NLD; r++) {
-            const bool ok = zin && th.ld_off[r] >= 0;
-            const long p = ok ? base + th.ld_off[r] : 0;
-            th.pf[0][r] = ok ? a.Us[p] : T(0);
-            th.pf[1][r] = ok ? a.Vs[p] : T(0);
-            th.pf[2][r] = ok ? a.Ws[p] : T(0);
+        for (int s = 0; s <= NHS; s++) {
+            const long off = s == 0 ? th.o_off : th.h_off[s - 1];
+            const bool ok = zin && off >= 0;
+            const long p = ok ? base + off : 0;
+            th.pf[0][s] = ok ? a.Us[p] : T(0);
+            th.pf[1][s] = ok ? a.Vs[p] : T(0);
+            th.pf[2][s] = ok ? a.Ws[p] : T(0);
         }
     }
 
-    __host__ __device__ static void store(T *lds, int q, const Thread &th)
+    // plane q (fetched) into LDS and into the own-cell queue
+    __host__ __device__ static void store(T *lds, int q, Thread &th)
     {
-        for (int r = 0; r < NLD; r++) {
-            const int e = th.ld_lds[r];
-            if (e < 0) continue;
-            for (int f = 0; f < 3; f++) lds[(f * NF + m5(q)) * PLANE + e] = th.pf[f][r];
+        const int e = (th.tid / TX + 2) * LX + th.tid % TX + 2;
+        Uf(lds, q, 0, e) = th.pf[0][0];
+        Vf(lds, q, 0, e) = th.pf[1][0];
+        Wf(lds, q, 0, e) = th.pf[2][0];
+        for (int s = 0; s < NHS; s++) {
+            const int h = th.h_lds[s];
+            if (h < 0) continue;
+            Uf(lds, q, 0, h) = th.pf[0][s + 1];
+            Vf(lds, q, 0, h) = th.pf[1][s + 1];
+            Wf(lds, q, 0, h) = th.pf[2][s + 1];
+        }
+        shift(th);
+    }
+
+    // advance the own-cell queue by one plane (the new top is the fetched plane)
+    __host__ __device__ static void shift(Thread &th)
+    {
+        for (int f = 0; f < 3; f++) {
+            th.own[f][0] = th.own[f][1];
+            th.own[f][1] = th.own[f][2];
+            th.own[f][2] = th.own[f][3];
+            th.own[f][3] = th.pf[f][0];
         }
     }
 
     __host__ __device__ static void fetch_p(const HsArgs<T> &a, int o, Thread &th)
     {
-        if (th.o_off >= 0) th.pP = a.Ps[plane_base(a.g, o) + th.o_off];
+        if (th.o_off >= 0) th.pPn = a.Ps[plane_base(a.g, o) + th.o_off];
     }
 
     __host__ __device__ static bool owned(const Thread &th, int x, int y, int z)
@@ -170,22 +214,22 @@ struct HsTile {
     // phase A: red dp on plane q
     __host__ __device__ static void red(const HsArgs<T> &a, T *lds, int q, Thread &th)
     {
-        if (th.r_row >= RYN) return;
+        if (th.r_row >= DYN) return;
         const int xs = th.X0 - 1, y = th.Y0 - 1 + th.r_row;
-        const int rx = ((xs + y + q) & 1) + 2 * th.r_k;
-        if (rx >= RXN) return;
-        const int x = xs + rx, ry = th.r_row, lx = rx + 1, ly = ry + 1;
+        const int dx = ((xs + y + q) & 1) + 2 * th.r_k;
+        if (dx >= DXN) return;
+        const int x = xs + dx, dy = th.r_row, lx = dx + 1, ly = dy + 1;
         const HsCoef<T> &c = a.c;
         T dp = T(0);
         if (visited<COLORING>(a.R, x, y, q)) {
-            const T dudx = c.dxi * (-F(lds, 0, q, ly, lx - 1) + F(lds, 0, q, ly, lx));
-            const T dvdy = c.dyi * (-F(lds, 1, q, ly - 1, lx) + F(lds, 1, q, ly, lx));
-            const T dwdz = c.dzi * (-F(lds, 2, q - 1, ly, lx) + F(lds, 2, q, ly, lx));
+            const T dudx = c.dxi * (-Uf(lds, q, ly, lx - 1) + Uf(lds, q, ly, lx));
+            const T dvdy = c.dyi * (-Vf(lds, q, ly - 1, lx) + Vf(lds, q, ly, lx));
+            const T dwdz = c.dzi * (-Wf(lds, q - 1, ly, lx) + Wf(lds, q, ly, lx));
             const T div = (dudx + dvdy) + dwdz;
             dp = (-c.omega * div) * c.dl;
             if (owned(th, x, y, q) && in_d3(a.g, x, y, q)) th.acc += (double)(div * div);
         }
-        Rd(lds, q, ry, rx) = dp;
+        D(lds, q, dy, dx) = dp;
     }
 
     // phase B, first part: black dp on plane p, on faces already corrected by
@@ -195,17 +239,17 @@ struct HsTile {
         if (th.b_row >= BYN) return;
         const int y = th.Y0 + th.b_row;
         const int bx = ((th.X0 + y + p + 1) & 1) + 2 * th.b_k;
-        if (bx >= BXN) return;
-        const int x = th.X0 + bx, by = th.b_row, rx = bx + 1, ry = by + 1, lx = bx + 2, ly = by + 2;
+        if (bx > TX) return;
+        const int x = th.X0 + bx, dx = bx + 1, dy = th.b_row + 1, lx = bx + 2, ly = th.b_row + 2;
         const HsCoef<T> &c = a.c;
         T dp = T(0);
         if (visited<COLORING>(a.R, x, y, p)) {
-            const T Ur = F(lds, 0, p, ly, lx) - Rd(lds, p, ry, rx + 1) * c.dxi;
-            const T Ul = F(lds, 0, p, ly, lx - 1) + Rd(lds, p, ry, rx - 1) * c.dxi;
-            const T Vr = F(lds, 1, p, ly, lx) - Rd(lds, p, ry + 1, rx) * c.dyi;
-            const T Vl = F(lds, 1, p, ly - 1, lx) + Rd(lds, p, ry - 1, rx) * c.dyi;
-            const T Wt = F(lds, 2, p, ly, lx) - Rd(lds, p + 1, ry, rx) * c.dzi;
-            const T Wb = F(lds, 2, p - 1, ly, lx) + Rd(lds, p - 1, ry, rx) * c.dzi;
+            const T Ur = Uf(lds, p, ly, lx) - D(lds, p, dy, dx + 1) * c.dxi;
+            const T Ul = Uf(lds, p, ly, lx - 1) + D(lds, p, dy, dx - 1) * c.dxi;
+            const T Vr = Vf(lds, p, ly, lx) - D(lds, p, dy + 1, dx) * c.dyi;
+            const T Vl = Vf(lds, p, ly - 1, lx) + D(lds, p, dy - 1, dx) * c.dyi;
+            const T Wt = Wf(lds, p, ly, lx) - D(lds, p + 1, dy, dx) * c.dzi;
+            const T Wb = Wf(lds, p - 1, ly, lx) + D(lds, p - 1, dy, dx) * c.dzi;
             const T dudx = c.dxi * (-Ul + Ur);
             const T dvdy = c.dyi * (-Vl + Vr);
             const T dwdz = c.dzi * (-Wb + Wt);
@@ -213,26 +257,24 @@ struct HsTile {
             dp = (-c.omega * div) * c.dl;
             if (owned(th, x, y, p) && in_d3(a.g, x, y, p)) th.acc += (double)(div * div);
         }
-        Bd(lds, p, by, bx) = dp;
+        D(lds, p, dy, dx) = dp;
     }
 
     // phase B, second part: final values of the owned cell on plane o
+    // (its U, V, W are own[.][0], its P is pP)
     __host__ __device__ static void output(const HsArgs<T> &a, T *lds, int o, Thread &th)
     {
         if (th.o_off < 0) return;
         const int tx = th.tid % TX, ty = th.tid / TX;
-        const int x = th.X0 + tx, y = th.Y0 + ty;
-        const int rx = tx + 1, ry = ty + 1, lx = tx + 2, ly = ty + 2;
+        const int x = th.X0 + tx, y = th.Y0 + ty, dx = tx + 1, dy = ty + 1;
         const HsCoef<T> &c = a.c;
         const bool isred = ((x + y + o) & 1) == 0;
         // correction of this cell and of its +x, +y, +z neighbours (0 outside R)
-        const T d0 = isred ? Rd(lds, o, ry, rx) : Bd(lds, o, ty, tx);
-        const T dx1 = isred ? Bd(lds, o, ty, tx + 1) : Rd(lds, o, ry, rx + 1);
-        const T dy1 = isred ? Bd(lds, o, ty + 1, tx) : Rd(lds, o, ry + 1, rx);
-        const T dz1 = isred ? Bd(lds, o + 1, ty, tx) : Rd(lds, o + 1, ry, rx);
+        const T d0 = D(lds, o, dy, dx), dx1 = D(lds, o, dy, dx + 1), dy1 = D(lds, o, dy + 1, dx);
+        const T dz1 = D(lds, o + 1, dy, dx);
         T pv = th.pP;
         if (visited<COLORING>(a.R, x, y, o)) pv += d0 * a.dti;
-        T u = F(lds, 0, o, ly, lx), v = F(lds, 1, o, ly, lx), w = F(lds, 2, o, ly, lx);
+        T u = th.own[0][0], v = th.own[1][0], w = th.own[2][0];
         if (isred) {
             u += d0 * c.dxi;
             u -= dx1 * c.dxi;
@@ -271,13 +313,15 @@ struct HsTile {
         for (int q = z0 - 1; q <= z1 + 3; q++) {
             for (int t = 0; t < NT; t++) {
                 if (q + 1 <= z1 + 2) fetch(a, q + 1, th[t]);
-                if (q - 3 >= z0) fetch_p(a, q - 3, th[t]);
+                if (q - 2 >= z0 && q - 2 <= z1) fetch_p(a, q - 2, th[t]);
                 if (q <= z1 + 2) red(a, lds, q, th[t]);
             }
             for (int t = 0; t < NT; t++) {
                 if (q - 1 >= z0 && q - 1 <= z1 + 1) black(a, lds, q - 1, th[t]);
                 if (q - 3 >= z0) output(a, lds, q - 3, th[t]);
                 if (q + 1 <= z1 + 2) store(lds, q + 1, th[t]);
+                else shift(th[t]);
+                th[t].pP = th[t].pPn;
             }
         }
         double s = 0.0;

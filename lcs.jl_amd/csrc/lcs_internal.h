@@ -120,6 +120,7 @@ struct Box {
     lcs_box_desc desc;
     Geom g;
     int esize;  // 4 or 8
+    int cus = 256;  // compute units of the device
     hipStream_t stream = nullptr;
     void *f[10] = {};  // LCS_FIELD_* device pointers (DU..DW lazily allocated)
     void *spare[4] = {};  // ping-pong partners of (U,V,W,P) for fused HSMAC sweeps
