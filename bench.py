@@ -58,6 +58,26 @@ def workload(name, n_override=None, particles_override=None):
     return cfg, desc
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of nbytes through the HIP runtime (its copy
+    kernel reads and writes exactly nbytes), for scaling traffic counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:  # DtoD
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return nbytes
+
+
 def cpu_baseline(name, n, steps):
     """Time the CPU oracle on a bounded sample (n^3 box, same parameters)."""
     import numpy as np
@@ -98,6 +118,11 @@ def main():
     ap.add_argument("--cpu-size", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--localize-interval", type=int, default=10,
+                    help="reorder particle storage by position every N steps (0: never; changes no result)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the run, copy a MIB-sized device buffer once (a plain copy kernel of known size "
+                         "for scaling hardware traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the ensemble and single-box "
@@ -111,7 +136,7 @@ def main():
     init_s = time.perf_counter() - t0
     print(f"# built {desc} in {init_s:.1f} s, {box.nbytes() / 2**30:.1f} GiB on the device", file=sys.stderr,
           flush=True)
-    scfg = L.step_config(cfg)
+    scfg = L.step_config(cfg, localize_interval=args.localize_interval)
     for s in range(args.warmup):
         log = L.step(box, scfg)
         print(f"# warmup {s}: itr {log['flow']['couple1']['itr']}/{log['flow']['couple2']['itr']} "
@@ -153,7 +178,7 @@ def main():
         "vs_baseline": None, "dtype": cfg.dtype.replace("f", "f"), "data": "synthetic (seeded spectrum init)",
         "config": {"workload": desc, "grid": list(cfg.grid.dims), "particles": sum(p.population.valid for p in
                                                                                      cfg.particles),
-                   "hsmac_sweeps_per_step": s_mean},
+                   "hsmac_sweeps_per_step": s_mean, "particle_localize_interval": args.localize_interval},
         "step_hbm": {"bytes_per_cell_step": b_cell, "achieved_GBs": value * b_cell / 1e9,
                      "frac": value * b_cell / 1e9 / PEAK_HBM_GBS},
         "roofline": {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -162,6 +187,8 @@ def main():
         "kernels": kernels,
     }
     box.close()
+    if args.copy_calibration:
+        out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration << 20)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_size, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -130,6 +130,10 @@ typedef struct {
     double dtmax;
     lcs_hsmac_params hsmac;
     lcs_forcing forcing;
+    /* lcs_particles_localize every profile before every localize_interval-th
+     * lcs_step call on the box (0: never).  No counterpart in the reference;
+     * it changes no result. */
+    int localize_interval;
 } lcs_step_config;
 
 /* The log of one step: the reference returns it as a NamedTuple
@@ -210,6 +214,13 @@ int lcs_particles_timestep(lcs_box *box, int profile, double dtmax, double *dt);
 /* Particles.makeindex! (src/particles/makeindex.jl:9-67): stage 1 bins xss,
  * stage 2 bins xss2; permutes all 17 properties; stable sort */
 int lcs_particles_makeindex(lcs_box *box, int profile, int stage);
+/* No counterpart in the reference.  Reorders the profile's device storage by
+ * the 8^3-cell tile of each particle's position, so the field gathers of
+ * motion! read nearby memory.  Results do not change, and every other entry
+ * point keeps presenting particles in their logical order (the reference's
+ * array order): upload/download map through the storage order, and makeindex
+ * restores the logical order before it sorts. */
+int lcs_particles_localize(lcs_box *box, int profile);
 /* the CellIndexBuffer after makeindex (src/particles/buffer.jl:45-75):
  * hashes (nvalid), starts/stops ((nc+2)^3, 1-based); any pointer may be NULL */
 int lcs_particles_cellindex(lcs_box *box, int profile, int64_t *hashes, int64_t *starts,

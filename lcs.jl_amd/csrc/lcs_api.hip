@@ -62,6 +62,16 @@ ProfScope::~ProfScope()
     if (e0 && e1) b.prof.entries[idx].pending.emplace_back(e0, e1);
 }
 
+void prof_uncount(Box &b, const char *name, int64_t launches, double bytes)
+{
+    if (!b.prof.on) return;
+    for (auto &e : b.prof.entries)
+        if (e.name == name) {
+            e.launches -= launches;
+            e.bytes -= bytes;
+        }
+}
+
 static void prof_collect(Box &b)
 {
     for (auto &e : b.prof.entries) {
@@ -224,7 +234,7 @@ int lcs_box_destroy(lcs_box *box)
     for (auto &pp : b->profiles) {
         for (void *p : pp.props)
             if (p) (void)hipFree(p);
-        void *extra[] = {pp.hashes, pp.perm, pp.starts, pp.stops, pp.scratch, pp.sort_tmp};
+        void *extra[] = {pp.hashes, pp.perm, pp.starts, pp.stops, pp.scratch, pp.sort_tmp, pp.loc, pp.loc_tmp};
         for (void *p : extra)
             if (p) (void)hipFree(p);
     }
@@ -462,10 +472,18 @@ static int prop_xfer(lcs_box *box, int profile, int prop, void *host, size_t byt
     if (bytes != need)
         return fail(LCS_ERR_INVALID, "property size mismatch: expected " + std::to_string(need) + " bytes");
     if (need == 0) return LCS_OK;
-    if (up)
-        LCS_HIP(hipMemcpyAsync(pp->props[prop], host, bytes, hipMemcpyHostToDevice, b.stream));
-    else
-        LCS_HIP(hipMemcpyAsync(host, pp->props[prop], bytes, hipMemcpyDeviceToHost, b.stream));
+    if (!pp->loc_on) {
+        if (up)
+            LCS_HIP(hipMemcpyAsync(pp->props[prop], host, bytes, hipMemcpyHostToDevice, b.stream));
+        else
+            LCS_HIP(hipMemcpyAsync(host, pp->props[prop], bytes, hipMemcpyDeviceToHost, b.stream));
+    } else if (up) {  // logical order on the host, storage order on the device
+        LCS_HIP(hipMemcpyAsync(pp->scratch, host, bytes, hipMemcpyHostToDevice, b.stream));
+        if ((rc = launch_p_loc_move(b, *pp, pp->scratch, pp->props[prop], es, true))) return rc;
+    } else {
+        if ((rc = launch_p_loc_move(b, *pp, pp->props[prop], pp->scratch, es, false))) return rc;
+        LCS_HIP(hipMemcpyAsync(host, pp->scratch, bytes, hipMemcpyDeviceToHost, b.stream));
+    }
     LCS_HIP(hipStreamSynchronize(b.stream));
     return LCS_OK;
 }
@@ -523,6 +541,12 @@ int lcs_particles_makeindex(lcs_box *box, int profile, int stage)
     return launch_p_makeindex(b, *pp, stage);
 }
 
+int lcs_particles_localize(lcs_box *box, int profile)
+{
+    WITH_PROFILE
+    return launch_p_localize(b, *pp);
+}
+
 int lcs_particles_cellindex(lcs_box *box, int profile, int64_t *hashes, int64_t *starts, int64_t *stops,
                             int64_t *ncells)
 {
@@ -551,6 +575,7 @@ int lcs_init_particles(lcs_box *box, int profile, uint64_t seed, double st_min, 
     WITH_PROFILE
     if (!(st_min > 0) || !(st_max >= st_min) || !(eps > 0))
         return fail(LCS_ERR_INVALID, "need 0 < st_min <= st_max and eps > 0");
+    pp->loc_on = false;  // every property is rewritten in logical order
     rc = launch_p_init(b, *pp, seed, st_min, st_max, eps, zero_slip);
     if (rc) return rc;
     LCS_HIP(hipStreamSynchronize(b.stream));
@@ -576,10 +601,15 @@ int lcs_step(lcs_box *box, const lcs_step_config *cfg, lcs_step_log *log)
     if (check_box(box) || !cfg) return fail(LCS_ERR_INVALID, "null argument");
     Box &b = *B(box);
     if (b.g.h < 3) return fail(LCS_ERR_INVALID, "a flow step needs halo_size >= 3");
+    if (cfg->localize_interval < 0) return fail(LCS_ERR_INVALID, "localize_interval must be >= 0");
     int rc;
+    int np = (int)b.profiles.size();
+    if (cfg->localize_interval > 0 && b.steps_host % cfg->localize_interval == 0)
+        for (int q = 0; q < np; q++)
+            if ((rc = launch_p_localize(b, b.profiles[q]))) return rc;
+    b.steps_host++;
     // timestep!: src/timestep.jl:152-173
     if ((rc = launch_cfl(b, cfg->cfl, cfg->dtmax))) return rc;
-    int np = (int)b.profiles.size();
     for (int q = 0; q < np; q++)
         if ((rc = launch_p_timestep(b, b.profiles[q], cfg->dtmax, &b.S->dtp[q]))) return rc;
     if ((rc = launch_finalize_dt(b, np, cfg->dtmax))) return rc;

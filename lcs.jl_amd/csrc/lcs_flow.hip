@@ -62,6 +62,39 @@ __device__ inline double block_max(double v, double *lds)
     return s;
 }
 
+// Workgroups are dispatched round-robin over the XCDs (linear id L runs on
+// XCD L % kXcds), each with its own L2.  Tile kernels take their tile from
+// this map instead of blockIdx, so each XCD works on one contiguous range of
+// tiles (x fastest) and neighbouring tiles share halo rows through one L2.
+// The map is a bijection for any kXcds, so results never depend on it.
+constexpr int kXcds = 8;
+__device__ inline void xcd_tile(int &bx, int &by, int &bz)
+{
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int n = gx * gy * gridDim.z, L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int per = n / kXcds, rem = n % kXcds, x = L % kXcds, i = L / kXcds;
+    const int t = x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+    bx = t % gx;
+    by = (t / gx) % gy;
+    bz = t / (gx * gy);
+}
+
+// Visits every interior cell once: rows (j, k) are spread over the grid and x
+// over the threads of a block, so the index arithmetic per cell is one add.
+template <typename F>
+__device__ inline void for_interior(const Geom &g, F &&f)
+{
+    const int rows = g.ny * g.nz;
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int j = r % g.ny + 1, k = r / g.ny + 1;
+        const long base = g.at(0, j, k);
+        for (int i = 1 + (int)threadIdx.x; i <= g.nx; i += blockDim.x) f(i, j, k, base + i);
+    }
+}
+
+// blocks for a for_interior launch: every row gets a block up to the cap
+static int grid_rows(const Geom &g, int cap = 4096) { return (int)std::max(1L, std::min<long>((long)g.ny * g.nz, cap)); }
+
 static int grid_for(long work, int block, int cap = 4096)
 {
     long g = (work + block - 1) / block;
@@ -220,27 +253,26 @@ __global__ void __launch_bounds__(TX *TY) k_rhs_tile(RhsArgs<T> a, DtRef dt)
     extern __shared__ __align__(16) unsigned char rhs_lds_raw[];
     T *lds = reinterpret_cast<T *>(rhs_lds_raw);
     typename Tile::Thread th;
-    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
-    const int X0 = blockIdx.x * TX + 1, Y0 = blockIdx.y * TY + 1, z0 = blockIdx.z * a.zc + 1;
-    const int z1 = min(z0 + a.zc - 1, a.g.nz);
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    Tile::init(a, bx, by, bz, threadIdx.x, th);
     const T rdt = FUSED ? T(dt.factor * (*dt.src)) : T(0);
-    for (int q = z0 - 3; q <= z0 + 3; q++) {
-        Tile::fetch(a, X0, Y0, q, tid, th);
-        Tile::store(lds, q, tid, th);
-    }
+    Tile::prologue_fetch(a, th);
+    Tile::prologue_store(a, lds, th);
+    Tile::prologue_queues(a, th);
     __syncthreads();
-    Tile::warmup(lds, z0, tx, ty, th);
+    const int z0 = th.z0, z1 = th.z1;
     for (int z = z0; z <= z1; z++) {
-        if (z < z1) Tile::fetch(a, X0, Y0, z + 4, tid, th);
-        Tile::phase_fluxes(lds, z, tid);
+        if (z < z1) Tile::fetch(a, z, th);
+        Tile::phase_fluxes(lds, z, threadIdx.x);
         __syncthreads();
-        Tile::template phase_output<FUSED>(a, lds, X0, Y0, z, tx, ty, rdt, th);
-        if (z < z1) Tile::store(lds, z + 4, tid, th);
+        Tile::template phase_output<FUSED>(a, lds, z, rdt, th);
+        if (z < z1) Tile::store(lds, z, th);
         __syncthreads();
     }
 }
 
-// Tile shapes: FP32 32x8 (60 KB of LDS, two workgroups per CU), FP64 32x4 (81 KB).
+// Tile shapes: FP32 32x8 (39 KB of LDS, four workgroups per CU), FP64 32x4 (53 KB).
 template <typename T>
 struct RhsShape;
 template <>
@@ -376,33 +408,43 @@ struct BufSets {
     T *b[4];  // spare partners
 };
 
-// One colour pass over a (thin) box, in place on the current buffer set.
+// One colour pass over one or two (thin) boxes, in place on the current
+// buffer set.  Two boxes share a launch only when no cell of one touches a
+// face of the other (the opposite face slabs of one axis), so the pair gives
+// the same result as the two passes in either order.
+__host__ __device__ inline int pass_count(const Box3 &r)
+{
+    if (r.hi[0] < r.lo[0] || r.hi[1] < r.lo[1] || r.hi[2] < r.lo[2]) return 0;
+    return ((r.hi[0] - r.lo[0] + 2) / 2) * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
+}
+
 template <typename T, int COLORING>
-__global__ void __launch_bounds__(256) k_hsmac_pass(Geom g, BufSets<T> bs, Box3 r, int color, HsCoef<T> c,
-                                                   DtRef dt, const DevScalars *S, double *part)
+__global__ void __launch_bounds__(256) k_hsmac_pass(Geom g, BufSets<T> bs, Box3 r0, Box3 r1, int n0, int n1,
+                                                   int color, HsCoef<T> c, DtRef dt, const DevScalars *S,
+                                                   double *part)
 {
     __shared__ double lds[16];
     if (S->hs_done) return;
     const bool alt = S->hs_par != 0;
     T *U = alt ? bs.b[0] : bs.a[0], *V = alt ? bs.b[1] : bs.a[1];
     T *W = alt ? bs.b[2] : bs.a[2], *P = alt ? bs.b[3] : bs.a[3];
-    const int a = r.lo[0];
-    const int hw = (r.hi[0] - r.lo[0] + 2) / 2;  // cld(n, 2)
-    const int ny = r.hi[1] - r.lo[1] + 1;
-    const long total = (long)hw * ny * (r.hi[2] - r.lo[2] + 1);
     const T dti = T(1.0 / (dt.factor * (*dt.src)));
     const long sy = g.sx, sz = g.sx * g.sy;
     double acc = 0.0;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
-        int ih = (int)(t % hw);
-        long q = t / hw;
-        int j = r.lo[1] + (int)(q % ny);
-        int k = r.lo[2] + (int)(q / ny);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n0 + n1; t += gridDim.x * blockDim.x) {
+        const bool second = t >= n0;
+        const Box3 r = second ? r1 : r0;
+        const int u = second ? t - n0 : t;
+        const int a = r.lo[0];
+        const int hw = (r.hi[0] - r.lo[0] + 2) / 2;  // cld(n, 2)
+        const int ny = r.hi[1] - r.lo[1] + 1;
+        const int ih = u % hw, q = u / hw;
+        const int j = r.lo[1] + q % ny, k = r.lo[2] + q / ny;
         // RedBlackFast index map: lib/parallel/src/foraxes.jl:100-105
-        int i = a + 2 * ih + ((j + k - a + color) & 1);
+        const int i = a + 2 * ih + ((j + k - a + color) & 1);
         if (i > r.hi[0]) continue;
         if (!visited<COLORING>(r, i, j, k)) continue;
-        long p = g.at(i, j, k);
+        const long p = g.at(i, j, k);
         T dudx = c.dxi * (-U[p - 1] + U[p]);
         T dvdy = c.dyi * (-V[p - sy] + V[p]);
         T dwdz = c.dzi * (-W[p - sz] + W[p]);
@@ -448,7 +490,9 @@ __global__ void __launch_bounds__(64 * TY) k_hsmac_sweep(Geom g, BufSets<T> bs, 
     a.dti = T(1.0 / (dt.factor * (*dt.src)));
     a.zc = zc;
     typename Tile::Thread th;
-    Tile::init(a, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x, th);
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    Tile::init(a, bx, by, bz, threadIdx.x, th);
     const int z0 = th.z0, z1 = th.z1;
     Tile::fetch(a, z0 - 2, th);
     Tile::store(lds, z0 - 2, th);
@@ -531,35 +575,34 @@ struct HsmacRun {
     int coloring;
     CheckArgs ca;
     int npass = 0;
+    std::vector<double> sweep_bytes;  // algorithmic bytes of each fused sweep launched in this call
 
-    int pass(const Box3 &r, int color)
+    // one colour over boxes r0 and r1 (see k_hsmac_pass for when two boxes may share a launch)
+    int pass(const Box3 &r0, const Box3 &r1, int color)
     {
-        int hw = (r.hi[0] - r.lo[0] + 2) / 2;
-        long total = (long)hw * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
-        if (hw <= 0 || total <= 0) {
+        const int n0 = pass_count(r0), n1 = pass_count(r1);
+        if (n0 + n1 == 0) {
             ca.cnt[npass++] = 0;
             return LCS_OK;
         }
-        int blocks = grid_for(total, 256, 4096);
+        int blocks = grid_for((long)n0 + n1, 256, 4096);
         double *part = b.part + (long)npass * kMaxPartBlocks;
         // algorithmic: half of the 8 element passes (U,V,W,P read+write) per cell-sweep
-        long cells = (long)(r.hi[0] - r.lo[0] + 1) * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
-        ProfScope ps(b, "hsmac_pass", 4.0 * sizeof(T) * cells);
+        auto cells = [](const Box3 &r) {
+            return r.hi[0] < r.lo[0] || r.hi[1] < r.lo[1] || r.hi[2] < r.lo[2]
+                       ? 0L
+                       : (long)(r.hi[0] - r.lo[0] + 1) * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
+        };
+        ProfScope ps(b, "hsmac_pass", 4.0 * sizeof(T) * (cells(r0) + cells(r1)));
         if (coloring == LCS_COLORING_RED_BLACK)
             hipLaunchKernelGGL((k_hsmac_pass<T, LCS_COLORING_RED_BLACK>), dim3(blocks), dim3(256), 0, b.stream,
-                               b.g, bs, r, color, c, dt, b.S, part);
+                               b.g, bs, r0, r1, n0, n1, color, c, dt, b.S, part);
         else
             hipLaunchKernelGGL((k_hsmac_pass<T, LCS_COLORING_RED_BLACK_FAST>), dim3(blocks), dim3(256), 0,
-                               b.stream, b.g, bs, r, color, c, dt, b.S, part);
+                               b.stream, b.g, bs, r0, r1, n0, n1, color, c, dt, b.S, part);
         LCS_HIP(hipGetLastError());
         ca.cnt[npass++] = blocks;
         return LCS_OK;
-    }
-    int sweep_box_passes(const Box3 &r)
-    {
-        int rc = pass(r, 0);
-        if (rc) return rc;
-        return pass(r, 1);
     }
     // one fused red+black sweep over a full box, then the parity flip
     template <int COLORING, int TY>
@@ -601,6 +644,7 @@ struct HsmacRun {
         double *part = b.part + (long)npass * kMaxPartBlocks;
         long cells = (long)(r.hi[0] - r.lo[0] + 1) * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
         ProfScope ps(b, "hsmac_sweep", 8.0 * sizeof(T) * cells);
+        sweep_bytes.push_back(8.0 * sizeof(T) * cells);
         hipLaunchKernelGGL((k_hsmac_sweep<T, COLORING, TY>), dim3(gx, gy, nzc), dim3(Tile::NT), lds, b.stream, g,
                            bs, r, c, dt, b.S, part, zc);
         LCS_HIP(hipGetLastError());
@@ -630,9 +674,18 @@ struct HsmacRun {
             {{d.lo[0], 2, 2}, {1, g.ny, g.nz}},
             {{g.nx + 1, 2, 2}, {d.hi[0], g.ny, g.nz}},
         };
-        for (int q = 0; q < 6; q++) {
-            rc = sweep_box_passes(f[q]);
-            if (rc) return rc;
+        // Opposite slabs of one axis share no face for n >= 2, so the
+        // reference's order (z- red, z- black, z+ red, z+ black, y- ...) gives
+        // the same result as (z- and z+ red), (z- and z+ black), (y- ...).
+        const bool pair = g.nx >= 2 && g.ny >= 2 && g.nz >= 2;
+        for (int q = 0; q < 6; q += 2) {
+            if (pair) {
+                if ((rc = pass(f[q], f[q + 1], 0)) || (rc = pass(f[q], f[q + 1], 1))) return rc;
+            } else {
+                const Box3 none = {{1, 1, 1}, {0, 0, 0}};
+                for (int s = 0; s < 2; s++)
+                    if ((rc = pass(f[q + s], none, 0)) || (rc = pass(f[q + s], none, 1))) return rc;
+            }
         }
         return LCS_OK;
     }
@@ -729,6 +782,15 @@ static int hsmac_t(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int
     hipLaunchKernelGGL(k_hsmac_end, dim3(1), dim3(1), 0, b.stream, b.S, slot);
     LCS_HIP(hipGetLastError());
     if (!polled_last && (rc = run.poll(done))) return rc;
+    if (done) {
+        // every sweep has exactly one fused launch; those after the converged
+        // sweep returned at once
+        const size_t executed = (size_t)b.S_host->hs_itr + 1;
+        double skipped_bytes = 0;
+        for (size_t q = executed; q < run.sweep_bytes.size(); q++) skipped_bytes += run.sweep_bytes[q];
+        if (run.sweep_bytes.size() > executed)
+            prof_uncount(b, "hsmac_sweep", (int64_t)(run.sweep_bytes.size() - executed), skipped_bytes);
+    }
     if (b.S_host->hs_par) {
         // the result lives in the spare set: swap it in
         for (int q = 0; q < 3; q++) std::swap(Us[q], b.spare[q]);
@@ -756,19 +818,14 @@ __global__ void __launch_bounds__(256) k_cfl(Geom g, const T *__restrict__ U, co
     __shared__ double lds[16];
     const long sy = g.sx, sz = g.sx * g.sy;
     double m[3] = {0.0, 0.0, 0.0};
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < g.cells(); t += (long)gridDim.x * blockDim.x) {
-        int i = (int)(t % g.nx) + 1;
-        long q = t / g.nx;
-        int j = (int)(q % g.ny) + 1;
-        int k = (int)(q / g.ny) + 1;
-        long p = g.at(i, j, k);
+    for_interior(g, [&](int, int, int, long p) {
         T uc = interp_o4(U[p - 2], U[p - 1], U[p], U[p + 1]);
         T vc = interp_o4(V[p - 2 * sy], V[p - sy], V[p], V[p + sy]);
         T wc = interp_o4(W[p - 2 * sz], W[p - sz], W[p], W[p + sz]);
         m[0] = fmax(m[0], fabs((double)uc));
         m[1] = fmax(m[1], fabs((double)vc));
         m[2] = fmax(m[2], fabs((double)wc));
-    }
+    });
     for (int a = 0; a < 3; a++) {
         double s = block_max(m[a], lds);
         if (threadIdx.x == 0) part[(long)a * kMaxPartBlocks + blockIdx.x] = s;
@@ -805,7 +862,7 @@ int launch_cfl(Box &b, double cfl, double dtmax)
     void *arr[3] = {b.f[0], b.f[1], b.f[2]};
     int rc = launch_ghost_fill(b, arr, 3, b.g.h, 0, nullptr);
     if (rc) return rc;
-    int blocks = grid_for(b.g.cells(), 256);
+    int blocks = grid_rows(b.g);
     ProfScope ps(b, "cfl", 3.0 * b.esize * b.g.cells());
     if (b.esize == 8)
         hipLaunchKernelGGL(k_cfl<double>, dim3(blocks), dim3(256), 0, b.stream, b.g, (const double *)b.f[0],
@@ -830,12 +887,7 @@ __global__ void __launch_bounds__(256) k_lf_means(Geom g, const T *__restrict__ 
     __shared__ double lds[16];
     const long sy = g.sx, sz = g.sx * g.sy;
     double s[4] = {0, 0, 0, 0};
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < g.cells(); t += (long)gridDim.x * blockDim.x) {
-        int i = (int)(t % g.nx) + 1;
-        long q = t / g.nx;
-        int j = (int)(q % g.ny) + 1;
-        int k = (int)(q / g.ny) + 1;
-        long p = g.at(i, j, k);
+    for_interior(g, [&](int, int, int, long p) {
         T u = interp_o2(U[p - 1], U[p]);
         T v = interp_o2(V[p - sy], V[p]);
         T w = interp_o2(W[p - sz], W[p]);
@@ -844,7 +896,7 @@ __global__ void __launch_bounds__(256) k_lf_means(Geom g, const T *__restrict__ 
         s[1] += (double)U[p];
         s[2] += (double)V[p];
         s[3] += (double)W[p];
-    }
+    });
     for (int a = 0; a < 4; a++) {
         double r = block_sum(s[a], lds);
         if (threadIdx.x == 0) part[(long)a * kMaxPartBlocks + blockIdx.x] = r;
@@ -879,12 +931,7 @@ __global__ void __launch_bounds__(256) k_lf_apply(Geom g, T *U, T *V, T *W, cons
     const T A = T(S->lf[4]), um = T(S->lf[1]), vm = T(S->lf[2]), wm = T(S->lf[3]);
     const T rdt = T(dt.factor * (*dt.src));
     double sp = 0.0;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < g.cells(); t += (long)gridDim.x * blockDim.x) {
-        int i = (int)(t % g.nx) + 1;
-        long q = t / g.nx;
-        int j = (int)(q % g.ny) + 1;
-        int k = (int)(q / g.ny) + 1;
-        long p = g.at(i, j, k);
+    for_interior(g, [&](int, int, int, long p) {
         T u = U[p], v = V[p], w = W[p];
         T usrc = A * (u - um), vsrc = A * (v - vm), wsrc = A * (w - wm);
         T pw = (u * usrc + v * vsrc) + w * wsrc;
@@ -892,7 +939,7 @@ __global__ void __launch_bounds__(256) k_lf_apply(Geom g, T *U, T *V, T *W, cons
         V[p] = v + vsrc * rdt;
         W[p] = w + wsrc * rdt;
         sp += (double)pw;
-    }
+    });
     double r = block_sum(sp, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
@@ -911,7 +958,7 @@ int launch_force_linear(Box &b, double power, DtRef dt)
     void *arr[3] = {b.f[0], b.f[1], b.f[2]};
     int rc = launch_ghost_fill(b, arr, 3, b.g.h, 1, nullptr);
     if (rc) return rc;
-    int blocks = grid_for(b.g.cells(), 256);
+    int blocks = grid_rows(b.g);
     double len = (double)b.g.cells();
     {
     ProfScope ps(b, "lf_means", 3.0 * b.esize * len);

@@ -94,6 +94,12 @@ struct ParticleProfile {
     void *sort_tmp = nullptr;       // keys/values ping-pong + histograms
     size_t sort_tmp_bytes = 0;
     bool indexed = false;
+    // Storage order.  Logical particle i (the order the C ABI presents, which
+    // is the reference's array order) lives in slot loc[i] when loc_on; the
+    // identity otherwise.  lcs_particles_localize reorders slots by position.
+    void *loc = nullptr;      // uint32 ntotal
+    void *loc_tmp = nullptr;  // uint32 ntotal
+    bool loc_on = false;
 };
 
 struct Ctx {
@@ -132,6 +138,7 @@ struct Box {
     std::vector<ParticleProfile> profiles;
     Profiler prof;
     size_t bytes = 0;
+    int64_t steps_host = 0;  // lcs_step calls on this box
     double Re() const { return desc.U0_m_s * desc.L0_m / desc.NU_m2_s; }
 };
 
@@ -155,6 +162,10 @@ struct ProfScope {
     ProfScope(Box &box, const char *name, double bytes);
     ~ProfScope();
 };
+// Takes back launches (and their algorithmic bytes) of a named kernel that
+// returned without work, e.g. HSMAC sweeps enqueued after convergence; their
+// time stays in the total.
+void prof_uncount(Box &b, const char *name, int64_t launches, double bytes);
 
 // ---- launchers (lcs_flow.hip) ----------------------------------------------
 // Where a kernel needs the stage dt it reads factor * (*dtsrc) on the device.
@@ -186,6 +197,12 @@ int launch_p_makeindex(Box &b, ParticleProfile &pp, int stage);
 int launch_p_init(Box &b, ParticleProfile &pp, uint64_t seed, double st_min, double st_max,
                   double eps, int zero_slip);
 int launch_finalize_dt(Box &b, int nprofiles, double dtmax);
+int launch_p_localize(Box &b, ParticleProfile &pp);
+// element moves through the storage map: dst[loc[i]] = src[i] (scatter) or
+// dst[i] = src[loc[i]] (gather), i < ntotal, elements of esize bytes
+int launch_p_loc_move(Box &b, ParticleProfile &pp, const void *src, void *dst, size_t esize, bool scatter);
+// puts every property back into logical order and switches the map off
+int launch_p_unlocalize(Box &b, ParticleProfile &pp);
 
 // ---- counter-based RNG shared with the oracle ------------------------------
 __host__ __device__ inline uint64_t mix64(uint64_t z)

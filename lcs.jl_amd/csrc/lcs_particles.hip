@@ -606,8 +606,184 @@ static int makeindex_t(Box &b, ParticleProfile &pp, int stage)
     return LCS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Storage order for locality.  Each particle is advanced independently, so the
+// slot a particle occupies changes no result; sorting slots by the 8^3-cell
+// tile of the position makes the trilinear gathers of k_p_stage hit the same
+// few cache lines within a wave instead of 24 random lines per particle.  The
+// logical order -- what upload/download and makeindex see -- is kept through
+// the map loc (logical index -> slot).
+// ---------------------------------------------------------------------------
+constexpr int kLocTile = 8;
+
+template <typename T>
+__global__ void k_loc_key(long n, const T *xs, const T *ys, const T *zs, T idx0, T idx1, T idx2, int n0, int n1,
+                          int n2, int t0, int t1, unsigned *keys, unsigned *vals)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const T x[3] = {xs[p], ys[p], zs[p]};
+    const T idx[3] = {idx0, idx1, idx2};
+    const int nn[3] = {n0, n1, n2};
+    int t[3];
+    for (int a = 0; a < 3; a++) {
+        T f = floor((x[a] - T(kDomainOrigin)) * idx[a]);
+        int c = f < T(0) ? 0 : (f >= T(nn[a]) ? nn[a] - 1 : (int)f);  // any position gets a tile
+        t[a] = c / kLocTile;
+    }
+    keys[p] = (unsigned)(t[0] + t0 * (t[1] + t1 * t[2]));
+    vals[p] = (unsigned)p;
+}
+
+// inv[slot_old] = slot_new for the sorted slots, identity past n
+__global__ void k_loc_inverse(long n, long ntotal, const unsigned *sorted_vals, unsigned *inv)
+{
+    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ntotal) return;
+    if (j < n)
+        inv[sorted_vals[j]] = (unsigned)j;
+    else
+        inv[j] = (unsigned)j;
+}
+
+// loc[i] = inv[loc[i]] (or inv[i] when the map was off)
+__global__ void k_loc_compose(long ntotal, const unsigned *inv, unsigned *loc, int loc_on)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntotal) return;
+    loc[i] = inv[loc_on ? loc[i] : (unsigned)i];
+}
+
+template <typename E>
+__global__ void k_loc_move(long ntotal, const unsigned *loc, const E *src, E *dst, int scatter)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntotal) return;
+    if (scatter)
+        dst[loc[i]] = src[i];
+    else
+        dst[i] = src[loc[i]];
+}
+
+int launch_p_loc_move(Box &b, ParticleProfile &pp, const void *src, void *dst, size_t esize, bool scatter)
+{
+    long nt = pp.desc.ntotal;
+    if (nt == 0) return LCS_OK;
+    if (esize == 8)
+        hipLaunchKernelGGL(k_loc_move<uint64_t>, dim3(pgrid(nt)), dim3(256), 0, b.stream, nt,
+                           (const unsigned *)pp.loc, (const uint64_t *)src, (uint64_t *)dst, (int)scatter);
+    else
+        hipLaunchKernelGGL(k_loc_move<uint32_t>, dim3(pgrid(nt)), dim3(256), 0, b.stream, nt,
+                           (const unsigned *)pp.loc, (const uint32_t *)src, (uint32_t *)dst, (int)scatter);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+int launch_p_unlocalize(Box &b, ParticleProfile &pp)
+{
+    if (!pp.loc_on) return LCS_OK;
+    for (int q = 0; q < LCS_N_PROPERTIES; q++) {
+        int rc = launch_p_loc_move(b, pp, pp.props[q], pp.scratch, q == LCS_PROP_ID ? 8 : b.esize, false);
+        if (rc) return rc;
+        std::swap(pp.props[q], pp.scratch);
+    }
+    pp.loc_on = false;
+    return LCS_OK;
+}
+
+// radix sort of (keys, slot) pairs over n items in pp.sort_tmp; returns the
+// buffers holding the sorted pairs
+static int sort_pairs(Box &b, ParticleProfile &pp, long n, int bits, unsigned **keys, unsigned **vals,
+                      unsigned **spare)
+{
+    int nblk = (int)std::min<long>(1024, std::max(1L, (n + 4 * kSortBlock - 1) / (4 * kSortBlock)));
+    long chunk = (n + nblk - 1) / nblk;
+    chunk = ((chunk + kSortBlock - 1) / kSortBlock) * kSortBlock;
+    if (chunk == 0) chunk = kSortBlock;
+    unsigned *kA = *keys, *vA = *vals, *kB = vA + std::max(1L, n), *vB = kB + std::max(1L, n),
+             *hist = vB + std::max(1L, n);
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(kSortBlock), 0, b.stream, kA, n, shift, chunk, hist, nblk);
+        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, b.stream, hist, 256L * nblk);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(kSortBlock), 0, b.stream, kA, vA, kB, vB, n, shift, chunk,
+                           hist, nblk);
+        LCS_HIP(hipGetLastError());
+        std::swap(kA, kB);
+        std::swap(vA, vB);
+    }
+    *keys = kA;
+    *vals = vA;
+    *spare = kB;
+    return LCS_OK;
+}
+
+template <typename T>
+static int localize_t(Box &b, ParticleProfile &pp)
+{
+    long n = pp.desc.nvalid, nt = pp.desc.ntotal;
+    if (n < 2) return LCS_OK;
+    if (!pp.loc) {
+        LCS_HIP(hipMalloc(&pp.loc, sizeof(unsigned) * (size_t)nt));
+        LCS_HIP(hipMalloc(&pp.loc_tmp, sizeof(unsigned) * (size_t)nt));
+        b.bytes += 2 * sizeof(unsigned) * (size_t)nt;
+    }
+    int nblk = (int)std::min<long>(1024, std::max(1L, (n + 4 * kSortBlock - 1) / (4 * kSortBlock)));
+    size_t need = sizeof(unsigned) * (4 * (size_t)n + 256 * (size_t)nblk);
+    if (need > pp.sort_tmp_bytes) {
+        if (pp.sort_tmp) LCS_HIP(hipFree(pp.sort_tmp));
+        pp.sort_tmp = nullptr;
+        LCS_HIP(hipMalloc(&pp.sort_tmp, need));
+        b.bytes += need - pp.sort_tmp_bytes;
+        pp.sort_tmp_bytes = need;
+    }
+    const int nn[3] = {b.g.nx, b.g.ny, b.g.nz};
+    int nt3[3];
+    long ntiles = 1;
+    for (int a = 0; a < 3; a++) {
+        nt3[a] = (nn[a] + kLocTile - 1) / kLocTile;
+        ntiles *= nt3[a];
+    }
+    int bits = 0;
+    while (bits < 32 && ((unsigned long long)(ntiles - 1) >> bits) != 0) bits++;
+    unsigned *keys = (unsigned *)pp.sort_tmp, *vals = keys + n, *spare = nullptr;
+    ProfScope ps(b, "particles_localize", (double)n * (17.0 * sizeof(T) + 16.0 * ((bits + 7) / 8)));
+    hipLaunchKernelGGL(k_loc_key<T>, dim3(pgrid(n)), dim3(256), 0, b.stream, n, (const T *)pp.props[LCS_PROP_X],
+                       (const T *)pp.props[LCS_PROP_Y], (const T *)pp.props[LCS_PROP_Z], T(1.0 / spacing(nn[0])),
+                       T(1.0 / spacing(nn[1])), T(1.0 / spacing(nn[2])), nn[0], nn[1], nn[2], nt3[0], nt3[1], keys,
+                       vals);
+    LCS_HIP(hipGetLastError());
+    int rc = sort_pairs(b, pp, n, bits, &keys, &vals, &spare);
+    if (rc) return rc;
+    for (int q = 0; q < LCS_N_PROPERTIES; q++) {
+        if (q == LCS_PROP_ID)
+            hipLaunchKernelGGL(k_permute<int64_t>, dim3(pgrid(nt)), dim3(256), 0, b.stream, n, nt, vals,
+                               (const int64_t *)pp.props[q], (int64_t *)pp.scratch);
+        else
+            hipLaunchKernelGGL(k_permute<T>, dim3(pgrid(nt)), dim3(256), 0, b.stream, n, nt, vals,
+                               (const T *)pp.props[q], (T *)pp.scratch);
+        LCS_HIP(hipGetLastError());
+        std::swap(pp.props[q], pp.scratch);
+    }
+    hipLaunchKernelGGL(k_loc_inverse, dim3(pgrid(nt)), dim3(256), 0, b.stream, n, nt, vals,
+                       (unsigned *)pp.loc_tmp);
+    hipLaunchKernelGGL(k_loc_compose, dim3(pgrid(nt)), dim3(256), 0, b.stream, nt, (const unsigned *)pp.loc_tmp,
+                       (unsigned *)pp.loc, (int)pp.loc_on);
+    LCS_HIP(hipGetLastError());
+    pp.loc_on = true;
+    (void)spare;
+    return LCS_OK;
+}
+
+int launch_p_localize(Box &b, ParticleProfile &pp)
+{
+    return b.esize == 8 ? localize_t<double>(b, pp) : localize_t<float>(b, pp);
+}
+
 int launch_p_makeindex(Box &b, ParticleProfile &pp, int stage)
 {
+    // makeindex! sorts the logical order stably: restore it first
+    int rc = launch_p_unlocalize(b, pp);
+    if (rc) return rc;
     return b.esize == 8 ? makeindex_t<double>(b, pp, stage) : makeindex_t<float>(b, pp, stage);
 }
 

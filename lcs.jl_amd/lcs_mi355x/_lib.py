@@ -60,7 +60,8 @@ class Forcing(C.Structure):
 
 
 class StepConfig(C.Structure):
-    _fields_ = [("cfl", C.c_double), ("dtmax", C.c_double), ("hsmac", HsmacParams), ("forcing", Forcing)]
+    _fields_ = [("cfl", C.c_double), ("dtmax", C.c_double), ("hsmac", HsmacParams), ("forcing", Forcing),
+                ("localize_interval", C.c_int)]
 
 
 class StepLog(C.Structure):
@@ -98,6 +99,7 @@ SIGNATURES = {
     "lcs_particles_update": [P, I, I, D],
     "lcs_particles_timestep": [P, I, D, C.POINTER(D)],
     "lcs_particles_makeindex": [P, I, I],
+    "lcs_particles_localize": [P, I],
     "lcs_particles_cellindex": [P, I, P, P, P, C.POINTER(C.c_int64)],
     "lcs_init_spectrum": [P, C.c_uint64, D, D],
     "lcs_init_particles": [P, I, C.c_uint64, D, D, D, I],

@@ -260,14 +260,23 @@ def makeindex(box, profile, stage=1):
     check(box.L.lcs_particles_makeindex(box.handle, profile, int(stage)))
 
 
+def localize(box, profile):
+    """Reorder the profile's device storage by position for gather locality.
+    Changes no result; upload/download keep the logical order."""
+    check(box.L.lcs_particles_localize(box.handle, profile))
+
+
 # ---------------------------------------------------------------------------
 # Whole step
 # ---------------------------------------------------------------------------
-def step_config(config: Config):
+def step_config(config: Config, localize_interval=0):
+    """The C step config of a Config; localize_interval > 0 reorders particle
+    storage by position every that many steps (no effect on results)."""
     f = config.flow.forcing
     fk = _lib.Forcing(_lib.FORCING[f.kind], getattr(f, "power", 0.0), getattr(f, "energy", 0.0),
                       getattr(f, "kmin", 0), getattr(f, "kmax", 0), getattr(f, "coarse_size", 0))
-    return _lib.StepConfig(config.timestep.cfl, config.timestep.dtmax, _hsmac_params(config.flow.couple), fk)
+    return _lib.StepConfig(config.timestep.cfl, config.timestep.dtmax, _hsmac_params(config.flow.couple), fk,
+                           int(localize_interval))
 
 
 def step(box, scfg, want_log=True):

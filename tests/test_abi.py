@@ -43,7 +43,8 @@ def test_struct_layouts_match_header():
     assert C.sizeof(_lib.HsmacParams) == 40 and _lib.HsmacParams.omega.offset == 16
     assert C.sizeof(_lib.ParticleDesc) == 40 and _lib.ParticleDesc.drag.offset == 32
     assert C.sizeof(_lib.Forcing) == 40
-    assert C.sizeof(_lib.StepConfig) == 16 + 40 + 40
+    assert C.sizeof(_lib.StepConfig) == 16 + 40 + 40 + 8  # + localize_interval, padded
+    assert _lib.StepConfig.localize_interval.offset == 96
     assert C.sizeof(_lib.StepLog) == 8 + 8 + 8 + 4 + 4 + 4 * 8
 
 
@@ -53,16 +54,17 @@ def test_c_struct_sizes_agree_with_ctypes(tmp_path):
     from lcs_mi355x import _lib
     src = tmp_path / "t.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "lcs_mi355x.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(lcs_box_desc),'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu ", sizeof(lcs_box_desc),'
                    ' sizeof(lcs_hsmac_params), sizeof(lcs_particle_desc), sizeof(lcs_forcing),'
                    ' sizeof(lcs_step_config), sizeof(lcs_step_log), offsetof(lcs_step_config, forcing),'
-                   ' offsetof(lcs_step_log, div1)); return 0;}\n')
+                   ' offsetof(lcs_step_log, div1)); printf("%zu\\n", offsetof(lcs_step_config, localize_interval));'
+                   ' return 0;}\n')
     exe = tmp_path / "t"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
     out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     assert out == [C.sizeof(_lib.BoxDesc), C.sizeof(_lib.HsmacParams), C.sizeof(_lib.ParticleDesc),
                    C.sizeof(_lib.Forcing), C.sizeof(_lib.StepConfig), C.sizeof(_lib.StepLog),
-                   _lib.StepConfig.forcing.offset, _lib.StepLog.div1.offset]
+                   _lib.StepConfig.forcing.offset, _lib.StepLog.div1.offset, _lib.StepConfig.localize_interval.offset]
 
 
 YAML = """

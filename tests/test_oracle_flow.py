@@ -313,6 +313,37 @@ def test_hsmac_matches_sequential_pass_replay(rng, tb, ov, itrmin, itrmax, epsp)
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("dims", [(6, 7, 8), (2, 5, 3), (9, 2, 4)])
+def test_overlap_faces_pairwise_order_is_exact(rng, dims):
+    """The device runs the overlap faces as (z- and z+ red), (z- and z+ black),
+    then y, then x.  Opposite slabs of one axis share no face for n >= 2, so
+    this equals the reference's slab-by-slab order (halo.jl:349-366) bit for bit
+    over the widest domain d1."""
+    h = 3
+    nx, ny, nz = dims
+    base = [rand_halo(rng, dims, h) for _ in range(4)]
+    A = [b.copy(order="F") for b in base]
+    B = [b.copy(order="F") for b in base]
+    dlo, dhi = (-1, -1, -1), (nx + 3, ny + 3, nz + 3)
+    clo, chi = (2, 2, 2), (nx, ny, nz)
+    faces = [((dlo[0], dlo[1], dlo[2]), (dhi[0], dhi[1], clo[2] - 1)),
+             ((dlo[0], dlo[1], chi[2] + 1), (dhi[0], dhi[1], dhi[2])),
+             ((dlo[0], dlo[1], clo[2]), (dhi[0], clo[1] - 1, chi[2])),
+             ((dlo[0], chi[1] + 1, clo[2]), (dhi[0], dhi[1], chi[2])),
+             ((dlo[0], clo[1], clo[2]), (clo[0] - 1, chi[1], chi[2])),
+             ((chi[0] + 1, clo[1], clo[2]), (dhi[0], chi[1], chi[2]))]
+    D = np.zeros_like(A[0])
+    for lo, hi in faces:  # reference order
+        for c in (0, 1):
+            O.hsmac_pass(*A, D, h, lo, hi, c, 0.01)
+    for q in (0, 2, 4):  # paired order
+        for c in (0, 1):
+            for lo, hi in faces[q:q + 2]:
+                O.hsmac_pass(*B, D, h, lo, hi, c, 0.01)
+    for a, b in zip(A, B):
+        assert np.array_equal(a, b)
+
+
 def test_hsmac_itr_semantics():
     """itr counts the sweeps before the converged one (hsmac.jl:122-125): a
     converged call performs itr+1 sweeps; an unconverged naive call returns
