@@ -234,7 +234,8 @@ int lcs_box_destroy(lcs_box *box)
     for (auto &pp : b->profiles) {
         for (void *p : pp.props)
             if (p) (void)hipFree(p);
-        void *extra[] = {pp.hashes, pp.perm, pp.starts, pp.stops, pp.scratch, pp.sort_tmp, pp.loc, pp.loc_tmp};
+        void *extra[] = {pp.hashes,  pp.perm,     pp.starts, pp.stops,  pp.scratch,
+                         pp.scratch_t, pp.sort_tmp, pp.loc,    pp.loc_tmp};
         for (void *p : extra)
             if (p) (void)hipFree(p);
     }
@@ -439,6 +440,7 @@ int lcs_particles_add(lcs_box *box, const lcs_particle_desc *d, int *profile)
     if (rc == LCS_OK) rc = device_alloc(b, &pp.starts, 8 * (size_t)ncell);
     if (rc == LCS_OK) rc = device_alloc(b, &pp.stops, 8 * (size_t)ncell);
     if (rc == LCS_OK) rc = device_alloc(b, &pp.scratch, 8 * nt);
+    if (rc == LCS_OK) rc = device_alloc(b, &pp.scratch_t, (size_t)b.esize * nt);
     if (rc == LCS_OK) {
         // ids = INVALID (src/particles/buffer.jl:19-20)
         std::vector<int64_t> inval(nt, kInvalidId);
@@ -449,7 +451,7 @@ int lcs_particles_add(lcs_box *box, const lcs_particle_desc *d, int *profile)
     if (rc != LCS_OK) {
         for (void *p : pp.props)
             if (p) (void)hipFree(p);
-        void *extra[] = {pp.hashes, pp.perm, pp.starts, pp.stops, pp.scratch};
+        void *extra[] = {pp.hashes, pp.perm, pp.starts, pp.stops, pp.scratch, pp.scratch_t};
         for (void *p : extra)
             if (p) (void)hipFree(p);
         return rc;

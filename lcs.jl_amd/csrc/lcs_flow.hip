@@ -79,16 +79,25 @@ __device__ inline void xcd_tile(int &bx, int &by, int &bz)
     bz = t / (gx * gy);
 }
 
-// Visits every interior cell once: rows (j, k) are spread over the grid and x
-// over the threads of a block, so the index arithmetic per cell is one add.
+// Visits every interior cell once.  Each block walks one contiguous run of
+// rows (j, k) with x over its threads, so the index arithmetic per cell is one
+// add and the y/z neighbours a stencil reads were just read by the same block
+// or by a block on the same XCD (runs are handed out XCD by XCD).
 template <typename F>
 __device__ inline void for_interior(const Geom &g, F &&f)
 {
-    const int rows = g.ny * g.nz;
-    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
-        const int j = r % g.ny + 1, k = r / g.ny + 1;
+    const int rows = g.ny * g.nz, nb = gridDim.x;
+    const int per_x = nb / kXcds, rem = nb % kXcds, x = blockIdx.x % kXcds, i0 = blockIdx.x / kXcds;
+    const int b = x < rem ? x * (per_x + 1) + i0 : rem * (per_x + 1) + (x - rem) * per_x + i0;
+    const int per = (rows + nb - 1) / nb, r0 = b * per, r1 = min(rows, r0 + per);
+    int j = r0 % g.ny + 1, k = r0 / g.ny + 1;
+    for (int r = r0; r < r1; r++) {
         const long base = g.at(0, j, k);
         for (int i = 1 + (int)threadIdx.x; i <= g.nx; i += blockDim.x) f(i, j, k, base + i);
+        if (++j > g.ny) {
+            j = 1;
+            k++;
+        }
     }
 }
 
@@ -326,7 +335,8 @@ static int rhs_t(Box &b, int stage_in, bool fused, void *const out[3], void *con
     a.P = (const T *)b.f[LCS_FIELD_P];
     for (int q = 0; q < 3; q++) {
         a.out[q] = (T *)out[q];
-        a.base[q] = fused ? (const T *)base[q] : nullptr;
+        // base == input (stage 1): read it from the tile's registers, not HBM
+        a.base[q] = (fused && base[q] != in[q]) ? (const T *)base[q] : nullptr;
         a.di[q] = c.di[q];
         a.di2[q] = c.di2[q];
     }

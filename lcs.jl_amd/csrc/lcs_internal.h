@@ -90,7 +90,8 @@ struct ParticleProfile {
     void *perm = nullptr;           // int32 ntotal
     void *starts = nullptr;         // int64 (nc+2)^3
     void *stops = nullptr;
-    void *scratch = nullptr;        // 8 bytes * ntotal (permutation staging)
+    void *scratch = nullptr;        // 8 bytes * ntotal: spare of the int64 ids, and transfer staging
+    void *scratch_t = nullptr;      // esize * ntotal: spare of the float properties
     void *sort_tmp = nullptr;       // keys/values ping-pong + histograms
     size_t sort_tmp_bytes = 0;
     bool indexed = false;
@@ -101,6 +102,10 @@ struct ParticleProfile {
     void *loc_tmp = nullptr;  // uint32 ntotal
     bool loc_on = false;
 };
+
+// The spare buffer a property is permuted into before the two are swapped:
+// the spare always has that property's element size, so swaps never mix sizes.
+inline void *&prop_spare(ParticleProfile &pp, int prop) { return prop == LCS_PROP_ID ? pp.scratch : pp.scratch_t; }
 
 struct Ctx {
     int device;

@@ -590,12 +590,12 @@ static int makeindex_t(Box &b, ParticleProfile &pp, int stage)
         for (int q = 0; q < LCS_N_PROPERTIES; q++) {
             if (q == LCS_PROP_ID)
                 hipLaunchKernelGGL(k_permute<int64_t>, dim3(pgrid(ntotal)), dim3(256), 0, b.stream, n, ntotal, vA,
-                                   (const int64_t *)pp.props[q], (int64_t *)pp.scratch);
+                                   (const int64_t *)pp.props[q], (int64_t *)prop_spare(pp, q));
             else
                 hipLaunchKernelGGL(k_permute<T>, dim3(pgrid(ntotal)), dim3(256), 0, b.stream, n, ntotal, vA,
-                                   (const T *)pp.props[q], (T *)pp.scratch);
+                                   (const T *)pp.props[q], (T *)prop_spare(pp, q));
             LCS_HIP(hipGetLastError());
-            std::swap(pp.props[q], pp.scratch);
+            std::swap(pp.props[q], prop_spare(pp, q));
         }
         LCS_HIP(hipMemcpyAsync(pp.perm, vA, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, b.stream));
     }
@@ -683,9 +683,9 @@ int launch_p_unlocalize(Box &b, ParticleProfile &pp)
 {
     if (!pp.loc_on) return LCS_OK;
     for (int q = 0; q < LCS_N_PROPERTIES; q++) {
-        int rc = launch_p_loc_move(b, pp, pp.props[q], pp.scratch, q == LCS_PROP_ID ? 8 : b.esize, false);
+        int rc = launch_p_loc_move(b, pp, pp.props[q], prop_spare(pp, q), q == LCS_PROP_ID ? 8 : b.esize, false);
         if (rc) return rc;
-        std::swap(pp.props[q], pp.scratch);
+        std::swap(pp.props[q], prop_spare(pp, q));
     }
     pp.loc_on = false;
     return LCS_OK;
@@ -757,12 +757,12 @@ static int localize_t(Box &b, ParticleProfile &pp)
     for (int q = 0; q < LCS_N_PROPERTIES; q++) {
         if (q == LCS_PROP_ID)
             hipLaunchKernelGGL(k_permute<int64_t>, dim3(pgrid(nt)), dim3(256), 0, b.stream, n, nt, vals,
-                               (const int64_t *)pp.props[q], (int64_t *)pp.scratch);
+                               (const int64_t *)pp.props[q], (int64_t *)prop_spare(pp, q));
         else
             hipLaunchKernelGGL(k_permute<T>, dim3(pgrid(nt)), dim3(256), 0, b.stream, n, nt, vals,
-                               (const T *)pp.props[q], (T *)pp.scratch);
+                               (const T *)pp.props[q], (T *)prop_spare(pp, q));
         LCS_HIP(hipGetLastError());
-        std::swap(pp.props[q], pp.scratch);
+        std::swap(pp.props[q], prop_spare(pp, q));
     }
     hipLaunchKernelGGL(k_loc_inverse, dim3(pgrid(nt)), dim3(256), 0, b.stream, n, nt, vals,
                        (unsigned *)pp.loc_tmp);

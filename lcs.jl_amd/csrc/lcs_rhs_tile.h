@@ -33,7 +33,7 @@ struct RhsArgs {
     Geom g;
     const T *U, *V, *W, *P;  // velocity of the stage + pressure
     T *out[3];               // fused: updated velocity; else dU/dt (interior layout)
-    const T *base[3];        // fused: velocity at the start of the step
+    const T *base[3];        // fused: velocity at the start of the step (null: the stage input itself)
     T di[3], di2[3], Rei;
     int zc;  // planes per chunk
 };
@@ -287,8 +287,8 @@ struct RhsTile {
                 const T fzs = sf(th.j1[c][0], th.j1[c][1], th.j3[c][0], th.j3[c][3]);
                 const T advect = (a.di[0] * fxs + a.di[1] * fys) + a.di[2] * fzs;
                 const T dq = (diff - pres) - advect;
-                if (FUSED)
-                    a.out[c][p] = a.base[c][p] + dq * dt;
+                if (FUSED)  // stage 1 updates its own input: the base is the register copy
+                    a.out[c][p] = (a.base[c] ? a.base[c][p] : ctr) + dq * dt;
                 else
                     a.out[c][g.interior(th.X0 + tx, th.Y0 + ty, z)] = dq;
             }

@@ -77,3 +77,24 @@ def test_makeindex_after_localize(rng):
         np.testing.assert_array_equal(b, a)
     for k in ("ids", "x", "y", "z"):
         np.testing.assert_array_equal(boxes[1].download_prop(0, k), boxes[0].download_prop(0, k), err_msg=k)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_repeated_reorders_keep_buffers_consistent(rng, dtype):
+    """makeindex and localize permute every property through a spare buffer of
+    the property's own element size; repeated calls in any order (FP32 has
+    4-byte properties beside the 8-byte ids) must leave every download intact."""
+    L = lcs()
+    box, prof, Us, props = particle_box(rng, dtype, 2000, dims=(32, 32, 32), ntotal=2100)
+    L.makeindex(box, prof, 1)
+    once = all_props(box, prof)
+    L.makeindex(box, prof, 1)  # stable sort of sorted data: unchanged
+    for k, v in all_props(box, prof).items():
+        np.testing.assert_array_equal(v, once[k], err_msg=k)
+    for _ in range(2):
+        L.localize(box, prof)
+        for k, v in all_props(box, prof).items():
+            np.testing.assert_array_equal(v, once[k], err_msg=k)
+    L.makeindex(box, prof, 1)
+    for k, v in all_props(box, prof).items():
+        np.testing.assert_array_equal(v, once[k], err_msg=k)

@@ -38,12 +38,13 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
-def _run(lib, shape, arrays, h, Re, out, base=None, dt=0.0, zc=None):
+def _run(lib, shape, arrays, h, Re, out, base=None, dt=0.0, zc=None, fused=None):
     nx, ny, nz = (s - 2 * h for s in arrays[0].shape)
     b = base if base is not None else (None, None, None)
+    fused = base is not None if fused is None else fused
     rc = lib.rhs_tile_emul(shape, arrays[0].dtype.itemsize, nx, ny, nz, h, C.c_double(Re),
                            *[_p(a) for a in arrays], *[_p(o) for o in out], *[_p(x) for x in b],
-                           int(base is not None), C.c_double(dt), zc or nz)
+                           int(fused), C.c_double(dt), zc or nz)
     assert rc == 0
 
 
@@ -85,3 +86,19 @@ def test_tile_rhs_fused_update(emul, dtype):
         o2 = o.copy()
         o2[h:-h, h:-h, h:-h] = b[h:-h, h:-h, h:-h]
         np.testing.assert_array_equal(o2, b)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_tile_rhs_fused_update_of_its_own_input(emul, dtype):
+    """Stage 1 updates the array it reads (base == input): with no base array
+    the kernel takes the base from its register copy of the input."""
+    h, dims, dt = 3, (21, 9, 10), 0.011
+    rng = np.random.default_rng(6)
+    A = [np.asfortranarray(rng.standard_normal(tuple(n + 2 * h for n in dims)).astype(dtype)) for _ in range(4)]
+    for a in A:
+        O.ghost_fill(a, h, h, True)
+    out = [np.zeros_like(A[0]) for _ in range(3)]
+    _run(emul, 1, A, h, 13.0, out, dt=dt, zc=3, fused=True)
+    dU = O.rhs(*A, h, 13.0)
+    for o, u, d in zip(out, A[:3], dU):
+        np.testing.assert_array_equal(o[h:-h, h:-h, h:-h], u[h:-h, h:-h, h:-h] + d * dtype(dt))
