@@ -244,6 +244,27 @@ int lcs_step(lcs_box *box, const lcs_step_config *cfg, lcs_step_log *log);
 /* the most recent step's log (synchronises) */
 int lcs_step_log_read(lcs_box *box, lcs_step_log *log);
 
+/* ---- batches of boxes ---------------------------------------------------- */
+/* No counterpart in the reference: an ensemble of independent boxes advanced
+ * together, one kernel launch per phase for all of them.  A box added to a
+ * batch is stepped only by lcs_batch_step (lcs_step and lcs_hsmac on it fail
+ * with LCS_ERR_STATE); its own entry points for data (upload, download,
+ * particles) keep working.  Each box keeps its own state, dt, step count and
+ * HSMAC convergence, and advances exactly as lcs_step would advance it alone.
+ * Boxes of one batch share the float type, halo size, cfl, dtmax, HSMAC
+ * parameters and forcing kind (none or linear-power); sizes, particle
+ * profiles and the forcing power may differ. */
+typedef struct lcs_batch lcs_batch;
+int lcs_batch_create(lcs_ctx *ctx, lcs_batch **out);
+int lcs_batch_destroy(lcs_batch *batch); /* detaches every box first */
+int lcs_batch_add(lcs_batch *batch, lcs_box *box, const lcs_step_config *cfg);
+int lcs_batch_remove(lcs_batch *batch, lcs_box *box);
+int lcs_batch_size(lcs_batch *batch, int *nboxes);
+/* one timestep! + evolve! of every box; logs (may be NULL) receives one entry
+ * per box in the order of addition (at most nlogs) */
+int lcs_batch_step(lcs_batch *batch, lcs_step_log *logs, int nlogs);
+int lcs_batch_sync(lcs_batch *batch);
+
 /* ---- measurement hook (no counterpart in the reference) ---------------------
  * enable != 0 starts recording HIP events on the box stream around every
  * kernel launch, keyed by kernel name, together with each launch's

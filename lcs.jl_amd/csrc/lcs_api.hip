@@ -225,6 +225,7 @@ int lcs_box_destroy(lcs_box *box)
 {
     if (!box) return LCS_OK;
     Box *b = B(box);
+    (void)batch_detach_box(*b);
     (void)hipSetDevice(b->ctx->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (void *p : b->f)
@@ -360,6 +361,7 @@ int lcs_hsmac(lcs_box *box, int stage, double dt_stage, const lcs_hsmac_params *
     if (check_box(box) || check_stage(stage)) return LCS_ERR_INVALID;
     if (!prm) return fail(LCS_ERR_INVALID, "null params");
     Box &b = *B(box);
+    if (b.batch) return fail(LCS_ERR_STATE, "box is stepped by a batch");
     int rc = set_phase_dt(b, dt_stage);
     if (rc) return rc;
     void **Us = stage == 1 ? &b.f[LCS_FIELD_U2] : &b.f[LCS_FIELD_U];
@@ -457,6 +459,7 @@ int lcs_particles_add(lcs_box *box, const lcs_particle_desc *d, int *profile)
         return rc;
     }
     b.profiles.push_back(pp);
+    batch_mark_dirty(b);
     if (profile) *profile = (int)b.profiles.size() - 1;
     return LCS_OK;
 }
@@ -540,12 +543,14 @@ int lcs_particles_makeindex(lcs_box *box, int profile, int stage)
 {
     WITH_PROFILE
     if (check_stage(stage)) return LCS_ERR_INVALID;
+    batch_mark_dirty(b);
     return launch_p_makeindex(b, *pp, stage);
 }
 
 int lcs_particles_localize(lcs_box *box, int profile)
 {
     WITH_PROFILE
+    batch_mark_dirty(b);
     return launch_p_localize(b, *pp);
 }
 
@@ -602,6 +607,7 @@ int lcs_step(lcs_box *box, const lcs_step_config *cfg, lcs_step_log *log)
 {
     if (check_box(box) || !cfg) return fail(LCS_ERR_INVALID, "null argument");
     Box &b = *B(box);
+    if (b.batch) return fail(LCS_ERR_STATE, "box is stepped by a batch");
     if (b.g.h < 3) return fail(LCS_ERR_INVALID, "a flow step needs halo_size >= 3");
     if (cfg->localize_interval < 0) return fail(LCS_ERR_INVALID, "localize_interval must be >= 0");
     int rc;

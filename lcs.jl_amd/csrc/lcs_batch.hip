@@ -1,0 +1,573 @@
+// Batches of boxes: an ensemble of independent periodic boxes of uneven size
+// advanced with one launch per phase (lcs_batch_*).  Each box keeps its own
+// state, step count and HSMAC convergence; the batch owns the stream, a
+// device table of per-box records (BoxRec) and the work tables that map
+// workgroups to (box, block) or (box, tile).  The per-box arithmetic is the
+// single-box kernels' own device code, so a box advanced in a batch gives the
+// same result as the same box advanced alone with lcs_step.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "lcs_internal.h"
+#include "lcs_hsmac_tile.h"
+#include "lcs_rhs_tile.h"
+
+namespace lcs {
+
+template <typename E>
+struct DevVec {
+    E *d = nullptr;
+    int n = 0, cap = 0;
+    int assign(const std::vector<E> &h, hipStream_t s)
+    {
+        n = (int)h.size();
+        if (n > cap) {
+            if (d) LCS_HIP(hipFree(d));
+            d = nullptr;
+            cap = std::max(n, 16);
+            LCS_HIP(hipMalloc(&d, sizeof(E) * (size_t)cap));
+        }
+        if (n) LCS_HIP(hipMemcpyAsync(d, h.data(), sizeof(E) * (size_t)n, hipMemcpyHostToDevice, s));
+        return LCS_OK;
+    }
+    void release()
+    {
+        if (d) (void)hipFree(d);
+        d = nullptr;
+        n = cap = 0;
+    }
+};
+
+struct Batch {
+    Ctx *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    int esize = 0;
+    lcs_hsmac_params hs{};
+    std::vector<Box *> boxes;
+    std::vector<lcs_step_config> cfgs;
+    std::vector<hipStream_t> own_streams;
+    bool dirty = true;
+    // device state
+    std::vector<BoxRec> tab_h;
+    DevVec<BoxRec> tab;
+    DevVec<WorkItem> ghost_face, ghost_full, ghost_one, rows, parts, diams, faces[3];
+    DevVec<TileItem> rhs_tiles, hs_tiles;
+    HsPoll *poll_d = nullptr, *poll_h = nullptr;
+    DevScalars *sc_d = nullptr, *sc_h = nullptr;
+    int cap_boxes = 0;
+};
+
+// Work items are listed box by box, then permuted so that workgroup L, which
+// the hardware runs on XCD L % kXcds, takes item t of one contiguous range per
+// XCD (the inverse of the order used by the single-box tile kernels).
+template <typename E>
+static std::vector<E> xcd_order(const std::vector<E> &v)
+{
+    const int n = (int)v.size(), X = 8;
+    std::vector<E> out(v.size());
+    const int per = n / X, rem = n % X;
+    for (int L = 0; L < n; L++) {
+        const int x = L % X, i = L / X;
+        const int t = x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+        out[L] = v[t];
+    }
+    return out;
+}
+
+static int blocks_of(long n, int per = 256) { return (int)((n + per - 1) / per); }
+
+template <typename T>
+static void fill_coefs(const Box &b, const lcs_hsmac_params &hs, BoxRec &r)
+{
+    const Geom &g = b.g;
+    const double d[3] = {spacing(g.nx), spacing(g.ny), spacing(g.nz)};
+    for (int a = 0; a < 3; a++) {
+        const T di = T(1.0 / d[a]);
+        r.rhs_di[a] = di;
+        r.rhs_di2[a] = (double)(T)(di * di);
+    }
+    r.rhs_Rei = (double)T(1.0 / b.Re());
+    const T dxi = T(1.0 / d[0]), dyi = T(1.0 / d[1]), dzi = T(1.0 / d[2]);
+    r.hs_dxi = dxi;
+    r.hs_dyi = dyi;
+    r.hs_dzi = dzi;
+    r.hs_dl = (double)(T)(T(1) / (T(2) * ((dxi * dxi + dyi * dyi) + dzi * dzi)));
+    r.hs_omega = (double)T(hs.omega);
+    r.dx = d[0];
+    r.dy = d[1];
+    r.dz = d[2];
+}
+
+template <typename T>
+static void fill_profile(const Box &b, const ParticleProfile &pp, DevProfile &p)
+{
+    p.nvalid = pp.desc.nvalid;
+    p.ntotal = pp.desc.ntotal;
+    for (int q = 0; q < LCS_N_PROPERTIES; q++) p.props[q] = pp.props[q];
+    p.mdx = (double)T(spacing(b.g.nx));
+    p.mdy = (double)T(spacing(b.g.ny));
+    p.mdz = (double)T(spacing(b.g.nz));
+    p.mRe = (double)T(b.Re());
+    p.mg = (double)T(pp.desc.gravity);
+    p.mkap = (double)T(((1.0 / 18.0) * pp.desc.density_ratio) * b.Re());
+    p.kap = ((1.0 / 18.0) * pp.desc.density_ratio) * b.Re();
+    p.drag = pp.desc.drag;
+}
+
+// the record of box i from the host mirror of its pointers
+static void refresh_record(Batch &B, int i)
+{
+    Box &b = *B.boxes[i];
+    BoxRec &r = B.tab_h[i];
+    for (int q = 0; q < 7; q++) r.f[q] = b.f[q];
+    for (int q = 0; q < 4; q++) r.spare[q] = b.spare[q];
+    for (int q = 0; q < (int)b.profiles.size(); q++)
+        for (int k = 0; k < LCS_N_PROPERTIES; k++) r.prof[q].props[k] = b.profiles[q].props[k];
+}
+
+// The table is small (a few KB per box): it is uploaded once the stream has
+// drained, so no kernel reads a record while it changes.
+static int upload_table(Batch &B)
+{
+    LCS_HIP(hipStreamSynchronize(B.stream));
+    int rc = B.tab.assign(B.tab_h, B.stream);
+    if (rc) return rc;
+    LCS_HIP(hipStreamSynchronize(B.stream));
+    return LCS_OK;
+}
+
+static int rebuild(Batch &B)
+{
+    const int n = (int)B.boxes.size();
+    if (n > B.cap_boxes) {
+        if (B.poll_d) LCS_HIP(hipFree(B.poll_d));
+        if (B.poll_h) LCS_HIP(hipHostFree(B.poll_h));
+        if (B.sc_d) LCS_HIP(hipFree(B.sc_d));
+        if (B.sc_h) LCS_HIP(hipHostFree(B.sc_h));
+        B.cap_boxes = std::max(n, 2 * B.cap_boxes);
+        LCS_HIP(hipMalloc(&B.poll_d, sizeof(HsPoll) * (size_t)B.cap_boxes));
+        LCS_HIP(hipHostMalloc((void **)&B.poll_h, sizeof(HsPoll) * (size_t)B.cap_boxes, hipHostMallocDefault));
+        LCS_HIP(hipMalloc(&B.sc_d, sizeof(DevScalars) * (size_t)B.cap_boxes));
+        LCS_HIP(hipHostMalloc((void **)&B.sc_h, sizeof(DevScalars) * (size_t)B.cap_boxes, hipHostMallocDefault));
+    }
+    B.tab_h.assign(n, BoxRec{});
+    std::vector<WorkItem> gface, gfull, gone, rows, parts, diams, faces[3];
+    std::vector<TileItem> rtiles, htiles;
+    const int domain = B.hs.time_blocking ? 1 : 3;
+    for (int i = 0; i < n; i++) {
+        Box &b = *B.boxes[i];
+        const Geom &g = b.g;
+        for (int q = 0; q < 4; q++)
+            if (!b.spare[q]) {
+                LCS_HIP(hipMalloc(&b.spare[q], (size_t)g.len * b.esize));
+                b.bytes += (size_t)g.len * b.esize;
+            }
+        BoxRec &r = B.tab_h[i];
+        r.g = g;
+        r.S = b.S;
+        r.part = b.part;
+        r.Re = b.Re();
+        r.power = B.cfgs[i].forcing.kind == LCS_FORCING_LINEAR_POWER ? B.cfgs[i].forcing.power : 0.0;
+        if (b.esize == 8)
+            fill_coefs<double>(b, B.hs, r);
+        else
+            fill_coefs<float>(b, B.hs, r);
+        // fused RHS tiles: 32 x (8 | 4), z-chunks of about 64 planes
+        const int rtx = 32, rty = b.esize == 8 ? 4 : 8;
+        const int rgx = (g.nx + rtx - 1) / rtx, rgy = (g.ny + rty - 1) / rty;
+        const int rgz = std::max(1, g.nz / 64);
+        r.rhs_zc = (g.nz + rgz - 1) / rgz;
+        for (int z = 0; z * r.rhs_zc < g.nz; z++)
+            for (int y = 0; y < rgy; y++)
+                for (int x = 0; x < rgx; x++) rtiles.push_back({i, x, y, z});
+        // fused HSMAC sweep tiles: 64 x 8 over the whole array
+        const int zplanes = g.nz + 2 * g.h;
+        const int hgx = (g.nx + 2 * g.h + 63) / 64, hgy = (g.ny + 2 * g.h + 7) / 8;
+        const int hgz = std::max(1, zplanes / 64);
+        r.hs_zc = (zplanes + hgz - 1) / hgz;
+        r.hs_tx = 64;
+        r.hs_ty = 8;
+        int nz_used = (zplanes + r.hs_zc - 1) / r.hs_zc;
+        r.hs_tiles = hgx * hgy * nz_used;
+        for (int z = 0; z < nz_used; z++)
+            for (int y = 0; y < hgy; y++)
+                for (int x = 0; x < hgx; x++) htiles.push_back({i, x, y, z});
+        // face passes of the overlap order
+        for (int a = 0; a < 3; a++) {
+            Box3 f0, f1;
+            hs_faces(g, domain, a, f0, f1);
+            r.face_blocks[a] = std::max(1, blocks_of(pass_count(f0) + pass_count(f1)));
+            for (int k = 0; k < r.face_blocks[a]; k++) faces[a].push_back({i, k});
+        }
+        // ghost fills
+        auto ghost = [&](int rr, int full, std::vector<WorkItem> &w) {
+            const long X = full ? g.nx + 2 * rr : g.nx, Y = full ? g.ny + 2 * rr : g.ny;
+            const long tot = 2L * rr * X * Y + 2L * rr * g.nz * X + 2L * rr * g.ny * g.nz;
+            for (int k = 0; k < blocks_of(tot); k++) w.push_back({i, k});
+        };
+        ghost(g.h, 0, gface);
+        ghost(g.h, 1, gfull);
+        ghost(1, 1, gone);
+        // row-walking reductions
+        r.row_blocks = std::min(g.ny * g.nz, kBatchRowBlocks);
+        for (int k = 0; k < r.row_blocks; k++) rows.push_back({i, k});
+        // particles
+        r.nprof = (int)b.profiles.size();
+        int blk = 0, dblk = 0;
+        for (int q = 0; q < r.nprof; q++) {
+            const ParticleProfile &pp = b.profiles[q];
+            DevProfile &p = r.prof[q];
+            if (b.esize == 8)
+                fill_profile<double>(b, pp, p);
+            else
+                fill_profile<float>(b, pp, p);
+            p.blk0 = blk;
+            p.nblk = blocks_of(pp.desc.nvalid);
+            p.dblk0 = dblk;
+            p.dnblk = std::max(1, blocks_of(pp.desc.ntotal));
+            blk += p.nblk;
+            dblk += p.dnblk;
+        }
+        for (int k = 0; k < blk; k++) parts.push_back({i, k});
+        for (int k = 0; k < dblk; k++) diams.push_back({i, k});
+        // every count of per-block partials must fit one partial slot
+        int most = std::max(r.hs_tiles, r.row_blocks);
+        for (int a = 0; a < 3; a++) most = std::max(most, r.face_blocks[a]);
+        for (int q = 0; q < r.nprof; q++) most = std::max(most, r.prof[q].dnblk);
+        if (most > kMaxPartBlocks) return fail(LCS_ERR_CAPACITY, "box too large for a batch");
+        refresh_record(B, i);
+    }
+    int rc;
+    if ((rc = upload_table(B))) return rc;
+    if ((rc = B.ghost_face.assign(xcd_order(gface), B.stream)) || (rc = B.ghost_full.assign(xcd_order(gfull), B.stream)) ||
+        (rc = B.ghost_one.assign(xcd_order(gone), B.stream)) || (rc = B.rows.assign(xcd_order(rows), B.stream)) ||
+        (rc = B.parts.assign(xcd_order(parts), B.stream)) || (rc = B.diams.assign(xcd_order(diams), B.stream)) ||
+        (rc = B.rhs_tiles.assign(xcd_order(rtiles), B.stream)) || (rc = B.hs_tiles.assign(xcd_order(htiles), B.stream)))
+        return rc;
+    for (int a = 0; a < 3; a++)
+        if ((rc = B.faces[a].assign(xcd_order(faces[a]), B.stream))) return rc;
+    LCS_HIP(hipStreamSynchronize(B.stream));  // host vectors above are temporaries
+    B.dirty = false;
+    return LCS_OK;
+}
+
+// ---- HSMAC over the whole batch (couple!: src/flows/couples/hsmac.jl:52-201) --
+static int b_fill(Batch &B, int set, int narr, int width, int hsmac)
+{
+    DevVec<WorkItem> &w = width == 1 ? B.ghost_one : B.ghost_full;
+    return launch_b_ghost_fill(B.stream, B.esize, B.tab.d, w.d, w.n, set, narr, width, 1, hsmac);
+}
+
+static int b_sweep(Batch &B, int set, int kind, int slot, double dtfac)
+{
+    int rc = launch_b_hsmac_sweep(B.stream, B.esize, B.hs.coloring, B.tab.d, B.hs_tiles.d, B.hs_tiles.n, set, kind,
+                                  slot, dtfac);
+    if (rc) return rc;
+    return launch_b_hsmac_ctl(B.stream, B.tab.d, (int)B.boxes.size(), kHsFlip, 0, nullptr);
+}
+
+// core sweep + paired face passes (overlap order); returns the pass kinds
+static int b_core_faces(Batch &B, int set, int domain, double dtfac, std::vector<int> &kinds)
+{
+    int rc = b_sweep(B, set, 0, 0, dtfac);
+    if (rc) return rc;
+    kinds.assign(1, 0);
+    for (int a = 0; a < 3; a++)
+        for (int c = 0; c < 2; c++) {
+            const int slot = (int)kinds.size();
+            rc = launch_b_hsmac_faces(B.stream, B.esize, B.hs.coloring, B.tab.d, B.faces[a].d, B.faces[a].n, set,
+                                      domain, a, c, slot, dtfac);
+            if (rc) return rc;
+            kinds.push_back(1 + a);
+        }
+    return LCS_OK;
+}
+
+static int b_check(Batch &B, const std::vector<int> &kinds)
+{
+    return launch_b_hsmac_check(B.stream, B.tab.d, (int)B.boxes.size(), kinds.data(), (int)kinds.size(), B.hs.epsp,
+                                B.hs.itrmin);
+}
+
+static int b_poll(Batch &B, bool &all_done)
+{
+    const int n = (int)B.boxes.size();
+    int rc = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsPoll, 0, B.poll_d);
+    if (rc) return rc;
+    LCS_HIP(hipMemcpyAsync(B.poll_h, B.poll_d, sizeof(HsPoll) * (size_t)n, hipMemcpyDeviceToHost, B.stream));
+    LCS_HIP(hipStreamSynchronize(B.stream));
+    all_done = true;
+    for (int i = 0; i < n; i++) all_done = all_done && B.poll_h[i].done;
+    return LCS_OK;
+}
+
+// set 1 acts on (U2, V2, W2, P) (stage 1), set 0 on (U, V, W, P) (stage 2)
+static int b_hsmac(Batch &B, int set, double dtfac, int slot)
+{
+    const int n = (int)B.boxes.size();
+    const lcs_hsmac_params &p = B.hs;
+    int rc = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsBegin, 0, nullptr);
+    if (rc) return rc;
+    bool done = false, polled_last = false;
+    std::vector<int> kinds;
+    if (!p.time_blocking) {  // _couple_naive!: hsmac.jl:77-129
+        for (int s = 0; s < p.itrmax && !done; s++) {
+            if ((rc = b_fill(B, set, p.overlap ? 3 : 4, 1, 1))) return rc;
+            if (p.overlap) {
+                if ((rc = b_core_faces(B, set, 3, dtfac, kinds))) return rc;
+            } else {
+                if ((rc = b_sweep(B, set, 3, 0, dtfac))) return rc;
+                kinds.assign(1, 0);
+            }
+            if ((rc = b_check(B, kinds))) return rc;
+            polled_last = false;
+            if (s >= p.itrmin) {
+                if ((rc = b_poll(B, done))) return rc;
+                polled_last = true;
+            }
+        }
+    } else {  // _couple_time_blocking!: hsmac.jl:131-201
+        for (int grp = 0; 3 * grp < p.itrmax && !done; grp++) {
+            if ((rc = b_fill(B, set, 3, B.boxes[0]->g.h, 1))) return rc;
+            if (p.overlap) {
+                if ((rc = b_core_faces(B, set, 1, dtfac, kinds))) return rc;
+            } else {
+                if ((rc = b_sweep(B, set, 1, 0, dtfac))) return rc;
+                kinds.assign(1, 0);
+            }
+            if ((rc = b_check(B, kinds))) return rc;
+            kinds.assign(1, 0);
+            if ((rc = b_sweep(B, set, 2, 0, dtfac)) || (rc = b_check(B, kinds))) return rc;
+            if ((rc = b_sweep(B, set, 3, 0, dtfac)) || (rc = b_check(B, kinds))) return rc;
+            polled_last = false;
+            if (3 * grp + 2 >= p.itrmin) {
+                if ((rc = b_poll(B, done))) return rc;
+                polled_last = true;
+            }
+        }
+    }
+    if ((rc = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsEnd, slot, nullptr))) return rc;
+    if (!polled_last && (rc = b_poll(B, done))) return rc;
+    // the final parity decides where each box's result lives: swap on the
+    // device (commit) and in the host mirror
+    if ((rc = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsCommit, set, nullptr))) return rc;
+    for (int i = 0; i < n; i++)
+        if (B.poll_h[i].par) {
+            Box &b = *B.boxes[i];
+            for (int q = 0; q < 4; q++) {
+                const int f = q < 3 ? 3 * set + q : LCS_FIELD_P;
+                std::swap(b.f[f], b.spare[q]);
+            }
+            refresh_record(B, i);
+        }
+    return LCS_OK;
+}
+
+static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
+{
+    const int n = (int)B.boxes.size();
+    if (n == 0) return LCS_OK;
+    int rc;
+    if (B.dirty && (rc = rebuild(B))) return rc;
+    // storage order of particles for locality, per box on its own schedule
+    bool moved = false;
+    for (int i = 0; i < n; i++) {
+        Box &b = *B.boxes[i];
+        const int iv = B.cfgs[i].localize_interval;
+        if (iv > 0 && b.steps_host % iv == 0 && !b.profiles.empty()) {
+            for (auto &pp : b.profiles)
+                if ((rc = launch_p_localize(b, pp))) return rc;
+            refresh_record(B, i);
+            moved = true;
+        }
+        b.steps_host++;
+    }
+    if (moved && (rc = upload_table(B))) return rc;
+    const lcs_step_config &c0 = B.cfgs[0];
+    const int h = B.boxes[0]->g.h;
+    // timestep!: CFL (FaceHalo fill first) and particle dt
+    if ((rc = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_face.d, B.ghost_face.n, 0, 3, h, 0, 0))) return rc;
+    if ((rc = launch_b_cfl(B.stream, B.esize, B.tab.d, B.rows.d, B.rows.n, n, c0.cfl, c0.dtmax))) return rc;
+    if ((rc = launch_b_particle_dt(B.stream, B.esize, B.tab.d, B.diams.d, B.diams.n, n, c0.dtmax))) return rc;
+    // stage 1
+    if ((rc = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 0, 4, h, 1, 0))) return rc;
+    if ((rc = launch_b_particles(B.stream, B.esize, B.tab.d, B.parts.d, B.parts.n, 1))) return rc;
+    if ((rc = launch_b_rhs(B.stream, B.esize, B.tab.d, B.rhs_tiles.d, B.rhs_tiles.n, 1))) return rc;
+    if ((rc = b_hsmac(B, 1, 0.5, 0))) return rc;
+    // stage 2
+    if ((rc = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 1, 4, h, 1, 0))) return rc;
+    if ((rc = launch_b_particles(B.stream, B.esize, B.tab.d, B.parts.d, B.parts.n, 2))) return rc;
+    if ((rc = launch_b_rhs(B.stream, B.esize, B.tab.d, B.rhs_tiles.d, B.rhs_tiles.n, 2))) return rc;
+    if (c0.forcing.kind == LCS_FORCING_LINEAR_POWER) {
+        if ((rc = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 0, 3, h, 1, 0)))
+            return rc;
+        if ((rc = launch_b_lf(B.stream, B.esize, B.tab.d, B.rows.d, B.rows.n, n))) return rc;
+    }
+    if ((rc = b_hsmac(B, 0, 1.0, 1))) return rc;
+    if ((rc = launch_b_state(B.stream, B.tab.d, n, logs ? B.sc_d : nullptr))) return rc;
+    if (logs) {
+        LCS_HIP(hipMemcpyAsync(B.sc_h, B.sc_d, sizeof(DevScalars) * (size_t)n, hipMemcpyDeviceToHost, B.stream));
+        LCS_HIP(hipStreamSynchronize(B.stream));
+        for (int i = 0; i < n && i < nlogs; i++) {
+            const DevScalars &s = B.sc_h[i];
+            lcs_step_log &l = logs[i];
+            l.step = s.step;
+            l.t = s.t;
+            l.dt = s.dt;
+            l.itr1 = s.itr[0];
+            l.itr2 = s.itr[1];
+            l.div1 = s.div[0];
+            l.div2 = s.div[1];
+            l.p = s.p;
+            l.e10 = s.e10;
+        }
+    }
+    return LCS_OK;
+}
+
+static bool same_hsmac(const lcs_hsmac_params &a, const lcs_hsmac_params &b)
+{
+    return a.time_blocking == b.time_blocking && a.overlap == b.overlap && a.coloring == b.coloring &&
+           a.omega == b.omega && a.epsp == b.epsp && a.itrmin == b.itrmin && a.itrmax == b.itrmax;
+}
+
+static int detach(Batch &B, int i)
+{
+    LCS_HIP(hipStreamSynchronize(B.stream));
+    Box &b = *B.boxes[i];
+    b.stream = B.own_streams[i];
+    b.batch = nullptr;
+    B.boxes.erase(B.boxes.begin() + i);
+    B.cfgs.erase(B.cfgs.begin() + i);
+    B.own_streams.erase(B.own_streams.begin() + i);
+    B.dirty = true;
+    return LCS_OK;
+}
+
+int batch_detach_box(Box &b)
+{
+    Batch *B = b.batch;
+    if (!B) return LCS_OK;
+    for (size_t i = 0; i < B->boxes.size(); i++)
+        if (B->boxes[i] == &b) return detach(*B, (int)i);
+    return LCS_OK;
+}
+
+void batch_mark_dirty(Box &b)
+{
+    if (b.batch) b.batch->dirty = true;
+}
+
+}  // namespace lcs
+
+using namespace lcs;
+
+static Batch *BT(lcs_batch *p) { return reinterpret_cast<Batch *>(p); }
+
+extern "C" {
+
+int lcs_batch_create(lcs_ctx *ctx, lcs_batch **out)
+{
+    if (!ctx || !out) return fail(LCS_ERR_INVALID, "null argument");
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    LCS_HIP(hipSetDevice(c->device));
+    Batch *B = new Batch;
+    B->ctx = c;
+    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete B;
+        return fail(LCS_ERR_HIP, "hipStreamCreate failed");
+    }
+    *out = reinterpret_cast<lcs_batch *>(B);
+    return LCS_OK;
+}
+
+int lcs_batch_destroy(lcs_batch *batch)
+{
+    if (!batch) return LCS_OK;
+    Batch *B = BT(batch);
+    (void)hipSetDevice(B->ctx->device);
+    while (!B->boxes.empty()) (void)detach(*B, (int)B->boxes.size() - 1);
+    DevVec<WorkItem> *w[] = {&B->ghost_face, &B->ghost_full, &B->ghost_one, &B->rows, &B->parts, &B->diams,
+                             &B->faces[0],   &B->faces[1],   &B->faces[2]};
+    for (auto *v : w) v->release();
+    B->rhs_tiles.release();
+    B->hs_tiles.release();
+    B->tab.release();
+    if (B->poll_d) (void)hipFree(B->poll_d);
+    if (B->poll_h) (void)hipHostFree(B->poll_h);
+    if (B->sc_d) (void)hipFree(B->sc_d);
+    if (B->sc_h) (void)hipHostFree(B->sc_h);
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+    delete B;
+    return LCS_OK;
+}
+
+int lcs_batch_add(lcs_batch *batch, lcs_box *box, const lcs_step_config *cfg)
+{
+    if (!batch || !box || !cfg) return fail(LCS_ERR_INVALID, "null argument");
+    Batch &B = *BT(batch);
+    Box &b = *reinterpret_cast<Box *>(box);
+    if (b.batch) return fail(LCS_ERR_STATE, "box is already in a batch");
+    if (b.ctx != B.ctx) return fail(LCS_ERR_INVALID, "box and batch belong to different contexts");
+    if (b.g.h < 3) return fail(LCS_ERR_INVALID, "a flow step needs halo_size >= 3");
+    if (b.g.nx < 8 || b.g.ny < 8 || b.g.nz < 8) return fail(LCS_ERR_INVALID, "batched boxes need n >= 8 per axis");
+    if (cfg->forcing.kind != LCS_FORCING_NONE && cfg->forcing.kind != LCS_FORCING_LINEAR_POWER)
+        return fail(LCS_ERR_INVALID, "batched steps support no forcing and linear-power forcing");
+    if (cfg->hsmac.coloring != LCS_COLORING_RED_BLACK_FAST && cfg->hsmac.coloring != LCS_COLORING_RED_BLACK)
+        return fail(LCS_ERR_INVALID, "unsupported coloring");
+    if (cfg->localize_interval < 0) return fail(LCS_ERR_INVALID, "localize_interval must be >= 0");
+    if (!B.boxes.empty()) {
+        if (b.esize != B.esize) return fail(LCS_ERR_INVALID, "all boxes of a batch share one float type");
+        const lcs_step_config &c0 = B.cfgs[0];
+        if (!same_hsmac(cfg->hsmac, B.hs) || cfg->cfl != c0.cfl || cfg->dtmax != c0.dtmax ||
+            cfg->forcing.kind != c0.forcing.kind || b.g.h != B.boxes[0]->g.h)
+            return fail(LCS_ERR_INVALID,
+                        "boxes of a batch share HSMAC parameters, cfl, dtmax, forcing kind and halo size");
+    } else {
+        B.esize = b.esize;
+        B.hs = cfg->hsmac;
+    }
+    LCS_HIP(hipStreamSynchronize(b.stream));
+    B.boxes.push_back(&b);
+    B.cfgs.push_back(*cfg);
+    B.own_streams.push_back(b.stream);
+    b.stream = B.stream;  // every launcher of the box now enqueues on the batch stream
+    b.batch = &B;
+    B.dirty = true;
+    return LCS_OK;
+}
+
+int lcs_batch_remove(lcs_batch *batch, lcs_box *box)
+{
+    if (!batch || !box) return fail(LCS_ERR_INVALID, "null argument");
+    Batch &B = *BT(batch);
+    Box *b = reinterpret_cast<Box *>(box);
+    for (size_t i = 0; i < B.boxes.size(); i++)
+        if (B.boxes[i] == b) return detach(B, (int)i);
+    return fail(LCS_ERR_INVALID, "box is not in this batch");
+}
+
+int lcs_batch_size(lcs_batch *batch, int *nboxes)
+{
+    if (!batch || !nboxes) return fail(LCS_ERR_INVALID, "null argument");
+    *nboxes = (int)BT(batch)->boxes.size();
+    return LCS_OK;
+}
+
+int lcs_batch_step(lcs_batch *batch, lcs_step_log *logs, int nlogs)
+{
+    if (!batch) return fail(LCS_ERR_INVALID, "null argument");
+    Batch &B = *BT(batch);
+    LCS_HIP(hipSetDevice(B.ctx->device));
+    return batch_step(B, logs, nlogs);
+}
+
+int lcs_batch_sync(lcs_batch *batch)
+{
+    if (!batch) return fail(LCS_ERR_INVALID, "null argument");
+    LCS_HIP(hipStreamSynchronize(BT(batch)->stream));
+    return LCS_OK;
+}
+
+}  // extern "C"

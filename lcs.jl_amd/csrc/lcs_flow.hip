@@ -84,11 +84,9 @@ __device__ inline void xcd_tile(int &bx, int &by, int &bz)
 // add and the y/z neighbours a stencil reads were just read by the same block
 // or by a block on the same XCD (runs are handed out XCD by XCD).
 template <typename F>
-__device__ inline void for_interior(const Geom &g, F &&f)
+__device__ inline void for_rows(const Geom &g, int b, int nb, F &&f)
 {
-    const int rows = g.ny * g.nz, nb = gridDim.x;
-    const int per_x = nb / kXcds, rem = nb % kXcds, x = blockIdx.x % kXcds, i0 = blockIdx.x / kXcds;
-    const int b = x < rem ? x * (per_x + 1) + i0 : rem * (per_x + 1) + (x - rem) * per_x + i0;
+    const int rows = g.ny * g.nz;
     const int per = (rows + nb - 1) / nb, r0 = b * per, r1 = min(rows, r0 + per);
     int j = r0 % g.ny + 1, k = r0 / g.ny + 1;
     for (int r = r0; r < r1; r++) {
@@ -99,6 +97,15 @@ __device__ inline void for_interior(const Geom &g, F &&f)
             k++;
         }
     }
+}
+
+template <typename F>
+__device__ inline void for_interior(const Geom &g, F &&f)
+{
+    const int nb = gridDim.x;
+    const int per_x = nb / kXcds, rem = nb % kXcds, x = blockIdx.x % kXcds, i0 = blockIdx.x / kXcds;
+    const int b = x < rem ? x * (per_x + 1) + i0 : rem * (per_x + 1) + (x - rem) * per_x + i0;
+    for_rows(g, b, nb, f);
 }
 
 // blocks for a for_interior launch: every row gets a block up to the cap
@@ -151,14 +158,10 @@ struct Ptr4 {
 // Inside an HSMAC call (S != nullptr) the fill is skipped once the call has
 // converged and acts on the spare buffer set B when the ping-pong parity says
 // the current state lives there.
-template <typename T>
-__global__ void k_ghost_fill(Geom g, Ptr4<T> A, Ptr4<T> Bs, int narr, int r, int full, long cz, long cy,
-                             long total, const DevScalars *S)
+// Ghost cell t of the fill (z-slabs, then y-slabs, then x-slabs): its
+// offset d and the offset s of its periodic source.
+__device__ inline void ghost_cell(const Geom &g, int r, int full, long cz, long cy, long t, long &d, long &s)
 {
-    if (S && S->hs_done) return;
-    const bool alt = S && S->hs_par;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
     const int X = full ? g.nx + 2 * r : g.nx;
     const int Y = full ? g.ny + 2 * r : g.ny;
     const int x0 = full ? 1 - r : 1, y0 = full ? 1 - r : 1;
@@ -191,8 +194,29 @@ __global__ void k_ghost_fill(Geom g, Ptr4<T> A, Ptr4<T> Bs, int narr, int r, int
         j = 1 + jj;
         k = 1 + kk;
     }
-    long d = g.at(i, j, k);
-    long s = g.at(wrap1(i, g.nx), wrap1(j, g.ny), wrap1(k, g.nz));
+    d = g.at(i, j, k);
+    s = g.at(wrap1(i, g.nx), wrap1(j, g.ny), wrap1(k, g.nz));
+}
+
+// ghost-cell counts of the z-, y- and x-slabs of a fill of width r
+__host__ __device__ inline void ghost_counts(const Geom &g, int r, int full, long &cz, long &cy, long &cx)
+{
+    const long X = full ? g.nx + 2 * r : g.nx, Y = full ? g.ny + 2 * r : g.ny;
+    cz = 2L * r * X * Y;
+    cy = 2L * r * g.nz * X;
+    cx = 2L * r * g.ny * g.nz;
+}
+
+template <typename T>
+__global__ void k_ghost_fill(Geom g, Ptr4<T> A, Ptr4<T> Bs, int narr, int r, int full, long cz, long cy,
+                             long total, const DevScalars *S)
+{
+    if (S && S->hs_done) return;
+    const bool alt = S && S->hs_par;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    long d, s;
+    ghost_cell(g, r, full, cz, cy, t, d, s);
     for (int a = 0; a < narr; a++) {
         T *q = alt ? Bs.p[a] : A.p[a];
         q[d] = q[s];
@@ -206,8 +230,8 @@ static int ghost_fill_t(Box &b, void *const *arrays, void *const *alt, int narra
     const Geom &g = b.g;
     if (r < 1 || r > g.h) return fail(LCS_ERR_INVALID, "ghost width must be in 1..halo_size");
     if (narrays < 1 || narrays > 4) return fail(LCS_ERR_INVALID, "1..4 arrays per ghost fill");
-    long X = full ? g.nx + 2 * r : g.nx, Y = full ? g.ny + 2 * r : g.ny;
-    long cz = 2L * r * X * Y, cy = 2L * r * g.nz * X, cx = 2L * r * g.ny * g.nz;
+    long cz, cy, cx;
+    ghost_counts(g, r, full, cz, cy, cx);
     long total = cz + cy + cx;
     Ptr4<T> A = {{nullptr, nullptr, nullptr, nullptr}}, Bs = A;
     for (int q = 0; q < narrays; q++) {
@@ -256,16 +280,11 @@ static RhsCoef<T> rhs_coef(const Box &b)
 
 // Each workgroup marches a TX x TY column up a chunk of zc planes.
 template <typename T, int TX, int TY, bool FUSED>
-__global__ void __launch_bounds__(TX *TY) k_rhs_tile(RhsArgs<T> a, DtRef dt)
+__device__ inline void rhs_tile_block(const RhsArgs<T> &a, int bx, int by, int bz, T rdt, T *lds)
 {
     using Tile = RhsTile<T, TX, TY>;
-    extern __shared__ __align__(16) unsigned char rhs_lds_raw[];
-    T *lds = reinterpret_cast<T *>(rhs_lds_raw);
     typename Tile::Thread th;
-    int bx, by, bz;
-    xcd_tile(bx, by, bz);
     Tile::init(a, bx, by, bz, threadIdx.x, th);
-    const T rdt = FUSED ? T(dt.factor * (*dt.src)) : T(0);
     Tile::prologue_fetch(a, th);
     Tile::prologue_store(a, lds, th);
     Tile::prologue_queues(a, th);
@@ -279,6 +298,16 @@ __global__ void __launch_bounds__(TX *TY) k_rhs_tile(RhsArgs<T> a, DtRef dt)
         if (z < z1) Tile::store(lds, z, th);
         __syncthreads();
     }
+}
+
+template <typename T, int TX, int TY, bool FUSED>
+__global__ void __launch_bounds__(TX *TY) k_rhs_tile(RhsArgs<T> a, DtRef dt)
+{
+    extern __shared__ __align__(16) unsigned char rhs_lds_raw[];
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const T rdt = FUSED ? T(dt.factor * (*dt.src)) : T(0);
+    rhs_tile_block<T, TX, TY, FUSED>(a, bx, by, bz, rdt, reinterpret_cast<T *>(rhs_lds_raw));
 }
 
 // Tile shapes: FP32 32x8 (39 KB of LDS, four workgroups per CU), FP64 32x4 (53 KB).
@@ -422,11 +451,6 @@ struct BufSets {
 // buffer set.  Two boxes share a launch only when no cell of one touches a
 // face of the other (the opposite face slabs of one axis), so the pair gives
 // the same result as the two passes in either order.
-__host__ __device__ inline int pass_count(const Box3 &r)
-{
-    if (r.hi[0] < r.lo[0] || r.hi[1] < r.lo[1] || r.hi[2] < r.lo[2]) return 0;
-    return ((r.hi[0] - r.lo[0] + 2) / 2) * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
-}
 
 template <typename T, int COLORING>
 __global__ void __launch_bounds__(256) k_hsmac_pass(Geom g, BufSets<T> bs, Box3 r0, Box3 r1, int n0, int n1,
@@ -474,34 +498,13 @@ __global__ void __launch_bounds__(256) k_hsmac_pass(Geom g, BufSets<T> bs, Box3 
 }
 
 // Fused red+black sweep over R, current -> other buffer set (the phase
-// functions and their schedule are in lcs_hsmac_tile.h).
+// functions and their schedule are in lcs_hsmac_tile.h).  Returns the
+// workgroup's div^2 sum (valid in thread 0).
 template <typename T, int COLORING, int TY>
-__global__ void __launch_bounds__(64 * TY) k_hsmac_sweep(Geom g, BufSets<T> bs, Box3 R, HsCoef<T> c, DtRef dt,
-                                                        DevScalars *S, double *part, int zc)
+__device__ inline double hsmac_sweep_block(const HsArgs<T> &a, int bx, int by, int bz, T *lds, double *red_lds)
 {
     using Tile = HsTile<T, COLORING, TY>;
-    extern __shared__ __align__(16) unsigned char hs_lds_raw[];
-    __shared__ double red_lds[16];
-    T *lds = reinterpret_cast<T *>(hs_lds_raw);
-    if (S->hs_done) return;
-    const bool par = S->hs_par != 0;
-    HsArgs<T> a;
-    a.g = g;
-    a.Us = par ? bs.b[0] : bs.a[0];
-    a.Vs = par ? bs.b[1] : bs.a[1];
-    a.Ws = par ? bs.b[2] : bs.a[2];
-    a.Ps = par ? bs.b[3] : bs.a[3];
-    a.Ud = par ? bs.a[0] : bs.b[0];
-    a.Vd = par ? bs.a[1] : bs.b[1];
-    a.Wd = par ? bs.a[2] : bs.b[2];
-    a.Pd = par ? bs.a[3] : bs.b[3];
-    a.R = R;
-    a.c = c;
-    a.dti = T(1.0 / (dt.factor * (*dt.src)));
-    a.zc = zc;
     typename Tile::Thread th;
-    int bx, by, bz;
-    xcd_tile(bx, by, bz);
     Tile::init(a, bx, by, bz, threadIdx.x, th);
     const int z0 = th.z0, z1 = th.z1;
     Tile::fetch(a, z0 - 2, th);
@@ -523,7 +526,40 @@ __global__ void __launch_bounds__(64 * TY) k_hsmac_sweep(Geom g, BufSets<T> bs, 
         th.pP = th.pPn;
         __syncthreads();
     }
-    const double s = block_sum(th.acc, red_lds);
+    return block_sum(th.acc, red_lds);
+}
+
+// the two buffer sets of a sweep: the parity says which one is current
+template <typename T>
+__device__ inline void hs_bind(HsArgs<T> &a, const BufSets<T> &bs, bool par)
+{
+    a.Us = par ? bs.b[0] : bs.a[0];
+    a.Vs = par ? bs.b[1] : bs.a[1];
+    a.Ws = par ? bs.b[2] : bs.a[2];
+    a.Ps = par ? bs.b[3] : bs.a[3];
+    a.Ud = par ? bs.a[0] : bs.b[0];
+    a.Vd = par ? bs.a[1] : bs.b[1];
+    a.Wd = par ? bs.a[2] : bs.b[2];
+    a.Pd = par ? bs.a[3] : bs.b[3];
+}
+
+template <typename T, int COLORING, int TY>
+__global__ void __launch_bounds__(64 * TY) k_hsmac_sweep(Geom g, BufSets<T> bs, Box3 R, HsCoef<T> c, DtRef dt,
+                                                        DevScalars *S, double *part, int zc)
+{
+    extern __shared__ __align__(16) unsigned char hs_lds_raw[];
+    __shared__ double red_lds[16];
+    if (S->hs_done) return;
+    HsArgs<T> a;
+    a.g = g;
+    hs_bind(a, bs, S->hs_par != 0);
+    a.R = R;
+    a.c = c;
+    a.dti = T(1.0 / (dt.factor * (*dt.src)));
+    a.zc = zc;
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const double s = hsmac_sweep_block<T, COLORING, TY>(a, bx, by, bz, reinterpret_cast<T *>(hs_lds_raw), red_lds);
     if (threadIdx.x == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
 }
 
@@ -822,13 +858,12 @@ int launch_hsmac(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int s
 // |interp_o4| per axis, then dt on the device.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_cfl(Geom g, const T *__restrict__ U, const T *__restrict__ V,
-                                            const T *__restrict__ W, double *part)
+__device__ inline void cfl_block(const Geom &g, const T *__restrict__ U, const T *__restrict__ V,
+                                 const T *__restrict__ W, int b, int nb, double m[3])
 {
-    __shared__ double lds[16];
     const long sy = g.sx, sz = g.sx * g.sy;
-    double m[3] = {0.0, 0.0, 0.0};
-    for_interior(g, [&](int, int, int, long p) {
+    m[0] = m[1] = m[2] = 0.0;
+    for_rows(g, b, nb, [&](int, int, int, long p) {
         T uc = interp_o4(U[p - 2], U[p - 1], U[p], U[p + 1]);
         T vc = interp_o4(V[p - 2 * sy], V[p - sy], V[p], V[p + sy]);
         T wc = interp_o4(W[p - 2 * sz], W[p - sz], W[p], W[p + sz]);
@@ -836,10 +871,43 @@ __global__ void __launch_bounds__(256) k_cfl(Geom g, const T *__restrict__ U, co
         m[1] = fmax(m[1], fabs((double)vc));
         m[2] = fmax(m[2], fabs((double)wc));
     });
+}
+
+__device__ inline int xcd_block()
+{
+    const int nb = gridDim.x;
+    const int per_x = nb / kXcds, rem = nb % kXcds, x = blockIdx.x % kXcds, i0 = blockIdx.x / kXcds;
+    return x < rem ? x * (per_x + 1) + i0 : rem * (per_x + 1) + (x - rem) * per_x + i0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_cfl(Geom g, const T *__restrict__ U, const T *__restrict__ V,
+                                            const T *__restrict__ W, double *part)
+{
+    __shared__ double lds[16];
+    double m[3];
+    cfl_block(g, U, V, W, xcd_block(), gridDim.x, m);
     for (int a = 0; a < 3; a++) {
         double s = block_max(m[a], lds);
         if (threadIdx.x == 0) part[(long)a * kMaxPartBlocks + blockIdx.x] = s;
     }
+}
+
+// dt from the per-axis maxima (src/flows/timestep.jl:56-64)
+__device__ inline void cfl_dt(DevScalars *S, const double mm[3], double cfl, double dtmax, double dx, double dy,
+                              double dz, double Re)
+{
+    double umax = mm[0] + 1e-10, vmax = mm[1] + 1e-10, wmax = mm[2] + 1e-10;
+    S->umax[0] = umax;
+    S->umax[1] = vmax;
+    S->umax[2] = wmax;
+    double dt = dtmax;
+    dt = fmin(dt, cfl * dx / umax);
+    dt = fmin(dt, cfl * dy / vmax);
+    dt = fmin(dt, cfl * dz / wmax);
+    dt = fmin(dt, 0.5 * ((dx / 2) * (dx / 2)) / (1 / Re));
+    S->dt_flow = dt;
+    S->dt = dt;
 }
 
 __global__ void k_cfl_finalize(const double *part, int nblk, double cfl, double dtmax, double dx, double dy,
@@ -852,19 +920,7 @@ __global__ void k_cfl_finalize(const double *part, int nblk, double cfl, double 
         for (int q = threadIdx.x; q < nblk; q += blockDim.x) m = fmax(m, part[(long)a * kMaxPartBlocks + q]);
         mm[a] = block_max(m, lds);
     }
-    if (threadIdx.x == 0) {
-        double umax = mm[0] + 1e-10, vmax = mm[1] + 1e-10, wmax = mm[2] + 1e-10;
-        S->umax[0] = umax;
-        S->umax[1] = vmax;
-        S->umax[2] = wmax;
-        double dt = dtmax;
-        dt = fmin(dt, cfl * dx / umax);
-        dt = fmin(dt, cfl * dy / vmax);
-        dt = fmin(dt, cfl * dz / wmax);
-        dt = fmin(dt, 0.5 * ((dx / 2) * (dx / 2)) / (1 / Re));
-        S->dt_flow = dt;
-        S->dt = dt;
-    }
+    if (threadIdx.x == 0) cfl_dt(S, mm, cfl, dtmax, dx, dy, dz, Re);
 }
 
 int launch_cfl(Box &b, double cfl, double dtmax)
@@ -891,13 +947,12 @@ int launch_cfl(Box &b, double cfl, double dtmax)
 // Constant-power linear forcing (src/flows/forcings/lf.jl:113-192)
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_lf_means(Geom g, const T *__restrict__ U, const T *__restrict__ V,
-                                                 const T *__restrict__ W, double *part)
+__device__ inline void lf_means_block(const Geom &g, const T *__restrict__ U, const T *__restrict__ V,
+                                      const T *__restrict__ W, int b, int nb, double s[4])
 {
-    __shared__ double lds[16];
     const long sy = g.sx, sz = g.sx * g.sy;
-    double s[4] = {0, 0, 0, 0};
-    for_interior(g, [&](int, int, int, long p) {
+    s[0] = s[1] = s[2] = s[3] = 0.0;
+    for_rows(g, b, nb, [&](int, int, int, long p) {
         T u = interp_o2(U[p - 1], U[p]);
         T v = interp_o2(V[p - sy], V[p]);
         T w = interp_o2(W[p - sz], W[p]);
@@ -907,10 +962,31 @@ __global__ void __launch_bounds__(256) k_lf_means(Geom g, const T *__restrict__ 
         s[2] += (double)V[p];
         s[3] += (double)W[p];
     });
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_lf_means(Geom g, const T *__restrict__ U, const T *__restrict__ V,
+                                                 const T *__restrict__ W, double *part)
+{
+    __shared__ double lds[16];
+    double s[4];
+    lf_means_block(g, U, V, W, xcd_block(), gridDim.x, s);
     for (int a = 0; a < 4; a++) {
         double r = block_sum(s[a], lds);
         if (threadIdx.x == 0) part[(long)a * kMaxPartBlocks + blockIdx.x] = r;
     }
+}
+
+// e and A = P0/(2e) from the four means (src/flows/forcings/lf.jl:120-160)
+__device__ inline void lf_coef(DevScalars *S, const double m[4], double len, double power)
+{
+    double u2 = m[0] / len, u = m[1] / len, v = m[2] / len, w = m[3] / len;
+    double e = 0.5 * (((u2 - u * u) - v * v) - w * w);
+    S->lf[0] = u2;
+    S->lf[1] = u;
+    S->lf[2] = v;
+    S->lf[3] = w;
+    S->lf[4] = power / (2 * e);
 }
 
 __global__ void k_lf_finalize(const double *part, int nblk, double len, double power, DevScalars *S)
@@ -922,26 +998,15 @@ __global__ void k_lf_finalize(const double *part, int nblk, double len, double p
         for (int q = threadIdx.x; q < nblk; q += blockDim.x) s += part[(long)a * kMaxPartBlocks + q];
         m[a] = block_sum(s, lds);
     }
-    if (threadIdx.x == 0) {
-        double u2 = m[0] / len, u = m[1] / len, v = m[2] / len, w = m[3] / len;
-        double e = 0.5 * (((u2 - u * u) - v * v) - w * w);
-        S->lf[0] = u2;
-        S->lf[1] = u;
-        S->lf[2] = v;
-        S->lf[3] = w;
-        S->lf[4] = power / (2 * e);
-    }
+    if (threadIdx.x == 0) lf_coef(S, m, len, power);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_lf_apply(Geom g, T *U, T *V, T *W, const DevScalars *S, DtRef dt,
-                                                 double *part)
+__device__ inline double lf_apply_block(const Geom &g, T *U, T *V, T *W, const DevScalars *S, T rdt, int b, int nb)
 {
-    __shared__ double lds[16];
     const T A = T(S->lf[4]), um = T(S->lf[1]), vm = T(S->lf[2]), wm = T(S->lf[3]);
-    const T rdt = T(dt.factor * (*dt.src));
     double sp = 0.0;
-    for_interior(g, [&](int, int, int, long p) {
+    for_rows(g, b, nb, [&](int, int, int, long p) {
         T u = U[p], v = V[p], w = W[p];
         T usrc = A * (u - um), vsrc = A * (v - vm), wsrc = A * (w - wm);
         T pw = (u * usrc + v * vsrc) + w * wsrc;
@@ -950,7 +1015,16 @@ __global__ void __launch_bounds__(256) k_lf_apply(Geom g, T *U, T *V, T *W, cons
         W[p] = w + wsrc * rdt;
         sp += (double)pw;
     });
-    double r = block_sum(sp, lds);
+    return sp;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_lf_apply(Geom g, T *U, T *V, T *W, const DevScalars *S, DtRef dt,
+                                                 double *part)
+{
+    __shared__ double lds[16];
+    const T rdt = T(dt.factor * (*dt.src));
+    double r = block_sum(lf_apply_block(g, U, V, W, S, rdt, xcd_block(), gridDim.x), lds);
     if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
@@ -1498,6 +1572,452 @@ static int init_spectrum_t(Box &b, uint64_t seed, double k0, double uprime)
 int launch_init_spectrum(Box &b, uint64_t seed, double k0, double uprime)
 {
     return b.esize == 8 ? init_spectrum_t<double>(b, seed, k0, uprime) : init_spectrum_t<float>(b, seed, k0, uprime);
+}
+
+// ===========================================================================
+// Batched kernels (lcs_batch_*): one launch covers every box of a batch.  A
+// work table maps each workgroup to (box, block of that box) or (box, tile);
+// pointers, sizes and coefficients come from the box's BoxRec at run time.
+// The per-box arithmetic is the single-box kernels' own device code.
+// ===========================================================================
+// arrays of a stage set: 0 -> (U, V, W, P), 1 -> (U2, V2, W2, P)
+__device__ inline void *set_array(const BoxRec &B, int set, int q) { return q < 3 ? B.f[3 * set + q] : B.f[6]; }
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_b_ghost_fill(const BoxRec *tab, const WorkItem *work, int set, int narr,
+                                                     int r, int full, int hsmac)
+{
+    const WorkItem w = work[blockIdx.x];
+    const BoxRec &B = tab[w.box];
+    if (hsmac && B.S->hs_done) return;
+    const bool alt = hsmac && B.S->hs_par;
+    const Geom g = B.g;
+    long cz, cy, cx;
+    ghost_counts(g, r, full, cz, cy, cx);
+    const long t = (long)w.blk * blockDim.x + threadIdx.x;
+    if (t >= cz + cy + cx) return;
+    long d, s;
+    ghost_cell(g, r, full, cz, cy, t, d, s);
+    for (int q = 0; q < narr; q++) {
+        T *A = (T *)(alt ? B.spare[q] : set_array(B, set, q));
+        A[d] = A[s];
+    }
+}
+
+int launch_b_ghost_fill(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *work, int nwork, int set,
+                        int narr, int r, int full, int hsmac)
+{
+    if (nwork == 0) return LCS_OK;
+    if (esize == 8)
+        hipLaunchKernelGGL(k_b_ghost_fill<double>, dim3(nwork), dim3(256), 0, s, tab, work, set, narr, r, full, hsmac);
+    else
+        hipLaunchKernelGGL(k_b_ghost_fill<float>, dim3(nwork), dim3(256), 0, s, tab, work, set, narr, r, full, hsmac);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+// CFL dt per box: rows work item blk of B.row_blocks, then one block per box
+template <typename T>
+__global__ void __launch_bounds__(256) k_b_cfl(const BoxRec *tab, const WorkItem *work)
+{
+    __shared__ double lds[16];
+    const WorkItem w = work[blockIdx.x];
+    const BoxRec &B = tab[w.box];
+    double m[3];
+    cfl_block(B.g, (const T *)B.f[0], (const T *)B.f[1], (const T *)B.f[2], w.blk, B.row_blocks, m);
+    for (int a = 0; a < 3; a++) {
+        double v = block_max(m[a], lds);
+        if (threadIdx.x == 0) B.part[(long)a * kMaxPartBlocks + w.blk] = v;
+    }
+}
+
+__global__ void k_b_cfl_fin(const BoxRec *tab, double cfl, double dtmax)
+{
+    __shared__ double lds[16];
+    const BoxRec &B = tab[blockIdx.x];
+    double mm[3];
+    for (int a = 0; a < 3; a++) {
+        double m = 0.0;
+        for (int q = threadIdx.x; q < B.row_blocks; q += blockDim.x) m = fmax(m, B.part[(long)a * kMaxPartBlocks + q]);
+        mm[a] = block_max(m, lds);
+    }
+    if (threadIdx.x == 0) cfl_dt(B.S, mm, cfl, dtmax, B.dx, B.dy, B.dz, B.Re);
+}
+
+int launch_b_cfl(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *rows, int nrows, int nbox, double cfl,
+                 double dtmax)
+{
+    if (esize == 8)
+        hipLaunchKernelGGL(k_b_cfl<double>, dim3(nrows), dim3(256), 0, s, tab, rows);
+    else
+        hipLaunchKernelGGL(k_b_cfl<float>, dim3(nrows), dim3(256), 0, s, tab, rows);
+    hipLaunchKernelGGL(k_b_cfl_fin, dim3(nbox), dim3(256), 0, s, tab, cfl, dtmax);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+// linear-power forcing per box (boxes with power 0 would divide by e: the
+// batch only forces boxes whose config asks for it, see lcs_batch.hip)
+template <typename T>
+__global__ void __launch_bounds__(256) k_b_lf_means(const BoxRec *tab, const WorkItem *work)
+{
+    __shared__ double lds[16];
+    const WorkItem w = work[blockIdx.x];
+    const BoxRec &B = tab[w.box];
+    double s4[4];
+    lf_means_block(B.g, (const T *)B.f[0], (const T *)B.f[1], (const T *)B.f[2], w.blk, B.row_blocks, s4);
+    for (int a = 0; a < 4; a++) {
+        double v = block_sum(s4[a], lds);
+        if (threadIdx.x == 0) B.part[(long)a * kMaxPartBlocks + w.blk] = v;
+    }
+}
+
+__global__ void k_b_lf_fin(const BoxRec *tab)
+{
+    __shared__ double lds[16];
+    const BoxRec &B = tab[blockIdx.x];
+    double m[4];
+    for (int a = 0; a < 4; a++) {
+        double v = 0.0;
+        for (int q = threadIdx.x; q < B.row_blocks; q += blockDim.x) v += B.part[(long)a * kMaxPartBlocks + q];
+        m[a] = block_sum(v, lds);
+    }
+    if (threadIdx.x == 0) lf_coef(B.S, m, (double)B.g.cells(), B.power);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_b_lf_apply(const BoxRec *tab, const WorkItem *work)
+{
+    __shared__ double lds[16];
+    const WorkItem w = work[blockIdx.x];
+    const BoxRec &B = tab[w.box];
+    const T rdt = T(B.S->dt);  // stage-2 dt
+    double v = block_sum(lf_apply_block(B.g, (T *)B.f[0], (T *)B.f[1], (T *)B.f[2], B.S, rdt, w.blk, B.row_blocks),
+                         lds);
+    if (threadIdx.x == 0) B.part[w.blk] = v;
+}
+
+__global__ void k_b_lf_mean(const BoxRec *tab)
+{
+    __shared__ double lds[16];
+    const BoxRec &B = tab[blockIdx.x];
+    double v = 0.0;
+    for (int q = threadIdx.x; q < B.row_blocks; q += blockDim.x) v += B.part[q];
+    v = block_sum(v, lds);
+    if (threadIdx.x == 0) B.S->p = v / (double)B.g.cells();
+}
+
+int launch_b_lf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *rows, int nrows, int nbox)
+{
+    if (esize == 8)
+        hipLaunchKernelGGL(k_b_lf_means<double>, dim3(nrows), dim3(256), 0, s, tab, rows);
+    else
+        hipLaunchKernelGGL(k_b_lf_means<float>, dim3(nrows), dim3(256), 0, s, tab, rows);
+    hipLaunchKernelGGL(k_b_lf_fin, dim3(nbox), dim3(256), 0, s, tab);
+    if (esize == 8)
+        hipLaunchKernelGGL(k_b_lf_apply<double>, dim3(nrows), dim3(256), 0, s, tab, rows);
+    else
+        hipLaunchKernelGGL(k_b_lf_apply<float>, dim3(nrows), dim3(256), 0, s, tab, rows);
+    hipLaunchKernelGGL(k_b_lf_mean, dim3(nbox), dim3(256), 0, s, tab);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+// fused RHS + update of one stage for every box
+template <typename T, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) k_b_rhs_tile(const BoxRec *tab, const TileItem *tiles, int stage)
+{
+    extern __shared__ __align__(16) unsigned char rhs_lds_raw[];
+    const TileItem t = tiles[blockIdx.x];
+    const BoxRec &B = tab[t.box];
+    RhsArgs<T> a;
+    a.g = B.g;
+    const int in = stage == 1 ? 0 : 3;
+    a.U = (const T *)B.f[in];
+    a.V = (const T *)B.f[in + 1];
+    a.W = (const T *)B.f[in + 2];
+    a.P = (const T *)B.f[6];
+    for (int q = 0; q < 3; q++) {
+        a.out[q] = (T *)B.f[(stage == 1 ? 3 : 0) + q];
+        a.base[q] = stage == 1 ? nullptr : (const T *)B.f[q];  // stage 1: base == input
+        a.di[q] = T(B.rhs_di[q]);
+        a.di2[q] = T(B.rhs_di2[q]);
+    }
+    a.Rei = T(B.rhs_Rei);
+    a.zc = B.rhs_zc;
+    const T rdt = T((stage == 1 ? 0.5 : 1.0) * B.S->dt);
+    rhs_tile_block<T, TX, TY, true>(a, t.bx, t.by, t.bz, rdt, reinterpret_cast<T *>(rhs_lds_raw));
+}
+
+template <typename T>
+static int b_rhs_t(hipStream_t s, const BoxRec *tab, const TileItem *tiles, int ntiles, int stage)
+{
+    constexpr int TX = RhsShape<T>::TX, TY = RhsShape<T>::TY;
+    using Tile = RhsTile<T, TX, TY>;
+    const size_t lds = Tile::LDS_ELEMS * sizeof(T);
+    static bool attr = false;
+    if (!attr) {
+        LCS_HIP(hipFuncSetAttribute((const void *)k_b_rhs_tile<T, TX, TY>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_b_rhs_tile<T, TX, TY>), dim3(ntiles), dim3(TX * TY), lds, s, tab, tiles, stage);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+int launch_b_rhs(hipStream_t s, int esize, const BoxRec *tab, const TileItem *tiles, int ntiles, int stage)
+{
+    if (ntiles == 0) return LCS_OK;
+    return esize == 8 ? b_rhs_t<double>(s, tab, tiles, ntiles, stage) : b_rhs_t<float>(s, tab, tiles, ntiles, stage);
+}
+
+template <typename T>
+__device__ inline void hs_coef(const BoxRec &B, HsCoef<T> &c)
+{
+    c.dxi = T(B.hs_dxi);
+    c.dyi = T(B.hs_dyi);
+    c.dzi = T(B.hs_dzi);
+    c.dl = T(B.hs_dl);
+    c.omega = T(B.hs_omega);
+}
+
+template <typename T, int COLORING, int TY>
+__global__ void __launch_bounds__(64 * TY) k_b_hsmac_sweep(const BoxRec *tab, const TileItem *tiles, int set, int kind,
+                                                          int slot, double dtfac)
+{
+    extern __shared__ __align__(16) unsigned char hs_lds_raw[];
+    __shared__ double red_lds[16];
+    const TileItem t = tiles[blockIdx.x];
+    const BoxRec &B = tab[t.box];
+    if (B.S->hs_done) return;
+    BufSets<T> bs;
+    for (int q = 0; q < 4; q++) {
+        bs.a[q] = (T *)set_array(B, set, q);
+        bs.b[q] = (T *)B.spare[q];
+    }
+    HsArgs<T> a;
+    a.g = B.g;
+    hs_bind(a, bs, B.S->hs_par != 0);
+    a.R = hs_region(B.g, kind);
+    hs_coef(B, a.c);
+    a.dti = T(1.0 / (dtfac * B.S->dt));
+    a.zc = B.hs_zc;
+    const double s = hsmac_sweep_block<T, COLORING, TY>(a, t.bx, t.by, t.bz, reinterpret_cast<T *>(hs_lds_raw), red_lds);
+    const int gx = (B.g.nx + 2 * B.g.h + 63) / 64, gy = (B.g.ny + 2 * B.g.h + TY - 1) / TY;
+    if (threadIdx.x == 0) B.part[(long)slot * kMaxPartBlocks + t.bx + gx * (t.by + gy * t.bz)] = s;
+}
+
+template <typename T, int COLORING>
+static int b_sweep_t(hipStream_t s, const BoxRec *tab, const TileItem *tiles, int ntiles, int set, int kind, int slot,
+                     double dtfac)
+{
+    constexpr int TY = 8;
+    using Tile = HsTile<T, COLORING, TY>;
+    const size_t lds = Tile::LDS_ELEMS * sizeof(T);
+    static bool attr = false;
+    if (!attr) {
+        LCS_HIP(hipFuncSetAttribute((const void *)k_b_hsmac_sweep<T, COLORING, TY>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_b_hsmac_sweep<T, COLORING, TY>), dim3(ntiles), dim3(Tile::NT), lds, s, tab, tiles, set, kind,
+                       slot, dtfac);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, const BoxRec *tab, const TileItem *tiles, int ntiles,
+                         int set, int kind, int slot, double dtfac)
+{
+    if (ntiles == 0) return LCS_OK;
+    if (esize == 8)
+        return coloring == LCS_COLORING_RED_BLACK
+                   ? b_sweep_t<double, LCS_COLORING_RED_BLACK>(s, tab, tiles, ntiles, set, kind, slot, dtfac)
+                   : b_sweep_t<double, LCS_COLORING_RED_BLACK_FAST>(s, tab, tiles, ntiles, set, kind, slot, dtfac);
+    return coloring == LCS_COLORING_RED_BLACK
+               ? b_sweep_t<float, LCS_COLORING_RED_BLACK>(s, tab, tiles, ntiles, set, kind, slot, dtfac)
+               : b_sweep_t<float, LCS_COLORING_RED_BLACK_FAST>(s, tab, tiles, ntiles, set, kind, slot, dtfac);
+}
+
+// one colour over both face slabs of one axis, in place on the current set
+template <typename T, int COLORING>
+__global__ void __launch_bounds__(256) k_b_hsmac_faces(const BoxRec *tab, const WorkItem *work, int set, int domain,
+                                                      int axis, int color, int slot, double dtfac)
+{
+    __shared__ double lds[16];
+    const WorkItem w = work[blockIdx.x];
+    const BoxRec &B = tab[w.box];
+    const DevScalars *S = B.S;
+    if (S->hs_done) return;
+    const bool alt = S->hs_par != 0;
+    T *U = (T *)(alt ? B.spare[0] : set_array(B, set, 0)), *V = (T *)(alt ? B.spare[1] : set_array(B, set, 1));
+    T *W = (T *)(alt ? B.spare[2] : set_array(B, set, 2)), *P = (T *)(alt ? B.spare[3] : set_array(B, set, 3));
+    HsCoef<T> c;
+    hs_coef(B, c);
+    const T dti = T(1.0 / (dtfac * S->dt));
+    const Geom g = B.g;
+    const long sy = g.sx, sz = g.sx * g.sy;
+    Box3 r0, r1;
+    hs_faces(g, domain, axis, r0, r1);
+    const int n0 = pass_count(r0), n1 = pass_count(r1);
+    double acc = 0.0;
+    const int t = w.blk * blockDim.x + threadIdx.x;
+    if (t < n0 + n1) {
+        const bool second = t >= n0;
+        const Box3 r = second ? r1 : r0;
+        const int u = second ? t - n0 : t;
+        const int a0 = r.lo[0];
+        const int hw = (r.hi[0] - r.lo[0] + 2) / 2;
+        const int ny = r.hi[1] - r.lo[1] + 1;
+        const int ih = u % hw, q = u / hw;
+        const int j = r.lo[1] + q % ny, k = r.lo[2] + q / ny;
+        const int i = a0 + 2 * ih + ((j + k - a0 + color) & 1);  // foraxes.jl:100-105
+        if (i <= r.hi[0] && visited<COLORING>(r, i, j, k)) {
+            const long p = g.at(i, j, k);
+            T dudx = c.dxi * (-U[p - 1] + U[p]);
+            T dvdy = c.dyi * (-V[p - sy] + V[p]);
+            T dwdz = c.dzi * (-W[p - sz] + W[p]);
+            T div = (dudx + dvdy) + dwdz;
+            T dp = (-c.omega * div) * c.dl;
+            P[p] += dp * dti;
+            U[p - 1] -= dp * c.dxi;
+            U[p] += dp * c.dxi;
+            V[p - sy] -= dp * c.dyi;
+            V[p] += dp * c.dyi;
+            W[p - sz] -= dp * c.dzi;
+            W[p] += dp * c.dzi;
+            if (in_d3(g, i, j, k)) acc += (double)(div * div);
+        }
+    }
+    const double s = block_sum(acc, lds);
+    if (threadIdx.x == 0) B.part[(long)slot * kMaxPartBlocks + w.blk - 0] = s;
+}
+
+int launch_b_hsmac_faces(hipStream_t s, int esize, int coloring, const BoxRec *tab, const WorkItem *work, int nwork,
+                         int set, int domain, int axis, int color, int slot, double dtfac)
+{
+    if (nwork == 0) return LCS_OK;
+#define B_FACES(T, C)                                                                                                \
+    hipLaunchKernelGGL((k_b_hsmac_faces<T, C>), dim3(nwork), dim3(256), 0, s, tab, work, set, domain, axis, color, \
+                       slot, dtfac)
+    if (esize == 8) {
+        if (coloring == LCS_COLORING_RED_BLACK)
+            B_FACES(double, LCS_COLORING_RED_BLACK);
+        else
+            B_FACES(double, LCS_COLORING_RED_BLACK_FAST);
+    } else {
+        if (coloring == LCS_COLORING_RED_BLACK)
+            B_FACES(float, LCS_COLORING_RED_BLACK);
+        else
+            B_FACES(float, LCS_COLORING_RED_BLACK_FAST);
+    }
+#undef B_FACES
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+// RMS-div test per box over the passes since the last check (hsmac.jl:118-128)
+struct BCheckKinds {
+    int k[kMaxSweepPasses];
+};
+
+__global__ void k_b_hsmac_check(const BoxRec *tab, BCheckKinds kinds, int npass, double epsp, int itrmin)
+{
+    __shared__ double lds[16];
+    const BoxRec &B = tab[blockIdx.x];
+    DevScalars *S = B.S;
+    if (S->hs_done) return;
+    double acc = 0.0;
+    for (int q = 0; q < npass; q++) {
+        const int kd = kinds.k[q];
+        const int cnt = kd == 0 ? B.hs_tiles : B.face_blocks[kd - 1];
+        const double *pp = B.part + (long)q * kMaxPartBlocks;
+        for (int m = threadIdx.x; m < cnt; m += blockDim.x) acc += pp[m];
+    }
+    const double s = block_sum(acc, lds);
+    if (threadIdx.x == 0) {
+        const Geom &g = B.g;
+        const double len3 = (double)(g.nx + 1) * (double)(g.ny + 1) * (double)(g.nz + 1);
+        const double dxmin = fmin(fmin(B.dx, B.dy), B.dz);
+        const double div = sqrt(s / len3);
+        const int itr = S->hs_itr;
+        S->hs_div = div;
+        if (div * dxmin < epsp && itrmin <= itr)
+            S->hs_done = 1;
+        else
+            S->hs_itr = itr + 1;
+    }
+}
+
+int launch_b_hsmac_check(hipStream_t s, const BoxRec *tab, int nbox, const int *kinds, int npass, double epsp,
+                         int itrmin)
+{
+    if (npass > kMaxSweepPasses) return fail(LCS_ERR_INVALID, "too many passes per check");
+    BCheckKinds k = {};
+    for (int q = 0; q < npass; q++) k.k[q] = kinds[q];
+    hipLaunchKernelGGL(k_b_hsmac_check, dim3(nbox), dim3(256), 0, s, tab, k, npass, epsp, itrmin);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+// begin / flip / end(slot) / commit(set) / poll, one thread per box
+__global__ void k_b_hsmac_ctl(BoxRec *tab, int nbox, int op, int arg, HsPoll *poll)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    BoxRec &B = tab[b];
+    DevScalars *S = B.S;
+    if (op == kHsBegin) {
+        S->hs_done = 0;
+        S->hs_itr = 0;
+        S->hs_div = 0.0;
+        S->hs_par = 0;
+    } else if (op == kHsFlip) {
+        if (!S->hs_done) S->hs_par ^= 1;
+    } else if (op == kHsEnd) {
+        S->itr[arg] = S->hs_itr;
+        S->div[arg] = S->hs_div;
+    } else if (op == kHsCommit) {
+        if (S->hs_par) {  // the result lives in the spares: swap them in
+            for (int q = 0; q < 4; q++) {
+                const int f = q < 3 ? 3 * arg + q : 6;
+                void *t = B.f[f];
+                B.f[f] = B.spare[q];
+                B.spare[q] = t;
+            }
+        }
+    } else if (op == kHsPoll) {
+        poll[b].done = S->hs_done;
+        poll[b].par = S->hs_par;
+        poll[b].itr = S->hs_itr;
+        poll[b].div = S->hs_div;
+    }
+}
+
+int launch_b_hsmac_ctl(hipStream_t s, BoxRec *tab, int nbox, int op, int arg, HsPoll *poll)
+{
+    hipLaunchKernelGGL(k_b_hsmac_ctl, dim3((nbox + 63) / 64), dim3(64), 0, s, tab, nbox, op, arg, poll);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+// evolve!(state) per box, and a copy of every box's scalars for the logs
+__global__ void k_b_state(const BoxRec *tab, int nbox, DevScalars *gather)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    DevScalars *S = tab[b].S;
+    S->step += 1;
+    S->t += S->dt;
+    if (gather) gather[b] = *S;
+}
+
+int launch_b_state(hipStream_t s, const BoxRec *tab, int nbox, DevScalars *gather)
+{
+    hipLaunchKernelGGL(k_b_state, dim3((nbox + 63) / 64), dim3(64), 0, s, tab, nbox, gather);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
 }
 
 }  // namespace lcs

@@ -65,6 +65,40 @@ __host__ __device__ inline bool in_d3(const Geom &g, int x, int y, int z)
     return x >= 1 && x <= g.nx + 1 && y >= 1 && y <= g.ny + 1 && z >= 1 && z <= g.nz + 1;
 }
 
+// cells of one colour pass over r: cld(nx, 2) per row (foraxes.jl:95-107)
+__host__ __device__ inline int pass_count(const Box3 &r)
+{
+    if (r.hi[0] < r.lo[0] || r.hi[1] < r.lo[1] || r.hi[2] < r.lo[2]) return 0;
+    return ((r.hi[0] - r.lo[0] + 2) / 2) * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
+}
+
+// HSMAC regions of a box: 0 core (2:n), 1 d1 (-1:n+3), 2 d2 (0:n+2), 3 d3 (1:n+1)
+// (hsmac.jl:97-100, 146-152)
+__host__ __device__ inline Box3 hs_region(const Geom &g, int kind)
+{
+    const int lo = kind == 0 ? 2 : (kind == 1 ? -1 : (kind == 2 ? 0 : 1));
+    const int up = kind == 0 ? 0 : (kind == 1 ? 3 : (kind == 2 ? 2 : 1));
+    Box3 r = {{lo, lo, lo}, {g.nx + up, g.ny + up, g.nz + up}};
+    return r;
+}
+
+// the two face slabs of one axis (0: z, 1: y, 2: x) of the overlap order
+// around the core (2:n)^3 within region `domain` (lib/topologies/src/halo.jl:349-366)
+__host__ __device__ inline void hs_faces(const Geom &g, int domain, int axis, Box3 &f0, Box3 &f1)
+{
+    const Box3 d = hs_region(g, domain);
+    if (axis == 0) {
+        f0 = {{d.lo[0], d.lo[1], d.lo[2]}, {d.hi[0], d.hi[1], 1}};
+        f1 = {{d.lo[0], d.lo[1], g.nz + 1}, {d.hi[0], d.hi[1], d.hi[2]}};
+    } else if (axis == 1) {
+        f0 = {{d.lo[0], d.lo[1], 2}, {d.hi[0], 1, g.nz}};
+        f1 = {{d.lo[0], g.ny + 1, 2}, {d.hi[0], d.hi[1], g.nz}};
+    } else {
+        f0 = {{d.lo[0], 2, 2}, {1, g.ny, g.nz}};
+        f1 = {{g.nx + 1, 2, 2}, {d.hi[0], g.ny, g.nz}};
+    }
+}
+
 template <typename T>
 struct HsArgs {
     Geom g;

@@ -126,6 +126,8 @@ struct Profiler {
     std::vector<hipEvent_t> pool;
 };
 
+struct Batch;
+
 struct Box {
     Ctx *ctx = nullptr;
     lcs_box_desc desc;
@@ -143,7 +145,8 @@ struct Box {
     std::vector<ParticleProfile> profiles;
     Profiler prof;
     size_t bytes = 0;
-    int64_t steps_host = 0;  // lcs_step calls on this box
+    int64_t steps_host = 0;  // steps taken by this box (lcs_step or a batch)
+    Batch *batch = nullptr;  // the batch that steps this box, if any
     double Re() const { return desc.U0_m_s * desc.L0_m / desc.NU_m2_s; }
 };
 
@@ -208,6 +211,78 @@ int launch_p_localize(Box &b, ParticleProfile &pp);
 int launch_p_loc_move(Box &b, ParticleProfile &pp, const void *src, void *dst, size_t esize, bool scatter);
 // puts every property back into logical order and switches the map off
 int launch_p_unlocalize(Box &b, ParticleProfile &pp);
+
+// ---- batches of boxes (lcs_batch_*) ----------------------------------------
+// A batch advances many independent boxes with one launch per phase.  Every
+// batched kernel reads its box's pointers, sizes and coefficients from a
+// device-resident BoxRec at run time, so the HSMAC ping-pong commit (a device
+// kernel) can swap buffers without the host and a captured step graph stays
+// valid.  Coefficients are stored as doubles holding the values rounded to
+// the field type on the host, exactly as the single-box launchers form them.
+struct DevProfile {
+    long nvalid, ntotal;
+    void *props[LCS_N_PROPERTIES];
+    double mdx, mdy, mdz, mRe, mg, mkap;  // MotionCoef (field-type values)
+    double kap;                           // taup/d^2 in double (particle dt)
+    int drag;
+    int blk0, nblk;    // particle blocks of this profile in the box's p_stage work (nvalid)
+    int dblk0, dnblk;  // blocks of this profile in the box's max-diameter work (ntotal)
+};
+struct BoxRec {
+    Geom g;
+    void *f[7];      // U V W U2 V2 W2 P, where they currently live
+    void *spare[4];  // ping-pong partners
+    DevScalars *S;
+    double *part;
+    double Re, power, dx, dy, dz;
+    double rhs_di[3], rhs_di2[3], rhs_Rei;
+    double hs_dxi, hs_dyi, hs_dzi, hs_dl, hs_omega;
+    int rhs_zc, hs_zc, hs_tx, hs_ty;
+    int hs_tiles;        // workgroups of one fused sweep
+    int face_blocks[3];  // blocks of one paired face pass, axes z, y, x
+    int row_blocks;      // blocks of the row-walking reductions (CFL, forcing)
+    int nprof;
+    DevProfile prof[kMaxProfiles];
+};
+struct WorkItem {
+    int box, blk;
+};
+struct TileItem {
+    int box, bx, by, bz;
+};
+// per-box HSMAC state read back by the host after each poll
+struct HsPoll {
+    int done, par, itr, pad;
+    double div;
+};
+constexpr int kBatchRowBlocks = 64;  // row-walking reduction blocks per batched box
+
+// batched launchers (lcs_flow.hip / lcs_particles.hip); all enqueue on `s`
+int launch_b_ghost_fill(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *work, int nwork, int set,
+                        int narr, int r, int full, int hsmac);
+int launch_b_cfl(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *rows, int nrows, int nbox,
+                 double cfl, double dtmax);
+int launch_b_lf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *rows, int nrows, int nbox);
+int launch_b_rhs(hipStream_t s, int esize, const BoxRec *tab, const TileItem *tiles, int ntiles, int stage);
+int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, const BoxRec *tab, const TileItem *tiles,
+                         int ntiles, int set, int kind, int slot, double dtfac);
+int launch_b_hsmac_faces(hipStream_t s, int esize, int coloring, const BoxRec *tab, const WorkItem *work,
+                         int nwork, int set, int domain, int axis, int color, int slot, double dtfac);
+// kinds of the passes since the last check: 0 = fused sweep, 1..3 = face pass of axis z, y, x
+int launch_b_hsmac_check(hipStream_t s, const BoxRec *tab, int nbox, const int *kinds, int npass, double epsp,
+                         int itrmin);
+int launch_b_hsmac_ctl(hipStream_t s, BoxRec *tab, int nbox, int op, int arg, HsPoll *poll);
+int launch_b_particles(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *work, int nwork, int stage);
+int launch_b_particle_dt(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *work, int nwork, int nbox,
+                         double dtmax);
+int launch_b_state(hipStream_t s, const BoxRec *tab, int nbox, DevScalars *gather);
+// HSMAC control ops for launch_b_hsmac_ctl
+enum { kHsBegin = 0, kHsFlip = 1, kHsEnd = 2, kHsCommit = 3, kHsPoll = 4 };
+
+// a box leaving its batch (lcs_box_destroy); a change of the box's buffers
+// that the batch's device table must pick up
+int batch_detach_box(Box &b);
+void batch_mark_dirty(Box &b);
 
 // ---- counter-based RNG shared with the oracle ------------------------------
 __host__ __device__ inline uint64_t mix64(uint64_t z)

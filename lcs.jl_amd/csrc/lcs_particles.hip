@@ -184,13 +184,10 @@ __global__ void k_p_update(long n, DtRef dt, T *xs, T *ys, T *zs, const T *xs0, 
 // xss2 = xss + dt uss, uss2 = uss + dt f.  Stage 2: wrap xss2 (shadow xss),
 // forces from Us2 at xss2/uss2, xss = xss + dt uss, uss = uss + dt f.
 template <typename T>
-__global__ void k_p_stage(Geom g, const T *U, const T *V, const T *W, MotionCoef<T> c, long n, int stage,
-                          DtRef dt, T *xs, T *ys, T *zs, T *xs2, T *ys2, T *zs2, T *us, T *vs, T *ws, T *us2,
-                          T *vs2, T *ws2, const T *diams, T *dudts, T *dvdts, T *dwdts)
+__device__ inline void p_stage_one(const Geom &g, const T *U, const T *V, const T *W, const MotionCoef<T> &c, long p,
+                                   int stage, T rdt, T *xs, T *ys, T *zs, T *xs2, T *ys2, T *zs2, T *us, T *vs,
+                                   T *ws, T *us2, T *vs2, T *ws2, const T *diams, T *dudts, T *dvdts, T *dwdts)
 {
-    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    T rdt = T(dt.factor * (*dt.src));
     T x = xs[p], y = ys[p], z = zs[p], x2 = xs2[p], y2 = ys2[p], z2 = zs2[p];
     T du, dv, dw;
     if (stage == 1) {
@@ -228,6 +225,17 @@ __global__ void k_p_stage(Geom g, const T *U, const T *V, const T *W, MotionCoef
     dudts[p] = du;
     dvdts[p] = dv;
     dwdts[p] = dw;
+}
+
+template <typename T>
+__global__ void k_p_stage(Geom g, const T *U, const T *V, const T *W, MotionCoef<T> c, long n, int stage,
+                          DtRef dt, T *xs, T *ys, T *zs, T *xs2, T *ys2, T *zs2, T *us, T *vs, T *ws, T *us2,
+                          T *vs2, T *ws2, const T *diams, T *dudts, T *dvdts, T *dwdts)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    p_stage_one(g, U, V, W, c, p, stage, T(dt.factor * (*dt.src)), xs, ys, zs, xs2, ys2, zs2, us, vs, ws, us2, vs2,
+                ws2, diams, dudts, dvdts, dwdts);
 }
 
 template <typename T>
@@ -848,6 +856,107 @@ int launch_p_init(Box &b, ParticleProfile &pp, uint64_t seed, double st_min, dou
 {
     return b.esize == 8 ? p_init_t<double>(b, pp, seed, st_min, st_max, eps, zero_slip)
                         : p_init_t<float>(b, pp, seed, st_min, st_max, eps, zero_slip);
+}
+
+// ---------------------------------------------------------------------------
+// Batched particle kernels (lcs_batch_*): work item (box, blk) covers 256
+// particles of one profile; each box lists its profiles' blocks back to back.
+// ---------------------------------------------------------------------------
+__device__ inline int b_profile(const BoxRec &B, int blk, bool diam)
+{
+    for (int q = 0; q < B.nprof; q++) {
+        const int b0 = diam ? B.prof[q].dblk0 : B.prof[q].blk0, nb = diam ? B.prof[q].dnblk : B.prof[q].nblk;
+        if (blk >= b0 && blk < b0 + nb) return q;
+    }
+    return -1;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_b_p_stage(const BoxRec *tab, const WorkItem *work, int stage)
+{
+    const WorkItem w = work[blockIdx.x];
+    const BoxRec &B = tab[w.box];
+    const int q = b_profile(B, w.blk, false);
+    if (q < 0) return;
+    const DevProfile &P = B.prof[q];
+    const long p = (long)(w.blk - P.blk0) * blockDim.x + threadIdx.x;
+    if (p >= P.nvalid) return;
+    MotionCoef<T> c;
+    c.dx = T(P.mdx);
+    c.dy = T(P.mdy);
+    c.dz = T(P.mdz);
+    c.Re = T(P.mRe);
+    c.g = T(P.mg);
+    c.kap = T(P.mkap);
+    c.drag = P.drag;
+    const int in = stage == 1 ? 0 : 3;
+    T *const *pr = (T *const *)P.props;
+    p_stage_one(B.g, (const T *)B.f[in], (const T *)B.f[in + 1], (const T *)B.f[in + 2], c, p, stage,
+                T((stage == 1 ? 0.5 : 1.0) * B.S->dt), pr[LCS_PROP_X], pr[LCS_PROP_Y], pr[LCS_PROP_Z],
+                pr[LCS_PROP_X2], pr[LCS_PROP_Y2], pr[LCS_PROP_Z2], pr[LCS_PROP_U], pr[LCS_PROP_V], pr[LCS_PROP_W],
+                pr[LCS_PROP_U2], pr[LCS_PROP_V2], pr[LCS_PROP_W2], (const T *)pr[LCS_PROP_DIAM], pr[LCS_PROP_DUDT],
+                pr[LCS_PROP_DVDT], pr[LCS_PROP_DWDT]);
+}
+
+int launch_b_particles(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *work, int nwork, int stage)
+{
+    if (nwork == 0) return LCS_OK;
+    if (esize == 8)
+        hipLaunchKernelGGL(k_b_p_stage<double>, dim3(nwork), dim3(256), 0, s, tab, work, stage);
+    else
+        hipLaunchKernelGGL(k_b_p_stage<float>, dim3(nwork), dim3(256), 0, s, tab, work, stage);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+// max diameter over ntotal per profile (partials in slot 3 + profile), then
+// the particle dt per profile and the step dt per box
+template <typename T>
+__global__ void __launch_bounds__(256) k_b_p_maxdiam(const BoxRec *tab, const WorkItem *work)
+{
+    __shared__ double lds[16];
+    const WorkItem w = work[blockIdx.x];
+    const BoxRec &B = tab[w.box];
+    const int q = b_profile(B, w.blk, true);
+    if (q < 0) return;
+    const DevProfile &P = B.prof[q];
+    const long p = (long)(w.blk - P.dblk0) * blockDim.x + threadIdx.x;
+    double m = p < P.ntotal ? (double)((const T *)P.props[LCS_PROP_DIAM])[p] : -INFINITY;
+    m = block_max_d(m, lds);
+    if (threadIdx.x == 0) B.part[(3L + q) * kMaxPartBlocks + (w.blk - P.dblk0)] = m;
+}
+
+__global__ void k_b_p_dt(const BoxRec *tab, double dtmax)
+{
+    __shared__ double lds[16];
+    const BoxRec &B = tab[blockIdx.x];
+    DevScalars *S = B.S;
+    for (int q = 0; q < B.nprof; q++) {
+        const DevProfile &P = B.prof[q];
+        double m = -INFINITY;
+        for (int k = threadIdx.x; k < P.dnblk; k += blockDim.x) m = fmax(m, B.part[(3L + q) * kMaxPartBlocks + k]);
+        m = block_max_d(m, lds);
+        if (threadIdx.x == 0) S->dtp[q] = fmin(0.5 * (P.kap * (m * m)), dtmax);
+    }
+    if (threadIdx.x == 0) {
+        double dt = S->dt_flow;
+        for (int q = 0; q < B.nprof; q++) dt = fmin(dt, S->dtp[q]);
+        S->dt = dt;
+    }
+}
+
+int launch_b_particle_dt(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *work, int nwork, int nbox,
+                         double dtmax)
+{
+    if (nwork > 0) {
+        if (esize == 8)
+            hipLaunchKernelGGL(k_b_p_maxdiam<double>, dim3(nwork), dim3(256), 0, s, tab, work);
+        else
+            hipLaunchKernelGGL(k_b_p_maxdiam<float>, dim3(nwork), dim3(256), 0, s, tab, work);
+    }
+    hipLaunchKernelGGL(k_b_p_dt, dim3(nbox), dim3(256), 0, s, tab, dtmax);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
 }
 
 }  // namespace lcs

@@ -105,6 +105,13 @@ SIGNATURES = {
     "lcs_init_particles": [P, I, C.c_uint64, D, D, D, I],
     "lcs_step": [P, C.POINTER(StepConfig), C.POINTER(StepLog)],
     "lcs_step_log_read": [P, C.POINTER(StepLog)],
+    "lcs_batch_create": [P, C.POINTER(P)],
+    "lcs_batch_destroy": [P],
+    "lcs_batch_add": [P, P, C.POINTER(StepConfig)],
+    "lcs_batch_remove": [P, P],
+    "lcs_batch_size": [P, C.POINTER(I)],
+    "lcs_batch_step": [P, P, I],
+    "lcs_batch_sync": [P],
     "lcs_profile": [P, I],
     "lcs_profile_read": [P, I, C.c_char_p, C.c_size_t, C.POINTER(D), C.POINTER(C.c_int64), C.POINTER(D)],
     "lcs_sync": [P],

@@ -288,6 +288,61 @@ def step(box, scfg, want_log=True):
     return log_dict(log)
 
 
+class Batch:
+    """An ensemble of independent boxes advanced together, one kernel launch
+    per phase for all of them (lcs_batch_*).  No counterpart in the reference,
+    whose ensembles are separate runs; each box advances as ``step`` would
+    advance it alone, and a box's result does not depend on the other boxes."""
+
+    def __init__(self, device=0):
+        self.L = _lib.load()
+        h = C.c_void_p()
+        check(self.L.lcs_batch_create(context(device), C.byref(h)))
+        self.handle = h
+        self.boxes = []
+        self._cfgs = []
+
+    def add(self, box, scfg):
+        check(self.L.lcs_batch_add(self.handle, box.handle, C.byref(scfg)))
+        self.boxes.append(box)
+        self._cfgs.append(scfg)
+
+    def remove(self, box):
+        check(self.L.lcs_batch_remove(self.handle, box.handle))
+        i = self.boxes.index(box)
+        del self.boxes[i], self._cfgs[i]
+
+    def __len__(self):
+        n = C.c_int()
+        check(self.L.lcs_batch_size(self.handle, C.byref(n)))
+        return n.value
+
+    def step(self, want_log=True):
+        """One timestep! + evolve! of every box; returns one log per box."""
+        n = len(self)
+        if not want_log:
+            check(self.L.lcs_batch_step(self.handle, None, 0))
+            return None
+        logs = (_lib.StepLog * max(n, 1))()
+        check(self.L.lcs_batch_step(self.handle, logs, n))
+        return [log_dict(logs[i]) for i in range(n)]
+
+    def sync(self):
+        check(self.L.lcs_batch_sync(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.L.lcs_batch_destroy(self.handle)
+            self.handle = None
+            self.boxes, self._cfgs = [], []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def read_log(box):
     log = _lib.StepLog()
     check(box.L.lcs_step_log_read(box.handle, C.byref(log)))

@@ -1,0 +1,99 @@
+"""Batched ensembles (lcs_batch_*): boxes of uneven size, particle counts and
+forcing power advanced with one launch per phase.
+
+* every box of a mixed batch follows the oracle's phase-by-phase restatement
+  of timestep! + evolve! for that box alone (same bar as the single-box step
+  test: sweep counts equal per step, fields within relative L2 1e-8 in FP64);
+* a box's result does not depend on the mix: alone in a batch and among other
+  boxes it is the same bit for bit (per-box reductions use per-box partials);
+* adding, removing and destroying boxes keeps the batch consistent.
+"""
+import numpy as np
+import pytest
+
+from conftest import interior, rel_l2
+from test_gpu_step import c1_config, oracle_from_device
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_configs(dtype="f64"):
+    from lcs_mi355x import config as C
+    cfgs = []
+    for dims, npart, power, profiles in [((24, 24, 24), 500, 0.5, 1), ((32, 16, 24), 1200, 0.3, 2),
+                                         ((16, 16, 16), 0, 0.7, 0)]:
+        cfg = c1_config(dims=dims, nparticles=max(npart, 1), forcing=C.ConstantPowerLF(power), dtype=dtype,
+                        profiles=profiles)
+        cfgs.append(cfg)
+    return cfgs
+
+
+def test_batched_boxes_follow_the_oracle():
+    import lcs_mi355x as L
+    cfgs = mixed_configs()
+    boxes = [L.build(c) for c in cfgs]
+    sims = [oracle_from_device(c, b) for c, b in zip(cfgs, boxes)]
+    batch = L.Batch()
+    for c, b in zip(cfgs, boxes):
+        batch.add(b, L.step_config(c, localize_interval=3))
+    for s in range(6):
+        logs = batch.step()
+        for q, (sim, log) in enumerate(zip(sims, logs)):
+            sim.timestep()
+            olog = sim.evolve()
+            assert log["flow"]["couple1"]["itr"] == olog["flow"]["couple1"]["itr"], (s, q)
+            assert log["flow"]["couple2"]["itr"] == olog["flow"]["couple2"]["itr"], (s, q)
+            assert np.isclose(log["dt"], sim.dt, rtol=1e-12), (s, q)
+            assert np.isclose(log["flow"]["forcing"]["p"], olog["flow"]["forcing"]["p"], rtol=1e-7), (s, q)
+    for q, (sim, b) in enumerate(zip(sims, boxes)):
+        for name, ref in zip(("U", "V", "W", "P"), (*sim.Us, sim.P)):
+            assert rel_l2(interior(b.download(name), 3), interior(ref, 3)) <= 1e-8, (q, name)
+    batch.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_box_result_does_not_depend_on_the_mix(dtype):
+    import lcs_mi355x as L
+    cfgs = mixed_configs(dtype)
+    alone = L.build(cfgs[0])
+    mix = [L.build(c) for c in cfgs]
+    b1, b2 = L.Batch(), L.Batch()
+    b1.add(alone, L.step_config(cfgs[0], localize_interval=2))
+    for c, b in zip(cfgs, mix):
+        b2.add(b, L.step_config(c, localize_interval=2))
+    for s in range(5):
+        la = b1.step()[0]
+        lm = b2.step()[0]
+        assert la == lm, s
+    for name in ("U", "V", "W", "P"):
+        np.testing.assert_array_equal(mix[0].download(name), alone.download(name), err_msg=name)
+    for k in ("ids", "x", "y", "z", "u", "v", "w"):
+        np.testing.assert_array_equal(mix[0].download_prop(0, k), alone.download_prop(0, k), err_msg=k)
+    b1.close()
+    b2.close()
+
+
+def test_batch_membership_changes():
+    import lcs_mi355x as L
+    from lcs_mi355x._lib import LCSError
+    cfgs = mixed_configs()
+    boxes = [L.build(c) for c in cfgs]
+    batch = L.Batch()
+    for c, b in zip(cfgs, boxes):
+        batch.add(b, L.step_config(c))
+    assert len(batch) == 3
+    batch.step()
+    with pytest.raises(LCSError):  # a batched box is stepped by its batch only
+        L.step(boxes[1], L.step_config(cfgs[1]))
+    # leave, step alone, come back: states keep advancing consistently
+    batch.remove(boxes[1])
+    assert len(batch) == 2
+    log = L.step(boxes[1], L.step_config(cfgs[1]))
+    assert log["step"] == 2
+    batch.add(boxes[1], L.step_config(cfgs[1]))
+    logs = batch.step()
+    assert [l["step"] for l in logs] == [2, 2, 3]
+    boxes[0].close()  # destroying a box detaches it
+    assert len(batch) == 2
+    batch.step()
+    batch.close()
