@@ -78,6 +78,91 @@ def copy_calibration(nbytes):
     return nbytes
 
 
+def ensemble_configs(nbox, seed=12345):
+    """Batched-ensemble workload: nbox FP32 periodic boxes, N uniform over
+    {32, 48, 64, 96, 128}, particle counts log-uniform in [1e3, 1e6] with Stokes
+    numbers log-uniform in [0.1, 10], seeded spectra with k0 from {2, 4, 6},
+    linear-power forcing P = 0.5 (Re scaled with N as Re = 53.3 (N/64)^(4/3))."""
+    import numpy as np
+    from lcs_mi355x import config as C
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nbox):
+        n = int(rng.choice([32, 48, 64, 96, 128]))
+        npart = int(round(10 ** rng.uniform(3.0, 6.0)))
+        k0 = float(rng.choice([2.0, 4.0, 6.0]))
+        Re = 53.3 * (n / 64.0) ** (4.0 / 3.0)
+        part = (C.ParticleConfig(C.Population(npart), C.ParticleParams(0.0, 843.0), 4, "linear",
+                                 C.ParticleInit(seed=2000 + i, velocity="zero-slip", size="stokes", st_min=0.1,
+                                                st_max=10.0, eps=0.5)),)
+        out.append(C.Config(grid=C.Grid((n, n, n), 3), timestep=C.TimeStep(0.3, 0.01), simulate=C.SimulateConfig(0),
+                            flow=C.FlowConfig(C.FlowParams(1.0, 1.0, 1.0 / Re), C.SpectrumInit(1000 + i, k0, 1.0),
+                                              C.HSMAC(True, True, "red-black-fast", 1.7, 1e-3, 6, 300),
+                                              C.ConstantPowerLF(0.5)),
+                            particles=part, dtype="f32"))
+    return out
+
+
+def bench_ensemble(args):
+    """Box-steps/s of the batched ensemble against the same boxes stepped one
+    at a time with lcs_step on the same GPU."""
+    import lcs_mi355x as L
+    cfgs = ensemble_configs(args.boxes)
+    t0 = time.perf_counter()
+    boxes = [L.build(c) for c in cfgs]
+    for b in boxes:
+        b.sync()
+    print(f"# built {len(boxes)} boxes in {time.perf_counter() - t0:.1f} s, "
+          f"{sum(b.nbytes() for b in boxes) / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+    scfgs = [L.step_config(c, localize_interval=args.localize_interval) for c in cfgs]
+    cells = sum(math.prod(c.grid.dims) for c in cfgs)
+    # serial: every box alone, one lcs_step at a time
+    for b, s in zip(boxes, scfgs):
+        L.step(b, s, want_log=False)
+        b.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.serial_steps):
+        for b, s in zip(boxes, scfgs):
+            L.step(b, s, want_log=False)
+    for b in boxes:
+        b.sync()
+    serial_s = time.perf_counter() - t0
+    serial = len(boxes) * args.serial_steps / serial_s
+    print(f"# serial: {serial:.1f} box-steps/s", file=sys.stderr, flush=True)
+    batch = L.Batch()
+    for b, s in zip(boxes, scfgs):
+        batch.add(b, s)
+    for _ in range(args.warmup):
+        batch.step(want_log=False)
+    batch.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.step(want_log=False)
+    batch.sync()
+    el = time.perf_counter() - t0
+    value = len(boxes) * args.steps / el
+    print(f"# batched: {value:.1f} box-steps/s ({value / serial:.2f}x serial)", file=sys.stderr, flush=True)
+    ns = sorted({c.grid.dims[0] for c in cfgs})
+    out = {
+        "metric": METRIC, "value": value, "unit": "box-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded spectra and particles per box)",
+        "config": {"workload": f"batched ensemble of {len(boxes)} FP32 boxes, N in {ns}, particles log-uniform "
+                               f"[1e3, 1e6], St log-uniform [0.1, 10], k0 in {{2, 4, 6}}, linear-power P=0.5",
+                   "boxes": len(boxes), "cells": cells,
+                   "particles": sum(c.particles[0].population.valid for c in cfgs),
+                   "particle_localize_interval": args.localize_interval},
+        "cell_steps_per_s": cells * args.steps / el,
+        "serial": {"value": serial, "unit": "box-steps/s", "steps_per_box": args.serial_steps,
+                   "how": "the same boxes, one lcs_step call per box at a time, same GPU"},
+        "speedup_vs_serial": value / serial,
+    }
+    batch.close()
+    for b in boxes:
+        b.close()
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(name, n, steps):
     """Time the CPU oracle on a bounded sample (n^3 box, same parameters)."""
     import numpy as np
@@ -113,7 +198,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--boxes", type=int, default=96, help="ensemble size (workload c4)")
+    ap.add_argument("--serial-steps", type=int, default=3, help="steps per box of the serial baseline (c4)")
     ap.add_argument("--size", type=int, default=None, help="override the grid size (testing only)")
     ap.add_argument("--cpu-size", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=30)
@@ -129,6 +216,8 @@ def main():
               f"paths shard by box across GPUs, which is not part of this build", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "c4":
+        return bench_ensemble(args)
     import lcs_mi355x as L
     cfg, desc = workload(args.workload, n_override=args.size)
     t0 = time.perf_counter()

@@ -52,7 +52,7 @@ struct Batch {
     std::vector<BoxRec> tab_h;
     DevVec<BoxRec> tab;
     DevVec<WorkItem> ghost_face, ghost_full, ghost_one, rows, parts, diams, faces[3];
-    DevVec<TileItem> rhs_tiles, hs_tiles;
+    DevVec<TileItem> rhs_tiles, hs_tiles[2];  // sweep tiles 64 and 32 wide
     HsPoll *poll_d = nullptr, *poll_h = nullptr;
     DevScalars *sc_d = nullptr, *sc_h = nullptr;
     int cap_boxes = 0;
@@ -76,6 +76,15 @@ static std::vector<E> xcd_order(const std::vector<E> &v)
 }
 
 static int blocks_of(long n, int per = 256) { return (int)((n + per - 1) / per); }
+
+// Width of the fused-sweep tiles of a box: the one that wastes fewer lanes
+// on the array's x extent (the results do not depend on it).
+static int sweep_width(const Batch &, const Geom &g)
+{
+    const int X = g.nx + 2 * g.h;
+    const double e64 = (double)X / (64.0 * ((X + 63) / 64)), e32 = (double)X / (32.0 * ((X + 31) / 32));
+    return e32 > e64 + 0.05 ? 32 : 64;
+}
 
 template <typename T>
 static void fill_coefs(const Box &b, const lcs_hsmac_params &hs, BoxRec &r)
@@ -153,7 +162,7 @@ static int rebuild(Batch &B)
     }
     B.tab_h.assign(n, BoxRec{});
     std::vector<WorkItem> gface, gfull, gone, rows, parts, diams, faces[3];
-    std::vector<TileItem> rtiles, htiles;
+    std::vector<TileItem> rtiles, htiles[2];
     const int domain = B.hs.time_blocking ? 1 : 3;
     for (int i = 0; i < n; i++) {
         Box &b = *B.boxes[i];
@@ -181,18 +190,18 @@ static int rebuild(Batch &B)
         for (int z = 0; z * r.rhs_zc < g.nz; z++)
             for (int y = 0; y < rgy; y++)
                 for (int x = 0; x < rgx; x++) rtiles.push_back({i, x, y, z});
-        // fused HSMAC sweep tiles: 64 x 8 over the whole array
+        // fused HSMAC sweep tiles: TX x 8 over the whole array, TX chosen per size
         const int zplanes = g.nz + 2 * g.h;
-        const int hgx = (g.nx + 2 * g.h + 63) / 64, hgy = (g.ny + 2 * g.h + 7) / 8;
+        r.hs_tx = sweep_width(B, g);
+        r.hs_ty = 8;
+        const int hgx = (g.nx + 2 * g.h + r.hs_tx - 1) / r.hs_tx, hgy = (g.ny + 2 * g.h + 7) / 8;
         const int hgz = std::max(1, zplanes / 64);
         r.hs_zc = (zplanes + hgz - 1) / hgz;
-        r.hs_tx = 64;
-        r.hs_ty = 8;
         int nz_used = (zplanes + r.hs_zc - 1) / r.hs_zc;
         r.hs_tiles = hgx * hgy * nz_used;
         for (int z = 0; z < nz_used; z++)
             for (int y = 0; y < hgy; y++)
-                for (int x = 0; x < hgx; x++) htiles.push_back({i, x, y, z});
+                for (int x = 0; x < hgx; x++) htiles[r.hs_tx == 64 ? 0 : 1].push_back({i, x, y, z});
         // face passes of the overlap order
         for (int a = 0; a < 3; a++) {
             Box3 f0, f1;
@@ -243,7 +252,9 @@ static int rebuild(Batch &B)
     if ((rc = B.ghost_face.assign(xcd_order(gface), B.stream)) || (rc = B.ghost_full.assign(xcd_order(gfull), B.stream)) ||
         (rc = B.ghost_one.assign(xcd_order(gone), B.stream)) || (rc = B.rows.assign(xcd_order(rows), B.stream)) ||
         (rc = B.parts.assign(xcd_order(parts), B.stream)) || (rc = B.diams.assign(xcd_order(diams), B.stream)) ||
-        (rc = B.rhs_tiles.assign(xcd_order(rtiles), B.stream)) || (rc = B.hs_tiles.assign(xcd_order(htiles), B.stream)))
+        (rc = B.rhs_tiles.assign(xcd_order(rtiles), B.stream)) ||
+        (rc = B.hs_tiles[0].assign(xcd_order(htiles[0]), B.stream)) ||
+        (rc = B.hs_tiles[1].assign(xcd_order(htiles[1]), B.stream)))
         return rc;
     for (int a = 0; a < 3; a++)
         if ((rc = B.faces[a].assign(xcd_order(faces[a]), B.stream))) return rc;
@@ -261,9 +272,11 @@ static int b_fill(Batch &B, int set, int narr, int width, int hsmac)
 
 static int b_sweep(Batch &B, int set, int kind, int slot, double dtfac)
 {
-    int rc = launch_b_hsmac_sweep(B.stream, B.esize, B.hs.coloring, B.tab.d, B.hs_tiles.d, B.hs_tiles.n, set, kind,
-                                  slot, dtfac);
-    if (rc) return rc;
+    int rc;
+    for (int w = 0; w < 2; w++)
+        if ((rc = launch_b_hsmac_sweep(B.stream, B.esize, B.hs.coloring, w == 0 ? 64 : 32, B.tab.d, B.hs_tiles[w].d,
+                                       B.hs_tiles[w].n, set, kind, slot, dtfac)))
+            return rc;
     return launch_b_hsmac_ctl(B.stream, B.tab.d, (int)B.boxes.size(), kHsFlip, 0, nullptr);
 }
 
@@ -492,7 +505,8 @@ int lcs_batch_destroy(lcs_batch *batch)
                              &B->faces[0],   &B->faces[1],   &B->faces[2]};
     for (auto *v : w) v->release();
     B->rhs_tiles.release();
-    B->hs_tiles.release();
+    B->hs_tiles[0].release();
+    B->hs_tiles[1].release();
     B->tab.release();
     if (B->poll_d) (void)hipFree(B->poll_d);
     if (B->poll_h) (void)hipHostFree(B->poll_h);

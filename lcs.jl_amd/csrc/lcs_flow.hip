@@ -500,30 +500,24 @@ __global__ void __launch_bounds__(256) k_hsmac_pass(Geom g, BufSets<T> bs, Box3 
 // Fused red+black sweep over R, current -> other buffer set (the phase
 // functions and their schedule are in lcs_hsmac_tile.h).  Returns the
 // workgroup's div^2 sum (valid in thread 0).
-template <typename T, int COLORING, int TY>
+template <typename T, int COLORING, int TY, int TX>
 __device__ inline double hsmac_sweep_block(const HsArgs<T> &a, int bx, int by, int bz, T *lds, double *red_lds)
 {
-    using Tile = HsTile<T, COLORING, TY>;
+    using Tile = HsTile<T, COLORING, TY, TX>;
     typename Tile::Thread th;
     Tile::init(a, bx, by, bz, threadIdx.x, th);
-    const int z0 = th.z0, z1 = th.z1;
-    Tile::fetch(a, z0 - 2, th);
-    Tile::store(lds, z0 - 2, th);
-    Tile::fetch(a, z0 - 1, th);
-    Tile::store(lds, z0 - 1, th);
+    Tile::prologue(a, lds, th);
     __syncthreads();
-    for (int q = z0 - 1; q <= z1 + 3; q++) {
-        if (q + 1 <= z1 + 2) Tile::fetch(a, q + 1, th);
-        if (q - 2 >= z0 && q - 2 <= z1) Tile::fetch_p(a, q - 2, th);
-        if (q <= z1 + 2) Tile::red(a, lds, q, th);
+    const int q1 = th.z1 + 3;
+    for (int q = th.z0 - 1; q <= q1; q += 2) {
+        Tile::step_a(a, lds, q, th, 0);
         __syncthreads();
-        if (q - 1 >= z0 && q - 1 <= z1 + 1) Tile::black(a, lds, q - 1, th);
-        if (q - 3 >= z0) Tile::output(a, lds, q - 3, th);
-        if (q + 1 <= z1 + 2)
-            Tile::store(lds, q + 1, th);
-        else
-            Tile::shift(th);
-        th.pP = th.pPn;
+        Tile::step_b(a, lds, q, th, 0);
+        __syncthreads();
+        if (q + 1 > q1) break;
+        Tile::step_a(a, lds, q + 1, th, 1);
+        __syncthreads();
+        Tile::step_b(a, lds, q + 1, th, 1);
         __syncthreads();
     }
     return block_sum(th.acc, red_lds);
@@ -559,7 +553,7 @@ __global__ void __launch_bounds__(64 * TY) k_hsmac_sweep(Geom g, BufSets<T> bs, 
     a.zc = zc;
     int bx, by, bz;
     xcd_tile(bx, by, bz);
-    const double s = hsmac_sweep_block<T, COLORING, TY>(a, bx, by, bz, reinterpret_cast<T *>(hs_lds_raw), red_lds);
+    const double s = hsmac_sweep_block<T, COLORING, TY, 64>(a, bx, by, bz, reinterpret_cast<T *>(hs_lds_raw), red_lds);
     if (threadIdx.x == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
 }
 
@@ -1782,9 +1776,9 @@ __device__ inline void hs_coef(const BoxRec &B, HsCoef<T> &c)
     c.omega = T(B.hs_omega);
 }
 
-template <typename T, int COLORING, int TY>
-__global__ void __launch_bounds__(64 * TY) k_b_hsmac_sweep(const BoxRec *tab, const TileItem *tiles, int set, int kind,
-                                                          int slot, double dtfac)
+template <typename T, int COLORING, int TY, int TX>
+__global__ void __launch_bounds__(TX *TY) k_b_hsmac_sweep(const BoxRec *tab, const TileItem *tiles, int set, int kind,
+                                                         int slot, double dtfac)
 {
     extern __shared__ __align__(16) unsigned char hs_lds_raw[];
     __shared__ double red_lds[16];
@@ -1803,41 +1797,50 @@ __global__ void __launch_bounds__(64 * TY) k_b_hsmac_sweep(const BoxRec *tab, co
     hs_coef(B, a.c);
     a.dti = T(1.0 / (dtfac * B.S->dt));
     a.zc = B.hs_zc;
-    const double s = hsmac_sweep_block<T, COLORING, TY>(a, t.bx, t.by, t.bz, reinterpret_cast<T *>(hs_lds_raw), red_lds);
-    const int gx = (B.g.nx + 2 * B.g.h + 63) / 64, gy = (B.g.ny + 2 * B.g.h + TY - 1) / TY;
+    const double s =
+        hsmac_sweep_block<T, COLORING, TY, TX>(a, t.bx, t.by, t.bz, reinterpret_cast<T *>(hs_lds_raw), red_lds);
+    const int gx = (B.g.nx + 2 * B.g.h + TX - 1) / TX, gy = (B.g.ny + 2 * B.g.h + TY - 1) / TY;
     if (threadIdx.x == 0) B.part[(long)slot * kMaxPartBlocks + t.bx + gx * (t.by + gy * t.bz)] = s;
 }
 
-template <typename T, int COLORING>
+template <typename T, int COLORING, int TX>
 static int b_sweep_t(hipStream_t s, const BoxRec *tab, const TileItem *tiles, int ntiles, int set, int kind, int slot,
                      double dtfac)
 {
     constexpr int TY = 8;
-    using Tile = HsTile<T, COLORING, TY>;
+    using Tile = HsTile<T, COLORING, TY, TX>;
     const size_t lds = Tile::LDS_ELEMS * sizeof(T);
     static bool attr = false;
     if (!attr) {
-        LCS_HIP(hipFuncSetAttribute((const void *)k_b_hsmac_sweep<T, COLORING, TY>,
+        LCS_HIP(hipFuncSetAttribute((const void *)k_b_hsmac_sweep<T, COLORING, TY, TX>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         attr = true;
     }
-    hipLaunchKernelGGL((k_b_hsmac_sweep<T, COLORING, TY>), dim3(ntiles), dim3(Tile::NT), lds, s, tab, tiles, set, kind,
-                       slot, dtfac);
+    hipLaunchKernelGGL((k_b_hsmac_sweep<T, COLORING, TY, TX>), dim3(ntiles), dim3(Tile::NT), lds, s, tab, tiles, set,
+                       kind, slot, dtfac);
     LCS_HIP(hipGetLastError());
     return LCS_OK;
 }
 
-int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, const BoxRec *tab, const TileItem *tiles, int ntiles,
-                         int set, int kind, int slot, double dtfac)
+template <typename T, int TX>
+static int b_sweep_c(hipStream_t s, int coloring, const BoxRec *tab, const TileItem *tiles, int ntiles, int set,
+                     int kind, int slot, double dtfac)
+{
+    return coloring == LCS_COLORING_RED_BLACK
+               ? b_sweep_t<T, LCS_COLORING_RED_BLACK, TX>(s, tab, tiles, ntiles, set, kind, slot, dtfac)
+               : b_sweep_t<T, LCS_COLORING_RED_BLACK_FAST, TX>(s, tab, tiles, ntiles, set, kind, slot, dtfac);
+}
+
+int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, int tx, const BoxRec *tab, const TileItem *tiles,
+                         int ntiles, int set, int kind, int slot, double dtfac)
 {
     if (ntiles == 0) return LCS_OK;
+    if (tx != 32 && tx != 64) return fail(LCS_ERR_INVALID, "sweep tiles are 32 or 64 wide");
     if (esize == 8)
-        return coloring == LCS_COLORING_RED_BLACK
-                   ? b_sweep_t<double, LCS_COLORING_RED_BLACK>(s, tab, tiles, ntiles, set, kind, slot, dtfac)
-                   : b_sweep_t<double, LCS_COLORING_RED_BLACK_FAST>(s, tab, tiles, ntiles, set, kind, slot, dtfac);
-    return coloring == LCS_COLORING_RED_BLACK
-               ? b_sweep_t<float, LCS_COLORING_RED_BLACK>(s, tab, tiles, ntiles, set, kind, slot, dtfac)
-               : b_sweep_t<float, LCS_COLORING_RED_BLACK_FAST>(s, tab, tiles, ntiles, set, kind, slot, dtfac);
+        return tx == 32 ? b_sweep_c<double, 32>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac)
+                        : b_sweep_c<double, 64>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac);
+    return tx == 32 ? b_sweep_c<float, 32>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac)
+                    : b_sweep_c<float, 64>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac);
 }
 
 // one colour over both face slabs of one axis, in place on the current set

@@ -8,14 +8,15 @@
 // two in-place passes apply them -- is formed by exactly one thread, and the
 // whole sweep costs one read and one write of (U, V, W, P).
 //
-// A workgroup owns a 64 x TY column of cells and a chunk of planes [z0, z1]
+// A workgroup owns a TX x TY column of cells and a chunk of planes [z0, z1]
 // and marches up in z.  At step q:
-//   phase A  fetch plane q+1 into registers; red dp on plane q over x in
+//   phase A  issue the loads of plane q+2 into registers (two planes in
+//            flight hide the memory latency); red dp on plane q over x in
 //            [X0-1, X1+2], y in [Y0-1, Y1+2] (the red neighbours of the black
 //            cells below);
 //   phase B  black dp on plane q-1 over x in [X0, X1+1], y in [Y0, Y1+1];
 //            final U, V, W, P of plane q-3 for the owned cells; plane q+1
-//            goes into LDS.
+//            (loaded during the previous step) goes into LDS.
 // LDS holds U and V for planes q-1..q+1, W for q-2..q+1 over x in
 // [X0-2, X1+2], y in [Y0-2, Y1+2], and one dp plane per z for q-3..q in
 // which red cells hold red dp and black cells black dp.  A thread keeps its
@@ -110,9 +111,9 @@ struct HsArgs {
     int zc;
 };
 
-template <typename T, int COLORING, int TY_>
+template <typename T, int COLORING, int TY_, int TX_ = 64>
 struct HsTile {
-    static constexpr int TX = 64, TY = TY_, NT = TX * TY;
+    static constexpr int TX = TX_, TY = TY_, NT = TX * TY;
     static constexpr int LX = TX + 4, LY = TY + 4, PLANE = LX * LY;  // fields: x in [X0-2, X1+2]
     static constexpr int DXN = TX + 3, DYN = TY + 3, DPL = DXN * DYN;  // dp: x in [X0-1, X1+2]
     static constexpr int OFF_V = 3 * PLANE, OFF_W = 6 * PLANE, OFF_D = 10 * PLANE;
@@ -129,7 +130,7 @@ struct HsTile {
         long h_off[NHS];  // their in-plane offsets, -1 outside the array
         long o_off;     // in-plane offset of the owned cell, -1 outside the array
         int r_row, r_k, b_row, b_k;
-        T pf[3][1 + NHS];  // fetched plane: own cell, halo elements
+        T pf[2][3][1 + NHS];  // two fetched planes in flight: own cell, halo elements
         T own[3][4];    // own cell's U, V, W at planes q-3..q
         T pP, pPn;      // P of the owned cell at planes q-3 and q-2
         double acc;
@@ -193,45 +194,46 @@ struct HsTile {
         th.acc = 0.0;
     }
 
-    __host__ __device__ static void fetch(const HsArgs<T> &a, int q, Thread &th)
+    // loads of plane q into register slot s
+    __host__ __device__ static void fetch(const HsArgs<T> &a, int q, Thread &th, int s)
     {
         const bool zin = plane_in(a.g, q);
         const long base = plane_base(a.g, q);
-        for (int s = 0; s <= NHS; s++) {
-            const long off = s == 0 ? th.o_off : th.h_off[s - 1];
+        for (int e = 0; e <= NHS; e++) {
+            const long off = e == 0 ? th.o_off : th.h_off[e - 1];
             const bool ok = zin && off >= 0;
             const long p = ok ? base + off : 0;
-            th.pf[0][s] = ok ? a.Us[p] : T(0);
-            th.pf[1][s] = ok ? a.Vs[p] : T(0);
-            th.pf[2][s] = ok ? a.Ws[p] : T(0);
+            th.pf[s][0][e] = ok ? a.Us[p] : T(0);
+            th.pf[s][1][e] = ok ? a.Vs[p] : T(0);
+            th.pf[s][2][e] = ok ? a.Ws[p] : T(0);
         }
     }
 
-    // plane q (fetched) into LDS and into the own-cell queue
-    __host__ __device__ static void store(T *lds, int q, Thread &th)
+    // plane q (register slot s) into LDS and into the own-cell queue
+    __host__ __device__ static void store(T *lds, int q, Thread &th, int s)
     {
         const int e = (th.tid / TX + 2) * LX + th.tid % TX + 2;
-        Uf(lds, q, 0, e) = th.pf[0][0];
-        Vf(lds, q, 0, e) = th.pf[1][0];
-        Wf(lds, q, 0, e) = th.pf[2][0];
-        for (int s = 0; s < NHS; s++) {
-            const int h = th.h_lds[s];
+        Uf(lds, q, 0, e) = th.pf[s][0][0];
+        Vf(lds, q, 0, e) = th.pf[s][1][0];
+        Wf(lds, q, 0, e) = th.pf[s][2][0];
+        for (int k = 0; k < NHS; k++) {
+            const int h = th.h_lds[k];
             if (h < 0) continue;
-            Uf(lds, q, 0, h) = th.pf[0][s + 1];
-            Vf(lds, q, 0, h) = th.pf[1][s + 1];
-            Wf(lds, q, 0, h) = th.pf[2][s + 1];
+            Uf(lds, q, 0, h) = th.pf[s][0][k + 1];
+            Vf(lds, q, 0, h) = th.pf[s][1][k + 1];
+            Wf(lds, q, 0, h) = th.pf[s][2][k + 1];
         }
-        shift(th);
+        shift(th, s);
     }
 
-    // advance the own-cell queue by one plane (the new top is the fetched plane)
-    __host__ __device__ static void shift(Thread &th)
+    // advance the own-cell queue by one plane (the new top comes from slot s)
+    __host__ __device__ static void shift(Thread &th, int s)
     {
         for (int f = 0; f < 3; f++) {
             th.own[f][0] = th.own[f][1];
             th.own[f][1] = th.own[f][2];
             th.own[f][2] = th.own[f][3];
-            th.own[f][3] = th.pf[f][0];
+            th.own[f][3] = th.pf[s][f][0];
         }
     }
 
@@ -331,36 +333,53 @@ struct HsTile {
         a.Wd[gi] = w;
     }
 
+    // One step of the march (the kernel runs it with s alternating 0, 1 so
+    // the register slots are compile-time constants).  SYNC is a workgroup
+    // barrier in the kernel; the host harness runs every thread's part A,
+    // then every thread's part B.
+    __host__ __device__ static void step_a(const HsArgs<T> &a, T *lds, int q, Thread &th, int s)
+    {
+        const int z0 = th.z0, z1 = th.z1;
+        if (q + 2 <= z1 + 2) fetch(a, q + 2, th, s ^ 1);
+        if (q - 2 >= z0 && q - 2 <= z1) fetch_p(a, q - 2, th);
+        if (q <= z1 + 2) red(a, lds, q, th);
+    }
+    __host__ __device__ static void step_b(const HsArgs<T> &a, T *lds, int q, Thread &th, int s)
+    {
+        const int z0 = th.z0, z1 = th.z1;
+        if (q - 1 >= z0 && q - 1 <= z1 + 1) black(a, lds, q - 1, th);
+        if (q - 3 >= z0) output(a, lds, q - 3, th);
+        if (q + 1 <= z1 + 2)
+            store(lds, q + 1, th, s);
+        else
+            shift(th, s);
+        th.pP = th.pPn;
+    }
+    // before the first step: planes z0-2 and z0-1 in LDS, plane z0 in flight (slot 0)
+    __host__ __device__ static void prologue(const HsArgs<T> &a, T *lds, Thread &th)
+    {
+        fetch(a, th.z0 - 2, th, 0);
+        store(lds, th.z0 - 2, th, 0);
+        fetch(a, th.z0 - 1, th, 0);
+        store(lds, th.z0 - 1, th, 0);
+        fetch(a, th.z0, th, 0);
+    }
+
     // The march of one workgroup for a host harness: the threads of each
-    // phase run one after the other; the kernel has a barrier where a phase
-    // ends.
+    // part run one after the other.
     static double emulate_block(const HsArgs<T> &a, T *lds, Thread *th, int bx, int by, int bz)
     {
         for (int t = 0; t < NT; t++) init(a, bx, by, bz, t, th[t]);
         const int z0 = th[0].z0, z1 = th[0].z1;
-        for (int t = 0; t < NT; t++) {
-            fetch(a, z0 - 2, th[t]);
-            store(lds, z0 - 2, th[t]);
-            fetch(a, z0 - 1, th[t]);
-            store(lds, z0 - 1, th[t]);
+        for (int t = 0; t < NT; t++) prologue(a, lds, th[t]);
+        int s = 0;
+        for (int q = z0 - 1; q <= z1 + 3; q++, s ^= 1) {
+            for (int t = 0; t < NT; t++) step_a(a, lds, q, th[t], s);
+            for (int t = 0; t < NT; t++) step_b(a, lds, q, th[t], s);
         }
-        for (int q = z0 - 1; q <= z1 + 3; q++) {
-            for (int t = 0; t < NT; t++) {
-                if (q + 1 <= z1 + 2) fetch(a, q + 1, th[t]);
-                if (q - 2 >= z0 && q - 2 <= z1) fetch_p(a, q - 2, th[t]);
-                if (q <= z1 + 2) red(a, lds, q, th[t]);
-            }
-            for (int t = 0; t < NT; t++) {
-                if (q - 1 >= z0 && q - 1 <= z1 + 1) black(a, lds, q - 1, th[t]);
-                if (q - 3 >= z0) output(a, lds, q - 3, th[t]);
-                if (q + 1 <= z1 + 2) store(lds, q + 1, th[t]);
-                else shift(th[t]);
-                th[t].pP = th[t].pPn;
-            }
-        }
-        double s = 0.0;
-        for (int t = 0; t < NT; t++) s += th[t].acc;
-        return s;
+        double sum = 0.0;
+        for (int t = 0; t < NT; t++) sum += th[t].acc;
+        return sum;
     }
 };
 

@@ -264,7 +264,8 @@ int launch_b_cfl(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *ro
                  double cfl, double dtmax);
 int launch_b_lf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *rows, int nrows, int nbox);
 int launch_b_rhs(hipStream_t s, int esize, const BoxRec *tab, const TileItem *tiles, int ntiles, int stage);
-int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, const BoxRec *tab, const TileItem *tiles,
+// tx: tile width of the fused sweep (32 or 64); the tiles listed must all be of that width
+int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, int tx, const BoxRec *tab, const TileItem *tiles,
                          int ntiles, int set, int kind, int slot, double dtfac);
 int launch_b_hsmac_faces(hipStream_t s, int esize, int coloring, const BoxRec *tab, const WorkItem *work,
                          int nwork, int set, int domain, int axis, int color, int slot, double dtfac);

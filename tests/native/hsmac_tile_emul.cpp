@@ -8,11 +8,11 @@
 
 using namespace lcs;
 
-template <typename T, int COLORING, int TY>
+template <typename T, int COLORING, int TY, int TX>
 static double run(int nx, int ny, int nz, int h, const int *lo, const int *hi, double dt, double omega,
                   void *const *src, void *const *dst, int zc)
 {
-    using Tile = HsTile<T, COLORING, TY>;
+    using Tile = HsTile<T, COLORING, TY, TX>;
     HsArgs<T> a;
     a.g = make_geom(nx, ny, nz, h);
     a.Us = (const T *)src[0];
@@ -46,25 +46,25 @@ static double run(int nx, int ny, int nz, int h, const int *lo, const int *hi, d
     return s;
 }
 
-extern "C" int hsmac_sweep_emul(int ty, int esize, int coloring, int nx, int ny, int nz, int h, const int *lo,
-                                const int *hi, double dt, double omega, void *const *src, void *const *dst, int zc,
-                                double *div2)
+extern "C" int hsmac_sweep_emul(int ty, int tx, int esize, int coloring, int nx, int ny, int nz, int h,
+                                const int *lo, const int *hi, double dt, double omega, void *const *src,
+                                void *const *dst, int zc, double *div2)
 {
-#define RUN(T, C, Y) *div2 = run<T, C, Y>(nx, ny, nz, h, lo, hi, dt, omega, src, dst, zc)
-#define BY_TY(T, C)               \
-    switch (ty) {                 \
-    case 4: RUN(T, C, 4); break;  \
-    case 8: RUN(T, C, 8); break;  \
-    default: return -1;           \
-    }
+#define RUN(T, C, Y, X) *div2 = run<T, C, Y, X>(nx, ny, nz, h, lo, hi, dt, omega, src, dst, zc)
+#define BY_SHAPE(T, C)                              \
+    if (ty == 4 && tx == 64) RUN(T, C, 4, 64);      \
+    else if (ty == 8 && tx == 64) RUN(T, C, 8, 64); \
+    else if (ty == 8 && tx == 32) RUN(T, C, 8, 32); \
+    else if (ty == 4 && tx == 32) RUN(T, C, 4, 32); \
+    else return -1;
     if (esize == 4) {
-        if (coloring == LCS_COLORING_RED_BLACK) { BY_TY(float, LCS_COLORING_RED_BLACK) }
-        else { BY_TY(float, LCS_COLORING_RED_BLACK_FAST) }
+        if (coloring == LCS_COLORING_RED_BLACK) { BY_SHAPE(float, LCS_COLORING_RED_BLACK) }
+        else { BY_SHAPE(float, LCS_COLORING_RED_BLACK_FAST) }
     } else {
-        if (coloring == LCS_COLORING_RED_BLACK) { BY_TY(double, LCS_COLORING_RED_BLACK) }
-        else { BY_TY(double, LCS_COLORING_RED_BLACK_FAST) }
+        if (coloring == LCS_COLORING_RED_BLACK) { BY_SHAPE(double, LCS_COLORING_RED_BLACK) }
+        else { BY_SHAPE(double, LCS_COLORING_RED_BLACK_FAST) }
     }
-#undef BY_TY
+#undef BY_SHAPE
 #undef RUN
     return 0;
 }

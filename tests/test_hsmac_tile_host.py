@@ -8,8 +8,8 @@ colour-0 pass followed by its colour-1 pass over R (hsmac.jl:1-35,
 lib/parallel/src/foraxes.jl:73-107) bit for bit -- every element of U, V, W,
 P, ghosts included -- and its div^2 sum over (1:n+1)^3 must equal the
 oracle's.  Boxes: the three time-blocking domains d1, d2, d3 and the overlap
-core (2:n)^3; ragged tiles in x and y, several z-chunkings, both colourings,
-both element types.
+core (2:n)^3; tiles 64 and 32 wide, ragged tiles in x and y, several
+z-chunkings, both colourings, both element types.
 """
 import ctypes as C
 import os
@@ -42,16 +42,18 @@ def _box(name, n):
     return np.asarray(lo, dtype=np.int32), np.asarray(hi, dtype=np.int32)
 
 
-CASES = [((8, 8, 8), 4, 14, "d1"), ((16, 12, 10), 4, 5, "d2"), ((70, 9, 7), 8, 3, "d3"), ((17, 5, 9), 8, 4, "core"),
-         ((20, 13, 11), 4, 1, "d1"), ((66, 20, 6), 8, 12, "d3"), ((5, 3, 4), 4, 2, "d1")]
+CASES = [((8, 8, 8), 4, 64, 14, "d1"), ((16, 12, 10), 4, 64, 5, "d2"), ((70, 9, 7), 8, 64, 3, "d3"),
+         ((17, 5, 9), 8, 64, 4, "core"), ((20, 13, 11), 4, 64, 1, "d1"), ((66, 20, 6), 8, 64, 12, "d3"),
+         ((5, 3, 4), 4, 64, 2, "d1"), ((40, 10, 9), 8, 32, 4, "d1"), ((27, 9, 6), 8, 32, 2, "core"),
+         ((33, 7, 5), 4, 32, 3, "d2")]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("coloring", [0, 1])
-@pytest.mark.parametrize("dims,ty,zc,box", CASES)
-def test_fused_sweep_equals_two_colour_passes(emul, dims, ty, zc, box, coloring, dtype):
+@pytest.mark.parametrize("dims,ty,tx,zc,box", CASES)
+def test_fused_sweep_equals_two_colour_passes(emul, dims, ty, tx, zc, box, coloring, dtype):
     h, dt, omega = 3, 0.01, 1.7
-    rng = np.random.default_rng(abs(hash((dims, ty, zc, box, coloring))) % 2**32)
+    rng = np.random.default_rng(abs(hash((dims, ty, tx, zc, box, coloring))) % 2**32)
     src = [np.asfortranarray(rng.standard_normal(tuple(n + 2 * h for n in dims)).astype(dtype)) for _ in range(4)]
     lo, hi = _box(box, dims)
     ref = [a.copy(order="F") for a in src]
@@ -65,7 +67,7 @@ def test_fused_sweep_equals_two_colour_passes(emul, dims, ty, zc, box, coloring,
     sp = (C.c_void_p * 4)(*[a.ctypes.data for a in src])
     dp = (C.c_void_p * 4)(*[a.ctypes.data for a in dst])
     s = C.c_double(0)
-    rc = emul.hsmac_sweep_emul(ty, np.dtype(dtype).itemsize, coloring, nx, ny, nz, h,
+    rc = emul.hsmac_sweep_emul(ty, tx, np.dtype(dtype).itemsize, coloring, nx, ny, nz, h,
                                lo.ctypes.data_as(C.c_void_p), hi.ctypes.data_as(C.c_void_p), C.c_double(dt),
                                C.c_double(omega), sp, dp, zc, C.byref(s))
     assert rc == 0
