@@ -251,9 +251,10 @@ int lcs_step_log_read(lcs_box *box, lcs_step_log *log);
  * with LCS_ERR_STATE); its own entry points for data (upload, download,
  * particles) keep working.  Each box keeps its own state, dt, step count and
  * HSMAC convergence, and advances exactly as lcs_step would advance it alone.
- * Boxes of one batch share the float type, halo size, cfl, dtmax, HSMAC
- * parameters and forcing kind (none or linear-power); sizes, particle
- * profiles and the forcing power may differ. */
+ * Boxes of one batch share the float type, halo size, cfl, dtmax and HSMAC
+ * parameters; sizes, particle profiles and forcing (kind and parameters) may
+ * differ.  Linear-power forcing runs batched; rc-energy boxes are forced one
+ * by one within the batched step. */
 typedef struct lcs_batch lcs_batch;
 int lcs_batch_create(lcs_ctx *ctx, lcs_batch **out);
 int lcs_batch_destroy(lcs_batch *batch); /* detaches every box first */
@@ -264,6 +265,57 @@ int lcs_batch_size(lcs_batch *batch, int *nboxes);
  * per box in the order of addition (at most nlogs) */
 int lcs_batch_step(lcs_batch *batch, lcs_step_log *logs, int nlogs);
 int lcs_batch_sync(lcs_batch *batch);
+
+/* ---- serving ---------------------------------------------------------------- */
+/* No counterpart in the reference.  A request loop over one batch: runs are
+ * submitted as requests, admitted in arrival order while a slot (max_live)
+ * and their device-memory footprint (memory_budget) fit, given their own box
+ * (seeded spectrum, one particle profile with log-uniform Stokes numbers,
+ * zero slip), and advanced together by lcs_server_step, one batched step per
+ * call.  A run that reaches its step count leaves the batch at once and its
+ * slot goes to the next request; its box stays readable (lcs_server_box)
+ * until lcs_server_release.  Times are seconds on the server's monotonic
+ * clock since lcs_server_create. */
+typedef struct lcs_server lcs_server;
+#define LCS_RUN_QUEUED 0
+#define LCS_RUN_LIVE 1
+#define LCS_RUN_DONE 2
+#define LCS_RUN_FAILED 3
+typedef struct {
+    int n;                /* cube edge: n^3 cells, halo 3 */
+    int dtype;            /* LCS_F32 / LCS_F64 */
+    int64_t particles;    /* particles of the run's one profile (0: flow only) */
+    double st_min, st_max; /* Stokes numbers, log-uniform */
+    int forcing;          /* LCS_FORCING_*: rc-energy uses kmin 1, kmax 2, coarse 8 */
+    double forcing_param; /* linear-power P0, or rc-energy energy */
+    int steps;            /* steps to run */
+    uint64_t seed;        /* spectrum and particle seed */
+    double k0;            /* spectrum peak wavenumber (u' = 1) */
+    double Re;            /* U0 L0 / nu with U0 = L0 = 1 */
+} lcs_request;
+typedef struct {
+    size_t memory_budget; /* bytes of device memory the live runs may hold */
+    int max_live;         /* runs in the batch at once */
+    double cfl, dtmax;
+    lcs_hsmac_params hsmac;
+    int localize_interval;
+} lcs_server_config;
+typedef struct {
+    int64_t id;
+    int status;           /* LCS_RUN_* */
+    int64_t steps_done;
+    double submit_s, start_s, finish_s;
+    size_t bytes;         /* device bytes held by the run */
+} lcs_run_status;
+int lcs_server_create(lcs_ctx *ctx, const lcs_server_config *cfg, lcs_server **out);
+int lcs_server_destroy(lcs_server *server);
+int lcs_server_submit(lcs_server *server, const lcs_request *req, int64_t *id);
+/* admit what fits, advance every live run one step, retire finished runs */
+int lcs_server_step(lcs_server *server, int *n_live, int *n_finished);
+int lcs_server_status(lcs_server *server, int64_t id, lcs_run_status *status);
+int lcs_server_counts(lcs_server *server, int *queued, int *live, size_t *memory_used);
+int lcs_server_box(lcs_server *server, int64_t id, lcs_box **box);
+int lcs_server_release(lcs_server *server, int64_t id);
 
 /* ---- measurement hook (no counterpart in the reference) ---------------------
  * enable != 0 starts recording HIP events on the box stream around every

@@ -51,7 +51,9 @@ struct Batch {
     // device state
     std::vector<BoxRec> tab_h;
     DevVec<BoxRec> tab;
-    DevVec<WorkItem> ghost_face, ghost_full, ghost_one, rows, parts, diams, faces[3];
+    DevVec<WorkItem> ghost_face, ghost_full, ghost_one, rows, lf_rows, parts, diams, faces[3];
+    DevVec<int> lf_boxes;            // boxes with linear-power forcing
+    std::vector<int> rcf_boxes;      // boxes with rc-energy forcing (forced one by one)
     DevVec<TileItem> rhs_tiles, hs_tiles[2];  // sweep tiles 64 and 32 wide
     HsPoll *poll_d = nullptr, *poll_h = nullptr;
     DevScalars *sc_d = nullptr, *sc_h = nullptr;
@@ -161,7 +163,9 @@ static int rebuild(Batch &B)
         LCS_HIP(hipHostMalloc((void **)&B.sc_h, sizeof(DevScalars) * (size_t)B.cap_boxes, hipHostMallocDefault));
     }
     B.tab_h.assign(n, BoxRec{});
-    std::vector<WorkItem> gface, gfull, gone, rows, parts, diams, faces[3];
+    std::vector<WorkItem> gface, gfull, gone, rows, lf_rows, parts, diams, faces[3];
+    std::vector<int> lf_boxes;
+    B.rcf_boxes.clear();
     std::vector<TileItem> rtiles, htiles[2];
     const int domain = B.hs.time_blocking ? 1 : 3;
     for (int i = 0; i < n; i++) {
@@ -221,6 +225,12 @@ static int rebuild(Batch &B)
         // row-walking reductions
         r.row_blocks = std::min(g.ny * g.nz, kBatchRowBlocks);
         for (int k = 0; k < r.row_blocks; k++) rows.push_back({i, k});
+        if (B.cfgs[i].forcing.kind == LCS_FORCING_LINEAR_POWER) {
+            lf_boxes.push_back(i);
+            for (int k = 0; k < r.row_blocks; k++) lf_rows.push_back({i, k});
+        } else if (B.cfgs[i].forcing.kind == LCS_FORCING_RC_ENERGY) {
+            B.rcf_boxes.push_back(i);
+        }
         // particles
         r.nprof = (int)b.profiles.size();
         int blk = 0, dblk = 0;
@@ -251,6 +261,7 @@ static int rebuild(Batch &B)
     if ((rc = upload_table(B))) return rc;
     if ((rc = B.ghost_face.assign(xcd_order(gface), B.stream)) || (rc = B.ghost_full.assign(xcd_order(gfull), B.stream)) ||
         (rc = B.ghost_one.assign(xcd_order(gone), B.stream)) || (rc = B.rows.assign(xcd_order(rows), B.stream)) ||
+        (rc = B.lf_rows.assign(xcd_order(lf_rows), B.stream)) || (rc = B.lf_boxes.assign(lf_boxes, B.stream)) ||
         (rc = B.parts.assign(xcd_order(parts), B.stream)) || (rc = B.diams.assign(xcd_order(diams), B.stream)) ||
         (rc = B.rhs_tiles.assign(xcd_order(rtiles), B.stream)) ||
         (rc = B.hs_tiles[0].assign(xcd_order(htiles[0]), B.stream)) ||
@@ -412,10 +423,17 @@ static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
     if ((rc = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 1, 4, h, 1, 0))) return rc;
     if ((rc = launch_b_particles(B.stream, B.esize, B.tab.d, B.parts.d, B.parts.n, 2))) return rc;
     if ((rc = launch_b_rhs(B.stream, B.esize, B.tab.d, B.rhs_tiles.d, B.rhs_tiles.n, 2))) return rc;
-    if (c0.forcing.kind == LCS_FORCING_LINEAR_POWER) {
+    // forcing! (stage 2 only): linear-power boxes batched, rc-energy boxes one by one
+    if (B.lf_boxes.n > 0) {
         if ((rc = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 0, 3, h, 1, 0)))
             return rc;
-        if ((rc = launch_b_lf(B.stream, B.esize, B.tab.d, B.rows.d, B.rows.n, n))) return rc;
+        if ((rc = launch_b_lf(B.stream, B.esize, B.tab.d, B.lf_rows.d, B.lf_rows.n, B.lf_boxes.d, B.lf_boxes.n)))
+            return rc;
+    }
+    for (int i : B.rcf_boxes) {
+        Box &b = *B.boxes[i];
+        const lcs_forcing &f = B.cfgs[i].forcing;
+        if ((rc = launch_force_rcf(b, f.energy, f.kmin, f.kmax, f.coarse_size, DtRef{&b.S->dt, 1.0}))) return rc;
     }
     if ((rc = b_hsmac(B, 0, 1.0, 1))) return rc;
     if ((rc = launch_b_state(B.stream, B.tab.d, n, logs ? B.sc_d : nullptr))) return rc;
@@ -501,9 +519,10 @@ int lcs_batch_destroy(lcs_batch *batch)
     Batch *B = BT(batch);
     (void)hipSetDevice(B->ctx->device);
     while (!B->boxes.empty()) (void)detach(*B, (int)B->boxes.size() - 1);
-    DevVec<WorkItem> *w[] = {&B->ghost_face, &B->ghost_full, &B->ghost_one, &B->rows, &B->parts, &B->diams,
-                             &B->faces[0],   &B->faces[1],   &B->faces[2]};
+    DevVec<WorkItem> *w[] = {&B->ghost_face, &B->ghost_full, &B->ghost_one, &B->rows,    &B->lf_rows,
+                             &B->parts,      &B->diams,      &B->faces[0],  &B->faces[1], &B->faces[2]};
     for (auto *v : w) v->release();
+    B->lf_boxes.release();
     B->rhs_tiles.release();
     B->hs_tiles[0].release();
     B->hs_tiles[1].release();
@@ -526,18 +545,17 @@ int lcs_batch_add(lcs_batch *batch, lcs_box *box, const lcs_step_config *cfg)
     if (b.ctx != B.ctx) return fail(LCS_ERR_INVALID, "box and batch belong to different contexts");
     if (b.g.h < 3) return fail(LCS_ERR_INVALID, "a flow step needs halo_size >= 3");
     if (b.g.nx < 8 || b.g.ny < 8 || b.g.nz < 8) return fail(LCS_ERR_INVALID, "batched boxes need n >= 8 per axis");
-    if (cfg->forcing.kind != LCS_FORCING_NONE && cfg->forcing.kind != LCS_FORCING_LINEAR_POWER)
-        return fail(LCS_ERR_INVALID, "batched steps support no forcing and linear-power forcing");
+    if (cfg->forcing.kind != LCS_FORCING_NONE && cfg->forcing.kind != LCS_FORCING_LINEAR_POWER &&
+        cfg->forcing.kind != LCS_FORCING_RC_ENERGY)
+        return fail(LCS_ERR_INVALID, "unknown forcing kind");
     if (cfg->hsmac.coloring != LCS_COLORING_RED_BLACK_FAST && cfg->hsmac.coloring != LCS_COLORING_RED_BLACK)
         return fail(LCS_ERR_INVALID, "unsupported coloring");
     if (cfg->localize_interval < 0) return fail(LCS_ERR_INVALID, "localize_interval must be >= 0");
     if (!B.boxes.empty()) {
         if (b.esize != B.esize) return fail(LCS_ERR_INVALID, "all boxes of a batch share one float type");
         const lcs_step_config &c0 = B.cfgs[0];
-        if (!same_hsmac(cfg->hsmac, B.hs) || cfg->cfl != c0.cfl || cfg->dtmax != c0.dtmax ||
-            cfg->forcing.kind != c0.forcing.kind || b.g.h != B.boxes[0]->g.h)
-            return fail(LCS_ERR_INVALID,
-                        "boxes of a batch share HSMAC parameters, cfl, dtmax, forcing kind and halo size");
+        if (!same_hsmac(cfg->hsmac, B.hs) || cfg->cfl != c0.cfl || cfg->dtmax != c0.dtmax || b.g.h != B.boxes[0]->g.h)
+            return fail(LCS_ERR_INVALID, "boxes of a batch share HSMAC parameters, cfl, dtmax and halo size");
     } else {
         B.esize = b.esize;
         B.hs = cfg->hsmac;

@@ -1650,8 +1650,7 @@ int launch_b_cfl(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *ro
     return LCS_OK;
 }
 
-// linear-power forcing per box (boxes with power 0 would divide by e: the
-// batch only forces boxes whose config asks for it, see lcs_batch.hip)
+// linear-power forcing of the boxes listed in `boxes` (their rows in `rows`)
 template <typename T>
 __global__ void __launch_bounds__(256) k_b_lf_means(const BoxRec *tab, const WorkItem *work)
 {
@@ -1666,10 +1665,10 @@ __global__ void __launch_bounds__(256) k_b_lf_means(const BoxRec *tab, const Wor
     }
 }
 
-__global__ void k_b_lf_fin(const BoxRec *tab)
+__global__ void k_b_lf_fin(const BoxRec *tab, const int *boxes)
 {
     __shared__ double lds[16];
-    const BoxRec &B = tab[blockIdx.x];
+    const BoxRec &B = tab[boxes[blockIdx.x]];
     double m[4];
     for (int a = 0; a < 4; a++) {
         double v = 0.0;
@@ -1691,28 +1690,30 @@ __global__ void __launch_bounds__(256) k_b_lf_apply(const BoxRec *tab, const Wor
     if (threadIdx.x == 0) B.part[w.blk] = v;
 }
 
-__global__ void k_b_lf_mean(const BoxRec *tab)
+__global__ void k_b_lf_mean(const BoxRec *tab, const int *boxes)
 {
     __shared__ double lds[16];
-    const BoxRec &B = tab[blockIdx.x];
+    const BoxRec &B = tab[boxes[blockIdx.x]];
     double v = 0.0;
     for (int q = threadIdx.x; q < B.row_blocks; q += blockDim.x) v += B.part[q];
     v = block_sum(v, lds);
     if (threadIdx.x == 0) B.S->p = v / (double)B.g.cells();
 }
 
-int launch_b_lf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *rows, int nrows, int nbox)
+int launch_b_lf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *rows, int nrows, const int *boxes,
+                int nbox)
 {
+    if (nbox == 0) return LCS_OK;
     if (esize == 8)
         hipLaunchKernelGGL(k_b_lf_means<double>, dim3(nrows), dim3(256), 0, s, tab, rows);
     else
         hipLaunchKernelGGL(k_b_lf_means<float>, dim3(nrows), dim3(256), 0, s, tab, rows);
-    hipLaunchKernelGGL(k_b_lf_fin, dim3(nbox), dim3(256), 0, s, tab);
+    hipLaunchKernelGGL(k_b_lf_fin, dim3(nbox), dim3(256), 0, s, tab, boxes);
     if (esize == 8)
         hipLaunchKernelGGL(k_b_lf_apply<double>, dim3(nrows), dim3(256), 0, s, tab, rows);
     else
         hipLaunchKernelGGL(k_b_lf_apply<float>, dim3(nrows), dim3(256), 0, s, tab, rows);
-    hipLaunchKernelGGL(k_b_lf_mean, dim3(nbox), dim3(256), 0, s, tab);
+    hipLaunchKernelGGL(k_b_lf_mean, dim3(nbox), dim3(256), 0, s, tab, boxes);
     LCS_HIP(hipGetLastError());
     return LCS_OK;
 }

@@ -262,7 +262,8 @@ int launch_b_ghost_fill(hipStream_t s, int esize, const BoxRec *tab, const WorkI
                         int narr, int r, int full, int hsmac);
 int launch_b_cfl(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *rows, int nrows, int nbox,
                  double cfl, double dtmax);
-int launch_b_lf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *rows, int nrows, int nbox);
+int launch_b_lf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *rows, int nrows, const int *boxes,
+                int nbox);
 int launch_b_rhs(hipStream_t s, int esize, const BoxRec *tab, const TileItem *tiles, int ntiles, int stage);
 // tx: tile width of the fused sweep (32 or 64); the tiles listed must all be of that width
 int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, int tx, const BoxRec *tab, const TileItem *tiles,

@@ -9,7 +9,8 @@ from ._lib import LCSArgumentError, LCSError, LIBPATH, load
 from .config import (Config, ConstantPowerLF, EnergyPreserveRCF, FlowParams, Grid, HSMAC, NoForcing,
                      ParticleConfig, SpectrumInit, TimeStep, from_dict, readconfig)
 from .core import (Batch, Box, boundary, boxmean, couple, fill_ghost, flow_timestep, forcing, localize, makeindex,
-                   motion, particle_timestep, particle_update, read_log, rhs, rhs_update, step, step_config, update)
+                   motion, particle_timestep, particle_update, read_log, rhs, rhs_update, Server, step, step_config,
+                   update)
 from .simulate import build, simulate
 
 __all__ = [
@@ -17,5 +18,5 @@ __all__ = [
     "LCSError", "LIBPATH", "NoForcing", "ParticleConfig", "SpectrumInit", "TimeStep", "boundary", "boxmean",
     "build", "config", "couple", "fill_ghost", "flow_timestep", "forcing", "from_dict", "load", "localize",
     "makeindex", "motion", "particle_timestep", "particle_update", "read_log", "readconfig", "rhs", "rhs_update",
-    "simulate", "step", "step_config", "update",
+    "Server", "simulate", "step", "step_config", "update",
 ]

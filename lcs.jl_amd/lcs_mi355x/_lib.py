@@ -69,6 +69,24 @@ class StepLog(C.Structure):
                 ("div1", C.c_double), ("div2", C.c_double), ("p", C.c_double), ("e10", C.c_double)]
 
 
+class Request(C.Structure):
+    _fields_ = [("n", C.c_int), ("dtype", C.c_int), ("particles", C.c_int64), ("st_min", C.c_double),
+                ("st_max", C.c_double), ("forcing", C.c_int), ("forcing_param", C.c_double), ("steps", C.c_int),
+                ("seed", C.c_uint64), ("k0", C.c_double), ("Re", C.c_double)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("memory_budget", C.c_size_t), ("max_live", C.c_int), ("cfl", C.c_double), ("dtmax", C.c_double),
+                ("hsmac", HsmacParams), ("localize_interval", C.c_int)]
+
+
+class RunStatus(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int), ("steps_done", C.c_int64), ("submit_s", C.c_double),
+                ("start_s", C.c_double), ("finish_s", C.c_double), ("bytes", C.c_size_t)]
+
+
+RUN_STATUS = {0: "queued", 1: "live", 2: "done", 3: "failed"}
+
 P = C.c_void_p
 I = C.c_int
 D = C.c_double
@@ -112,6 +130,14 @@ SIGNATURES = {
     "lcs_batch_size": [P, C.POINTER(I)],
     "lcs_batch_step": [P, P, I],
     "lcs_batch_sync": [P],
+    "lcs_server_create": [P, C.POINTER(ServerConfig), C.POINTER(P)],
+    "lcs_server_destroy": [P],
+    "lcs_server_submit": [P, C.POINTER(Request), C.POINTER(C.c_int64)],
+    "lcs_server_step": [P, C.POINTER(I), C.POINTER(I)],
+    "lcs_server_status": [P, C.c_int64, C.POINTER(RunStatus)],
+    "lcs_server_counts": [P, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_size_t)],
+    "lcs_server_box": [P, C.c_int64, C.POINTER(P)],
+    "lcs_server_release": [P, C.c_int64],
     "lcs_profile": [P, I],
     "lcs_profile_read": [P, I, C.c_char_p, C.c_size_t, C.POINTER(D), C.POINTER(C.c_int64), C.POINTER(D)],
     "lcs_sync": [P],

@@ -343,6 +343,88 @@ class Batch:
             pass
 
 
+class _BorrowedBox(Box):
+    """A box owned by the server (valid until the request is released)."""
+
+    def __init__(self, handle, dims, dtype):
+        self.L = _lib.load()
+        self.handle = handle
+        self.dims = tuple(dims)
+        self.h = 3
+        self.dtype = _np_dtype(dtype)
+        self.profiles = []
+
+    def close(self):
+        self.handle = None
+
+
+class Server:
+    """Serving loop over one batch (lcs_server_*): runs are submitted as
+    requests, admitted by slot and device-memory footprint, advanced together
+    one batched step per ``step()``, and retired when they reach their step
+    count.  No counterpart in the reference."""
+
+    def __init__(self, memory_budget=200 * 2**30, max_live=128, cfl=0.3, dtmax=0.01, hsmac=None,
+                 localize_interval=10, device=0):
+        self.L = _lib.load()
+        hs = hsmac or HSMAC(True, True, "red-black-fast", 1.7, 1e-3, 6, 300)
+        cfg = _lib.ServerConfig(int(memory_budget), int(max_live), float(cfl), float(dtmax), _hsmac_params(hs),
+                                int(localize_interval))
+        h = C.c_void_p()
+        check(self.L.lcs_server_create(context(device), C.byref(cfg), C.byref(h)))
+        self.handle = h
+        self._req = {}
+
+    def submit(self, n, steps, seed, particles=0, st=(1.0, 1.0), forcing="linear-power", forcing_param=0.5,
+               k0=4.0, Re=53.3, dtype="f32"):
+        r = _lib.Request(int(n), _lib.LCS_F64 if _np_dtype(dtype) == np.float64 else _lib.LCS_F32, int(particles),
+                         float(st[0]), float(st[1]), _lib.FORCING[forcing], float(forcing_param), int(steps),
+                         int(seed) & (2**64 - 1), float(k0), float(Re))
+        i = C.c_int64()
+        check(self.L.lcs_server_submit(self.handle, C.byref(r), C.byref(i)))
+        self._req[i.value] = (int(n), _np_dtype(dtype))
+        return i.value
+
+    def step(self):
+        """Admit what fits, advance every live run one step; returns (live, finished)."""
+        live, fin = C.c_int(), C.c_int()
+        check(self.L.lcs_server_step(self.handle, C.byref(live), C.byref(fin)))
+        return live.value, fin.value
+
+    def status(self, rid):
+        s = _lib.RunStatus()
+        check(self.L.lcs_server_status(self.handle, int(rid), C.byref(s)))
+        return {"id": s.id, "status": _lib.RUN_STATUS[s.status], "steps_done": s.steps_done,
+                "submit_s": s.submit_s, "start_s": s.start_s, "finish_s": s.finish_s, "bytes": s.bytes}
+
+    def counts(self):
+        q, lv, m = C.c_int(), C.c_int(), C.c_size_t()
+        check(self.L.lcs_server_counts(self.handle, C.byref(q), C.byref(lv), C.byref(m)))
+        return {"queued": q.value, "live": lv.value, "memory_used": m.value}
+
+    def box(self, rid):
+        """The run's box (fields and particles readable until release)."""
+        h = C.c_void_p()
+        check(self.L.lcs_server_box(self.handle, int(rid), C.byref(h)))
+        n, dt = self._req[rid]
+        return _BorrowedBox(h, (n, n, n), dt)
+
+    def release(self, rid):
+        check(self.L.lcs_server_release(self.handle, int(rid)))
+        self._req.pop(rid, None)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.L.lcs_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def read_log(box):
     log = _lib.StepLog()
     check(box.L.lcs_step_log_read(box.handle, C.byref(log)))

@@ -57,14 +57,18 @@ def test_c_struct_sizes_agree_with_ctypes(tmp_path):
                    'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu ", sizeof(lcs_box_desc),'
                    ' sizeof(lcs_hsmac_params), sizeof(lcs_particle_desc), sizeof(lcs_forcing),'
                    ' sizeof(lcs_step_config), sizeof(lcs_step_log), offsetof(lcs_step_config, forcing),'
-                   ' offsetof(lcs_step_log, div1)); printf("%zu\\n", offsetof(lcs_step_config, localize_interval));'
-                   ' return 0;}\n')
+                   ' offsetof(lcs_step_log, div1)); printf("%zu %zu %zu %zu %zu %zu\\n",'
+                   ' offsetof(lcs_step_config, localize_interval), sizeof(lcs_request), offsetof(lcs_request, seed),'
+                   ' sizeof(lcs_server_config), offsetof(lcs_server_config, localize_interval),'
+                   ' sizeof(lcs_run_status)); return 0;}\n')
     exe = tmp_path / "t"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
     out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     assert out == [C.sizeof(_lib.BoxDesc), C.sizeof(_lib.HsmacParams), C.sizeof(_lib.ParticleDesc),
                    C.sizeof(_lib.Forcing), C.sizeof(_lib.StepConfig), C.sizeof(_lib.StepLog),
-                   _lib.StepConfig.forcing.offset, _lib.StepLog.div1.offset, _lib.StepConfig.localize_interval.offset]
+                   _lib.StepConfig.forcing.offset, _lib.StepLog.div1.offset, _lib.StepConfig.localize_interval.offset,
+                   C.sizeof(_lib.Request), _lib.Request.seed.offset, C.sizeof(_lib.ServerConfig),
+                   _lib.ServerConfig.localize_interval.offset, C.sizeof(_lib.RunStatus)]
 
 
 YAML = """

@@ -1,8 +1,9 @@
 """Batched ensembles (lcs_batch_*): boxes of uneven size, particle counts and
 forcing power advanced with one launch per phase.
 
-* every box of a mixed batch follows the oracle's phase-by-phase restatement
-  of timestep! + evolve! for that box alone (same bar as the single-box step
+* every box of a mixed batch (linear-power and rc-energy forcing, with and
+  without particles) follows the oracle's phase-by-phase restatement of
+  timestep! + evolve! for that box alone (same bar as the single-box step
   test: sweep counts equal per step, fields within relative L2 1e-8 in FP64);
 * a box's result does not depend on the mix: alone in a batch and among other
   boxes it is the same bit for bit (per-box reductions use per-box partials);
@@ -20,10 +21,11 @@ pytestmark = pytest.mark.gpu
 def mixed_configs(dtype="f64"):
     from lcs_mi355x import config as C
     cfgs = []
-    for dims, npart, power, profiles in [((24, 24, 24), 500, 0.5, 1), ((32, 16, 24), 1200, 0.3, 2),
-                                         ((16, 16, 16), 0, 0.7, 0)]:
-        cfg = c1_config(dims=dims, nparticles=max(npart, 1), forcing=C.ConstantPowerLF(power), dtype=dtype,
-                        profiles=profiles)
+    for dims, npart, forcing, profiles in [((24, 24, 24), 500, C.ConstantPowerLF(0.5), 1),
+                                           ((32, 16, 24), 1200, C.ConstantPowerLF(0.3), 2),
+                                           ((16, 16, 16), 0, C.ConstantPowerLF(0.7), 0),
+                                           ((24, 24, 24), 400, C.EnergyPreserveRCF(0.649, 1, 2, 6), 1)]:
+        cfg = c1_config(dims=dims, nparticles=max(npart, 1), forcing=forcing, dtype=dtype, profiles=profiles)
         cfgs.append(cfg)
     return cfgs
 
@@ -81,19 +83,19 @@ def test_batch_membership_changes():
     batch = L.Batch()
     for c, b in zip(cfgs, boxes):
         batch.add(b, L.step_config(c))
-    assert len(batch) == 3
+    assert len(batch) == 4
     batch.step()
     with pytest.raises(LCSError):  # a batched box is stepped by its batch only
         L.step(boxes[1], L.step_config(cfgs[1]))
     # leave, step alone, come back: states keep advancing consistently
     batch.remove(boxes[1])
-    assert len(batch) == 2
+    assert len(batch) == 3
     log = L.step(boxes[1], L.step_config(cfgs[1]))
     assert log["step"] == 2
     batch.add(boxes[1], L.step_config(cfgs[1]))
     logs = batch.step()
-    assert [l["step"] for l in logs] == [2, 2, 3]
+    assert [l["step"] for l in logs] == [2, 2, 2, 3]
     boxes[0].close()  # destroying a box detaches it
-    assert len(batch) == 2
+    assert len(batch) == 3
     batch.step()
     batch.close()
