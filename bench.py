@@ -163,6 +163,112 @@ def bench_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+def serve_requests(nreq, seed=4242):
+    """Served-stream workload: N in {32, 48, ..., 128}, particle count
+    log-uniform in [1e3, 2e6], one Stokes number per run log-uniform in
+    [0.1, 10], forcing linear-power (P = 0.5) or rc-energy (energy 0.649),
+    step count uniform in [50, 500], a seed per request."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nreq):
+        n = int(rng.choice(np.arange(32, 129, 16)))
+        st = float(10 ** rng.uniform(-1.0, 1.0))
+        rc = bool(rng.integers(0, 2))
+        out.append(dict(n=n, particles=int(round(10 ** rng.uniform(3.0, np.log10(2e6)))), st=(st, st),
+                        forcing="rc-energy" if rc else "linear-power", forcing_param=0.649 if rc else 0.5,
+                        steps=int(rng.integers(50, 501)), seed=10_000 + i, k0=4.0,
+                        Re=53.3 * (n / 64.0) ** (4.0 / 3.0), dtype="f32"))
+    return out
+
+
+def bench_serve(args):
+    """Continuous batching of a Poisson request stream: completion latency
+    percentiles and box-steps/s, beside the same requests' serial rate
+    (lcs_step one run at a time) measured on a sample."""
+    import numpy as np
+    import lcs_mi355x as L
+    reqs = serve_requests(args.requests)
+    rng = np.random.default_rng(99)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, len(reqs)))
+    S = L.Server(max_live=args.max_live, localize_interval=args.localize_interval)
+    done, lat, steps_total, live_ids = 0, [], 0, set()
+    nxt = 0
+    t0 = time.perf_counter()
+    last_report = t0
+    while done < len(reqs):
+        now = time.perf_counter() - t0
+        while nxt < len(reqs) and arrivals[nxt] <= now:
+            live_ids.add(S.submit(**reqs[nxt]))
+            nxt += 1
+        if not live_ids:
+            time.sleep(max(0.0, min(0.01, arrivals[nxt] - now)))
+            continue
+        S.step()
+        for rid in list(live_ids):
+            st = S.status(rid)
+            if st["status"] in ("done", "failed"):
+                live_ids.discard(rid)
+                done += 1
+                if st["status"] == "done":
+                    lat.append(st["finish_s"] - st["submit_s"])
+                    steps_total += st["steps_done"]
+                S.release(rid)
+        if time.perf_counter() - last_report > 20:
+            last_report = time.perf_counter()
+            c = S.counts()
+            print(f"# t={last_report - t0:.0f}s submitted {nxt} done {done} live {c['live']} queued {c['queued']}",
+                  file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    S.close()
+    rate = steps_total / wall
+    # serial reference on a sample: the first few requests, a few steps each, one at a time
+    sample = reqs[: args.serial_sample]
+    boxes = []
+    for q, r in enumerate(sample):
+        from lcs_mi355x import config as C
+        cfg = C.Config(grid=C.Grid((r["n"],) * 3, 3), timestep=C.TimeStep(0.3, 0.01), simulate=C.SimulateConfig(0),
+                       flow=C.FlowConfig(C.FlowParams(1.0, 1.0, 1.0 / r["Re"]), C.SpectrumInit(r["seed"], r["k0"], 1.0),
+                                         C.HSMAC(True, True, "red-black-fast", 1.7, 1e-3, 6, 300),
+                                         C.ConstantPowerLF(0.5) if r["forcing"] == "linear-power"
+                                         else C.EnergyPreserveRCF(0.649, 1, 2, 8)),
+                       particles=(C.ParticleConfig(C.Population(r["particles"]), C.ParticleParams(0.0, 843.0), 4,
+                                                   "linear", C.ParticleInit(seed=r["seed"], velocity="zero-slip",
+                                                                            size="stokes", st_min=r["st"][0],
+                                                                            st_max=r["st"][1], eps=0.5)),),
+                       dtype="f32")
+        b = L.build(cfg)
+        boxes.append((b, L.step_config(cfg, localize_interval=args.localize_interval)))
+    for b, s in boxes:
+        L.step(b, s, want_log=False)
+        b.sync()
+    ts = time.perf_counter()
+    for b, s in boxes:
+        for _ in range(args.serial_steps):
+            L.step(b, s, want_log=False)
+        b.sync()
+    serial = len(boxes) * args.serial_steps / (time.perf_counter() - ts)
+    for b, _ in boxes:
+        b.close()
+    lat = np.array(lat)
+    out = {
+        "metric": METRIC, "value": rate, "unit": "box-steps/s", "n_gpus": 1, "steps": int(steps_total),
+        "warmup": 0, "ms_per_step": 1e3 / rate, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (per-request seeds)",
+        "config": {"workload": f"served stream of {len(reqs)} Poisson requests at {args.rate}/s, N in 32..128 "
+                               f"step 16, particles log-uniform [1e3, 2e6], St log-uniform [0.1, 10], forcing "
+                               f"linear-power or rc-energy, steps uniform [50, 500]",
+                   "max_live": args.max_live, "particle_localize_interval": args.localize_interval},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "mean": float(lat.mean()), "max": float(lat.max())},
+        "wall_s": wall, "completed": int(len(lat)),
+        "serial": {"value": serial, "unit": "box-steps/s", "sample": f"first {len(boxes)} requests, "
+                                                                      f"{args.serial_steps} steps each, one lcs_step at a time"},
+        "speedup_vs_serial": rate / serial,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(name, n, steps):
     """Time the CPU oracle on a bounded sample (n^3 box, same parameters)."""
     import numpy as np
@@ -198,7 +304,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--requests", type=int, default=2000, help="served requests (workload c5)")
+    ap.add_argument("--rate", type=float, default=8.0, help="Poisson arrival rate, requests/s (c5)")
+    ap.add_argument("--max-live", type=int, default=128, help="runs in the batch at once (c5)")
+    ap.add_argument("--serial-sample", type=int, default=16, help="requests of the serial reference (c5)")
     ap.add_argument("--boxes", type=int, default=96, help="ensemble size (workload c4)")
     ap.add_argument("--serial-steps", type=int, default=3, help="steps per box of the serial baseline (c4)")
     ap.add_argument("--size", type=int, default=None, help="override the grid size (testing only)")
@@ -218,6 +328,8 @@ def main():
 
     if args.workload == "c4":
         return bench_ensemble(args)
+    if args.workload == "c5":
+        return bench_serve(args)
     import lcs_mi355x as L
     cfg, desc = workload(args.workload, n_override=args.size)
     t0 = time.perf_counter()

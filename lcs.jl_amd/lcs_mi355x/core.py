@@ -346,13 +346,14 @@ class Batch:
 class _BorrowedBox(Box):
     """A box owned by the server (valid until the request is released)."""
 
-    def __init__(self, handle, dims, dtype):
+    def __init__(self, handle, dims, dtype, particles=0):
         self.L = _lib.load()
         self.handle = handle
         self.dims = tuple(dims)
         self.h = 3
         self.dtype = _np_dtype(dtype)
-        self.profiles = []
+        # a served run holds one particle profile (grid ratio 4) when it has particles
+        self.profiles = [dict(nvalid=int(particles), ntotal=int(particles), grid_ratio=4)] if particles > 0 else []
 
     def close(self):
         self.handle = None
@@ -382,7 +383,7 @@ class Server:
                          int(seed) & (2**64 - 1), float(k0), float(Re))
         i = C.c_int64()
         check(self.L.lcs_server_submit(self.handle, C.byref(r), C.byref(i)))
-        self._req[i.value] = (int(n), _np_dtype(dtype))
+        self._req[i.value] = (int(n), _np_dtype(dtype), int(particles))
         return i.value
 
     def step(self):
@@ -406,8 +407,8 @@ class Server:
         """The run's box (fields and particles readable until release)."""
         h = C.c_void_p()
         check(self.L.lcs_server_box(self.handle, int(rid), C.byref(h)))
-        n, dt = self._req[rid]
-        return _BorrowedBox(h, (n, n, n), dt)
+        n, dt, npart = self._req[rid]
+        return _BorrowedBox(h, (n, n, n), dt, npart)
 
     def release(self, rid):
         check(self.L.lcs_server_release(self.handle, int(rid)))
