@@ -197,6 +197,34 @@ int lcs_force_rcf(lcs_box *box, double energy, int kmin, int kmax, int coarse_si
  * receives 3*c^3 values (U then V then W), column-major, as doubles */
 int lcs_boxmean(lcs_box *box, int coarse_size, double *coarse3);
 
+/* ---- flow statistics -------------------------------------------------------- */
+/* Flows.FlowStat (src/flows/stat.jl:1-42): summaries (mean, max, min) of the
+ * raw U, V, W interior values (LCS.Summary, src/stat.jl:4-22) and the derived
+ * turbulence quantities of stat.jl:132-154. */
+typedef struct {
+    double u_mean, u_max, u_min;
+    double v_mean, v_max, v_min;
+    double w_mean, w_max, w_min;
+    double k;         /* 0.5 (<u2> - |mean|^2) */
+    double eps;       /* <2 S_ij S_ij> / Re */
+    double tau_eta;   /* Kolmogorov time */
+    double l_mu;      /* Taylor microscale */
+    double l_eta;     /* Kolmogorov length */
+    double Re_lambda;
+    double urms;
+    double k_eta;     /* (n/2) l_eta, n the smallest dimension */
+    double skew;      /* -<dudx^3> / <dudx^2>^1.5 */
+    double flat;      /* <dudx^4> / <dudx^2>^2 */
+    double L;         /* integral length; 0 unless requested */
+    double C_eps;     /* eps L / u'^3 */
+} lcs_flow_stats;
+
+/* Flows.stat(buf, state, config, topo) (src/flows/stat.jl:53-155): FullHalo
+ * ghost fill on U, V, W, then the statistics of the current velocity.
+ * integral_length != 0 also computes the integral length (stat.jl:157-182,
+ * config flag flow.stat.integral_length).  Synchronises. */
+int lcs_flow_stat(lcs_box *box, int integral_length, lcs_flow_stats *out);
+
 /* ---- particles ------------------------------------------------------------ */
 /* Particles.ParticleBuffer (src/particles/buffer.jl:124-135) for one profile;
  * profile index returned in *profile (0-based, in call order) */

@@ -2,6 +2,7 @@
 // per-phase entry points mirroring the reference's positional methods, and
 // the whole-step driver (LCS.timestep! + LCS.evolve!).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -391,6 +392,37 @@ int lcs_force_rcf(lcs_box *box, double energy, int kmin, int kmax, int c, double
         (rc = read_scalars(b)))
         return rc;
     if (p) *p = b.S_host->p;
+    return LCS_OK;
+}
+
+int lcs_flow_stat(lcs_box *box, int integral_length, lcs_flow_stats *out)
+{
+    if (check_box(box) || !out) return fail(LCS_ERR_INVALID, "null argument");
+    Box &b = *B(box);
+    double *dev = b.part + 16L * kMaxPartBlocks;  // 15 results after the partial slots the passes use
+    int rc = launch_flow_stat(b, integral_length, dev);
+    if (rc) return rc;
+    double r[15] = {0};
+    LCS_HIP(hipMemcpyAsync(r, dev, sizeof(double) * (integral_length ? 15 : 14), hipMemcpyDeviceToHost, b.stream));
+    LCS_HIP(hipStreamSynchronize(b.stream));
+    // derived quantities: src/flows/stat.jl:132-154
+    lcs_flow_stats &o = *out;
+    o.u_mean = r[0], o.u_max = r[1], o.u_min = r[2];
+    o.v_mean = r[3], o.v_max = r[4], o.v_min = r[5];
+    o.w_mean = r[6], o.w_max = r[7], o.w_min = r[8];
+    const double u2 = r[9], s2 = r[10], d2 = r[11], d3 = r[12], d4 = r[13], Re = b.Re();
+    o.urms = std::sqrt(u2 / 3);
+    o.k = 0.5 * (u2 - o.u_mean * o.u_mean - o.v_mean * o.v_mean - o.w_mean * o.w_mean);
+    o.eps = s2 / Re;
+    o.tau_eta = std::sqrt(1 / o.eps / Re);
+    o.l_mu = std::sqrt(15 * o.urms * o.urms / o.eps / Re);
+    o.l_eta = std::pow(1 / o.eps / (Re * Re * Re), 0.25);
+    o.Re_lambda = o.l_mu * o.urms * Re;
+    o.k_eta = (std::min(b.g.nx, std::min(b.g.ny, b.g.nz)) / 2.0) * o.l_eta;
+    o.skew = -d3 / std::pow(d2, 1.5);
+    o.flat = d4 / (d2 * d2);
+    o.L = integral_length ? r[14] : 0.0;
+    o.C_eps = o.eps * o.L / (o.urms * o.urms * o.urms);
     return LCS_OK;
 }
 

@@ -1031,6 +1031,176 @@ __global__ void k_mean_to(const double *part, int nblk, double len, double *dst)
     if (threadIdx.x == 0) *dst = s / len;
 }
 
+// ---------------------------------------------------------------------------
+// Flow statistics (Flows.stat, src/flows/stat.jl:67-155): one pass over the
+// interior for the summaries of U, V, W and the means of u2, 2 S_ij S_ij and
+// the powers of dudx; the integral length (stat.jl:157-182) in its own pass.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline T diff1_o4_half(T a, T b, T c, T d)  // src/diff.jl:13-15
+{
+    return T(1.0 / 24.0) * (((a - T(27) * b) + T(27) * c) - d);
+}
+
+constexpr int kStatQ = 14;  // U, V, W (sum, max, min), then u2, s2, d2, d3, d4 sums
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_flow_stat(Geom g, const T *__restrict__ U, const T *__restrict__ V,
+                                                  const T *__restrict__ W, T dxi, T dyi, T dzi, double *part)
+{
+    __shared__ double lds[16];
+    const long sy = g.sx, sz = g.sx * g.sy;
+    double s[kStatQ];
+    for (int a = 0; a < 3; a++) {
+        s[3 * a] = 0.0;
+        s[3 * a + 1] = -INFINITY;
+        s[3 * a + 2] = INFINITY;
+    }
+    for (int a = 9; a < kStatQ; a++) s[a] = 0.0;
+    for_rows(g, xcd_block(), gridDim.x, [&](int, int, int, long p) {
+        const T u = U[p], v = V[p], w = W[p];
+        const T u2 = (u * u + v * v) + w * w;
+        const T dudx = dxi * diff1_o4_half(U[p - 2], U[p - 1], U[p], U[p + 1]);
+        const T dvdy = dyi * diff1_o4_half(V[p - 2 * sy], V[p - sy], V[p], V[p + sy]);
+        const T dwdz = dzi * diff1_o4_half(W[p - 2 * sz], W[p - sz], W[p], W[p + sz]);
+        const T dudy = dyi * diff1_o4_half(U[p - 2 * sy], U[p - sy], U[p], U[p + sy]);
+        const T dvdx = dxi * diff1_o4_half(V[p - sy - 1], V[p - sy], V[p - sy + 1], V[p - sy + 2]);
+        const T dvdz = dzi * diff1_o4_half(V[p - sy - sz], V[p - sy], V[p - sy + sz], V[p - sy + 2 * sz]);
+        const T dwdy = dyi * diff1_o4_half(W[p - 2 * sy], W[p - sy], W[p], W[p + sy]);
+        const T dwdx = dxi * diff1_o4_half(W[p - sz - 1], W[p - sz], W[p - sz + 1], W[p - sz + 2]);
+        const T dudz = dzi * diff1_o4_half(U[p - 2 * sz], U[p - sz], U[p], U[p + sz]);
+        const T s11 = T(0.5) * (dudx + dudx), s22 = T(0.5) * (dvdy + dvdy), s33 = T(0.5) * (dwdz + dwdz);
+        const T s12 = T(0.5) * (dudy + dvdx), s13 = T(0.5) * (dudz + dwdx), s23 = T(0.5) * (dvdz + dwdy);
+        const T s2 = T(2) * (((s11 * s11 + s22 * s22) + s33 * s33) + T(2) * ((s12 * s12 + s13 * s13) + s23 * s23));
+        const T d2 = dudx * dudx;
+        const T vals[3] = {u, v, w};
+        for (int a = 0; a < 3; a++) {
+            s[3 * a] += (double)vals[a];
+            s[3 * a + 1] = fmax(s[3 * a + 1], (double)vals[a]);
+            s[3 * a + 2] = fmin(s[3 * a + 2], (double)vals[a]);
+        }
+        s[9] += (double)u2;
+        s[10] += (double)s2;
+        s[11] += (double)d2;
+        s[12] += (double)(d2 * dudx);
+        s[13] += (double)(d2 * d2);
+    });
+    for (int a = 0; a < kStatQ; a++) {
+        const int kind = a < 9 ? a % 3 : 0;  // 0 sum, 1 max, 2 min
+        double r = kind == 0 ? block_sum(s[a], lds) : kind == 1 ? block_max(s[a], lds) : -block_max(-s[a], lds);
+        if (threadIdx.x == 0) part[(long)a * kMaxPartBlocks + blockIdx.x] = r;
+    }
+}
+
+__global__ void k_flow_stat_fin(const double *part, int nblk, double len, double *out)
+{
+    __shared__ double lds[16];
+    for (int a = 0; a < kStatQ; a++) {
+        const int kind = a < 9 ? a % 3 : 0;
+        double r = kind == 0 ? 0.0 : kind == 1 ? -INFINITY : INFINITY;
+        for (int q = threadIdx.x; q < nblk; q += blockDim.x) {
+            const double x = part[(long)a * kMaxPartBlocks + q];
+            r = kind == 0 ? r + x : kind == 1 ? fmax(r, x) : fmin(r, x);
+        }
+        r = kind == 0 ? block_sum(r, lds) : kind == 1 ? block_max(r, lds) : -block_max(-r, lds);
+        if (threadIdx.x == 0) out[a] = kind == 0 ? r / len : r;
+    }
+}
+
+// Integral length (stat.jl:157-182).  The reference forms R = irfft(|rfft(U)|^2)
+// along x, i.e. the circular autocorrelation R[r] = sum_i U[i] U[i+r], and
+// sums lags 0..n/2 up to the first negative one: Ls = acc dx / R[0].  Here
+// each block takes a run of rows; a row sits twice in LDS (so i+r needs no
+// wrap) and each thread forms one lag directly, 256 lags at a time, stopping
+// at the first chunk that holds a negative lag.
+template <typename T>
+__global__ void __launch_bounds__(256) k_integral_length(Geom g, const T *__restrict__ U, double dx, double *part)
+{
+    extern __shared__ double row[];  // 2 nx
+    __shared__ double lag[256];
+    __shared__ double lds[16];
+    __shared__ int stop;
+    const int nx = g.nx, mid = nx / 2 + 1, rows = g.ny * g.nz;
+    const int per = (rows + gridDim.x - 1) / gridDim.x, r0 = blockIdx.x * per, r1 = min(rows, r0 + per);
+    double sum_ls = 0.0;
+    for (int r = r0; r < r1; r++) {
+        const long base = g.at(1, r % g.ny + 1, r / g.ny + 1);
+        for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+            const double v = (double)U[base + i];
+            row[i] = v;
+            row[i + nx] = v;
+        }
+        __syncthreads();
+        double acc = 0.0, rzero = 0.0;
+        for (int l0 = 0; l0 < mid; l0 += 256) {
+            const int l = l0 + (int)threadIdx.x;
+            double c = 0.0;
+            if (l < mid)
+                for (int i = 0; i < nx; i++) c += row[i] * row[i + l];
+            lag[threadIdx.x] = c;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int brk = 0;
+                for (int q = 0; q < 256 && l0 + q < mid; q++) {
+                    if (l0 + q == 0) rzero = lag[0];
+                    if (lag[q] < 0) {
+                        brk = 1;
+                        break;
+                    }
+                    acc += lag[q];
+                }
+                stop = brk;
+            }
+            __syncthreads();
+            if (stop) break;
+        }
+        if (threadIdx.x == 0) sum_ls += acc * dx / rzero;
+        __syncthreads();  // the row is rewritten next
+    }
+    double s = block_sum(threadIdx.x == 0 ? sum_ls : 0.0, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+int launch_flow_stat(Box &b, int integral_length, double *out_dev)
+{
+    void *arr[3] = {b.f[0], b.f[1], b.f[2]};
+    int rc = launch_ghost_fill(b, arr, 3, b.g.h, 1, nullptr);  // FullHalo on Us (stat.jl:79)
+    if (rc) return rc;
+    const Geom &g = b.g;
+    const int blocks = grid_rows(g);
+    const double dxi = 1.0 / spacing(g.nx), dyi = 1.0 / spacing(g.ny), dzi = 1.0 / spacing(g.nz);
+    {
+        ProfScope ps(b, "flow_stat", 3.0 * b.esize * (double)g.cells());
+        if (b.esize == 8)
+            hipLaunchKernelGGL(k_flow_stat<double>, dim3(blocks), dim3(256), 0, b.stream, g, (const double *)b.f[0],
+                               (const double *)b.f[1], (const double *)b.f[2], dxi, dyi, dzi, b.part);
+        else
+            hipLaunchKernelGGL(k_flow_stat<float>, dim3(blocks), dim3(256), 0, b.stream, g, (const float *)b.f[0],
+                               (const float *)b.f[1], (const float *)b.f[2], (float)dxi, (float)dyi, (float)dzi,
+                               b.part);
+        LCS_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_flow_stat_fin, dim3(1), dim3(256), 0, b.stream, b.part, blocks, (double)g.cells(), out_dev);
+    LCS_HIP(hipGetLastError());
+    if (!integral_length) return LCS_OK;
+    if (g.nx > 4096) return fail(LCS_ERR_INVALID, "integral length needs nx <= 4096");
+    double *part = b.part + (long)kStatQ * kMaxPartBlocks;
+    const size_t shm = sizeof(double) * 2 * (size_t)g.nx;
+    {
+        ProfScope ps(b, "integral_length", (double)b.esize * (double)g.cells());
+        if (b.esize == 8)
+            hipLaunchKernelGGL(k_integral_length<double>, dim3(blocks), dim3(256), shm, b.stream, g,
+                               (const double *)b.f[0], spacing(g.nx), part);
+        else
+            hipLaunchKernelGGL(k_integral_length<float>, dim3(blocks), dim3(256), shm, b.stream, g,
+                               (const float *)b.f[0], spacing(g.nx), part);
+        LCS_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_mean_to, dim3(1), dim3(256), 0, b.stream, part, blocks, (double)g.ny * g.nz, out_dev + kStatQ);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
 int launch_force_linear(Box &b, double power, DtRef dt)
 {
     void *arr[3] = {b.f[0], b.f[1], b.f[2]};

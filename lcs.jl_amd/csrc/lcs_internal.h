@@ -192,6 +192,9 @@ int launch_cfl(Box &b, double cfl, double dtmax);  // writes S->umax, S->dt_flow
 int launch_hsmac(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int log_slot);
 int launch_force_linear(Box &b, double power, DtRef dt);  // writes S->lf, S->p
 int launch_boxmean(Box &b, int c, double *coarse_dev);
+// Flows.stat raw values into out_dev[15] (device): U, V, W mean/max/min, means of
+// u2, s2, dudx^2..4, then the integral length when asked
+int launch_flow_stat(Box &b, int integral_length, double *out_dev);
 int launch_force_rcf(Box &b, double energy, int kmin, int kmax, int c, DtRef dt);
 int launch_init_spectrum(Box &b, uint64_t seed, double k0, double uprime);
 

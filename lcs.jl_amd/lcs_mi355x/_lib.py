@@ -80,6 +80,14 @@ class ServerConfig(C.Structure):
                 ("hsmac", HsmacParams), ("localize_interval", C.c_int)]
 
 
+FLOW_STAT_FIELDS = ["u_mean", "u_max", "u_min", "v_mean", "v_max", "v_min", "w_mean", "w_max", "w_min", "k", "eps",
+                    "tau_eta", "l_mu", "l_eta", "Re_lambda", "urms", "k_eta", "skew", "flat", "L", "C_eps"]
+
+
+class FlowStat(C.Structure):
+    _fields_ = [(n, C.c_double) for n in FLOW_STAT_FIELDS]
+
+
 class RunStatus(C.Structure):
     _fields_ = [("id", C.c_int64), ("status", C.c_int), ("steps_done", C.c_int64), ("submit_s", C.c_double),
                 ("start_s", C.c_double), ("finish_s", C.c_double), ("bytes", C.c_size_t)]
@@ -109,6 +117,7 @@ SIGNATURES = {
     "lcs_force_linear": [P, D, D, C.POINTER(D)],
     "lcs_force_rcf": [P, D, I, I, I, D, C.POINTER(D)],
     "lcs_boxmean": [P, I, P],
+    "lcs_flow_stat": [P, I, C.POINTER(FlowStat)],
     "lcs_particles_add": [P, C.POINTER(ParticleDesc), C.POINTER(I)],
     "lcs_particles_upload": [P, I, I, P, C.c_size_t],
     "lcs_particles_download": [P, I, I, P, C.c_size_t],

@@ -238,6 +238,18 @@ def boxmean(box, coarse_size):
     return [out[a * c ** 3:(a + 1) * c ** 3].reshape((c, c, c), order="F") for a in range(3)]
 
 
+def flow_stat(box, integral_length=False):
+    """Flows.stat (src/flows/stat.jl:53-155): statistics of the box's current
+    velocity (after a FullHalo ghost fill).  Returns a dict with the
+    reference's FlowStat fields; u, v, w are (mean, max, min) summaries."""
+    st = _lib.FlowStat()
+    check(box.L.lcs_flow_stat(box.handle, int(bool(integral_length)), C.byref(st)))
+    d = {n: getattr(st, n) for n in _lib.FLOW_STAT_FIELDS}
+    out = {c: {"mean": d.pop(c + "_mean"), "max": d.pop(c + "_max"), "min": d.pop(c + "_min")} for c in "uvw"}
+    out.update(d)
+    return out
+
+
 def boundary(box, profile, stage):
     check(box.L.lcs_particles_boundary(box.handle, profile, int(stage)))
 

@@ -535,6 +535,119 @@ double o_force_linear(int nx, int ny, int nz, int h, double power, double dt, re
 }
 
 /* ------------------------------------------------------------------------- */
+/* Flow statistics: Flows.stat, src/flows/stat.jl:67-155.                     */
+/* Reads U,V,W with their ghosts as they are (the reference fills FullHalo    */
+/* first, stat.jl:79; the caller does that here).  out[15]:                   */
+/*   0-8   mean, max, min of U, V, W over the interior (LCS.Summary,          */
+/*         src/stat.jl:20-22);                                                */
+/*   9-13  interior means of u2 = u^2+v^2+w^2 (raw face values), s2 =          */
+/*         2 S_ij S_ij, and dudx^2, dudx^3, dudx^4 (stat.jl:84-125);           */
+/*   14    the integral length (stat.jl:157-182) if integral_length, else 0.   */
+/* ------------------------------------------------------------------------- */
+static inline real diff1_o4(real a, real b, real c, real d) /* src/diff.jl:13-15 */
+{
+    return R(1.0 / 24.0) * (((a - R(27) * b) + R(27) * c) - d);
+}
+
+void o_flow_stat(int nx, int ny, int nz, int h, const real *U, const real *V, const real *W,
+                 int integral_length, double *out)
+{
+    grid_t g = mkgrid(nx, ny, nz, h);
+    const long sy = g.sx, sz = g.sx * g.sy;
+    const real dxi = R(1.0 / spacing(nx)), dyi = R(1.0 / spacing(ny)), dzi = R(1.0 / spacing(nz));
+    double *part = (double *)calloc((size_t)nz * 14, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; k++) {
+        double s[14];
+        for (int a = 0; a < 3; a++) {
+            s[3 * a] = 0.0;
+            s[3 * a + 1] = -INFINITY;
+            s[3 * a + 2] = INFINITY;
+        }
+        for (int a = 9; a < 14; a++) s[a] = 0.0;
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                size_t p = AT(g, i, j, k);
+                real u = U[p], v = V[p], w = W[p];
+                real u2 = (u * u + v * v) + w * w;
+                real dudx = dxi * diff1_o4(U[p - 2], U[p - 1], U[p], U[p + 1]);
+                real dvdy = dyi * diff1_o4(V[p - 2 * sy], V[p - sy], V[p], V[p + sy]);
+                real dwdz = dzi * diff1_o4(W[p - 2 * sz], W[p - sz], W[p], W[p + sz]);
+                real dudy = dyi * diff1_o4(U[p - 2 * sy], U[p - sy], U[p], U[p + sy]);
+                /* V[(i-1):(i+2), j-1, k] */
+                real dvdx = dxi * diff1_o4(V[p - sy - 1], V[p - sy], V[p - sy + 1], V[p - sy + 2]);
+                /* V[i, j-1, (k-1):(k+2)] */
+                real dvdz = dzi * diff1_o4(V[p - sy - sz], V[p - sy], V[p - sy + sz], V[p - sy + 2 * sz]);
+                real dwdy = dyi * diff1_o4(W[p - 2 * sy], W[p - sy], W[p], W[p + sy]);
+                /* W[(i-1):(i+2), j, k-1] */
+                real dwdx = dxi * diff1_o4(W[p - sz - 1], W[p - sz], W[p - sz + 1], W[p - sz + 2]);
+                real dudz = dzi * diff1_o4(U[p - 2 * sz], U[p - sz], U[p], U[p + sz]);
+                real s11 = R(0.5) * (dudx + dudx), s22 = R(0.5) * (dvdy + dvdy), s33 = R(0.5) * (dwdz + dwdz);
+                real s12 = R(0.5) * (dudy + dvdx), s13 = R(0.5) * (dudz + dwdx), s23 = R(0.5) * (dvdz + dwdy);
+                real s2 = R(2) * (((s11 * s11 + s22 * s22) + s33 * s33) +
+                                  R(2) * ((s12 * s12 + s13 * s13) + s23 * s23));
+                real d2 = dudx * dudx;
+                real vals[3] = {u, v, w};
+                for (int a = 0; a < 3; a++) {
+                    s[3 * a] += (double)vals[a];
+                    s[3 * a + 1] = fmax(s[3 * a + 1], (double)vals[a]);
+                    s[3 * a + 2] = fmin(s[3 * a + 2], (double)vals[a]);
+                }
+                s[9] += (double)u2;
+                s[10] += (double)s2;
+                s[11] += (double)d2;
+                s[12] += (double)(d2 * dudx);
+                s[13] += (double)(d2 * d2);
+            }
+        for (int a = 0; a < 14; a++) part[14 * (k - 1) + a] = s[a];
+    }
+    double len = (double)nx * (double)ny * (double)nz;
+    for (int a = 0; a < 14; a++) {
+        double r = part[a];
+        for (int k = 1; k < nz; k++) {
+            double x = part[14 * k + a];
+            if (a < 9 && a % 3 == 1)
+                r = fmax(r, x);
+            else if (a < 9 && a % 3 == 2)
+                r = fmin(r, x);
+            else
+                r += x;
+        }
+        out[a] = (a < 9 && a % 3 != 0) ? r : r / len;
+    }
+    free(part);
+    out[14] = 0.0;
+    if (!integral_length) return;
+    /* integral_length: stat.jl:157-182.  R = irfft(|rfft(U)|^2) along x is the
+     * circular autocorrelation R[r] = sum_i U[i] U[i+r]; lags 0..n/2 are
+     * summed up to the first negative one, Ls = acc dx / R[0], L = mean(Ls). */
+    const int mid = nx / 2 + 1;
+    const double dx = spacing(nx);
+    double *ls = (double *)calloc((size_t)nz, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; k++) {
+        double acc_k = 0.0;
+        for (int j = 1; j <= ny; j++) {
+            const real *row = U + AT(g, 1, j, k);
+            double acc = 0.0, r0 = 0.0;
+            for (int r = 0; r < mid; r++) {
+                double c = 0.0;
+                for (int i = 0; i < nx; i++) c += (double)row[i] * (double)row[(i + r) % nx];
+                if (r == 0) r0 = c;
+                if (c < 0) break;
+                acc += c;
+            }
+            acc_k += acc * dx / r0;
+        }
+        ls[k - 1] = acc_k;
+    }
+    double L = 0.0;
+    for (int k = 0; k < nz; k++) L += ls[k];
+    free(ls);
+    out[14] = L / ((double)ny * (double)nz);
+}
+
+/* ------------------------------------------------------------------------- */
 /* rc-energy forcing: src/flows/forcings/rcf.jl:21-191,                      */
 /* lib/fouriertools/src/FourierTools.jl:9-208.                                */
 /* ------------------------------------------------------------------------- */

@@ -80,6 +80,7 @@ def _declare(L):
         "o_spectrum_modes": (I, [U64, D, I, P, P, I]),
         "o_init_spectrum": (None, [I, I, I, I, U64, D, D, P, P, P, P]),
         "o_init_particles": (None, [U64, Lg, D, D, D, D, D, P, P, P, P, P]),
+        "o_flow_stat": (None, [I, I, I, I, P, P, P, I, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -166,6 +167,41 @@ def force_linear(U, V, W, h, power, dt):
     m = np.zeros(5)
     p = lib(U.dtype).o_force_linear(nx, ny, nz, h, power, dt, _p(U), _p(V), _p(W), _p(m))
     return p, m
+
+
+def flow_stat_sums(U, V, W, h, integral_length=False):
+    """Raw interior statistics of Flows.stat (src/flows/stat.jl:67-130), read
+    with the ghosts as they are: (means, max, min of U, V, W), the means of
+    u2, s2, dudx^2, dudx^3, dudx^4, and the integral length (0 unless asked)."""
+    nx, ny, nz = dims_of(U, h)
+    out = np.zeros(15)
+    lib(U.dtype).o_flow_stat(nx, ny, nz, h, _p(U), _p(V), _p(W), int(bool(integral_length)), out.ctypes.data)
+    return out
+
+
+def flow_stat_derived(sums, dims, Re):
+    """The derived quantities of Flows.stat: src/flows/stat.jl:132-154."""
+    um, vm, wm = sums[0], sums[3], sums[6]
+    u2, s2, d2, d3, d4, L = sums[9:15]
+    urms = np.sqrt(u2 / 3)
+    k = 0.5 * (u2 - um**2 - vm**2 - wm**2)
+    eps = s2 / Re
+    tau_eta = np.sqrt(1 / eps / Re)
+    l_mu = np.sqrt(15 * urms**2 / eps / Re)
+    l_eta = (1 / eps / Re**3) ** 0.25
+    return {"u": {"mean": um, "max": sums[1], "min": sums[2]}, "v": {"mean": vm, "max": sums[4], "min": sums[5]},
+            "w": {"mean": wm, "max": sums[7], "min": sums[8]}, "k": k, "eps": eps, "tau_eta": tau_eta,
+            "l_mu": l_mu, "l_eta": l_eta, "Re_lambda": l_mu * urms * Re, "urms": urms,
+            "k_eta": (min(dims) / 2) * l_eta, "skew": -d3 / d2 ** 1.5, "flat": d4 / d2**2, "L": L,
+            "C_eps": eps * L / urms**3}
+
+
+def flow_stat(U, V, W, h, Re, integral_length=False):
+    """Flows.stat (src/flows/stat.jl:67-155) on copies: FullHalo(h) on U, V, W, then the statistics."""
+    Us = [np.array(a, order="F") for a in (U, V, W)]
+    for a in Us:
+        ghost_fill(a, h, h, True)
+    return flow_stat_derived(flow_stat_sums(*Us, h, integral_length), dims_of(U, h), Re)
 
 
 def boxmean(U, h, coarse):

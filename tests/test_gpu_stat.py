@@ -1,0 +1,90 @@
+"""Flow statistics on the device (lcs_flow_stat, Flows.stat:
+src/flows/stat.jl:53-182).
+
+* the device's statistics of a seeded field equal the oracle's (FP64: relative
+  1e-12 on every quantity, extrema exact; FP32: relative 1e-5), integral length
+  included, on a non-cubic box;
+* one sine mode uploaded to the device gives the closed forms (flatness 3/2,
+  skewness 0, k = A^2/4, integral length from (n/2) cos(theta r));
+* long runs: a 32^3 box stepped 1000 times (t = 10, forced
+  to a stationary state) keeps kinetic energy and dissipation within 0.5 % of
+  the FP64 oracle's at every 100th step, for FP64 and for FP32 boxes.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle.driver import OracleSim
+from test_gpu_step import c1_config
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("k", "eps", "tau_eta", "l_mu", "l_eta", "Re_lambda", "urms", "k_eta", "skew", "flat", "L", "C_eps")
+
+
+@pytest.mark.parametrize("dtype,rtol", [("f64", 1e-12), ("f32", 1e-5)])
+def test_stat_matches_oracle(dtype, rtol):
+    import lcs_mi355x as L
+    cfg = c1_config(dims=(24, 20, 28), dtype=dtype, profiles=0)
+    box = L.build(cfg)
+    dev = L.flow_stat(box, integral_length=True)
+    Us = [box.download(n) for n in ("U", "V", "W")]
+    ref = O.flow_stat(*Us, 3, 53.3, integral_length=True)
+    for c in "uvw":
+        assert dev[c]["max"] == ref[c]["max"] and dev[c]["min"] == ref[c]["min"], c
+        assert dev[c]["mean"] == pytest.approx(ref[c]["mean"], rel=1e-9, abs=1e-12), c
+    for k in KEYS:
+        assert dev[k] == pytest.approx(ref[k], rel=rtol), k
+    assert dev["L"] > 0 and 1.0 < dev["flat"] < 10.0
+    box.close()
+
+
+def test_single_mode_closed_forms_on_device():
+    import lcs_mi355x as L
+    n, m, A, h = 30, 4, 0.8, 3
+    cfg = c1_config(dims=(n, 8, 6), dtype="f64", profiles=0)
+    box = L.build(cfg)
+    dx = 2 * np.pi / n
+    x_face = (np.arange(n + 2 * h) + 1 - h) * dx
+    U = np.zeros((n + 2 * h, 8 + 2 * h, 6 + 2 * h), order="F")
+    U[...] = (A * np.sin(m * x_face))[:, None, None]
+    zero = np.zeros_like(U)
+    box.upload("U", U)
+    box.upload("V", zero)
+    box.upload("W", zero)
+    st = L.flow_stat(box, integral_length=True)
+    theta = m * dx
+    G = (1 / dx) * (27 * 2 * np.sin(theta / 2) - 2 * np.sin(3 * theta / 2)) / 24
+    assert st["flat"] == pytest.approx(1.5, rel=1e-12)
+    assert abs(st["skew"]) < 1e-12
+    assert st["k"] == pytest.approx(A**2 / 4, rel=1e-12)
+    assert st["eps"] == pytest.approx(A**2 * G**2 / 53.3, rel=1e-12)
+    lags = np.cos(theta * np.arange(n // 2 + 1))
+    stop = int(np.argmax(lags < 0))
+    assert st["L"] == pytest.approx(dx * lags[:stop].sum(), rel=1e-12)
+    box.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_long_run_energy_and_dissipation_within_half_percent(dtype):
+    import lcs_mi355x as L
+    cfg = c1_config(dims=(32, 32, 32), dtype=dtype, profiles=0)
+    box = L.build(cfg)
+    Us = [box.download(n).astype(np.float64) for n in ("U", "V", "W")]
+    sim = OracleSim(cfg, np.float64, Us, box.download("P").astype(np.float64), [])
+    scfg = L.step_config(cfg)
+    worst = {"k": 0.0, "eps": 0.0}
+    for s in range(1, 1001):
+        L.step(box, scfg, want_log=False)
+        sim.timestep()
+        sim.evolve()
+        if s % 100 == 0:
+            dev = L.flow_stat(box)
+            ref = O.flow_stat(*sim.Us, 3, 53.3)
+            for k in worst:
+                err = abs(dev[k] - ref[k]) / abs(ref[k])
+                worst[k] = max(worst[k], err)
+                assert err <= 5e-3, (s, k, dev[k], ref[k])
+    print(f"{dtype}: largest relative deviation over 1000 steps: k {worst['k']:.2e}, eps {worst['eps']:.2e}",
+          flush=True)
+    box.close()
