@@ -186,8 +186,8 @@ static int rebuild(Batch &B)
             fill_coefs<double>(b, B.hs, r);
         else
             fill_coefs<float>(b, B.hs, r);
-        // fused RHS tiles: 32 x (8 | 4), z-chunks of about 64 planes
-        const int rtx = 32, rty = b.esize == 8 ? 4 : 8;
+        // fused RHS tiles (kBatchRhsTX x batch_rhs_ty), z-chunks of about 64 planes
+        const int rtx = kBatchRhsTX, rty = batch_rhs_ty(b.esize);
         const int rgx = (g.nx + rtx - 1) / rtx, rgy = (g.ny + rty - 1) / rty;
         const int rgz = std::max(1, g.nz / 64);
         r.rhs_zc = (g.nz + rgz - 1) / rgz;

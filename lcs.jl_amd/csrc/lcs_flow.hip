@@ -315,7 +315,7 @@ template <typename T>
 struct RhsShape;
 template <>
 struct RhsShape<float> {
-    static constexpr int TX = 32, TY = 8;
+    static constexpr int TX = 64, TY = 16;
 };
 template <>
 struct RhsShape<double> {
@@ -695,7 +695,10 @@ struct HsmacRun {
     }
     int fused(const Box3 &r)
     {
-        constexpr int TY = 8;
+        // 64 x 16 columns: the 2-cell halo ring costs 20 rows per 16 owned
+        // (12 per 8 with TY = 8), and one 1024-thread workgroup per CU keeps
+        // fewer tiles per XCD competing for its L2
+        constexpr int TY = 16;
         return coloring == LCS_COLORING_RED_BLACK ? fused_launch<LCS_COLORING_RED_BLACK, TY>(r)
                                                   : fused_launch<LCS_COLORING_RED_BLACK_FAST, TY>(r);
     }
@@ -1917,7 +1920,7 @@ __global__ void __launch_bounds__(TX *TY) k_b_rhs_tile(const BoxRec *tab, const 
 template <typename T>
 static int b_rhs_t(hipStream_t s, const BoxRec *tab, const TileItem *tiles, int ntiles, int stage)
 {
-    constexpr int TX = RhsShape<T>::TX, TY = RhsShape<T>::TY;
+    constexpr int TX = kBatchRhsTX, TY = batch_rhs_ty(sizeof(T));
     using Tile = RhsTile<T, TX, TY>;
     const size_t lds = Tile::LDS_ELEMS * sizeof(T);
     static bool attr = false;

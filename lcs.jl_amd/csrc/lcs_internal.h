@@ -258,7 +258,11 @@ struct HsPoll {
     int done, par, itr, pad;
     double div;
 };
-constexpr int kBatchRowBlocks = 64;  // row-walking reduction blocks per batched box
+constexpr int kBatchRowBlocks = 64;
+// fused RHS tile of the batched kernels: narrower than the single-box tile so
+// 32^3 boxes do not leave most of a workgroup idle
+constexpr int kBatchRhsTX = 32;
+__host__ __device__ constexpr int batch_rhs_ty(int esize) { return esize == 8 ? 4 : 8; }  // row-walking reduction blocks per batched box
 
 // batched launchers (lcs_flow.hip / lcs_particles.hip); all enqueue on `s`
 int launch_b_ghost_fill(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *work, int nwork, int set,

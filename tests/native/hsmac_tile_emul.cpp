@@ -52,7 +52,8 @@ extern "C" int hsmac_sweep_emul(int ty, int tx, int esize, int coloring, int nx,
 {
 #define RUN(T, C, Y, X) *div2 = run<T, C, Y, X>(nx, ny, nz, h, lo, hi, dt, omega, src, dst, zc)
 #define BY_SHAPE(T, C)                              \
-    if (ty == 4 && tx == 64) RUN(T, C, 4, 64);      \
+    if (ty == 16 && tx == 64) RUN(T, C, 16, 64);    \
+    else if (ty == 4 && tx == 64) RUN(T, C, 4, 64); \
     else if (ty == 8 && tx == 64) RUN(T, C, 8, 64); \
     else if (ty == 8 && tx == 32) RUN(T, C, 8, 32); \
     else if (ty == 4 && tx == 32) RUN(T, C, 4, 32); \

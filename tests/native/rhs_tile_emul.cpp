@@ -57,12 +57,16 @@ extern "C" int rhs_tile_emul(int shape, int esize, int nx, int ny, int nz, int h
         case 0: return RUN(float, 64, 4);
         case 1: return RUN(float, 8, 4);
         case 2: return RUN(float, 16, 2);
+        case 3: return RUN(float, 64, 16);
+        case 4: return RUN(float, 32, 8);
         }
     } else {
         switch (shape) {
         case 0: return RUN(double, 64, 4);
         case 1: return RUN(double, 8, 4);
         case 2: return RUN(double, 16, 2);
+        case 3: return RUN(double, 64, 16);
+        case 4: return RUN(double, 32, 8);
         }
     }
 #undef RUN

@@ -45,7 +45,7 @@ def _box(name, n):
 CASES = [((8, 8, 8), 4, 64, 14, "d1"), ((16, 12, 10), 4, 64, 5, "d2"), ((70, 9, 7), 8, 64, 3, "d3"),
          ((17, 5, 9), 8, 64, 4, "core"), ((20, 13, 11), 4, 64, 1, "d1"), ((66, 20, 6), 8, 64, 12, "d3"),
          ((5, 3, 4), 4, 64, 2, "d1"), ((40, 10, 9), 8, 32, 4, "d1"), ((27, 9, 6), 8, 32, 2, "core"),
-         ((33, 7, 5), 4, 32, 3, "d2")]
+         ((33, 7, 5), 4, 32, 3, "d2"), ((70, 20, 6), 16, 64, 3, "d1"), ((12, 35, 5), 16, 64, 2, "core")]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])

@@ -18,7 +18,7 @@ import pytest
 from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SHAPES = {0: (64, 4), 1: (8, 4), 2: (16, 2)}
+SHAPES = {0: (64, 4), 1: (8, 4), 2: (16, 2), 3: (64, 16), 4: (32, 8)}
 
 
 @pytest.fixture(scope="module")
@@ -49,7 +49,7 @@ def _run(lib, shape, arrays, h, Re, out, base=None, dt=0.0, zc=None, fused=None)
 
 
 CASES = [((16, 16, 16), 1, 16), ((20, 13, 11), 1, 4), ((70, 9, 7), 0, 3), ((17, 5, 9), 2, 5),
-         ((8, 8, 8), 0, 8), ((33, 3, 12), 2, 1), ((5, 7, 6), 1, 2)]
+         ((8, 8, 8), 0, 8), ((33, 3, 12), 2, 1), ((5, 7, 6), 1, 2), ((70, 20, 6), 3, 4), ((40, 17, 5), 4, 2)]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
