@@ -116,6 +116,11 @@ def bench_ensemble(args):
           f"{sum(b.nbytes() for b in boxes) / 2**30:.1f} GiB", file=sys.stderr, flush=True)
     scfgs = [L.step_config(c, localize_interval=args.localize_interval) for c in cfgs]
     cells = sum(math.prod(c.grid.dims) for c in cfgs)
+    if not args.no_autotune:  # the serial baseline runs each box with its best single-box tiles
+        t0 = time.perf_counter()
+        for b in boxes:
+            L.autotune(b)
+        print(f"# autotuned the serial boxes in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     # serial: every box alone, one lcs_step at a time
     for b, s in zip(boxes, scfgs):
         L.step(b, s, want_log=False)
@@ -238,6 +243,8 @@ def bench_serve(args):
                                                                             st_max=r["st"][1], eps=0.5)),),
                        dtype="f32")
         b = L.build(cfg)
+        if not args.no_autotune:
+            L.autotune(b)
         boxes.append((b, L.step_config(cfg, localize_interval=args.localize_interval)))
     for b, s in boxes:
         L.step(b, s, want_log=False)
@@ -315,6 +322,7 @@ def main():
     ap.add_argument("--cpu-size", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="use the size rule for tile shapes")
     ap.add_argument("--localize-interval", type=int, default=10,
                     help="reorder particle storage by position every N steps (0: never; changes no result)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -337,6 +345,11 @@ def main():
     init_s = time.perf_counter() - t0
     print(f"# built {desc} in {init_s:.1f} s, {box.nbytes() / 2**30:.1f} GiB on the device", file=sys.stderr,
           flush=True)
+    t0 = time.perf_counter()
+    tiles = L.get_tiles(box) if args.no_autotune else L.autotune(box)
+    tune_s = time.perf_counter() - t0
+    print(f"# tiles {tiles} ({'size rule' if args.no_autotune else f'autotuned in {tune_s:.1f} s'})",
+          file=sys.stderr, flush=True)
     scfg = L.step_config(cfg, localize_interval=args.localize_interval)
     for s in range(args.warmup):
         log = L.step(box, scfg)
@@ -366,7 +379,8 @@ def main():
     if os.path.exists(tf):
         with open(tf) as fh:
             t = json.load(fh)
-        if t.get("workload") == args.workload and t.get("size") == cfg.grid.dims[0]:
+        if (t.get("workload") == args.workload and t.get("size") == cfg.grid.dims[0]
+                and t.get("tiles") == {k: list(v) for k, v in tiles.items()}):
             traffic = t.get("bytes_per_launch")
     kernels = {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1],
                    "algo_GBs": (v[2] / (v[0] * 1e-3) / 1e9) if v[0] > 0 else None} for k, v in prof.items()}
@@ -379,7 +393,8 @@ def main():
         "vs_baseline": None, "dtype": cfg.dtype.replace("f", "f"), "data": "synthetic (seeded spectrum init)",
         "config": {"workload": desc, "grid": list(cfg.grid.dims), "particles": sum(p.population.valid for p in
                                                                                      cfg.particles),
-                   "hsmac_sweeps_per_step": s_mean, "particle_localize_interval": args.localize_interval},
+                   "hsmac_sweeps_per_step": s_mean, "particle_localize_interval": args.localize_interval,
+                   "tiles": {k: list(v) for k, v in tiles.items()}, "autotuned": not args.no_autotune},
         "step_hbm": {"bytes_per_cell_step": b_cell, "achieved_GBs": value * b_cell / 1e9,
                      "frac": value * b_cell / 1e9 / PEAK_HBM_GBS},
         "roofline": {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -197,6 +197,29 @@ int lcs_force_rcf(lcs_box *box, double energy, int kmin, int kmax, int coarse_si
  * receives 3*c^3 values (U then V then W), column-major, as doubles */
 int lcs_boxmean(lcs_box *box, int coarse_size, double *coarse3);
 
+/* ---- tile shapes ------------------------------------------------------------ */
+/* No counterpart in the reference (its kernels are KernelAbstractions
+ * workgroups of a fixed size).  The fused RHS and HSMAC sweep kernels march
+ * tx x ty columns of cells up in z; the shape changes speed, never results.
+ * Sweep shapes: (64|32) x (16|8).  RHS shapes: FP32 (64|32) x (16|8), FP64
+ * 64x8, 32x8, 32x4.  A zero shape means the size rule. */
+typedef struct {
+    int sweep_tx, sweep_ty;
+    int rhs_tx, rhs_ty;
+} lcs_tile_shapes;
+
+/* Times every candidate shape on a scratch box with this box's x-y extent (at
+ * most 128 planes deep) and keeps the fastest for this box; the choice is
+ * cached per device, size and float type, so later boxes of the same size
+ * take it at once.  Falls back to the size rule if the scratch box does not
+ * fit in device memory.  chosen may be NULL. */
+int lcs_autotune(lcs_box *box, lcs_tile_shapes *chosen);
+/* Force shapes (NULL: back to the size rule); LCS_ERR_INVALID if a shape is
+ * not one of the candidates. */
+int lcs_set_tiles(lcs_box *box, const lcs_tile_shapes *shapes);
+/* The shapes the next launches will use (the size rule resolved). */
+int lcs_get_tiles(lcs_box *box, lcs_tile_shapes *shapes);
+
 /* ---- flow statistics -------------------------------------------------------- */
 /* Flows.FlowStat (src/flows/stat.jl:1-42): summaries (mean, max, min) of the
  * raw U, V, W interior values (LCS.Summary, src/stat.jl:4-22) and the derived

@@ -3,6 +3,9 @@
 // the whole-step driver (LCS.timestep! + LCS.evolve!).
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cstring>
 #include <string>
 
@@ -392,6 +395,156 @@ int lcs_force_rcf(lcs_box *box, double energy, int kmin, int kmax, int c, double
         (rc = read_scalars(b)))
         return rc;
     if (p) *p = b.S_host->p;
+    return LCS_OK;
+}
+
+// ---- tile shapes ---------------------------------------------------------------
+static bool shape_ok(int esize, int kind, int tx, int ty)
+{
+    int c[8][2];
+    const int n = tile_candidates(esize, kind, c, 8);
+    for (int i = 0; i < n; i++)
+        if (c[i][0] == tx && c[i][1] == ty) return true;
+    return false;
+}
+
+// One time-blocked sweep group (three fused sweeps with their fills and
+// checks) and one fused stage-1 RHS on scratch box t, with shapes s.
+static int time_shapes(Box &t, const TileShapes &s, int coloring, float &sweep_ms, float &rhs_ms)
+{
+    t.tiles = s;
+    const lcs_hsmac_params p = {1, 0, coloring, 1.7, 0.0, 3, 3};  // epsp 0: exactly itrmax sweeps
+    void *out[3] = {t.f[LCS_FIELD_U2], t.f[LCS_FIELD_V2], t.f[LCS_FIELD_W2]};
+    void *base[3] = {t.f[LCS_FIELD_U], t.f[LCS_FIELD_V], t.f[LCS_FIELD_W]};
+    hipEvent_t e[3];
+    for (auto &x : e) LCS_HIP(hipEventCreate(&x));
+    int rc = LCS_OK;
+    for (int rep = 0; rep < 2 && rc == LCS_OK; rep++) {  // the first round sets kernel attributes
+        if (hipEventRecord(e[0], t.stream) != hipSuccess) rc = fail(LCS_ERR_HIP, "hipEventRecord failed");
+        if (rc == LCS_OK) rc = launch_hsmac(t, &t.f[LCS_FIELD_U], DtRef{&t.S->dt_phase, 1.0}, p, -1);
+        if (rc == LCS_OK && hipEventRecord(e[1], t.stream) != hipSuccess) rc = fail(LCS_ERR_HIP, "hipEventRecord failed");
+        if (rc == LCS_OK) rc = launch_rhs(t, 1, true, out, base, DtRef{&t.S->dt_phase, 0.5});
+        if (rc == LCS_OK && hipEventRecord(e[2], t.stream) != hipSuccess) rc = fail(LCS_ERR_HIP, "hipEventRecord failed");
+        if (rc == LCS_OK && hipEventSynchronize(e[2]) != hipSuccess) rc = fail(LCS_ERR_HIP, "hipEventSynchronize failed");
+    }
+    if (rc == LCS_OK && (hipEventElapsedTime(&sweep_ms, e[0], e[1]) != hipSuccess ||
+                         hipEventElapsedTime(&rhs_ms, e[1], e[2]) != hipSuccess))
+        rc = fail(LCS_ERR_HIP, "hipEventElapsedTime failed");
+    for (auto &x : e) (void)hipEventDestroy(x);
+    return rc;
+}
+
+struct TuneKey {
+    int device, nx, ny, nz, h, esize;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(device, nx, ny, nz, h, esize) < std::tie(o.device, o.nx, o.ny, o.nz, o.h, o.esize);
+    }
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, TileShapes> g_tune_cache;
+
+static void to_c(const TileShapes &t, lcs_tile_shapes *o)
+{
+    if (!o) return;
+    o->sweep_tx = t.sweep_tx;
+    o->sweep_ty = t.sweep_ty;
+    o->rhs_tx = t.rhs_tx;
+    o->rhs_ty = t.rhs_ty;
+}
+
+int lcs_autotune(lcs_box *box, lcs_tile_shapes *chosen)
+{
+    if (check_box(box)) return LCS_ERR_INVALID;
+    Box &b = *B(box);
+    const TuneKey key{b.ctx->device, b.g.nx, b.g.ny, b.g.nz, b.g.h, b.esize};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            b.tiles = it->second;
+            to_c(b.tiles, chosen);
+            return LCS_OK;
+        }
+    }
+    if (b.g.h < 3) {  // time-blocked sweeps need a 3-wide halo: keep the rule
+        b.tiles = tiles_default(b.g, b.esize);
+        to_c(b.tiles, chosen);
+        return LCS_OK;
+    }
+    lcs_box_desc d = b.desc;
+    d.nz = std::min(d.nz, 128);
+    lcs_box *tb = nullptr;
+    if (lcs_box_create(reinterpret_cast<lcs_ctx *>(b.ctx), &d, &tb) != LCS_OK) {
+        b.tiles = tiles_default(b.g, b.esize);  // no room for a scratch box: keep the rule
+        to_c(b.tiles, chosen);
+        return LCS_OK;
+    }
+    Box &t = *B(tb);
+    int rc = set_phase_dt(t, 1e-3);
+    TileShapes best = tiles_default(b.g, b.esize);
+    int sw[8][2], rh[8][2];
+    const int nsw = tile_candidates(b.esize, 0, sw, 8), nrh = tile_candidates(b.esize, 1, rh, 8);
+    float best_sweep = 1e30f, best_rhs = 1e30f;
+    // sweep shapes with the rule's RHS shape, then RHS shapes with the best sweep
+    for (int i = 0; i < nsw && rc == LCS_OK; i++) {
+        TileShapes s = best;
+        s.sweep_tx = sw[i][0];
+        s.sweep_ty = sw[i][1];
+        float ms_sw = 0, ms_rh = 0;
+        rc = time_shapes(t, s, LCS_COLORING_RED_BLACK_FAST, ms_sw, ms_rh);
+        if (rc == LCS_OK && ms_sw < best_sweep) {
+            best_sweep = ms_sw;
+            best.sweep_tx = s.sweep_tx;
+            best.sweep_ty = s.sweep_ty;
+        }
+    }
+    for (int i = 0; i < nrh && rc == LCS_OK; i++) {
+        TileShapes s = best;
+        s.rhs_tx = rh[i][0];
+        s.rhs_ty = rh[i][1];
+        float ms_sw = 0, ms_rh = 0;
+        rc = time_shapes(t, s, LCS_COLORING_RED_BLACK_FAST, ms_sw, ms_rh);
+        if (rc == LCS_OK && ms_rh < best_rhs) {
+            best_rhs = ms_rh;
+            best.rhs_tx = s.rhs_tx;
+            best.rhs_ty = s.rhs_ty;
+        }
+    }
+    const int drc = lcs_box_destroy(tb);
+    if (rc) return rc;
+    if (drc) return drc;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = best;
+    }
+    b.tiles = best;
+    to_c(best, chosen);
+    return LCS_OK;
+}
+
+int lcs_set_tiles(lcs_box *box, const lcs_tile_shapes *sh)
+{
+    if (check_box(box)) return LCS_ERR_INVALID;
+    Box &b = *B(box);
+    if (!sh) {
+        b.tiles = TileShapes{};
+        return LCS_OK;
+    }
+    if (!shape_ok(b.esize, 0, sh->sweep_tx, sh->sweep_ty)) return fail(LCS_ERR_INVALID, "unsupported sweep tile shape");
+    if (!shape_ok(b.esize, 1, sh->rhs_tx, sh->rhs_ty)) return fail(LCS_ERR_INVALID, "unsupported RHS tile shape");
+    b.tiles.sweep_tx = sh->sweep_tx;
+    b.tiles.sweep_ty = sh->sweep_ty;
+    b.tiles.rhs_tx = sh->rhs_tx;
+    b.tiles.rhs_ty = sh->rhs_ty;
+    return LCS_OK;
+}
+
+int lcs_get_tiles(lcs_box *box, lcs_tile_shapes *sh)
+{
+    if (check_box(box) || !sh) return fail(LCS_ERR_INVALID, "null argument");
+    Box &b = *B(box);
+    to_c(b.tiles.sweep_tx ? b.tiles : tiles_default(b.g, b.esize), sh);
     return LCS_OK;
 }
 

@@ -311,21 +311,9 @@ __global__ void __launch_bounds__(TX *TY) k_rhs_tile(RhsArgs<T> a, DtRef dt)
 }
 
 // Tile shapes: FP32 32x8 (39 KB of LDS, four workgroups per CU), FP64 32x4 (53 KB).
-template <typename T>
-struct RhsShape;
-template <>
-struct RhsShape<float> {
-    static constexpr int TX = 64, TY = 16;
-};
-template <>
-struct RhsShape<double> {
-    static constexpr int TX = 32, TY = 4;
-};
-
-template <typename T, bool FUSED>
+template <typename T, bool FUSED, int TX, int TY>
 static int rhs_tile_launch(Box &b, const RhsArgs<T> &a0, DtRef dt)
 {
-    constexpr int TX = RhsShape<T>::TX, TY = RhsShape<T>::TY;
     using Tile = RhsTile<T, TX, TY>;
     static_assert(Tile::LDS_ELEMS * sizeof(T) <= 160 * 1024, "tile exceeds the LDS of a CU");
     const Geom &g = a0.g;
@@ -348,6 +336,61 @@ static int rhs_tile_launch(Box &b, const RhsArgs<T> &a0, DtRef dt)
     hipLaunchKernelGGL((k_rhs_tile<T, TX, TY, FUSED>), dim3(gx, gy, gz), dim3(TX * TY), lds, b.stream, a, dt);
     LCS_HIP(hipGetLastError());
     return LCS_OK;
+}
+
+// Tile shapes (tx, ty) the fused kernels are built for.  Sweep: 64 or 32
+// wide, 16 or 8 tall.  RHS: FP32 up to 64 x 16; FP64 up to 64 x 8 (LDS).
+static const int kSweepShapes[][2] = {{64, 16}, {64, 8}, {32, 16}, {32, 8}};
+static const int kRhsShapesF[][2] = {{64, 16}, {64, 8}, {32, 16}, {32, 8}};
+static const int kRhsShapesD[][2] = {{64, 8}, {32, 8}, {32, 4}};
+
+int tile_candidates(int esize, int kind, int out[][2], int cap)
+{
+    const int(*src)[2] = kind == 0 ? kSweepShapes : (esize == 8 ? kRhsShapesD : kRhsShapesF);
+    const int n = kind == 0 ? 4 : (esize == 8 ? 3 : 4);
+    for (int i = 0; i < n && i < cap; i++) {
+        out[i][0] = src[i][0];
+        out[i][1] = src[i][1];
+    }
+    return n;
+}
+
+// Size rule used until a box is tuned: tall tiles only where the planes are
+// large enough to fill the chip with one 1024-thread workgroup per CU; sweep
+// width by lane efficiency over the whole array row.
+TileShapes tiles_default(const Geom &g, int esize)
+{
+    TileShapes t;
+    const bool big = (long)g.nx * g.ny >= 256L * 256L;
+    const int X = g.nx + 2 * g.h;
+    const double e64 = (double)X / (64.0 * ((X + 63) / 64)), e32 = (double)X / (32.0 * ((X + 31) / 32));
+    t.sweep_tx = e32 > e64 + 0.05 ? 32 : 64;
+    t.sweep_ty = big ? 16 : 8;
+    if (esize == 4) {
+        t.rhs_tx = big ? 64 : 32;
+        t.rhs_ty = big ? 16 : 8;
+    } else {
+        t.rhs_tx = 32;
+        t.rhs_ty = 4;
+    }
+    return t;
+}
+
+template <typename T, bool FUSED>
+static int rhs_shape(Box &b, const RhsArgs<T> &a, DtRef dt)
+{
+    const TileShapes t = b.tiles.rhs_tx ? b.tiles : tiles_default(b.g, b.esize);
+    if constexpr (sizeof(T) == 4) {
+        if (t.rhs_tx == 64 && t.rhs_ty == 16) return rhs_tile_launch<T, FUSED, 64, 16>(b, a, dt);
+        if (t.rhs_tx == 64 && t.rhs_ty == 8) return rhs_tile_launch<T, FUSED, 64, 8>(b, a, dt);
+        if (t.rhs_tx == 32 && t.rhs_ty == 16) return rhs_tile_launch<T, FUSED, 32, 16>(b, a, dt);
+        if (t.rhs_tx == 32 && t.rhs_ty == 8) return rhs_tile_launch<T, FUSED, 32, 8>(b, a, dt);
+    } else {
+        if (t.rhs_tx == 64 && t.rhs_ty == 8) return rhs_tile_launch<T, FUSED, 64, 8>(b, a, dt);
+        if (t.rhs_tx == 32 && t.rhs_ty == 8) return rhs_tile_launch<T, FUSED, 32, 8>(b, a, dt);
+        if (t.rhs_tx == 32 && t.rhs_ty == 4) return rhs_tile_launch<T, FUSED, 32, 4>(b, a, dt);
+    }
+    return fail(LCS_ERR_INVALID, "unsupported RHS tile shape");
 }
 
 template <typename T>
@@ -374,7 +417,7 @@ static int rhs_t(Box &b, int stage_in, bool fused, void *const out[3], void *con
     // algorithmic bytes: read U,V,W,P (+ the three base fields in stage 2), write 3
     ProfScope ps(b, fused ? "rhs_update" : "rhs",
                  (double)g.cells() * sizeof(T) * (fused && stage_in == 2 ? 10 : 7));
-    return fused ? rhs_tile_launch<T, true>(b, a, dt) : rhs_tile_launch<T, false>(b, a, dt);
+    return fused ? rhs_shape<T, true>(b, a, dt) : rhs_shape<T, false>(b, a, dt);
 }
 
 int launch_rhs(Box &b, int stage_in, bool fused, void *const out[3], void *const base[3], DtRef dt)
@@ -537,9 +580,9 @@ __device__ inline void hs_bind(HsArgs<T> &a, const BufSets<T> &bs, bool par)
     a.Pd = par ? bs.a[3] : bs.b[3];
 }
 
-template <typename T, int COLORING, int TY>
-__global__ void __launch_bounds__(64 * TY) k_hsmac_sweep(Geom g, BufSets<T> bs, Box3 R, HsCoef<T> c, DtRef dt,
-                                                        DevScalars *S, double *part, int zc)
+template <typename T, int COLORING, int TY, int TX>
+__global__ void __launch_bounds__(TX *TY) k_hsmac_sweep(Geom g, BufSets<T> bs, Box3 R, HsCoef<T> c, DtRef dt,
+                                                       DevScalars *S, double *part, int zc)
 {
     extern __shared__ __align__(16) unsigned char hs_lds_raw[];
     __shared__ double red_lds[16];
@@ -553,7 +596,7 @@ __global__ void __launch_bounds__(64 * TY) k_hsmac_sweep(Geom g, BufSets<T> bs, 
     a.zc = zc;
     int bx, by, bz;
     xcd_tile(bx, by, bz);
-    const double s = hsmac_sweep_block<T, COLORING, TY, 64>(a, bx, by, bz, reinterpret_cast<T *>(hs_lds_raw), red_lds);
+    const double s = hsmac_sweep_block<T, COLORING, TY, TX>(a, bx, by, bz, reinterpret_cast<T *>(hs_lds_raw), red_lds);
     if (threadIdx.x == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
 }
 
@@ -645,10 +688,10 @@ struct HsmacRun {
         return LCS_OK;
     }
     // one fused red+black sweep over a full box, then the parity flip
-    template <int COLORING, int TY>
+    template <int COLORING, int TY, int TX>
     int fused_launch(const Box3 &r)
     {
-        using Tile = HsTile<T, COLORING, TY>;
+        using Tile = HsTile<T, COLORING, TY, TX>;
         static_assert(Tile::LDS_ELEMS * sizeof(T) <= 160 * 1024, "tile exceeds the LDS of a CU");
         const Geom &g = b.g;
         const int gx = (g.nx + 2 * g.h + Tile::TX - 1) / Tile::TX, gy = (g.ny + 2 * g.h + TY - 1) / TY;
@@ -657,10 +700,10 @@ struct HsmacRun {
         const size_t lds = Tile::LDS_ELEMS * sizeof(T);
         static int per_cu = 0;  // resident workgroups per CU, per template instance
         if (!per_cu) {
-            LCS_HIP(hipFuncSetAttribute((const void *)k_hsmac_sweep<T, COLORING, TY>,
+            LCS_HIP(hipFuncSetAttribute((const void *)k_hsmac_sweep<T, COLORING, TY, TX>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            LCS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_hsmac_sweep<T, COLORING, TY>, Tile::NT,
-                                                                 lds));
+            LCS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_hsmac_sweep<T, COLORING, TY, TX>,
+                                                                 Tile::NT, lds));
             if (per_cu < 1) per_cu = 1;
         }
         // z-chunks: a chunk re-reads five halo planes and the last round of
@@ -685,7 +728,7 @@ struct HsmacRun {
         long cells = (long)(r.hi[0] - r.lo[0] + 1) * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
         ProfScope ps(b, "hsmac_sweep", 8.0 * sizeof(T) * cells);
         sweep_bytes.push_back(8.0 * sizeof(T) * cells);
-        hipLaunchKernelGGL((k_hsmac_sweep<T, COLORING, TY>), dim3(gx, gy, nzc), dim3(Tile::NT), lds, b.stream, g,
+        hipLaunchKernelGGL((k_hsmac_sweep<T, COLORING, TY, TX>), dim3(gx, gy, nzc), dim3(Tile::NT), lds, b.stream, g,
                            bs, r, c, dt, b.S, part, zc);
         LCS_HIP(hipGetLastError());
         ca.cnt[npass++] = (int)(cols * nzc);
@@ -693,14 +736,21 @@ struct HsmacRun {
         LCS_HIP(hipGetLastError());
         return LCS_OK;
     }
+    template <int COLORING>
+    int fused_shape(const Box3 &r, int tx, int ty)
+    {
+        if (tx == 64 && ty == 16) return fused_launch<COLORING, 16, 64>(r);
+        if (tx == 64 && ty == 8) return fused_launch<COLORING, 8, 64>(r);
+        if (tx == 32 && ty == 16) return fused_launch<COLORING, 16, 32>(r);
+        if (tx == 32 && ty == 8) return fused_launch<COLORING, 8, 32>(r);
+        return fail(LCS_ERR_INVALID, "unsupported sweep tile shape");
+    }
+    // one fused sweep with the box's tile shape
     int fused(const Box3 &r)
     {
-        // 64 x 16 columns: the 2-cell halo ring costs 20 rows per 16 owned
-        // (12 per 8 with TY = 8), and one 1024-thread workgroup per CU keeps
-        // fewer tiles per XCD competing for its L2
-        constexpr int TY = 16;
-        return coloring == LCS_COLORING_RED_BLACK ? fused_launch<LCS_COLORING_RED_BLACK, TY>(r)
-                                                  : fused_launch<LCS_COLORING_RED_BLACK_FAST, TY>(r);
+        const TileShapes t = b.tiles.sweep_tx ? b.tiles : tiles_default(b.g, b.esize);
+        return coloring == LCS_COLORING_RED_BLACK ? fused_shape<LCS_COLORING_RED_BLACK>(r, t.sweep_tx, t.sweep_ty)
+                                                  : fused_shape<LCS_COLORING_RED_BLACK_FAST>(r, t.sweep_tx, t.sweep_ty);
     }
     // Topologies.faces order z-, z+, y-, y+, x-, x+ (lib/topologies/src/halo.jl:349-366)
     int sweep_core_faces(const Box3 &d)

@@ -128,6 +128,16 @@ struct Profiler {
 
 struct Batch;
 
+// Tile shapes of the fused RHS and HSMAC sweep kernels of one box.  Results
+// never depend on them (every shape computes the same values); 0 means the
+// size rule of tiles_default.  lcs_autotune picks them by timing.
+struct TileShapes {
+    int sweep_tx = 0, sweep_ty = 0, rhs_tx = 0, rhs_ty = 0;
+};
+TileShapes tiles_default(const Geom &g, int esize);
+// candidate (tx, ty) pairs: kind 0 sweep, 1 RHS; returns the count
+int tile_candidates(int esize, int kind, int out[][2], int cap);
+
 struct Box {
     Ctx *ctx = nullptr;
     lcs_box_desc desc;
@@ -147,6 +157,7 @@ struct Box {
     size_t bytes = 0;
     int64_t steps_host = 0;  // steps taken by this box (lcs_step or a batch)
     Batch *batch = nullptr;  // the batch that steps this box, if any
+    TileShapes tiles;        // fused-kernel tile shapes (0: size rule)
     double Re() const { return desc.U0_m_s * desc.L0_m / desc.NU_m2_s; }
 };
 

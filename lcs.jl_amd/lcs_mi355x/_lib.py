@@ -84,6 +84,10 @@ FLOW_STAT_FIELDS = ["u_mean", "u_max", "u_min", "v_mean", "v_max", "v_min", "w_m
                     "tau_eta", "l_mu", "l_eta", "Re_lambda", "urms", "k_eta", "skew", "flat", "L", "C_eps"]
 
 
+class TileShapes(C.Structure):
+    _fields_ = [("sweep_tx", C.c_int), ("sweep_ty", C.c_int), ("rhs_tx", C.c_int), ("rhs_ty", C.c_int)]
+
+
 class FlowStat(C.Structure):
     _fields_ = [(n, C.c_double) for n in FLOW_STAT_FIELDS]
 
@@ -118,6 +122,9 @@ SIGNATURES = {
     "lcs_force_rcf": [P, D, I, I, I, D, C.POINTER(D)],
     "lcs_boxmean": [P, I, P],
     "lcs_flow_stat": [P, I, C.POINTER(FlowStat)],
+    "lcs_autotune": [P, C.POINTER(TileShapes)],
+    "lcs_set_tiles": [P, C.POINTER(TileShapes)],
+    "lcs_get_tiles": [P, C.POINTER(TileShapes)],
     "lcs_particles_add": [P, C.POINTER(ParticleDesc), C.POINTER(I)],
     "lcs_particles_upload": [P, I, I, P, C.c_size_t],
     "lcs_particles_download": [P, I, I, P, C.c_size_t],

@@ -238,6 +238,37 @@ def boxmean(box, coarse_size):
     return [out[a * c ** 3:(a + 1) * c ** 3].reshape((c, c, c), order="F") for a in range(3)]
 
 
+def _shapes(t):
+    return {"sweep": (t.sweep_tx, t.sweep_ty), "rhs": (t.rhs_tx, t.rhs_ty)}
+
+
+def autotune(box):
+    """Time the candidate tile shapes of the fused RHS and sweep kernels for
+    this box's size and keep the fastest (cached per size).  No result
+    changes.  Returns {"sweep": (tx, ty), "rhs": (tx, ty)}."""
+    t = _lib.TileShapes()
+    check(box.L.lcs_autotune(box.handle, C.byref(t)))
+    return _shapes(t)
+
+
+def set_tiles(box, sweep=None, rhs=None):
+    """Force tile shapes; with no arguments, return to the size rule."""
+    if sweep is None and rhs is None:
+        check(box.L.lcs_set_tiles(box.handle, None))
+        return
+    cur = get_tiles(box)
+    sweep = sweep or cur["sweep"]
+    rhs = rhs or cur["rhs"]
+    t = _lib.TileShapes(int(sweep[0]), int(sweep[1]), int(rhs[0]), int(rhs[1]))
+    check(box.L.lcs_set_tiles(box.handle, C.byref(t)))
+
+
+def get_tiles(box):
+    t = _lib.TileShapes()
+    check(box.L.lcs_get_tiles(box.handle, C.byref(t)))
+    return _shapes(t)
+
+
 def flow_stat(box, integral_length=False):
     """Flows.stat (src/flows/stat.jl:53-155): statistics of the box's current
     velocity (after a FullHalo ghost fill).  Returns a dict with the
