@@ -209,7 +209,7 @@ typedef struct {
 } lcs_tile_shapes;
 
 /* Times every candidate shape on a scratch box with this box's x-y extent (at
- * most 128 planes deep) and keeps the fastest for this box; the choice is
+ * most 256 planes deep) and keeps the fastest for this box; the choice is
  * cached per device, size and float type, so later boxes of the same size
  * take it at once.  Falls back to the size rule if the scratch box does not
  * fit in device memory.  chosen may be NULL. */

@@ -419,17 +419,23 @@ static int time_shapes(Box &t, const TileShapes &s, int coloring, float &sweep_m
     hipEvent_t e[3];
     for (auto &x : e) LCS_HIP(hipEventCreate(&x));
     int rc = LCS_OK;
-    for (int rep = 0; rep < 2 && rc == LCS_OK; rep++) {  // the first round sets kernel attributes
+    sweep_ms = rhs_ms = 1e30f;
+    for (int rep = 0; rep < 4 && rc == LCS_OK; rep++) {  // round 0 sets kernel attributes; best of the rest
         if (hipEventRecord(e[0], t.stream) != hipSuccess) rc = fail(LCS_ERR_HIP, "hipEventRecord failed");
         if (rc == LCS_OK) rc = launch_hsmac(t, &t.f[LCS_FIELD_U], DtRef{&t.S->dt_phase, 1.0}, p, -1);
         if (rc == LCS_OK && hipEventRecord(e[1], t.stream) != hipSuccess) rc = fail(LCS_ERR_HIP, "hipEventRecord failed");
         if (rc == LCS_OK) rc = launch_rhs(t, 1, true, out, base, DtRef{&t.S->dt_phase, 0.5});
         if (rc == LCS_OK && hipEventRecord(e[2], t.stream) != hipSuccess) rc = fail(LCS_ERR_HIP, "hipEventRecord failed");
         if (rc == LCS_OK && hipEventSynchronize(e[2]) != hipSuccess) rc = fail(LCS_ERR_HIP, "hipEventSynchronize failed");
+        float a = 0, b = 0;
+        if (rc == LCS_OK && (hipEventElapsedTime(&a, e[0], e[1]) != hipSuccess ||
+                             hipEventElapsedTime(&b, e[1], e[2]) != hipSuccess))
+            rc = fail(LCS_ERR_HIP, "hipEventElapsedTime failed");
+        if (rc == LCS_OK && rep > 0) {
+            sweep_ms = std::min(sweep_ms, a);
+            rhs_ms = std::min(rhs_ms, b);
+        }
     }
-    if (rc == LCS_OK && (hipEventElapsedTime(&sweep_ms, e[0], e[1]) != hipSuccess ||
-                         hipEventElapsedTime(&rhs_ms, e[1], e[2]) != hipSuccess))
-        rc = fail(LCS_ERR_HIP, "hipEventElapsedTime failed");
     for (auto &x : e) (void)hipEventDestroy(x);
     return rc;
 }
@@ -472,8 +478,10 @@ int lcs_autotune(lcs_box *box, lcs_tile_shapes *chosen)
         to_c(b.tiles, chosen);
         return LCS_OK;
     }
+    // a scratch box as deep as the box, up to 256 planes (the kernels' z-chunk
+    // choice depends on the depth)
     lcs_box_desc d = b.desc;
-    d.nz = std::min(d.nz, 128);
+    d.nz = std::min(d.nz, 256);
     lcs_box *tb = nullptr;
     if (lcs_box_create(reinterpret_cast<lcs_ctx *>(b.ctx), &d, &tb) != LCS_OK) {
         b.tiles = tiles_default(b.g, b.esize);  // no room for a scratch box: keep the rule
