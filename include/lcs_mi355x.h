@@ -316,6 +316,12 @@ int lcs_batch_size(lcs_batch *batch, int *nboxes);
  * per box in the order of addition (at most nlogs) */
 int lcs_batch_step(lcs_batch *batch, lcs_step_log *logs, int nlogs);
 int lcs_batch_sync(lcs_batch *batch);
+/* Each step's runs of launches between convergence polls (stage prefix up to
+ * couple!, the HSMAC groups before itrmin, one HSMAC group, stage 2 up to the
+ * linear-power forcing) are captured as hipGraphs on the second step after a
+ * membership change and replayed from then on; on by default.  Results are
+ * the same either way. */
+int lcs_batch_set_graphs(lcs_batch *batch, int on);
 
 /* ---- serving ---------------------------------------------------------------- */
 /* No counterpart in the reference.  A request loop over one batch: runs are

@@ -7,6 +7,7 @@
 // same result as the same box advanced alone with lcs_step.
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <vector>
 
 #include "lcs_internal.h"
@@ -54,11 +55,52 @@ struct Batch {
     DevVec<WorkItem> ghost_face, ghost_full, ghost_one, rows, lf_rows, parts, diams, faces[3];
     DevVec<int> lf_boxes;            // boxes with linear-power forcing
     std::vector<int> rcf_boxes;      // boxes with rc-energy forcing (forced one by one)
-    DevVec<TileItem> rhs_tiles, hs_tiles[2];  // sweep tiles 64 and 32 wide
+    DevVec<TileItem> rhs_tiles, hs_tiles[4];  // sweep tiles by shape: kBatchSweepShapes
     HsPoll *poll_d = nullptr, *poll_h = nullptr;
     DevScalars *sc_d = nullptr, *sc_h = nullptr;
     int cap_boxes = 0;
+    // hipGraphs of the runs of launches between host polls, keyed by segment
+    // id; captured on the second step after a membership change (the first
+    // runs eagerly and sets the kernels' attributes) and replayed afterwards
+    bool graphs = true, warm = false;
+    std::map<int, hipGraphExec_t> gexec;
 };
+
+static void drop_graphs(Batch &B)
+{
+    for (auto &kv : B.gexec) (void)hipGraphExecDestroy(kv.second);
+    B.gexec.clear();
+    B.warm = false;
+}
+
+// Enqueues a run of launches with no host synchronisation inside, through its
+// captured graph once the batch is warm.  The launches take every per-box
+// value from the device table or the device scalars, so a replay is the same
+// work as the eager run.
+template <typename F>
+static int segment(Batch &B, int id, F &&enqueue)
+{
+    if (!B.graphs || !B.warm) return enqueue();
+    auto it = B.gexec.find(id);
+    if (it == B.gexec.end()) {
+        LCS_HIP(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(B.stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(LCS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) return fail(LCS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        it = B.gexec.emplace(id, x).first;
+    }
+    LCS_HIP(hipGraphLaunch(it->second, B.stream));
+    return LCS_OK;
+}
 
 // Work items are listed box by box, then permuted so that workgroup L, which
 // the hardware runs on XCD L % kXcds, takes item t of one contiguous range per
@@ -81,11 +123,18 @@ static int blocks_of(long n, int per = 256) { return (int)((n + per - 1) / per);
 
 // Width of the fused-sweep tiles of a box: the one that wastes fewer lanes
 // on the array's x extent (the results do not depend on it).
-static int sweep_width(const Batch &, const Geom &g)
+// Sweep tile shapes of a batch, one launch per shape.  Width by lane
+// efficiency over the array row; 16 rows only for boxes whose planes are at
+// least kBatchTallPlane^2 (smaller boxes fill the chip better with 8).
+static const int kBatchSweepShapes[4][2] = {{64, 8}, {32, 8}, {64, 16}, {32, 16}};
+constexpr int kBatchTallPlane = 100000;
+static int sweep_shape(const Geom &g)
 {
-    const int X = g.nx + 2 * g.h;
+    const int X = g.nx + 2 * g.h, Y = g.ny + 2 * g.h;
     const double e64 = (double)X / (64.0 * ((X + 63) / 64)), e32 = (double)X / (32.0 * ((X + 31) / 32));
-    return e32 > e64 + 0.05 ? 32 : 64;
+    const int wide = e32 > e64 + 0.05 ? 1 : 0;
+    const int tall = (long)X * Y >= (long)kBatchTallPlane * kBatchTallPlane ? 2 : 0;
+    return tall + wide;
 }
 
 template <typename T>
@@ -150,6 +199,7 @@ static int upload_table(Batch &B)
 
 static int rebuild(Batch &B)
 {
+    drop_graphs(B);
     const int n = (int)B.boxes.size();
     if (n > B.cap_boxes) {
         if (B.poll_d) LCS_HIP(hipFree(B.poll_d));
@@ -166,7 +216,7 @@ static int rebuild(Batch &B)
     std::vector<WorkItem> gface, gfull, gone, rows, lf_rows, parts, diams, faces[3];
     std::vector<int> lf_boxes;
     B.rcf_boxes.clear();
-    std::vector<TileItem> rtiles, htiles[2];
+    std::vector<TileItem> rtiles, htiles[4];
     const int domain = B.hs.time_blocking ? 1 : 3;
     for (int i = 0; i < n; i++) {
         Box &b = *B.boxes[i];
@@ -196,16 +246,17 @@ static int rebuild(Batch &B)
                 for (int x = 0; x < rgx; x++) rtiles.push_back({i, x, y, z});
         // fused HSMAC sweep tiles: TX x 8 over the whole array, TX chosen per size
         const int zplanes = g.nz + 2 * g.h;
-        r.hs_tx = sweep_width(B, g);
-        r.hs_ty = 8;
-        const int hgx = (g.nx + 2 * g.h + r.hs_tx - 1) / r.hs_tx, hgy = (g.ny + 2 * g.h + 7) / 8;
+        const int shape = sweep_shape(g);
+        r.hs_tx = kBatchSweepShapes[shape][0];
+        r.hs_ty = kBatchSweepShapes[shape][1];
+        const int hgx = (g.nx + 2 * g.h + r.hs_tx - 1) / r.hs_tx, hgy = (g.ny + 2 * g.h + r.hs_ty - 1) / r.hs_ty;
         const int hgz = std::max(1, zplanes / 64);
         r.hs_zc = (zplanes + hgz - 1) / hgz;
         int nz_used = (zplanes + r.hs_zc - 1) / r.hs_zc;
         r.hs_tiles = hgx * hgy * nz_used;
         for (int z = 0; z < nz_used; z++)
             for (int y = 0; y < hgy; y++)
-                for (int x = 0; x < hgx; x++) htiles[r.hs_tx == 64 ? 0 : 1].push_back({i, x, y, z});
+                for (int x = 0; x < hgx; x++) htiles[shape].push_back({i, x, y, z});
         // face passes of the overlap order
         for (int a = 0; a < 3; a++) {
             Box3 f0, f1;
@@ -263,10 +314,10 @@ static int rebuild(Batch &B)
         (rc = B.ghost_one.assign(xcd_order(gone), B.stream)) || (rc = B.rows.assign(xcd_order(rows), B.stream)) ||
         (rc = B.lf_rows.assign(xcd_order(lf_rows), B.stream)) || (rc = B.lf_boxes.assign(lf_boxes, B.stream)) ||
         (rc = B.parts.assign(xcd_order(parts), B.stream)) || (rc = B.diams.assign(xcd_order(diams), B.stream)) ||
-        (rc = B.rhs_tiles.assign(xcd_order(rtiles), B.stream)) ||
-        (rc = B.hs_tiles[0].assign(xcd_order(htiles[0]), B.stream)) ||
-        (rc = B.hs_tiles[1].assign(xcd_order(htiles[1]), B.stream)))
+        (rc = B.rhs_tiles.assign(xcd_order(rtiles), B.stream)))
         return rc;
+    for (int q = 0; q < 4; q++)
+        if ((rc = B.hs_tiles[q].assign(xcd_order(htiles[q]), B.stream))) return rc;
     for (int a = 0; a < 3; a++)
         if ((rc = B.faces[a].assign(xcd_order(faces[a]), B.stream))) return rc;
     LCS_HIP(hipStreamSynchronize(B.stream));  // host vectors above are temporaries
@@ -284,9 +335,10 @@ static int b_fill(Batch &B, int set, int narr, int width, int hsmac)
 static int b_sweep(Batch &B, int set, int kind, int slot, double dtfac)
 {
     int rc;
-    for (int w = 0; w < 2; w++)
-        if ((rc = launch_b_hsmac_sweep(B.stream, B.esize, B.hs.coloring, w == 0 ? 64 : 32, B.tab.d, B.hs_tiles[w].d,
-                                       B.hs_tiles[w].n, set, kind, slot, dtfac)))
+    for (int q = 0; q < 4; q++)
+        if ((rc = launch_b_hsmac_sweep(B.stream, B.esize, B.hs.coloring, kBatchSweepShapes[q][0],
+                                       kBatchSweepShapes[q][1], B.tab.d, B.hs_tiles[q].d, B.hs_tiles[q].n, set, kind,
+                                       slot, dtfac)))
             return rc;
     return launch_b_hsmac_ctl(B.stream, B.tab.d, (int)B.boxes.size(), kHsFlip, 0, nullptr);
 }
@@ -326,50 +378,60 @@ static int b_poll(Batch &B, bool &all_done)
     return LCS_OK;
 }
 
-// set 1 acts on (U2, V2, W2, P) (stage 1), set 0 on (U, V, W, P) (stage 2)
-static int b_hsmac(Batch &B, int set, double dtfac, int slot)
+// One iteration group: _couple_naive! does one sweep per check (hsmac.jl:77-129),
+// _couple_time_blocking! three sweeps over shrinking regions after one fill
+// (hsmac.jl:131-201).
+static int b_group(Batch &B, int set, double dtfac)
+{
+    const lcs_hsmac_params &p = B.hs;
+    std::vector<int> kinds;
+    int rc;
+    if (!p.time_blocking) {
+        if ((rc = b_fill(B, set, p.overlap ? 3 : 4, 1, 1))) return rc;
+        if (p.overlap) {
+            if ((rc = b_core_faces(B, set, 3, dtfac, kinds))) return rc;
+        } else {
+            if ((rc = b_sweep(B, set, 3, 0, dtfac))) return rc;
+            kinds.assign(1, 0);
+        }
+        return b_check(B, kinds);
+    }
+    if ((rc = b_fill(B, set, 3, B.boxes[0]->g.h, 1))) return rc;
+    if (p.overlap) {
+        if ((rc = b_core_faces(B, set, 1, dtfac, kinds))) return rc;
+    } else {
+        if ((rc = b_sweep(B, set, 1, 0, dtfac))) return rc;
+        kinds.assign(1, 0);
+    }
+    if ((rc = b_check(B, kinds))) return rc;
+    kinds.assign(1, 0);
+    if ((rc = b_sweep(B, set, 2, 0, dtfac)) || (rc = b_check(B, kinds))) return rc;
+    if ((rc = b_sweep(B, set, 3, 0, dtfac)) || (rc = b_check(B, kinds))) return rc;
+    return LCS_OK;
+}
+
+// set 1 acts on (U2, V2, W2, P) (stage 1), set 0 on (U, V, W, P) (stage 2).
+// Groups before the first one that may end the iteration (itrmin) run
+// unconditionally with the begin kernel in one segment; after that each group
+// is one segment followed by a poll of every box's convergence flag.
+static int b_hsmac(Batch &B, int set, double dtfac, int slot, int seg_id)
 {
     const int n = (int)B.boxes.size();
     const lcs_hsmac_params &p = B.hs;
-    int rc = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsBegin, 0, nullptr);
+    const int ngroups = p.time_blocking ? (p.itrmax + 2) / 3 : p.itrmax;
+    int pre = 0;  // groups with no poll after them
+    while (pre < ngroups && (p.time_blocking ? 3 * pre + 2 : pre) < p.itrmin) pre++;
+    int rc = segment(B, seg_id, [&]() {
+        int r = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsBegin, 0, nullptr);
+        for (int g = 0; g < pre && r == LCS_OK; g++) r = b_group(B, set, dtfac);
+        return r;
+    });
     if (rc) return rc;
     bool done = false, polled_last = false;
-    std::vector<int> kinds;
-    if (!p.time_blocking) {  // _couple_naive!: hsmac.jl:77-129
-        for (int s = 0; s < p.itrmax && !done; s++) {
-            if ((rc = b_fill(B, set, p.overlap ? 3 : 4, 1, 1))) return rc;
-            if (p.overlap) {
-                if ((rc = b_core_faces(B, set, 3, dtfac, kinds))) return rc;
-            } else {
-                if ((rc = b_sweep(B, set, 3, 0, dtfac))) return rc;
-                kinds.assign(1, 0);
-            }
-            if ((rc = b_check(B, kinds))) return rc;
-            polled_last = false;
-            if (s >= p.itrmin) {
-                if ((rc = b_poll(B, done))) return rc;
-                polled_last = true;
-            }
-        }
-    } else {  // _couple_time_blocking!: hsmac.jl:131-201
-        for (int grp = 0; 3 * grp < p.itrmax && !done; grp++) {
-            if ((rc = b_fill(B, set, 3, B.boxes[0]->g.h, 1))) return rc;
-            if (p.overlap) {
-                if ((rc = b_core_faces(B, set, 1, dtfac, kinds))) return rc;
-            } else {
-                if ((rc = b_sweep(B, set, 1, 0, dtfac))) return rc;
-                kinds.assign(1, 0);
-            }
-            if ((rc = b_check(B, kinds))) return rc;
-            kinds.assign(1, 0);
-            if ((rc = b_sweep(B, set, 2, 0, dtfac)) || (rc = b_check(B, kinds))) return rc;
-            if ((rc = b_sweep(B, set, 3, 0, dtfac)) || (rc = b_check(B, kinds))) return rc;
-            polled_last = false;
-            if (3 * grp + 2 >= p.itrmin) {
-                if ((rc = b_poll(B, done))) return rc;
-                polled_last = true;
-            }
-        }
+    for (int g = pre; g < ngroups && !done; g++) {
+        if ((rc = segment(B, seg_id + 1, [&]() { return b_group(B, set, dtfac); }))) return rc;
+        if ((rc = b_poll(B, done))) return rc;
+        polled_last = true;
     }
     if ((rc = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsEnd, slot, nullptr))) return rc;
     if (!polled_last && (rc = b_poll(B, done))) return rc;
@@ -410,33 +472,44 @@ static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
     if (moved && (rc = upload_table(B))) return rc;
     const lcs_step_config &c0 = B.cfgs[0];
     const int h = B.boxes[0]->g.h;
-    // timestep!: CFL (FaceHalo fill first) and particle dt
-    if ((rc = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_face.d, B.ghost_face.n, 0, 3, h, 0, 0))) return rc;
-    if ((rc = launch_b_cfl(B.stream, B.esize, B.tab.d, B.rows.d, B.rows.n, n, c0.cfl, c0.dtmax))) return rc;
-    if ((rc = launch_b_particle_dt(B.stream, B.esize, B.tab.d, B.diams.d, B.diams.n, n, c0.dtmax))) return rc;
-    // stage 1
-    if ((rc = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 0, 4, h, 1, 0))) return rc;
-    if ((rc = launch_b_particles(B.stream, B.esize, B.tab.d, B.parts.d, B.parts.n, 1))) return rc;
-    if ((rc = launch_b_rhs(B.stream, B.esize, B.tab.d, B.rhs_tiles.d, B.rhs_tiles.n, 1))) return rc;
-    if ((rc = b_hsmac(B, 1, 0.5, 0))) return rc;
-    // stage 2
-    if ((rc = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 1, 4, h, 1, 0))) return rc;
-    if ((rc = launch_b_particles(B.stream, B.esize, B.tab.d, B.parts.d, B.parts.n, 2))) return rc;
-    if ((rc = launch_b_rhs(B.stream, B.esize, B.tab.d, B.rhs_tiles.d, B.rhs_tiles.n, 2))) return rc;
-    // forcing! (stage 2 only): linear-power boxes batched, rc-energy boxes one by one
-    if (B.lf_boxes.n > 0) {
-        if ((rc = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 0, 3, h, 1, 0)))
-            return rc;
-        if ((rc = launch_b_lf(B.stream, B.esize, B.tab.d, B.lf_rows.d, B.lf_rows.n, B.lf_boxes.d, B.lf_boxes.n)))
-            return rc;
-    }
+    // timestep! (CFL after a FaceHalo fill, particle dt) and stage 1 up to couple!
+    rc = segment(B, 1, [&]() {
+        int r;
+        if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_face.d, B.ghost_face.n, 0, 3, h, 0, 0)) ||
+            (r = launch_b_cfl(B.stream, B.esize, B.tab.d, B.rows.d, B.rows.n, n, c0.cfl, c0.dtmax)) ||
+            (r = launch_b_particle_dt(B.stream, B.esize, B.tab.d, B.diams.d, B.diams.n, n, c0.dtmax)) ||
+            (r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 0, 4, h, 1, 0)) ||
+            (r = launch_b_particles(B.stream, B.esize, B.tab.d, B.parts.d, B.parts.n, 1)) ||
+            (r = launch_b_rhs(B.stream, B.esize, B.tab.d, B.rhs_tiles.d, B.rhs_tiles.n, 1)))
+            return r;
+        return LCS_OK;
+    });
+    if (rc || (rc = b_hsmac(B, 1, 0.5, 0, 10))) return rc;
+    // stage 2 up to forcing!: linear-power boxes batched in the segment,
+    // rc-energy boxes one by one after it (their launches take the host's
+    // current array pointers)
+    rc = segment(B, 2, [&]() {
+        int r;
+        if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 1, 4, h, 1, 0)) ||
+            (r = launch_b_particles(B.stream, B.esize, B.tab.d, B.parts.d, B.parts.n, 2)) ||
+            (r = launch_b_rhs(B.stream, B.esize, B.tab.d, B.rhs_tiles.d, B.rhs_tiles.n, 2)))
+            return r;
+        if (B.lf_boxes.n > 0) {
+            if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 0, 3, h, 1, 0)) ||
+                (r = launch_b_lf(B.stream, B.esize, B.tab.d, B.lf_rows.d, B.lf_rows.n, B.lf_boxes.d, B.lf_boxes.n)))
+                return r;
+        }
+        return LCS_OK;
+    });
+    if (rc) return rc;
     for (int i : B.rcf_boxes) {
         Box &b = *B.boxes[i];
         const lcs_forcing &f = B.cfgs[i].forcing;
         if ((rc = launch_force_rcf(b, f.energy, f.kmin, f.kmax, f.coarse_size, DtRef{&b.S->dt, 1.0}))) return rc;
     }
-    if ((rc = b_hsmac(B, 0, 1.0, 1))) return rc;
+    if ((rc = b_hsmac(B, 0, 1.0, 1, 20))) return rc;
     if ((rc = launch_b_state(B.stream, B.tab.d, n, logs ? B.sc_d : nullptr))) return rc;
+    B.warm = true;
     if (logs) {
         LCS_HIP(hipMemcpyAsync(B.sc_h, B.sc_d, sizeof(DevScalars) * (size_t)n, hipMemcpyDeviceToHost, B.stream));
         LCS_HIP(hipStreamSynchronize(B.stream));
@@ -518,14 +591,14 @@ int lcs_batch_destroy(lcs_batch *batch)
     if (!batch) return LCS_OK;
     Batch *B = BT(batch);
     (void)hipSetDevice(B->ctx->device);
+    drop_graphs(*B);
     while (!B->boxes.empty()) (void)detach(*B, (int)B->boxes.size() - 1);
     DevVec<WorkItem> *w[] = {&B->ghost_face, &B->ghost_full, &B->ghost_one, &B->rows,    &B->lf_rows,
                              &B->parts,      &B->diams,      &B->faces[0],  &B->faces[1], &B->faces[2]};
     for (auto *v : w) v->release();
     B->lf_boxes.release();
     B->rhs_tiles.release();
-    B->hs_tiles[0].release();
-    B->hs_tiles[1].release();
+    for (auto &t : B->hs_tiles) t.release();
     B->tab.release();
     if (B->poll_d) (void)hipFree(B->poll_d);
     if (B->poll_h) (void)hipHostFree(B->poll_h);
@@ -593,6 +666,16 @@ int lcs_batch_step(lcs_batch *batch, lcs_step_log *logs, int nlogs)
     Batch &B = *BT(batch);
     LCS_HIP(hipSetDevice(B.ctx->device));
     return batch_step(B, logs, nlogs);
+}
+
+int lcs_batch_set_graphs(lcs_batch *batch, int on)
+{
+    if (!batch) return fail(LCS_ERR_INVALID, "null argument");
+    Batch &B = *BT(batch);
+    LCS_HIP(hipStreamSynchronize(B.stream));
+    drop_graphs(B);
+    B.graphs = on != 0;
+    return LCS_OK;
 }
 
 int lcs_batch_sync(lcs_batch *batch)

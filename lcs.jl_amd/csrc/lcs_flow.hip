@@ -2027,11 +2027,10 @@ __global__ void __launch_bounds__(TX *TY) k_b_hsmac_sweep(const BoxRec *tab, con
     if (threadIdx.x == 0) B.part[(long)slot * kMaxPartBlocks + t.bx + gx * (t.by + gy * t.bz)] = s;
 }
 
-template <typename T, int COLORING, int TX>
+template <typename T, int COLORING, int TX, int TY>
 static int b_sweep_t(hipStream_t s, const BoxRec *tab, const TileItem *tiles, int ntiles, int set, int kind, int slot,
                      double dtfac)
 {
-    constexpr int TY = 8;
     using Tile = HsTile<T, COLORING, TY, TX>;
     const size_t lds = Tile::LDS_ELEMS * sizeof(T);
     static bool attr = false;
@@ -2046,25 +2045,32 @@ static int b_sweep_t(hipStream_t s, const BoxRec *tab, const TileItem *tiles, in
     return LCS_OK;
 }
 
-template <typename T, int TX>
+template <typename T, int TX, int TY>
 static int b_sweep_c(hipStream_t s, int coloring, const BoxRec *tab, const TileItem *tiles, int ntiles, int set,
                      int kind, int slot, double dtfac)
 {
     return coloring == LCS_COLORING_RED_BLACK
-               ? b_sweep_t<T, LCS_COLORING_RED_BLACK, TX>(s, tab, tiles, ntiles, set, kind, slot, dtfac)
-               : b_sweep_t<T, LCS_COLORING_RED_BLACK_FAST, TX>(s, tab, tiles, ntiles, set, kind, slot, dtfac);
+               ? b_sweep_t<T, LCS_COLORING_RED_BLACK, TX, TY>(s, tab, tiles, ntiles, set, kind, slot, dtfac)
+               : b_sweep_t<T, LCS_COLORING_RED_BLACK_FAST, TX, TY>(s, tab, tiles, ntiles, set, kind, slot, dtfac);
 }
 
-int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, int tx, const BoxRec *tab, const TileItem *tiles,
+template <typename T>
+static int b_sweep_shape(hipStream_t s, int coloring, int tx, int ty, const BoxRec *tab, const TileItem *tiles,
                          int ntiles, int set, int kind, int slot, double dtfac)
 {
+    if (tx == 64 && ty == 8) return b_sweep_c<T, 64, 8>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac);
+    if (tx == 32 && ty == 8) return b_sweep_c<T, 32, 8>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac);
+    if (tx == 64 && ty == 16) return b_sweep_c<T, 64, 16>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac);
+    if (tx == 32 && ty == 16) return b_sweep_c<T, 32, 16>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac);
+    return fail(LCS_ERR_INVALID, "unsupported sweep tile shape");
+}
+
+int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, int tx, int ty, const BoxRec *tab,
+                         const TileItem *tiles, int ntiles, int set, int kind, int slot, double dtfac)
+{
     if (ntiles == 0) return LCS_OK;
-    if (tx != 32 && tx != 64) return fail(LCS_ERR_INVALID, "sweep tiles are 32 or 64 wide");
-    if (esize == 8)
-        return tx == 32 ? b_sweep_c<double, 32>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac)
-                        : b_sweep_c<double, 64>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac);
-    return tx == 32 ? b_sweep_c<float, 32>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac)
-                    : b_sweep_c<float, 64>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac);
+    return esize == 8 ? b_sweep_shape<double>(s, coloring, tx, ty, tab, tiles, ntiles, set, kind, slot, dtfac)
+                      : b_sweep_shape<float>(s, coloring, tx, ty, tab, tiles, ntiles, set, kind, slot, dtfac);
 }
 
 // one colour over both face slabs of one axis, in place on the current set

@@ -284,8 +284,8 @@ int launch_b_lf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *row
                 int nbox);
 int launch_b_rhs(hipStream_t s, int esize, const BoxRec *tab, const TileItem *tiles, int ntiles, int stage);
 // tx: tile width of the fused sweep (32 or 64); the tiles listed must all be of that width
-int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, int tx, const BoxRec *tab, const TileItem *tiles,
-                         int ntiles, int set, int kind, int slot, double dtfac);
+int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, int tx, int ty, const BoxRec *tab,
+                         const TileItem *tiles, int ntiles, int set, int kind, int slot, double dtfac);
 int launch_b_hsmac_faces(hipStream_t s, int esize, int coloring, const BoxRec *tab, const WorkItem *work,
                          int nwork, int set, int domain, int axis, int color, int slot, double dtfac);
 // kinds of the passes since the last check: 0 = fused sweep, 1..3 = face pass of axis z, y, x

@@ -370,6 +370,10 @@ class Batch:
         check(self.L.lcs_batch_step(self.handle, logs, n))
         return [log_dict(logs[i]) for i in range(n)]
 
+    def set_graphs(self, on):
+        """Replay each step's launch runs as captured hipGraphs (default on)."""
+        check(self.L.lcs_batch_set_graphs(self.handle, int(bool(on))))
+
     def sync(self):
         check(self.L.lcs_batch_sync(self.handle))
 
