@@ -379,8 +379,9 @@ def main():
     if os.path.exists(tf):
         with open(tf) as fh:
             t = json.load(fh)
+        own = {"hsmac_sweep": "sweep", "rhs_update": "rhs"}.get(name)  # the kernel's own tile shape
         if (t.get("workload") == args.workload and t.get("size") == cfg.grid.dims[0]
-                and t.get("tiles") == {k: list(v) for k, v in tiles.items()}):
+                and own and t.get("tile") == list(tiles[own])):
             traffic = t.get("bytes_per_launch")
     kernels = {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1],
                    "algo_GBs": (v[2] / (v[0] * 1e-3) / 1e9) if v[0] > 0 else None} for k, v in prof.items()}
