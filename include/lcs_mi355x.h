@@ -202,10 +202,12 @@ int lcs_boxmean(lcs_box *box, int coarse_size, double *coarse3);
  * workgroups of a fixed size).  The fused RHS and HSMAC sweep kernels march
  * tx x ty columns of cells up in z; the shape changes speed, never results.
  * Sweep shapes: (64|32) x (16|8).  RHS shapes: FP32 (64|32) x (16|8), FP64
- * 64x8, 32x8, 32x4.  A zero shape means the size rule. */
+ * 64x8, 32x8, 32x4.  A zero sweep shape means the size rule for all of them. */
 typedef struct {
     int sweep_tx, sweep_ty;
     int rhs_tx, rhs_ty;
+    int sweep2_tx, sweep2_ty; /* FP32: the d2 and d3 sweeps of a time-blocked group before itrmin as one
+                                 pass, 32x24, 64x12 or 32x16; 0x0 = two single sweeps */
 } lcs_tile_shapes;
 
 /* Times every candidate shape on a scratch box with this box's x-y extent (at

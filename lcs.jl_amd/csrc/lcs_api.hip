@@ -413,7 +413,9 @@ static bool shape_ok(int esize, int kind, int tx, int ty)
 static int time_shapes(Box &t, const TileShapes &s, int coloring, float &sweep_ms, float &rhs_ms)
 {
     t.tiles = s;
-    const lcs_hsmac_params p = {1, 0, coloring, 1.7, 0.0, 3, 3};  // epsp 0: exactly itrmax sweeps
+    // one time-blocked group (epsp 0: exactly itrmax sweeps); itrmin 3 lets its
+    // d2 and d3 sweeps run as one pass where the shapes ask for it
+    const lcs_hsmac_params p = {1, 0, coloring, 1.7, 0.0, 3, 3};
     void *out[3] = {t.f[LCS_FIELD_U2], t.f[LCS_FIELD_V2], t.f[LCS_FIELD_W2]};
     void *base[3] = {t.f[LCS_FIELD_U], t.f[LCS_FIELD_V], t.f[LCS_FIELD_W]};
     hipEvent_t e[3];
@@ -457,6 +459,8 @@ static void to_c(const TileShapes &t, lcs_tile_shapes *o)
     o->sweep_ty = t.sweep_ty;
     o->rhs_tx = t.rhs_tx;
     o->rhs_ty = t.rhs_ty;
+    o->sweep2_tx = t.sweep2_tx;
+    o->sweep2_ty = t.sweep2_ty;
 }
 
 int lcs_autotune(lcs_box *box, lcs_tile_shapes *chosen)
@@ -494,7 +498,8 @@ int lcs_autotune(lcs_box *box, lcs_tile_shapes *chosen)
     int sw[8][2], rh[8][2];
     const int nsw = tile_candidates(b.esize, 0, sw, 8), nrh = tile_candidates(b.esize, 1, rh, 8);
     float best_sweep = 1e30f, best_rhs = 1e30f;
-    // sweep shapes with the rule's RHS shape, then RHS shapes with the best sweep
+    // sweep shapes (single sweeps), then the two-sweep pass, then RHS shapes
+    best.sweep2_tx = best.sweep2_ty = 0;
     for (int i = 0; i < nsw && rc == LCS_OK; i++) {
         TileShapes s = best;
         s.sweep_tx = sw[i][0];
@@ -505,6 +510,21 @@ int lcs_autotune(lcs_box *box, lcs_tile_shapes *chosen)
             best_sweep = ms_sw;
             best.sweep_tx = s.sweep_tx;
             best.sweep_ty = s.sweep_ty;
+        }
+    }
+    int s2[8][2];
+    const int ns2 = tile_candidates(b.esize, 2, s2, 8);
+    float best_group = 1e30f;
+    for (int i = 0; i < ns2 && rc == LCS_OK; i++) {  // the two-sweep pass, or none
+        TileShapes s = best;
+        s.sweep2_tx = s2[i][0];
+        s.sweep2_ty = s2[i][1];
+        float ms_sw = 0, ms_rh = 0;
+        rc = time_shapes(t, s, LCS_COLORING_RED_BLACK_FAST, ms_sw, ms_rh);
+        if (rc == LCS_OK && ms_sw < best_group) {
+            best_group = ms_sw;
+            best.sweep2_tx = s.sweep2_tx;
+            best.sweep2_ty = s.sweep2_ty;
         }
     }
     for (int i = 0; i < nrh && rc == LCS_OK; i++) {
@@ -541,10 +561,14 @@ int lcs_set_tiles(lcs_box *box, const lcs_tile_shapes *sh)
     }
     if (!shape_ok(b.esize, 0, sh->sweep_tx, sh->sweep_ty)) return fail(LCS_ERR_INVALID, "unsupported sweep tile shape");
     if (!shape_ok(b.esize, 1, sh->rhs_tx, sh->rhs_ty)) return fail(LCS_ERR_INVALID, "unsupported RHS tile shape");
+    if ((sh->sweep2_tx || sh->sweep2_ty) && !shape_ok(b.esize, 2, sh->sweep2_tx, sh->sweep2_ty))
+        return fail(LCS_ERR_INVALID, "unsupported two-sweep tile shape");
     b.tiles.sweep_tx = sh->sweep_tx;
     b.tiles.sweep_ty = sh->sweep_ty;
     b.tiles.rhs_tx = sh->rhs_tx;
     b.tiles.rhs_ty = sh->rhs_ty;
+    b.tiles.sweep2_tx = sh->sweep2_tx;
+    b.tiles.sweep2_ty = sh->sweep2_ty;
     return LCS_OK;
 }
 

@@ -12,6 +12,7 @@
 
 #include "lcs_internal.h"
 #include "lcs_hsmac_tile.h"
+#include "lcs_hsmac_tile2.h"
 #include "lcs_rhs_tile.h"
 
 namespace lcs {
@@ -343,11 +344,13 @@ static int rhs_tile_launch(Box &b, const RhsArgs<T> &a0, DtRef dt)
 static const int kSweepShapes[][2] = {{64, 16}, {64, 8}, {32, 16}, {32, 8}};
 static const int kRhsShapesF[][2] = {{64, 16}, {64, 8}, {32, 16}, {32, 8}};
 static const int kRhsShapesD[][2] = {{64, 8}, {32, 8}, {32, 4}};
+// two-sweep pass (FP32 only; (0, 0) = two single sweeps)
+static const int kSweep2Shapes[][2] = {{32, 24}, {64, 12}, {32, 16}, {0, 0}};
 
 int tile_candidates(int esize, int kind, int out[][2], int cap)
 {
-    const int(*src)[2] = kind == 0 ? kSweepShapes : (esize == 8 ? kRhsShapesD : kRhsShapesF);
-    const int n = kind == 0 ? 4 : (esize == 8 ? 3 : 4);
+    const int(*src)[2] = kind == 0 ? kSweepShapes : kind == 2 ? kSweep2Shapes : (esize == 8 ? kRhsShapesD : kRhsShapesF);
+    const int n = kind == 0 ? 4 : kind == 2 ? (esize == 8 ? 0 : 4) : (esize == 8 ? 3 : 4);
     for (int i = 0; i < n && i < cap; i++) {
         out[i][0] = src[i][0];
         out[i][1] = src[i][1];
@@ -366,6 +369,8 @@ TileShapes tiles_default(const Geom &g, int esize)
     const double e64 = (double)X / (64.0 * ((X + 63) / 64)), e32 = (double)X / (32.0 * ((X + 31) / 32));
     t.sweep_tx = e32 > e64 + 0.05 ? 32 : 64;
     t.sweep_ty = big ? 16 : 8;
+    t.sweep2_tx = esize == 4 && big ? 32 : 0;
+    t.sweep2_ty = esize == 4 && big ? 24 : 0;
     if (esize == 4) {
         t.rhs_tx = big ? 64 : 32;
         t.rhs_ty = big ? 16 : 8;
@@ -600,6 +605,73 @@ __global__ void __launch_bounds__(TX *TY) k_hsmac_sweep(Geom g, BufSets<T> bs, B
     if (threadIdx.x == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
 }
 
+// Two fused sweeps, RA then RB, current -> other buffer set (phase functions
+// and schedule in lcs_hsmac_tile2.h).  The div^2 sums of the two sweeps go to
+// their own partial slots.
+template <typename T, int COLORING, int TY, int TX>
+__device__ inline void hsmac_sweep2_block(const Hs2Args<T> &a, int bx, int by, int bz, T *lds, double *red_lds,
+                                          double &sa, double &sb)
+{
+    using Tile = HsTile2<T, COLORING, TY, TX>;
+    typename Tile::Thread th;
+    Tile::init(a, bx, by, bz, threadIdx.x, th);
+    Tile::prologue(a, lds, th);
+    __syncthreads();
+    const int q0 = Tile::first_step(th), q1 = Tile::last_step(th);
+    for (int q = q0; q <= q1; q += 2) {
+        Tile::step_1(a, lds, q, th, 0);
+        __syncthreads();
+        Tile::step_2(a, lds, q, th, 0);
+        __syncthreads();
+        if (q + 1 > q1) break;
+        Tile::step_1(a, lds, q + 1, th, 1);
+        __syncthreads();
+        Tile::step_2(a, lds, q + 1, th, 1);
+        __syncthreads();
+    }
+    sa = block_sum(th.accA, red_lds);
+    sb = block_sum(th.accB, red_lds);
+}
+
+template <typename T>
+__device__ inline void hs2_bind(Hs2Args<T> &a, const BufSets<T> &bs, bool par)
+{
+    a.Us = par ? bs.b[0] : bs.a[0];
+    a.Vs = par ? bs.b[1] : bs.a[1];
+    a.Ws = par ? bs.b[2] : bs.a[2];
+    a.Ps = par ? bs.b[3] : bs.a[3];
+    a.Ud = par ? bs.a[0] : bs.b[0];
+    a.Vd = par ? bs.a[1] : bs.b[1];
+    a.Wd = par ? bs.a[2] : bs.b[2];
+    a.Pd = par ? bs.a[3] : bs.b[3];
+}
+
+template <typename T, int COLORING, int TY, int TX>
+__global__ void __launch_bounds__(TX *TY) k_hsmac_sweep2(Geom g, BufSets<T> bs, Box3 RA, Box3 RB, HsCoef<T> c,
+                                                        DtRef dt, DevScalars *S, double *partA, double *partB, int zc)
+{
+    extern __shared__ __align__(16) unsigned char hs_lds_raw[];
+    __shared__ double red_lds[16];
+    if (S->hs_done) return;
+    Hs2Args<T> a;
+    a.g = g;
+    hs2_bind(a, bs, S->hs_par != 0);
+    a.RA = RA;
+    a.RB = RB;
+    a.c = c;
+    a.dti = T(1.0 / (dt.factor * (*dt.src)));
+    a.zc = zc;
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    double sa, sb;
+    hsmac_sweep2_block<T, COLORING, TY, TX>(a, bx, by, bz, reinterpret_cast<T *>(hs_lds_raw), red_lds, sa, sb);
+    const int L = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    if (threadIdx.x == 0) {
+        partA[L] = sa;
+        partB[L] = sb;
+    }
+}
+
 struct CheckArgs {
     int npass;
     int cnt[kMaxSweepPasses];
@@ -659,6 +731,9 @@ struct HsmacRun {
     CheckArgs ca;
     int npass = 0;
     std::vector<double> sweep_bytes;  // algorithmic bytes of each fused sweep launched in this call
+    std::vector<int> sweep_first;     // for each launch of sweep_bytes: index of its first sweep
+    std::vector<const char *> sweep_name;
+    int sweeps = 0;                   // sweeps enqueued so far in this call
 
     // one colour over boxes r0 and r1 (see k_hsmac_pass for when two boxes may share a launch)
     int pass(const Box3 &r0, const Box3 &r1, int color)
@@ -728,6 +803,8 @@ struct HsmacRun {
         long cells = (long)(r.hi[0] - r.lo[0] + 1) * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
         ProfScope ps(b, "hsmac_sweep", 8.0 * sizeof(T) * cells);
         sweep_bytes.push_back(8.0 * sizeof(T) * cells);
+        sweep_first.push_back(sweeps++);
+        sweep_name.push_back("hsmac_sweep");
         hipLaunchKernelGGL((k_hsmac_sweep<T, COLORING, TY, TX>), dim3(gx, gy, nzc), dim3(Tile::NT), lds, b.stream, g,
                            bs, r, c, dt, b.S, part, zc);
         LCS_HIP(hipGetLastError());
@@ -780,6 +857,96 @@ struct HsmacRun {
                     if ((rc = pass(f[q + s], none, 0)) || (rc = pass(f[q + s], none, 1))) return rc;
             }
         }
+        return LCS_OK;
+    }
+    // two fused sweeps RA then RB; partial slots npass (RA) and npass+1 (RB)
+    template <int COLORING, int TY, int TX>
+    int fused2_launch(const Box3 &ra, const Box3 &rb)
+    {
+        using Tile = HsTile2<T, COLORING, TY, TX>;
+        static_assert(Tile::LDS_ELEMS * sizeof(T) + 256 <= 160 * 1024, "tile exceeds the LDS of a CU");
+        const Geom &g = b.g;
+        const int gx = (g.nx + 2 * g.h + TX - 1) / TX, gy = (g.ny + 2 * g.h + TY - 1) / TY;
+        const int zplanes = g.nz + 2 * g.h;
+        const long cols = (long)gx * gy;
+        const size_t lds = Tile::LDS_ELEMS * sizeof(T);
+        static int per_cu = 0;
+        if (!per_cu) {
+            LCS_HIP(hipFuncSetAttribute((const void *)k_hsmac_sweep2<T, COLORING, TY, TX>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            LCS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_hsmac_sweep2<T, COLORING, TY, TX>,
+                                                                 Tile::NT, lds));
+            if (per_cu < 1) per_cu = 1;
+        }
+        // z-chunks as for one sweep; a chunk here re-reads about nine halo planes
+        const long slots = (long)b.cus * per_cu;
+        int nzc = 1;
+        double best = 1e300;
+        for (int n = 1; n <= 64 && n <= zplanes; n++) {
+            if (cols * n > kMaxPartBlocks) break;
+            const long zc_n = (zplanes + n - 1) / n;
+            const double cost = (double)((cols * n + slots - 1) / slots) * (double)(zc_n + 9);
+            if (cost < best * 0.999) {
+                best = cost;
+                nzc = n;
+            }
+        }
+        if (cols * nzc > kMaxPartBlocks || npass + 2 > kMaxSweepPasses)
+            return fail(LCS_ERR_INVALID, "grid too large for the two-sweep HSMAC kernel");
+        const int zc = (zplanes + nzc - 1) / nzc;
+        nzc = (zplanes + zc - 1) / zc;
+        double *pa = b.part + (long)npass * kMaxPartBlocks, *pb = pa + kMaxPartBlocks;
+        auto cells = [](const Box3 &r) {
+            return (double)(r.hi[0] - r.lo[0] + 1) * (r.hi[1] - r.lo[1] + 1) * (r.hi[2] - r.lo[2] + 1);
+        };
+        const double bytes = 8.0 * sizeof(T) * (cells(ra) + cells(rb));
+        ProfScope ps(b, "hsmac_sweep2", bytes);
+        sweep_bytes.push_back(bytes);
+        sweep_first.push_back(sweeps);
+        sweep_name.push_back("hsmac_sweep2");
+        sweeps += 2;
+        hipLaunchKernelGGL((k_hsmac_sweep2<T, COLORING, TY, TX>), dim3(gx, gy, nzc), dim3(Tile::NT), lds, b.stream,
+                           g, bs, ra, rb, c, dt, b.S, pa, pb, zc);
+        LCS_HIP(hipGetLastError());
+        ca.cnt[npass++] = (int)(cols * nzc);
+        ca.cnt[npass++] = (int)(cols * nzc);
+        hipLaunchKernelGGL(k_hsmac_flip, dim3(1), dim3(1), 0, b.stream, b.S);
+        LCS_HIP(hipGetLastError());
+        return LCS_OK;
+    }
+    // the d2 and d3 sweeps of a group as one pass, if this box has a two-sweep shape
+    bool has_fused2() const
+    {
+        const TileShapes t = b.tiles.sweep_tx ? b.tiles : tiles_default(b.g, b.esize);
+        return sizeof(T) == 4 && t.sweep2_tx > 0;
+    }
+    template <int COLORING>
+    int fused2_shape(const Box3 &ra, const Box3 &rb, int tx, int ty)
+    {
+        if constexpr (sizeof(T) == 4) {
+            if (tx == 32 && ty == 24) return fused2_launch<COLORING, 24, 32>(ra, rb);
+            if (tx == 64 && ty == 12) return fused2_launch<COLORING, 12, 64>(ra, rb);
+            if (tx == 32 && ty == 16) return fused2_launch<COLORING, 16, 32>(ra, rb);
+        }
+        return fail(LCS_ERR_INVALID, "unsupported two-sweep tile shape");
+    }
+    int fused2(const Box3 &ra, const Box3 &rb)
+    {
+        const TileShapes t = b.tiles.sweep_tx ? b.tiles : tiles_default(b.g, b.esize);
+        return coloring == LCS_COLORING_RED_BLACK
+                   ? fused2_shape<LCS_COLORING_RED_BLACK>(ra, rb, t.sweep2_tx, t.sweep2_ty)
+                   : fused2_shape<LCS_COLORING_RED_BLACK_FAST>(ra, rb, t.sweep2_tx, t.sweep2_ty);
+    }
+    // one convergence check over partial slots [first, first + count)
+    int check_slots(int first, int count)
+    {
+        CheckArgs c2 = ca;
+        c2.npass = count;
+        for (int i = 0; i < count; i++) c2.cnt[i] = ca.cnt[first + i];
+        ProfScope ps(b, "hsmac_check", 0.0);
+        hipLaunchKernelGGL(k_hsmac_check, dim3(1), dim3(256), 0, b.stream, b.part + (long)first * kMaxPartBlocks, c2,
+                           b.S);
+        LCS_HIP(hipGetLastError());
         return LCS_OK;
     }
     int check()
@@ -861,10 +1028,17 @@ static int hsmac_t(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int
             rc = prm.overlap ? run.sweep_core_faces(d1) : run.fused(d1);
             if (rc) return rc;
             if ((rc = run.check())) return rc;
-            if ((rc = run.fused(d2))) return rc;
-            if ((rc = run.check())) return rc;
-            if ((rc = run.fused(d3))) return rc;
-            if ((rc = run.check())) return rc;
+            if (3 * grp + 2 < prm.itrmin && run.has_fused2()) {
+                // neither the d2 nor the d3 sweep can end the iteration: one pass
+                if ((rc = run.fused2(d2, d3)) || (rc = run.check_slots(0, 1)) || (rc = run.check_slots(1, 1)))
+                    return rc;
+                run.npass = 0;
+            } else {
+                if ((rc = run.fused(d2))) return rc;
+                if ((rc = run.check())) return rc;
+                if ((rc = run.fused(d3))) return rc;
+                if ((rc = run.check())) return rc;
+            }
             polled_last = false;
             if (3 * grp + 2 >= prm.itrmin) {
                 if ((rc = run.poll(done))) return rc;
@@ -878,11 +1052,9 @@ static int hsmac_t(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int
     if (done) {
         // every sweep has exactly one fused launch; those after the converged
         // sweep returned at once
-        const size_t executed = (size_t)b.S_host->hs_itr + 1;
-        double skipped_bytes = 0;
-        for (size_t q = executed; q < run.sweep_bytes.size(); q++) skipped_bytes += run.sweep_bytes[q];
-        if (run.sweep_bytes.size() > executed)
-            prof_uncount(b, "hsmac_sweep", (int64_t)(run.sweep_bytes.size() - executed), skipped_bytes);
+        const int executed = b.S_host->hs_itr + 1;
+        for (size_t q = 0; q < run.sweep_bytes.size(); q++)
+            if (run.sweep_first[q] >= executed) prof_uncount(b, run.sweep_name[q], 1, run.sweep_bytes[q]);
     }
     if (b.S_host->hs_par) {
         // the result lives in the spare set: swap it in

@@ -133,9 +133,10 @@ struct Batch;
 // size rule of tiles_default.  lcs_autotune picks them by timing.
 struct TileShapes {
     int sweep_tx = 0, sweep_ty = 0, rhs_tx = 0, rhs_ty = 0;
+    int sweep2_tx = 0, sweep2_ty = 0;  // two-sweep pass; 0: two single sweeps
 };
 TileShapes tiles_default(const Geom &g, int esize);
-// candidate (tx, ty) pairs: kind 0 sweep, 1 RHS; returns the count
+// candidate (tx, ty) pairs: kind 0 sweep, 1 RHS, 2 two-sweep pass; returns the count
 int tile_candidates(int esize, int kind, int out[][2], int cap);
 
 struct Box {

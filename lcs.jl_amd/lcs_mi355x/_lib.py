@@ -85,7 +85,8 @@ FLOW_STAT_FIELDS = ["u_mean", "u_max", "u_min", "v_mean", "v_max", "v_min", "w_m
 
 
 class TileShapes(C.Structure):
-    _fields_ = [("sweep_tx", C.c_int), ("sweep_ty", C.c_int), ("rhs_tx", C.c_int), ("rhs_ty", C.c_int)]
+    _fields_ = [("sweep_tx", C.c_int), ("sweep_ty", C.c_int), ("rhs_tx", C.c_int), ("rhs_ty", C.c_int),
+                ("sweep2_tx", C.c_int), ("sweep2_ty", C.c_int)]
 
 
 class FlowStat(C.Structure):

@@ -239,27 +239,29 @@ def boxmean(box, coarse_size):
 
 
 def _shapes(t):
-    return {"sweep": (t.sweep_tx, t.sweep_ty), "rhs": (t.rhs_tx, t.rhs_ty)}
+    return {"sweep": (t.sweep_tx, t.sweep_ty), "rhs": (t.rhs_tx, t.rhs_ty), "sweep2": (t.sweep2_tx, t.sweep2_ty)}
 
 
 def autotune(box):
     """Time the candidate tile shapes of the fused RHS and sweep kernels for
     this box's size and keep the fastest (cached per size).  No result
-    changes.  Returns {"sweep": (tx, ty), "rhs": (tx, ty)}."""
+    changes.  Returns {"sweep": (tx, ty), "rhs": (tx, ty), "sweep2": (tx, ty)}."""
     t = _lib.TileShapes()
     check(box.L.lcs_autotune(box.handle, C.byref(t)))
     return _shapes(t)
 
 
-def set_tiles(box, sweep=None, rhs=None):
-    """Force tile shapes; with no arguments, return to the size rule."""
-    if sweep is None and rhs is None:
+def set_tiles(box, sweep=None, rhs=None, sweep2=None):
+    """Force tile shapes (sweep2=(0, 0): no two-sweep pass); with no
+    arguments, return to the size rule."""
+    if sweep is None and rhs is None and sweep2 is None:
         check(box.L.lcs_set_tiles(box.handle, None))
         return
     cur = get_tiles(box)
     sweep = sweep or cur["sweep"]
     rhs = rhs or cur["rhs"]
-    t = _lib.TileShapes(int(sweep[0]), int(sweep[1]), int(rhs[0]), int(rhs[1]))
+    sweep2 = cur["sweep2"] if sweep2 is None else sweep2
+    t = _lib.TileShapes(int(sweep[0]), int(sweep[1]), int(rhs[0]), int(rhs[1]), int(sweep2[0]), int(sweep2[1]))
     check(box.L.lcs_set_tiles(box.handle, C.byref(t)))
 
 

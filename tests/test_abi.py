@@ -61,7 +61,8 @@ def test_c_struct_sizes_agree_with_ctypes(tmp_path):
                    ' offsetof(lcs_step_config, localize_interval), sizeof(lcs_request), offsetof(lcs_request, seed),'
                    ' sizeof(lcs_server_config), offsetof(lcs_server_config, localize_interval),'
                    ' sizeof(lcs_run_status)); printf("%zu %zu %zu\\n", sizeof(lcs_flow_stats),'
-                   ' offsetof(lcs_flow_stats, C_eps), sizeof(lcs_tile_shapes)); return 0;}\n')
+                   ' offsetof(lcs_flow_stats, C_eps), sizeof(lcs_tile_shapes));'
+                   ' printf("%zu\\n", offsetof(lcs_tile_shapes, sweep2_ty)); return 0;}\n')
     exe = tmp_path / "t"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
     out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
@@ -70,7 +71,8 @@ def test_c_struct_sizes_agree_with_ctypes(tmp_path):
                    _lib.StepConfig.forcing.offset, _lib.StepLog.div1.offset, _lib.StepConfig.localize_interval.offset,
                    C.sizeof(_lib.Request), _lib.Request.seed.offset, C.sizeof(_lib.ServerConfig),
                    _lib.ServerConfig.localize_interval.offset, C.sizeof(_lib.RunStatus), C.sizeof(_lib.FlowStat),
-                   _lib.FlowStat.C_eps.offset, C.sizeof(_lib.TileShapes)]
+                   _lib.FlowStat.C_eps.offset, C.sizeof(_lib.TileShapes),
+                   _lib.TileShapes.sweep2_ty.offset]
 
 
 YAML = """
