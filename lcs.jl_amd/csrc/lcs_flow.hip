@@ -369,8 +369,7 @@ TileShapes tiles_default(const Geom &g, int esize)
     const double e64 = (double)X / (64.0 * ((X + 63) / 64)), e32 = (double)X / (32.0 * ((X + 31) / 32));
     t.sweep_tx = e32 > e64 + 0.05 ? 32 : 64;
     t.sweep_ty = big ? 16 : 8;
-    t.sweep2_tx = esize == 4 && big ? 32 : 0;
-    t.sweep2_ty = esize == 4 && big ? 24 : 0;
+    t.sweep2_tx = t.sweep2_ty = 0;  // the two-sweep pass only where the tuner measured it faster
     if (esize == 4) {
         t.rhs_tx = big ? 64 : 32;
         t.rhs_ty = big ? 16 : 8;
