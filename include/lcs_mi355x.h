@@ -197,6 +197,46 @@ int lcs_force_rcf(lcs_box *box, double energy, int kmin, int kmax, int coarse_si
  * receives 3*c^3 values (U then V then W), column-major, as doubles */
 int lcs_boxmean(lcs_box *box, int coarse_size, double *coarse3);
 
+/* ---- particle statistics ------------------------------------------------------ */
+/* Particles.Stat (src/particles/config.jl:66-108): RDF and spectrum settings. */
+typedef struct {
+    double contact_tolerance; /* rdf.contact_tolerance */
+    int bin_size;             /* rdf.bin_size: number of log-spaced bins, 1..1024 */
+    double min_radius;        /* rdf.min_radius */
+    int search_cell_size;     /* rdf.search_cell_size: +-cells searched; rmax = it x min spacing */
+    int max_wavenumber;       /* spectrum.max_wavenumber (0: no spectrum) */
+    int nsamples;             /* spectrum.nsamples: directions per shell */
+} lcs_particle_stat_params;
+
+/* Particles.ParticleStat (src/particles/stat.jl:46-75) minus its arrays:
+ * LCS.Summary (mean, max, min) of x, y, z, u, v, w, diam over 1:nvalid. */
+typedef struct {
+    double x_mean, x_max, x_min, y_mean, y_max, y_min, z_mean, z_max, z_min;
+    double u_mean, u_max, u_min, v_mean, v_max, v_min, w_mean, w_max, w_min;
+    double diam_mean, diam_max, diam_min;
+    double taup;  /* of the mean diameter */
+    double St;    /* taup / tau_eta */
+    double gr_contact;
+    int64_t npairs_contact;
+    int nbins, nshells;
+} lcs_particle_stats;
+
+/* Particles.stat(fstat, buf, state, config, topo, iprofile)
+ * (src/particles/stat.jl:80-155): re-bins the profile (makeindex!), then the
+ * summaries, the radial distribution function over the cell index
+ * (rdf_assume_ci, src/particles/rdf.jl:1-103) and the number-density spectrum
+ * (src/particles/spectrum.jl:11-78).  tau_eta is FlowStat.tau_eta.  Arrays
+ * (any may be NULL): edges[bin_size+1], npairs[bin_size], gr[bin_size],
+ * wavenumbers[nshells], Enp[nshells] with nshells the wavenumbers of
+ * (1, 2, 3, 4, 6, 8, ..., 768) up to max_wavenumber.  Synchronises. */
+int lcs_particle_stat(lcs_box *box, int profile, const lcs_particle_stat_params *params, double tau_eta,
+                      lcs_particle_stats *out, double *edges, int64_t *npairs, double *gr, double *wavenumbers,
+                      double *Enp);
+/* Particles.density (src/particles/stat.jl:171-217): re-bins the profile, then
+ * the SPH density of each of the nvalid particles (cubic spline, h = smallest
+ * grid spacing), in the order the re-binning leaves.  Synchronises. */
+int lcs_particle_density(lcs_box *box, int profile, double *density);
+
 /* ---- tile shapes ------------------------------------------------------------ */
 /* No counterpart in the reference (its kernels are KernelAbstractions
  * workgroups of a fixed size).  The fused RHS and HSMAC sweep kernels march

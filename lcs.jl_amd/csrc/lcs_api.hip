@@ -786,6 +786,24 @@ int lcs_particles_cellindex(lcs_box *box, int profile, int64_t *hashes, int64_t 
     return LCS_OK;
 }
 
+int lcs_particle_stat(lcs_box *box, int profile, const lcs_particle_stat_params *params, double tau_eta,
+                      lcs_particle_stats *out, double *edges, int64_t *npairs, double *gr, double *wavenumbers,
+                      double *Enp)
+{
+    WITH_PROFILE
+    if (!params || !out) return fail(LCS_ERR_INVALID, "null argument");
+    batch_mark_dirty(b);  // re-binning moves the properties
+    return particle_stat(b, *pp, *params, tau_eta, *out, edges, npairs, gr, wavenumbers, Enp);
+}
+
+int lcs_particle_density(lcs_box *box, int profile, double *density)
+{
+    WITH_PROFILE
+    if (!density && pp->desc.nvalid > 0) return fail(LCS_ERR_INVALID, "null argument");
+    batch_mark_dirty(b);  // re-binning moves the properties
+    return particle_density(b, *pp, density);
+}
+
 int lcs_init_spectrum(lcs_box *box, uint64_t seed, double k0, double uprime)
 {
     if (check_box(box)) return LCS_ERR_INVALID;

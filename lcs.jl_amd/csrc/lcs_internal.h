@@ -207,6 +207,10 @@ int launch_boxmean(Box &b, int c, double *coarse_dev);
 // Flows.stat raw values into out_dev[15] (device): U, V, W mean/max/min, means of
 // u2, s2, dudx^2..4, then the integral length when asked
 int launch_flow_stat(Box &b, int integral_length, double *out_dev);
+// Particles.stat / Particles.density of one profile (re-bin first)
+int particle_stat(Box &b, ParticleProfile &pp, const lcs_particle_stat_params &p, double tau_eta,
+                  lcs_particle_stats &o, double *edges, int64_t *npairs, double *gr, double *wavenumbers, double *Enp);
+int particle_density(Box &b, ParticleProfile &pp, double *out);
 int launch_force_rcf(Box &b, double energy, int kmin, int kmax, int c, DtRef dt);
 int launch_init_spectrum(Box &b, uint64_t seed, double k0, double uprime);
 

@@ -9,7 +9,7 @@ from ._lib import LCSArgumentError, LCSError, LIBPATH, load
 from .config import (Config, ConstantPowerLF, EnergyPreserveRCF, FlowParams, Grid, HSMAC, NoForcing,
                      ParticleConfig, SpectrumInit, TimeStep, from_dict, readconfig)
 from .core import (Batch, Box, autotune, boundary, boxmean, couple, fill_ghost, flow_stat, flow_timestep, forcing, localize,
-                   get_tiles, makeindex, motion, particle_timestep, particle_update, read_log, rhs, rhs_update, Server,
+                   get_tiles, makeindex, motion, particle_density, particle_stat, particle_timestep, particle_update, read_log, rhs, rhs_update, Server,
                    set_tiles, step, step_config, update)
 from .simulate import build, simulate
 
@@ -17,6 +17,6 @@ __all__ = [
     "Batch", "Box", "Config", "ConstantPowerLF", "EnergyPreserveRCF", "FlowParams", "Grid", "HSMAC", "LCSArgumentError",
     "LCSError", "LIBPATH", "NoForcing", "ParticleConfig", "SpectrumInit", "TimeStep", "autotune", "boundary", "boxmean",
     "build", "config", "couple", "fill_ghost", "flow_stat", "flow_timestep", "forcing", "from_dict", "get_tiles", "load", "localize",
-    "makeindex", "motion", "particle_timestep", "particle_update", "read_log", "readconfig", "rhs", "rhs_update",
+    "makeindex", "motion", "particle_density", "particle_stat", "particle_timestep", "particle_update", "read_log", "readconfig", "rhs", "rhs_update",
     "Server", "set_tiles", "simulate", "step", "step_config", "update",
 ]

@@ -84,6 +84,20 @@ FLOW_STAT_FIELDS = ["u_mean", "u_max", "u_min", "v_mean", "v_max", "v_min", "w_m
                     "tau_eta", "l_mu", "l_eta", "Re_lambda", "urms", "k_eta", "skew", "flat", "L", "C_eps"]
 
 
+class ParticleStatParams(C.Structure):
+    _fields_ = [("contact_tolerance", C.c_double), ("bin_size", C.c_int), ("min_radius", C.c_double),
+                ("search_cell_size", C.c_int), ("max_wavenumber", C.c_int), ("nsamples", C.c_int)]
+
+
+PSTAT_SUMMARIES = [f"{c}_{k}" for c in ("x", "y", "z", "u", "v", "w", "diam") for k in ("mean", "max", "min")]
+
+
+class ParticleStats(C.Structure):
+    _fields_ = ([(n, C.c_double) for n in PSTAT_SUMMARIES] +
+                [("taup", C.c_double), ("St", C.c_double), ("gr_contact", C.c_double),
+                 ("npairs_contact", C.c_int64), ("nbins", C.c_int), ("nshells", C.c_int)])
+
+
 class TileShapes(C.Structure):
     _fields_ = [("sweep_tx", C.c_int), ("sweep_ty", C.c_int), ("rhs_tx", C.c_int), ("rhs_ty", C.c_int),
                 ("sweep2_tx", C.c_int), ("sweep2_ty", C.c_int)]
@@ -124,6 +138,8 @@ SIGNATURES = {
     "lcs_boxmean": [P, I, P],
     "lcs_flow_stat": [P, I, C.POINTER(FlowStat)],
     "lcs_autotune": [P, C.POINTER(TileShapes)],
+    "lcs_particle_stat": [P, I, C.POINTER(ParticleStatParams), C.c_double, C.POINTER(ParticleStats), P, P, P, P, P],
+    "lcs_particle_density": [P, I, P],
     "lcs_set_tiles": [P, C.POINTER(TileShapes)],
     "lcs_get_tiles": [P, C.POINTER(TileShapes)],
     "lcs_particles_add": [P, C.POINTER(ParticleDesc), C.POINTER(I)],

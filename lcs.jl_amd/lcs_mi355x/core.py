@@ -238,6 +238,43 @@ def boxmean(box, coarse_size):
     return [out[a * c ** 3:(a + 1) * c ** 3].reshape((c, c, c), order="F") for a in range(3)]
 
 
+SPECTRUM_WAVENUMBERS = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768)
+
+
+def particle_stat(box, profile, tau_eta, contact_tolerance=0.01, bin_size=16, min_radius=1.25e-4,
+                  search_cell_size=2, max_wavenumber=768, nsamples=64):
+    """Particles.stat (src/particles/stat.jl:80-155) of one profile: re-bins it
+    (makeindex!), then summaries, taup/St, the radial distribution function
+    over the cell index and the number-density spectrum.  Defaults are those of
+    cfgs/N1024-1.lcs-yaml."""
+    prm = _lib.ParticleStatParams(float(contact_tolerance), int(bin_size), float(min_radius), int(search_cell_size),
+                                  int(max_wavenumber), int(nsamples))
+    nsh = sum(1 for k in SPECTRUM_WAVENUMBERS if k <= max_wavenumber) if max_wavenumber > 0 else 0
+    edges = np.zeros(bin_size + 1)
+    npairs = np.zeros(bin_size, np.int64)
+    gr = np.zeros(bin_size)
+    wn = np.zeros(max(nsh, 1))
+    enp = np.zeros(max(nsh, 1))
+    st = _lib.ParticleStats()
+    check(box.L.lcs_particle_stat(box.handle, int(profile), C.byref(prm), float(tau_eta), C.byref(st),
+                                  edges.ctypes.data, npairs.ctypes.data, gr.ctypes.data, wn.ctypes.data,
+                                  enp.ctypes.data))
+    out = {c: {k: getattr(st, f"{c}_{k}") for k in ("mean", "max", "min")} for c in ("x", "y", "z", "u", "v", "w", "diam")}
+    out.update(taup=st.taup, St=st.St, rdf=dict(gr_contact=st.gr_contact, npairs_contact=st.npairs_contact,
+                                                 edges=edges, gr=gr, npairs=npairs),
+               spectrum=dict(wavenumbers=wn[:nsh], Enp=enp[:nsh]))
+    return out
+
+
+def particle_density(box, profile):
+    """Particles.density (src/particles/stat.jl:171-217): re-bins the profile,
+    then the SPH density of every valid particle in the re-binned order."""
+    nv = box.profiles[profile]["nvalid"]
+    out = np.zeros(max(nv, 1))
+    check(box.L.lcs_particle_density(box.handle, int(profile), out.ctypes.data))
+    return out[:nv]
+
+
 def _shapes(t):
     return {"sweep": (t.sweep_tx, t.sweep_ty), "rhs": (t.rhs_tx, t.rhs_ty), "sweep2": (t.sweep2_tx, t.sweep2_ty)}
 

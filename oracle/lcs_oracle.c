@@ -648,6 +648,148 @@ void o_flow_stat(int nx, int ny, int nz, int h, const real *U, const real *V, co
 }
 
 /* ------------------------------------------------------------------------- */
+/* Particle statistics on the cell index: src/particles/rdf.jl:1-103,          */
+/* src/particles/stat.jl:171-231 (SPH density), src/particles/spectrum.jl.     */
+/* Positions, ids, hashes and 1-based starts/stops are those makeindex!        */
+/* leaves (hash = 1 + ci1 + (nc1+2) ci2 + (nc1+2)(nc2+2) ci3, axes 0:nc+1).     */
+/* ------------------------------------------------------------------------- */
+static void decode_hash(int64_t hsh, const int nc[3], int ci[3])
+{
+    int64_t r = hsh - 1;
+    ci[0] = (int)(r % (nc[0] + 2));
+    r /= nc[0] + 2;
+    ci[1] = (int)(r % (nc[1] + 2));
+    ci[2] = (int)(r / (nc[1] + 2));
+}
+
+/* Utils.norm (src/utils/fast.jl:42-46) of a difference, in the particle type */
+static inline real pnorm(real dx, real dy, real dz) { return (real)sqrt((double)((dx * dx + dy * dy) + dz * dz)); }
+
+/* rdf_assume_ci: pair counts per log-spaced bin (edges[k] < d <= edges[k+1])
+ * and contact pairs |d/diam - 1| < tol, over pairs id1 < id2 found within
+ * +-ncell coarse cells of a particle whose cell is inside the domain. */
+void o_rdf(int nx, int ny, int nz, int grid_ratio, long n, const real *xs, const real *ys, const real *zs,
+           const int64_t *ids, const int64_t *hashes, const int64_t *starts, const int64_t *stops, double diam,
+           double tol, int nbins, const double *edges, int ncell, int64_t *npairs, int64_t *ncontact)
+{
+    const int nc[3] = {nx / grid_ratio, ny / grid_ratio, nz / grid_ratio};
+    for (int k = 0; k < nbins; k++) npairs[k] = 0;
+    *ncontact = 0;
+    for (long i1 = 0; i1 < n; i1++) {
+        int ci[3];
+        decode_hash(hashes[i1], nc, ci);
+        if (ci[0] < 1 || ci[0] > nc[0] || ci[1] < 1 || ci[1] > nc[1] || ci[2] < 1 || ci[2] > nc[2]) continue;
+        for (int di = -ncell; di <= ncell; di++)
+            for (int dj = -ncell; dj <= ncell; dj++)
+                for (int dk = -ncell; dk <= ncell; dk++) {
+                    const int c2[3] = {ci[0] + di, ci[1] + dj, ci[2] + dk};
+                    if (c2[0] < 0 || c2[0] > nc[0] + 1 || c2[1] < 0 || c2[1] > nc[1] + 1 || c2[2] < 0 ||
+                        c2[2] > nc[2] + 1)
+                        continue;
+                    const int64_t h2 = 1 + c2[0] + (int64_t)(nc[0] + 2) * (c2[1] + (int64_t)(nc[1] + 2) * c2[2]);
+                    for (int64_t i2 = starts[h2 - 1]; i2 <= stops[h2 - 1]; i2++) {
+                        const long j = (long)i2 - 1;
+                        if (!(ids[i1] < ids[j])) continue;
+                        const double d = (double)pnorm(xs[i1] - xs[j], ys[i1] - ys[j], zs[i1] - zs[j]);
+                        /* searchsortedfirst(edges, d) - 1: first edge >= d */
+                        int lo = 0, hi = nbins + 1;
+                        while (lo < hi) {
+                            const int m = (lo + hi) / 2;
+                            if (edges[m] < d)
+                                lo = m + 1;
+                            else
+                                hi = m;
+                        }
+                        const int k = lo; /* 1-based bin = lo (0-based index of the first edge >= d) */
+                        if (k >= 1 && k <= nbins) npairs[k - 1]++;
+                        if (fabs(d / diam - 1) < tol) (*ncontact)++;
+                    }
+                }
+    }
+}
+
+/* cubic_spline_kernel: src/particles/stat.jl:220-231 */
+static double cubic_spline(double r, double h)
+{
+    const double q = r / h, factor = 1 / (M_PI * (h * h * h));
+    if (q < 1) return factor * ((1 - 1.5 * (q * q)) + 0.75 * ((q * q) * q));
+    if (q < 2) return (factor * 0.25) * (((2 - q) * (2 - q)) * (2 - q));
+    return 0.0;
+}
+
+/* density: src/particles/stat.jl:171-217 (h = smallest fine spacing) */
+void o_sph_density(int nx, int ny, int nz, int grid_ratio, long n, const real *xs, const real *ys, const real *zs,
+                   const int64_t *hashes, const int64_t *starts, const int64_t *stops, double *dens)
+{
+    const int nc[3] = {nx / grid_ratio, ny / grid_ratio, nz / grid_ratio};
+    const double h = fmin(fmin(spacing(nx), spacing(ny)), spacing(nz));
+    for (long i1 = 0; i1 < n; i1++) {
+        int ci[3];
+        decode_hash(hashes[i1], nc, ci);
+        double rho = 0.0;
+        if (ci[0] >= 1 && ci[0] <= nc[0] && ci[1] >= 1 && ci[1] <= nc[1] && ci[2] >= 1 && ci[2] <= nc[2]) {
+            for (int di = -1; di <= 1; di++)
+                for (int dj = -1; dj <= 1; dj++)
+                    for (int dk = -1; dk <= 1; dk++) {
+                        const int c2[3] = {ci[0] + di, ci[1] + dj, ci[2] + dk};
+                        const int64_t h2 =
+                            1 + c2[0] + (int64_t)(nc[0] + 2) * (c2[1] + (int64_t)(nc[1] + 2) * c2[2]);
+                        for (int64_t i2 = starts[h2 - 1]; i2 <= stops[h2 - 1]; i2++) {
+                            const long j = (long)i2 - 1;
+                            const double d = (double)pnorm(xs[i1] - xs[j], ys[i1] - ys[j], zs[i1] - zs[j]);
+                            if (d < 2 * h) rho += cubic_spline(d, h);
+                        }
+                    }
+        }
+        dens[i1] = rho;
+    }
+}
+
+/* _fibonacci_direction: src/particles/spectrum.jl:3-9 (i 0-based) */
+void o_fib_dir(int i, int nsamples, double out[3])
+{
+    const double phi = M_PI * (3 - sqrt(5.0));
+    const double y = 1 - ((double)i / (double)(nsamples - 1)) * 2;
+    const double radius = sqrt(1 - y * y);
+    const double theta = phi * i;
+    out[0] = cos(theta) * radius;
+    out[1] = y;
+    out[2] = sin(theta) * radius;
+}
+
+/* spectrum: src/particles/spectrum.jl:11-78.  wavenumbers[nshells]; cos/sin
+ * sums per (shell, direction) accumulated per particle in double; Enp out. */
+void o_pspectrum(long n, const real *xs, const real *ys, const real *zs, int nshells, const int *wavenumbers,
+                 int nsamples, double nvalid_g, double *Enp)
+{
+    double *cs = (double *)calloc((size_t)nshells * nsamples * 2, sizeof(double));
+    for (int b = 0; b < nshells; b++)
+        for (int i = 0; i < nsamples; i++) {
+            double dir[3];
+            o_fib_dir(i, nsamples, dir);
+            const double kk = wavenumbers[b], k[3] = {kk * dir[0], kk * dir[1], kk * dir[2]};
+            double c = 0, sn = 0;
+            for (long p = 0; p < n; p++) {
+                const double kx = (k[0] * (double)xs[p] + k[1] * (double)ys[p]) + k[2] * (double)zs[p];
+                c += cos(kx);
+                sn += sin(kx);
+            }
+            cs[2 * (b * nsamples + i)] = c;
+            cs[2 * (b * nsamples + i) + 1] = sn;
+        }
+    for (int b = 0; b < nshells; b++) {
+        double phi = 0.0;
+        for (int i = 0; i < nsamples; i++) {
+            const double c = cs[2 * (b * nsamples + i)], sn = cs[2 * (b * nsamples + i) + 1];
+            phi += (c * c + sn * sn) / (nvalid_g * nvalid_g) - 1 / nvalid_g;
+        }
+        const double kk = wavenumbers[b];
+        Enp[b] = phi / nsamples * 4 * M_PI * (kk * kk);
+    }
+    free(cs);
+}
+
+/* ------------------------------------------------------------------------- */
 /* rc-energy forcing: src/flows/forcings/rcf.jl:21-191,                      */
 /* lib/fouriertools/src/FourierTools.jl:9-208.                                */
 /* ------------------------------------------------------------------------- */

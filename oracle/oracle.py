@@ -81,6 +81,10 @@ def _declare(L):
         "o_init_spectrum": (None, [I, I, I, I, U64, D, D, P, P, P, P]),
         "o_init_particles": (None, [U64, Lg, D, D, D, D, D, P, P, P, P, P]),
         "o_flow_stat": (None, [I, I, I, I, P, P, P, I, P]),
+        "o_rdf": (None, [I, I, I, I, Lg, P, P, P, P, P, P, P, D, D, I, P, I, P, P]),
+        "o_sph_density": (None, [I, I, I, I, Lg, P, P, P, P, P, P, P]),
+        "o_fib_dir": (None, [I, I, P]),
+        "o_pspectrum": (None, [Lg, P, P, P, I, P, I, D, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -202,6 +206,58 @@ def flow_stat(U, V, W, h, Re, integral_length=False):
     for a in Us:
         ghost_fill(a, h, h, True)
     return flow_stat_derived(flow_stat_sums(*Us, h, integral_length), dims_of(U, h), Re)
+
+
+# particle statistics (src/particles/rdf.jl, spectrum.jl, stat.jl)
+SPECTRUM_WAVENUMBERS = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768)
+
+
+def rdf_edges(rmin, rmax, nbins):
+    """logrange(rmin, rmax, nbins + 1): log-spaced, ends exact."""
+    e = np.exp(np.linspace(np.log(rmin), np.log(rmax), nbins + 1))
+    e[0], e[-1] = rmin, rmax
+    return e
+
+
+def rdf_counts(dims, grid_ratio, xss, ids, hashes, starts, stops, diam, tol, edges, ncell):
+    """rdf_assume_ci pair counts (per bin) and contact pairs, on makeindex!'s output."""
+    n = len(hashes)
+    nb = len(edges) - 1
+    npairs = np.zeros(nb, np.int64)
+    ncont = C.c_int64(0)
+    xs = [np.ascontiguousarray(a) for a in xss]
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    ids, hashes, starts, stops = (np.ascontiguousarray(a, dtype=np.int64) for a in (ids, hashes, starts, stops))
+    lib(xs[0].dtype).o_rdf(*dims, grid_ratio, n, *[_p(a) for a in xs], _p(ids), _p(hashes), _p(starts), _p(stops),
+                           float(diam), float(tol), nb, _p(e), int(ncell), _p(npairs), C.addressof(ncont))
+    return npairs, ncont.value
+
+
+def sph_density(dims, grid_ratio, xss, hashes, starts, stops):
+    n = len(hashes)
+    out = np.zeros(n)
+    xs = [np.ascontiguousarray(a) for a in xss]
+    hashes, starts, stops = (np.ascontiguousarray(a, dtype=np.int64) for a in (hashes, starts, stops))
+    lib(xs[0].dtype).o_sph_density(*dims, grid_ratio, n, *[_p(a) for a in xs], _p(hashes), _p(starts), _p(stops),
+                                   _p(out))
+    return out
+
+
+def fib_dir(i, nsamples):
+    out = np.zeros(3)
+    lib(np.float64).o_fib_dir(int(i), int(nsamples), _p(out))
+    return out
+
+
+def pspectrum(xss, max_wavenumber, nsamples, nvalid_g=None):
+    ks = [k for k in SPECTRUM_WAVENUMBERS if k <= max_wavenumber]
+    xs = [np.ascontiguousarray(a) for a in xss]
+    n = len(xs[0])
+    wn = np.asarray(ks, dtype=np.int32)
+    out = np.zeros(len(ks))
+    lib(xs[0].dtype).o_pspectrum(n, *[_p(a) for a in xs], len(ks), _p(wn), int(nsamples),
+                                 float(n if nvalid_g is None else nvalid_g), _p(out))
+    return np.asarray(ks, dtype=np.float64), out
 
 
 def boxmean(U, h, coarse):

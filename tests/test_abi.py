@@ -62,7 +62,8 @@ def test_c_struct_sizes_agree_with_ctypes(tmp_path):
                    ' sizeof(lcs_server_config), offsetof(lcs_server_config, localize_interval),'
                    ' sizeof(lcs_run_status)); printf("%zu %zu %zu\\n", sizeof(lcs_flow_stats),'
                    ' offsetof(lcs_flow_stats, C_eps), sizeof(lcs_tile_shapes));'
-                   ' printf("%zu\\n", offsetof(lcs_tile_shapes, sweep2_ty)); return 0;}\n')
+                   ' printf("%zu %zu %zu %zu\\n", offsetof(lcs_tile_shapes, sweep2_ty), sizeof(lcs_particle_stat_params),'
+                   ' sizeof(lcs_particle_stats), offsetof(lcs_particle_stats, npairs_contact)); return 0;}\n')
     exe = tmp_path / "t"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
     out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
@@ -72,7 +73,8 @@ def test_c_struct_sizes_agree_with_ctypes(tmp_path):
                    C.sizeof(_lib.Request), _lib.Request.seed.offset, C.sizeof(_lib.ServerConfig),
                    _lib.ServerConfig.localize_interval.offset, C.sizeof(_lib.RunStatus), C.sizeof(_lib.FlowStat),
                    _lib.FlowStat.C_eps.offset, C.sizeof(_lib.TileShapes),
-                   _lib.TileShapes.sweep2_ty.offset]
+                   _lib.TileShapes.sweep2_ty.offset, C.sizeof(_lib.ParticleStatParams), C.sizeof(_lib.ParticleStats),
+                   _lib.ParticleStats.npairs_contact.offset]
 
 
 YAML = """
