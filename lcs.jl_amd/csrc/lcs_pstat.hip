@@ -5,29 +5,28 @@
 // (src/particles/stat.jl:171-231).  Both entry points re-bin the profile first
 // (makeindex!), as the reference does.
 //
-// Distances are Utils.norm (src/utils/fast.jl:42-46) in the particle type with
-// every multiply and add rounded on its own (no contraction), so pair counts
-// and densities are those of the reference's arithmetic, bit for bit.
+// Distances are Utils.norm (src/utils/fast.jl:42-46) in the particle type.
+// This file is compiled without contraction of multiplies into adds (the
+// pragma below) and with correctly rounded square roots (sqrtf/sqrt; HIP's
+// default), so every operation rounds as the reference's does and pair counts
+// and densities are the reference's arithmetic, bit for bit.
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
 #include "lcs_internal.h"
 
+#pragma clang fp contract(off)
+
 namespace lcs {
 
-// ---- exact (uncontracted) arithmetic in the particle type -------------------------
-__device__ inline float rmul(float a, float b) { return __fmul_rn(a, b); }
-__device__ inline float radd(float a, float b) { return __fadd_rn(a, b); }
-__device__ inline float rsqrt_n(float a) { return __fsqrt_rn(a); }
-__device__ inline double rmul(double a, double b) { return __dmul_rn(a, b); }
-__device__ inline double radd(double a, double b) { return __dadd_rn(a, b); }
-__device__ inline double rsqrt_n(double a) { return __dsqrt_rn(a); }
+__device__ inline float psqrt(float a) { return sqrtf(a); }
+__device__ inline double psqrt(double a) { return sqrt(a); }
 
 template <typename T>
 __device__ inline double pnorm(T dx, T dy, T dz)
 {
-    return (double)rsqrt_n(radd(radd(rmul(dx, dx), rmul(dy, dy)), rmul(dz, dz)));
+    return (double)psqrt((dx * dx + dy * dy) + dz * dz);
 }
 
 __device__ inline void decode_hash(int64_t hsh, const int nc[3], int ci[3])
@@ -165,17 +164,16 @@ __global__ void __launch_bounds__(256) k_ps_density(long n, const T *xs, const T
                     for (int64_t j = j1 - 1; j <= j2 - 1; j++) {
                         const double d = pnorm<T>(x1 - xs[j], y1 - ys[j], z1 - zs[j]);
                         if (!(d < h2x)) continue;
-                        // cubic_spline_kernel (stat.jl:220-231), every operation rounded
+                        // cubic_spline_kernel (stat.jl:220-231)
                         const double q = d / h;
                         double w;
                         if (q < 1) {
-                            const double q2 = __dmul_rn(q, q), q3 = __dmul_rn(q2, q);
-                            w = __dmul_rn(factor, __dadd_rn(__dsub_rn(1.0, __dmul_rn(1.5, q2)), __dmul_rn(0.75, q3)));
+                            w = factor * ((1 - 1.5 * (q * q)) + 0.75 * ((q * q) * q));
                         } else {
-                            const double t = __dsub_rn(2.0, q);
-                            w = __dmul_rn(__dmul_rn(factor, 0.25), __dmul_rn(__dmul_rn(t, t), t));
+                            const double t = 2 - q;
+                            w = (factor * 0.25) * ((t * t) * t);
                         }
-                        rho = __dadd_rn(rho, w);
+                        rho += w;
                     }
                 }
     }
