@@ -10,12 +10,37 @@
 #include <cmath>
 #include <cstring>
 
+#include <map>
+#include <mutex>
+
 #include "lcs_internal.h"
 #include "lcs_hsmac_tile.h"
 #include "lcs_hsmac_tile2.h"
 #include "lcs_rhs_tile.h"
 
 namespace lcs {
+
+// Launch set-up of a kernel with more dynamic LDS than the default: its LDS
+// limit is raised and its resident workgroups per CU measured once per
+// (kernel, device).
+int kernel_setup(const void *func, int threads, size_t lds, int *per_cu)
+{
+    static std::mutex mu;
+    static std::map<std::pair<const void *, int>, int> done;
+    int dev = 0;
+    LCS_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_pair(func, dev);
+    auto it = done.find(key);
+    if (it == done.end()) {
+        LCS_HIP(hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        int pc = 0;
+        LCS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, func, threads, lds));
+        it = done.emplace(key, std::max(pc, 1)).first;
+    }
+    if (per_cu) *per_cu = it->second;
+    return LCS_OK;
+}
 
 // ---------------------------------------------------------------------------
 // small device helpers
@@ -328,12 +353,8 @@ static int rhs_tile_launch(Box &b, const RhsArgs<T> &a0, DtRef dt)
     a.zc = (g.nz + gz - 1) / gz;
     gz = (g.nz + a.zc - 1) / a.zc;
     const size_t lds = Tile::LDS_ELEMS * sizeof(T);
-    static bool attr_set = false;  // per template instance; the attribute is a property of the function
-    if (!attr_set) {
-        LCS_HIP(hipFuncSetAttribute((const void *)k_rhs_tile<T, TX, TY, FUSED>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        attr_set = true;
-    }
+    int rc = kernel_setup((const void *)k_rhs_tile<T, TX, TY, FUSED>, TX * TY, lds, nullptr);
+    if (rc) return rc;
     hipLaunchKernelGGL((k_rhs_tile<T, TX, TY, FUSED>), dim3(gx, gy, gz), dim3(TX * TY), lds, b.stream, a, dt);
     LCS_HIP(hipGetLastError());
     return LCS_OK;
@@ -772,14 +793,9 @@ struct HsmacRun {
         const int zplanes = g.nz + 2 * g.h;
         const long cols = (long)gx * gy;
         const size_t lds = Tile::LDS_ELEMS * sizeof(T);
-        static int per_cu = 0;  // resident workgroups per CU, per template instance
-        if (!per_cu) {
-            LCS_HIP(hipFuncSetAttribute((const void *)k_hsmac_sweep<T, COLORING, TY, TX>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            LCS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_hsmac_sweep<T, COLORING, TY, TX>,
-                                                                 Tile::NT, lds));
-            if (per_cu < 1) per_cu = 1;
-        }
+        int per_cu = 1;  // resident workgroups per CU
+        int rc = kernel_setup((const void *)k_hsmac_sweep<T, COLORING, TY, TX>, Tile::NT, lds, &per_cu);
+        if (rc) return rc;
         // z-chunks: a chunk re-reads five halo planes and the last round of
         // workgroups may be partial; pick the count with the fewest
         // plane-steps on the busiest CU slot
@@ -869,14 +885,9 @@ struct HsmacRun {
         const int zplanes = g.nz + 2 * g.h;
         const long cols = (long)gx * gy;
         const size_t lds = Tile::LDS_ELEMS * sizeof(T);
-        static int per_cu = 0;
-        if (!per_cu) {
-            LCS_HIP(hipFuncSetAttribute((const void *)k_hsmac_sweep2<T, COLORING, TY, TX>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            LCS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_hsmac_sweep2<T, COLORING, TY, TX>,
-                                                                 Tile::NT, lds));
-            if (per_cu < 1) per_cu = 1;
-        }
+        int per_cu = 1;
+        int rc = kernel_setup((const void *)k_hsmac_sweep2<T, COLORING, TY, TX>, Tile::NT, lds, &per_cu);
+        if (rc) return rc;
         // z-chunks as for one sweep; a chunk here re-reads about nine halo planes
         const long slots = (long)b.cus * per_cu;
         int nzc = 1;
@@ -2144,12 +2155,8 @@ static int b_rhs_t(hipStream_t s, const BoxRec *tab, const TileItem *tiles, int 
     constexpr int TX = kBatchRhsTX, TY = batch_rhs_ty(sizeof(T));
     using Tile = RhsTile<T, TX, TY>;
     const size_t lds = Tile::LDS_ELEMS * sizeof(T);
-    static bool attr = false;
-    if (!attr) {
-        LCS_HIP(hipFuncSetAttribute((const void *)k_b_rhs_tile<T, TX, TY>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds));
-        attr = true;
-    }
+    int rc = kernel_setup((const void *)k_b_rhs_tile<T, TX, TY>, TX * TY, lds, nullptr);
+    if (rc) return rc;
     hipLaunchKernelGGL((k_b_rhs_tile<T, TX, TY>), dim3(ntiles), dim3(TX * TY), lds, s, tab, tiles, stage);
     LCS_HIP(hipGetLastError());
     return LCS_OK;
@@ -2204,12 +2211,8 @@ static int b_sweep_t(hipStream_t s, const BoxRec *tab, const TileItem *tiles, in
 {
     using Tile = HsTile<T, COLORING, TY, TX>;
     const size_t lds = Tile::LDS_ELEMS * sizeof(T);
-    static bool attr = false;
-    if (!attr) {
-        LCS_HIP(hipFuncSetAttribute((const void *)k_b_hsmac_sweep<T, COLORING, TY, TX>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        attr = true;
-    }
+    int rc = kernel_setup((const void *)k_b_hsmac_sweep<T, COLORING, TY, TX>, Tile::NT, lds, nullptr);
+    if (rc) return rc;
     hipLaunchKernelGGL((k_b_hsmac_sweep<T, COLORING, TY, TX>), dim3(ntiles), dim3(Tile::NT), lds, s, tab, tiles, set,
                        kind, slot, dtfac);
     LCS_HIP(hipGetLastError());
