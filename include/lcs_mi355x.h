@@ -159,6 +159,9 @@ int lcs_box_create(lcs_ctx *ctx, const lcs_box_desc *desc, lcs_box **out);
 int lcs_box_destroy(lcs_box *box);
 /* bytes of device memory the box holds now */
 int lcs_box_bytes(const lcs_box *box, size_t *bytes);
+/* the descriptor the box was created with and its number of particle
+ * profiles (nprofiles may be NULL) */
+int lcs_box_describe(const lcs_box *box, lcs_box_desc *desc, int *nprofiles);
 
 /* host <-> device copies in the reference layout (blocking) */
 int lcs_field_upload(lcs_box *box, int field, const void *host, size_t bytes);
@@ -294,6 +297,8 @@ int lcs_flow_stat(lcs_box *box, int integral_length, lcs_flow_stats *out);
 /* Particles.ParticleBuffer (src/particles/buffer.jl:124-135) for one profile;
  * profile index returned in *profile (0-based, in call order) */
 int lcs_particles_add(lcs_box *box, const lcs_particle_desc *desc, int *profile);
+/* the descriptor a profile was added with (nvalid = state.particles[i].nvalid) */
+int lcs_particles_describe(const lcs_box *box, int profile, lcs_particle_desc *desc);
 int lcs_particles_upload(lcs_box *box, int profile, int prop, const void *host, size_t bytes);
 int lcs_particles_download(lcs_box *box, int profile, int prop, void *host, size_t bytes);
 /* Particles.boundary! (src/particles/boundary.jl:33-120), single process */
@@ -336,6 +341,9 @@ int lcs_init_particles(lcs_box *box, int profile, uint64_t seed, double st_min, 
 int lcs_step(lcs_box *box, const lcs_step_config *cfg, lcs_step_log *log);
 /* the most recent step's log (synchronises) */
 int lcs_step_log_read(lcs_box *box, lcs_step_log *log);
+/* state.step = step; state.t = t, as LCSIO.load_resume_state! sets them on
+ * resume (src/LCSIO/load.jl:1-21).  state.dt is recomputed by the next step. */
+int lcs_set_state(lcs_box *box, int64_t step, double t);
 
 /* ---- batches of boxes ---------------------------------------------------- */
 /* No counterpart in the reference: an ensemble of independent boxes advanced

@@ -24,6 +24,12 @@ int fail(int code, const std::string &msg)
 
 __global__ void k_set_double(double *dst, double v) { *dst = v; }
 
+__global__ void k_set_state(DevScalars *S, int64_t step, double t)
+{
+    S->step = step;
+    S->t = t;
+}
+
 __global__ void k_advance_state(DevScalars *S)
 {
     // evolve!(state): src/evolve.jl:10-13
@@ -259,6 +265,15 @@ int lcs_box_bytes(const lcs_box *box, size_t *bytes)
 {
     if (check_box(box) || !bytes) return fail(LCS_ERR_INVALID, "null argument");
     *bytes = B(box)->bytes;
+    return LCS_OK;
+}
+
+int lcs_box_describe(const lcs_box *box, lcs_box_desc *desc, int *nprofiles)
+{
+    if (check_box(box) || !desc) return fail(LCS_ERR_INVALID, "null argument");
+    const Box &b = *B(box);
+    *desc = b.desc;
+    if (nprofiles) *nprofiles = (int)b.profiles.size();
     return LCS_OK;
 }
 
@@ -710,6 +725,15 @@ static int prop_xfer(lcs_box *box, int profile, int prop, void *host, size_t byt
     return LCS_OK;
 }
 
+int lcs_particles_describe(const lcs_box *box, int profile, lcs_particle_desc *desc)
+{
+    if (check_box(box) || !desc) return fail(LCS_ERR_INVALID, "null argument");
+    const Box &b = *B(box);
+    if (profile < 0 || profile >= (int)b.profiles.size()) return fail(LCS_ERR_STATE, "no such particle profile");
+    *desc = b.profiles[profile].desc;
+    return LCS_OK;
+}
+
 int lcs_particles_upload(lcs_box *box, int profile, int prop, const void *host, size_t bytes)
 {
     return prop_xfer(box, profile, prop, const_cast<void *>(host), bytes, true);
@@ -894,6 +918,18 @@ int lcs_step(lcs_box *box, const lcs_step_config *cfg, lcs_step_log *log)
         if ((rc = read_scalars(b))) return rc;
         fill_log(*b.S_host, log);
     }
+    return LCS_OK;
+}
+
+int lcs_set_state(lcs_box *box, int64_t step, double t)
+{
+    if (check_box(box)) return fail(LCS_ERR_INVALID, "null argument");
+    Box &b = *B(box);
+    if (step < 0) return fail(LCS_ERR_INVALID, "step must be >= 0");
+    LCS_HIP(hipSetDevice(b.ctx->device));
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(1), 0, b.stream, b.S, step, t);
+    LCS_HIP(hipGetLastError());
+    LCS_HIP(hipStreamSynchronize(b.stream));
     return LCS_OK;
 }
 

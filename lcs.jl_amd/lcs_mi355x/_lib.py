@@ -125,6 +125,7 @@ SIGNATURES = {
     "lcs_box_create": [P, C.POINTER(BoxDesc), C.POINTER(P)],
     "lcs_box_destroy": [P],
     "lcs_box_bytes": [P, C.POINTER(C.c_size_t)],
+    "lcs_box_describe": [P, C.POINTER(BoxDesc), C.POINTER(I)],
     "lcs_field_upload": [P, I, P, C.c_size_t],
     "lcs_field_download": [P, I, P, C.c_size_t],
     "lcs_fill_ghost": [P, C.c_uint, I, I],
@@ -143,6 +144,7 @@ SIGNATURES = {
     "lcs_set_tiles": [P, C.POINTER(TileShapes)],
     "lcs_get_tiles": [P, C.POINTER(TileShapes)],
     "lcs_particles_add": [P, C.POINTER(ParticleDesc), C.POINTER(I)],
+    "lcs_particles_describe": [P, I, C.POINTER(ParticleDesc)],
     "lcs_particles_upload": [P, I, I, P, C.c_size_t],
     "lcs_particles_download": [P, I, I, P, C.c_size_t],
     "lcs_particles_boundary": [P, I, I],
@@ -156,6 +158,7 @@ SIGNATURES = {
     "lcs_init_particles": [P, I, C.c_uint64, D, D, D, I],
     "lcs_step": [P, C.POINTER(StepConfig), C.POINTER(StepLog)],
     "lcs_step_log_read": [P, C.POINTER(StepLog)],
+    "lcs_set_state": [P, C.c_int64, D],
     "lcs_batch_create": [P, C.POINTER(P)],
     "lcs_batch_destroy": [P],
     "lcs_batch_add": [P, P, C.POINTER(StepConfig)],

@@ -110,11 +110,23 @@ class SpectrumInit:
 
 
 @dataclass(frozen=True)
+class RestartInit:  # LCS.Restart (src/config.jl:235-238), kind "restart" (src/flows/config.jl:27)
+    file: str
+    kind: str = "restart"
+
+
+@dataclass(frozen=True)
+class FlowStatConfig:  # Flows.Stat (src/flows/config.jl:182-185)
+    integral_length: bool = False
+
+
+@dataclass(frozen=True)
 class FlowConfig:  # src/flows/config.jl:199-206
     params: FlowParams
     init: object
     couple: HSMAC
     forcing: object
+    stat: FlowStatConfig = field(default_factory=FlowStatConfig)
 
 
 @dataclass(frozen=True)
@@ -138,12 +150,26 @@ class ParticleParams:  # src/particles/config.jl:35-41
 @dataclass(frozen=True)
 class ParticleInit:
     seed: int = 0
-    velocity: str = "rest"  # "rest" | "zero-slip"
-    size: str = "const"  # "const" | "stokes"
+    velocity: str = "rest"  # "rest" | "zero-slip" | "restart"
+    size: str = "const"  # "const" | "stokes" | "restart"
     diam_m: float = 0.0
     st_min: float = 1.0
     st_max: float = 1.0
     eps: float = 0.5
+    # Particles.ParticleRestart (src/particles/config.jl:150-154): id, position,
+    # velocity and size all read from restart_file, profile restart_profile (1-based)
+    restart_file: Optional[str] = None
+    restart_profile: int = 1
+
+
+@dataclass(frozen=True)
+class ParticleStatConfig:  # Particles.Stat, RDF, Spectrum (src/particles/config.jl:66-108)
+    contact_tolerance: float = 0.01
+    bin_size: int = 16
+    min_radius: float = 1.25e-4
+    search_cell_size: int = 2
+    max_wavenumber: int = 768
+    nsamples: int = 64
 
 
 @dataclass(frozen=True)
@@ -153,6 +179,7 @@ class ParticleConfig:  # src/particles/config.jl:272-282
     grid_ratio: int
     drag: str
     init: ParticleInit = field(default_factory=ParticleInit)
+    stat: ParticleStatConfig = field(default_factory=ParticleStatConfig)
 
     @property
     def drag_id(self):
@@ -171,6 +198,7 @@ class Config:  # src/config.jl:173-194
     backend: str = "mi355x"
     dtype: str = "f64"
     outdir: str = "output"
+    resume: bool = False  # set by lcsio.init only (src/LCSIO/config.jl:44-77)
 
 
 def _forcing(d):
@@ -197,13 +225,53 @@ def _flow_init(d):
     kind, p = _kind(d, "flow.init")
     if kind == "spectrum":
         return SpectrumInit(int(p.get("seed", 0)), float(p.get("k0", 4.0)), float(p.get("uprime", 1.0)))
-    raise ValueError(f"flow.init kind {kind!r} is not available on the mi355x backend (use 'spectrum')")
+    if kind == "restart":
+        return RestartInit(str(p["file"]))
+    raise ValueError(f"flow.init kind {kind!r} is not available on the mi355x backend (use 'spectrum' or 'restart')")
+
+
+def _particle_restart(init):
+    """The ParticleRestart variants of one profile's init, or None.  This
+    backend reads id, position, velocity and size from one file together."""
+    kinds = {k: _kind(init[k], k) for k in ("id", "position", "velocity", "size") if k in init}
+    rs = {k: p for k, (kind, p) in kinds.items() if kind == "restart"}
+    if not rs:
+        return None
+    if len(rs) != 4:
+        raise ValueError("particle init: id, position, velocity and size must all be 'restart' or none of them")
+    srcs = {(str(p["file"]), int(p["iprofile"])) for p in rs.values()}
+    if len(srcs) != 1:
+        raise ValueError("particle init: every 'restart' variant must name the same file and iprofile")
+    return srcs.pop()
+
+
+def _particle_stat(d):
+    if not d:
+        return ParticleStatConfig()
+    r = d.get("rdf") or {}
+    sp = d.get("spectrum") or {}
+    dflt = ParticleStatConfig()
+    return ParticleStatConfig(float(r.get("contact_tolerance", dflt.contact_tolerance)),
+                              int(r.get("bin_size", dflt.bin_size)), float(r.get("min_radius", dflt.min_radius)),
+                              int(r.get("search_cell_size", dflt.search_cell_size)),
+                              int(sp.get("max_wavenumber", dflt.max_wavenumber)),
+                              int(sp.get("nsamples", dflt.nsamples)))
 
 
 def _particle(d, L0_m):
     pop = d["population"]
     prm = d["params"]
     init = d.get("init") or {}
+    restart = _particle_restart(init)
+    if restart is not None:
+        drag, _ = _kind(d.get("drag", {"kind": "linear"}), "drag")
+        return ParticleConfig(Population(int(pop["valid"]), float(pop.get("invalid_ratio", 0.0)),
+                                         float(pop.get("exchange_ratio", 0.0))),
+                              ParticleParams(float(prm["gravity"]), float(prm["density_ratio"])),
+                              int(d["cell"]["grid_ratio"]), drag,
+                              ParticleInit(velocity="restart", size="restart", restart_file=restart[0],
+                                           restart_profile=restart[1]),
+                              _particle_stat(d.get("stat")))
     seed = 0
     pos_kind, pos_p = _kind(init.get("position", {"kind": "random"}), "position")
     if pos_kind != "random":
@@ -218,7 +286,7 @@ def _particle(d, L0_m):
     return ParticleConfig(Population(int(pop["valid"]), float(pop.get("invalid_ratio", 0.0)),
                                      float(pop.get("exchange_ratio", 0.0))),
                           ParticleParams(float(prm["gravity"]), float(prm["density_ratio"])),
-                          int(d["cell"]["grid_ratio"]), drag, pinit)
+                          int(d["cell"]["grid_ratio"]), drag, pinit, _particle_stat(d.get("stat")))
 
 
 def from_dict(x):
@@ -230,7 +298,8 @@ def from_dict(x):
     g = x["grid"]
     f = x["flow"]
     fp = FlowParams(float(f["params"]["U0_m_s"]), float(f["params"]["L0_m"]), float(f["params"]["NU_m2_s"]))
-    flow = FlowConfig(fp, _flow_init(f["init"]), _couple(f["couple"]), _forcing(f["forcing"]))
+    fstat = FlowStatConfig(bool((f.get("stat") or {}).get("integral_length", False)))
+    flow = FlowConfig(fp, _flow_init(f["init"]), _couple(f["couple"]), _forcing(f["forcing"]), fstat)
     parts = ()
     if mode == "flow-particle" and x.get("particles"):
         parts = tuple(_particle(p, fp.L0_m) for p in x["particles"])
@@ -241,10 +310,78 @@ def from_dict(x):
                                           int(s.get("interval_restart", 0)), int(s.get("interval_gc", 0)),
                                           int(s.get("num_restart_files", 1))),
                   flow=flow, particles=parts, mode=mode, backend=backend, dtype=str(x.get("dtype", "f64")),
-                  outdir=str(x.get("outdir", "output")))
+                  outdir=str(x.get("outdir", "output")), resume=bool(x.get("resume", False)))
 
 
-def readconfig(path):
-    """LCSIO.readconfig (src/LCSIO/config.jl:6-31) without the patch expression."""
+def _forcing_dict(f):
+    if isinstance(f, NoForcing):
+        return {"kind": "no"}
+    if isinstance(f, ConstantPowerLF):
+        return {"kind": f.kind, "params": {"power": f.power}}
+    return {"kind": f.kind, "params": {"energy": f.energy, "kmin": f.kmin, "kmax": f.kmax,
+                                       "coarse_size": f.coarse_size}}
+
+
+def _particle_dict(p: ParticleConfig):
+    i = p.init
+    if i.restart_file is not None:
+        r = {"kind": "restart", "params": {"file": i.restart_file, "iprofile": i.restart_profile}}
+        init = {"id": r, "position": r, "velocity": r, "size": r}
+    else:
+        if i.size == "stokes":
+            size = {"kind": "stokes", "params": {"st_min": i.st_min, "st_max": i.st_max, "eps": i.eps}}
+        else:
+            size = {"kind": "const", "params": {"diam_m": i.diam_m}}
+        init = {"id": {"kind": "generate"}, "position": {"kind": "random", "params": {"seed": i.seed}},
+                "velocity": {"kind": i.velocity}, "size": size}
+    s = p.stat
+    return {"population": {"valid": p.population.valid, "invalid_ratio": p.population.invalid_ratio,
+                           "exchange_ratio": p.population.exchange_ratio},
+            "params": {"gravity": p.params.gravity, "density_ratio": p.params.density_ratio},
+            "cell": {"grid_ratio": p.grid_ratio},
+            "stat": {"rdf": {"contact_tolerance": s.contact_tolerance, "bin_size": s.bin_size,
+                             "min_radius": s.min_radius, "search_cell_size": s.search_cell_size},
+                     "spectrum": {"max_wavenumber": s.max_wavenumber, "nsamples": s.nsamples}},
+            "drag": {"kind": p.drag}, "init": init}
+
+
+def to_dict(c: Config):
+    """PolySerde.serialize(LCS.Config, c) for the mi355x backend: a mapping
+    that from_dict reads back to an equal Config.  ``resume`` is left out, as
+    LCSIO.init leaves it out of the saved config (src/LCSIO/config.jl:66-70)."""
+    f = c.flow
+    if isinstance(f.init, RestartInit):
+        finit = {"kind": "restart", "params": {"file": f.init.file}}
+    else:
+        finit = {"kind": "spectrum", "params": {"seed": f.init.seed, "k0": f.init.k0, "uprime": f.init.uprime}}
+    h = f.couple
+    d = {"outdir": c.outdir, "mode": {"kind": c.mode}, "backend": {"kind": c.backend}, "dtype": c.dtype,
+         "grid": {"dims": list(c.grid.dims), "halo_size": c.grid.halo_size},
+         "timestep": {"cfl": c.timestep.cfl, "dtmax": c.timestep.dtmax},
+         "simulate": {"last_step": c.simulate.last_step, "interval_stat": c.simulate.interval_stat,
+                      "interval_restart": c.simulate.interval_restart, "interval_gc": c.simulate.interval_gc,
+                      "num_restart_files": c.simulate.num_restart_files},
+         "flow": {"params": {"U0_m_s": f.params.U0_m_s, "L0_m": f.params.L0_m, "NU_m2_s": f.params.NU_m2_s},
+                  "init": finit,
+                  "couple": {"kind": "hsmac", "params": {"time_blocking": h.time_blocking, "overlap": h.overlap,
+                                                         "omega": h.omega, "epsp": h.epsp, "itrmin": h.itrmin,
+                                                         "itrmax": h.itrmax, "coloring": {"kind": h.coloring}}},
+                  "forcing": _forcing_dict(f.forcing),
+                  "stat": {"integral_length": f.stat.integral_length}}}
+    if c.particles:
+        d["particles"] = [_particle_dict(p) for p in c.particles]
+    return d
+
+
+def readconfig(path, patch=None):
+    """LCSIO.readconfig(file; patch) (src/LCSIO/config.jl:6-31): a YAML file
+    that sets ``resume`` is refused, since resuming is chosen by the
+    LCS_RESUME environment variable."""
     with open(path) as fh:
-        return from_dict(yaml.safe_load(fh))
+        d = yaml.safe_load(fh)
+    if "resume" in d:
+        raise ValueError("resume must not be set in configuration, resume mode is controlled by LCS_RESUME "
+                         "environment variable")
+    d["resume"] = False
+    c = from_dict(d)
+    return patch(c) if patch is not None else c

@@ -512,6 +512,12 @@ class Server:
             pass
 
 
+def set_state(box, step, t):
+    """state.step = step; state.t = t (LCSIO.load_resume_state!,
+    src/LCSIO/load.jl:1-21)."""
+    check(box.L.lcs_set_state(box.handle, int(step), float(t)))
+
+
 def read_log(box):
     log = _lib.StepLog()
     check(box.L.lcs_step_log_read(box.handle, C.byref(log)))

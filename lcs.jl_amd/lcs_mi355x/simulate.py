@@ -1,14 +1,16 @@
 """``LCS.simulate(config)`` for the mi355x backend (src/simulate.jl:13-90).
 
-Builds the box (``LCS.Buffer``), runs the builder-defined initialisers, then
-loops ``timestep!`` + ``evolve!`` up to ``simulate.last_step`` and prints the
-per-step log the way ``printstate``/``printlog`` do (src/stat.jl:48-85).
-Statistics and restart IO (src/LCSIO) are outside this backend's scope.
+LCSIO.init (output directory, resume policy), the box (``LCS.Buffer``), the
+initialisers (``LCS.init!``: builder-defined spectrum and Stokes-number
+initial conditions, or a restart file), then ``timestep!`` + ``evolve!`` from
+state.step through ``simulate.last_step`` with the per-step log printed the
+way ``printstate``/``printlog`` do (src/stat.jl:48-85), statistics saved every
+``interval_stat`` steps and restart files every ``interval_restart`` steps.
 """
 import sys
 
-from . import core
-from .config import Config, SpectrumInit, readconfig
+from . import core, lcsio
+from .config import Config, RestartInit, SpectrumInit, readconfig
 
 
 def build(config: Config, device=0):
@@ -16,14 +18,22 @@ def build(config: Config, device=0):
     g = config.grid
     box = core.Box(g.dims, g.halo_size, config.flow.params, config.dtype, device)
     init = config.flow.init
-    if not isinstance(init, SpectrumInit):
-        raise ValueError("the mi355x backend initialises flows with kind 'spectrum'")
-    box.init_spectrum(init.seed, init.k0, init.uprime)
-    core.fill_ghost(box, ("U", "V", "W", "P"), g.halo_size, True)
+    if isinstance(init, RestartInit):
+        lcsio.load_flow_fields(init.file, box)  # src/flows/inits/init.jl:16-19
+    elif isinstance(init, SpectrumInit):
+        box.init_spectrum(init.seed, init.k0, init.uprime)
+        core.fill_ghost(box, ("U", "V", "W", "P"), g.halo_size, True)
+    else:
+        raise ValueError("the mi355x backend initialises flows with kind 'spectrum' or 'restart'")
     for prof in config.particles:
-        i = box.add_particles(prof.population.valid, prof.population.ntotal, prof.params.gravity,
-                              prof.params.density_ratio, prof.drag, prof.grid_ratio)
+        box.add_particles(prof.population.valid, prof.population.ntotal, prof.params.gravity,
+                          prof.params.density_ratio, prof.drag, prof.grid_ratio)
+    for i, prof in enumerate(config.particles):
         pi = prof.init
+        if pi.restart_file is not None:
+            # ParticleRestart(file, iprofile) fills profile iprofile (src/particles/init.jl:22-31,61-70)
+            lcsio.load_particles(pi.restart_file, box, pi.restart_profile - 1)
+            continue
         if pi.size == "stokes":
             st_min, st_max, eps = pi.st_min, pi.st_max, pi.eps
         elif pi.size == "const":
@@ -40,21 +50,57 @@ def build(config: Config, device=0):
     return box
 
 
+def stat(box, config: Config):
+    """LCS.stat (src/stat.jl:92-105): flow statistics, then per profile the
+    particle statistics with that flow's tau_eta.  Returns (flow, particles)."""
+    flow = core.flow_stat(box, config.flow.stat.integral_length)
+    parts = []
+    for i, p in enumerate(config.particles):
+        s = p.stat
+        parts.append(core.particle_stat(box, i, flow["tau_eta"], s.contact_tolerance, s.bin_size, s.min_radius,
+                                        s.search_cell_size, s.max_wavenumber, s.nsamples))
+    return flow, parts
+
+
+def _print_log(log):
+    f = log["flow"]
+    print(f"step {log['step']}  t {log['t']:.6g}  dt {log['dt']:.6g}  "
+          f"couple1 itr {f['couple1']['itr']} div {f['couple1']['div']:.4g}  "
+          f"couple2 itr {f['couple2']['itr']} div {f['couple2']['div']:.4g}  p {f['forcing']['p']:.4g}",
+          file=sys.stdout, flush=True)
+
+
 def simulate(config, callbacks=(), quiet=False, device=0):
+    """The loop of src/simulate.jl:54-88: steps state.step..last_step (so a
+    fresh run takes last_step + 1 steps, as the reference's does)."""
     if isinstance(config, str):
         config = readconfig(config)
+    config = lcsio.init(config)
     box = build(config, device)
+    if config.resume:
+        step, t = lcsio.load_resume_state(config)  # LCS.init_state! (src/init.jl:11-15)
+        core.set_state(box, step, t)
     scfg = core.step_config(config)
+    sim = config.simulate
     logs = []
-    for _ in range(config.simulate.last_step):
+    first = core.read_log(box)["step"]
+    for _ in range(first, sim.last_step + 1):
         log = core.step(box, scfg)
         logs.append(log)
         if not quiet:
-            f = log["flow"]
-            print(f"step {log['step']}  t {log['t']:.6g}  dt {log['dt']:.6g}  "
-                  f"couple1 itr {f['couple1']['itr']} div {f['couple1']['div']:.4g}  "
-                  f"couple2 itr {f['couple2']['itr']} div {f['couple2']['div']:.4g}  p {f['forcing']['p']:.4g}",
-                  file=sys.stdout, flush=True)
+            _print_log(log)
+        s = log["step"]
+        if lcsio.oninterval(s, sim.interval_stat):
+            flow, parts = stat(box, config)
+            state = lcsio.state_tree(box, log)
+            if not quiet:
+                print(f"stat {s}  k {flow['k']:.6g}  eps {flow['eps']:.6g}  Re_lambda {flow['Re_lambda']:.6g}",
+                      file=sys.stdout, flush=True)
+            lcsio.savestat(config, state, lcsio.stat_tree(state, flow, parts),
+                           lcsio.log_tree(log, config.flow.forcing.kind, [p["nvalid"] for p in box.profiles]))
+        if lcsio.oninterval(s, sim.interval_restart):
+            lcsio.saverestart(box, config, s)
+            lcsio.cleanrestart(s, config)
         for cb in callbacks:
             cb(box, log)
     return box, logs
