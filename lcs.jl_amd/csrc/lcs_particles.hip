@@ -446,28 +446,36 @@ __global__ void __launch_bounds__(kSortBlock) k_rs_hist(const unsigned *keys, lo
     hist[(long)threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// exclusive scan of m unsigned counts in one block of 1024 threads
-__global__ void __launch_bounds__(1024) k_rs_scan(unsigned *hist, long m)
+// Exclusive scan of each digit's row of per-block counts, hist[d * nblk + blk]:
+// one workgroup per digit, coalesced chunks of 256 with a wave scan each.
+// The row totals go to tot[d]; k_rs_scatter adds the digits' exclusive
+// prefix of those totals.
+__global__ void __launch_bounds__(256) k_rs_scan_rows(unsigned *hist, int nblk, unsigned *tot)
 {
-    __shared__ unsigned tot[1024];
-    long seg = (m + 1023) / 1024;
-    long lo = threadIdx.x * seg, hi = std::min(m, lo + seg);
-    unsigned s = 0;
-    for (long q = lo; q < hi; q++) s += hist[q];
-    tot[threadIdx.x] = s;
+    __shared__ unsigned wsum[4];
+    __shared__ unsigned carry;
+    unsigned *row = hist + (long)blockIdx.x * nblk;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry = 0u;
     __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        unsigned v = threadIdx.x >= (unsigned)o ? tot[threadIdx.x - o] : 0u;
+    for (int base = 0; base < nblk; base += 256) {
+        const int q = base + (int)threadIdx.x;
+        const unsigned v = q < nblk ? row[q] : 0u;
+        unsigned x = v;  // inclusive scan within the wave
+        for (int o = 1; o < 64; o <<= 1) {
+            unsigned y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
         __syncthreads();
-        tot[threadIdx.x] += v;
+        unsigned pre = carry;
+        for (int k = 0; k < w; k++) pre += wsum[k];
+        if (q < nblk) row[q] = pre + x - v;
+        __syncthreads();  // every thread has read carry and wsum
+        if (threadIdx.x == 255) carry = pre + x;
         __syncthreads();
     }
-    unsigned run = tot[threadIdx.x] - s;
-    for (long q = lo; q < hi; q++) {
-        unsigned v = hist[q];
-        hist[q] = run;
-        run += v;
-    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
 }
 
 // Stable scatter: items keep their input order within a digit.  Ranks inside
@@ -475,12 +483,26 @@ __global__ void __launch_bounds__(1024) k_rs_scan(unsigned *hist, long m)
 // per-wave digit counts in LDS.
 __global__ void __launch_bounds__(kSortBlock) k_rs_scatter(const unsigned *kin, const unsigned *vin, unsigned *kout,
                                                          unsigned *vout, long n, int shift, long chunk,
-                                                         const unsigned *hist, int nblk)
+                                                         const unsigned *hist, int nblk, const unsigned *tot)
 {
     __shared__ unsigned run[256];
     __shared__ unsigned wc[4][256];
+    __shared__ unsigned dsum[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    run[threadIdx.x] = hist[(long)threadIdx.x * nblk + blockIdx.x];
+    {
+        // start of digit threadIdx.x: exclusive prefix of the digit totals
+        const unsigned tv = tot[threadIdx.x];
+        unsigned x = tv;
+        for (int o = 1; o < 64; o <<= 1) {
+            unsigned y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) dsum[w] = x;
+        __syncthreads();
+        unsigned pre = x - tv;
+        for (int k = 0; k < w; k++) pre += dsum[k];
+        run[threadIdx.x] = hist[(long)threadIdx.x * nblk + blockIdx.x] + pre;
+    }
     long lo = (long)blockIdx.x * chunk, hi = std::min(n, lo + chunk);
     const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     for (long base = lo; base < hi; base += kSortBlock) {
@@ -562,7 +584,7 @@ static int makeindex_t(Box &b, ParticleProfile &pp, int stage)
     long chunk = (n + nblk - 1) / nblk;
     chunk = ((chunk + kSortBlock - 1) / kSortBlock) * kSortBlock;
     if (chunk == 0) chunk = kSortBlock;
-    size_t need = sizeof(unsigned) * (4 * (size_t)std::max(1L, n) + 256 * (size_t)nblk);
+    size_t need = sizeof(unsigned) * (4 * (size_t)std::max(1L, n) + 256 * (size_t)nblk + 256);
     if (need > pp.sort_tmp_bytes) {
         if (pp.sort_tmp) LCS_HIP(hipFree(pp.sort_tmp));
         pp.sort_tmp = nullptr;
@@ -571,7 +593,7 @@ static int makeindex_t(Box &b, ParticleProfile &pp, int stage)
         pp.sort_tmp_bytes = need;
     }
     unsigned *kA = (unsigned *)pp.sort_tmp, *vA = kA + std::max(1L, n), *kB = vA + std::max(1L, n),
-             *vB = kB + std::max(1L, n), *hist = vB + std::max(1L, n);
+             *vB = kB + std::max(1L, n), *hist = vB + std::max(1L, n), *tot = hist + 256L * nblk;
     int x = stage == 1 ? LCS_PROP_X : LCS_PROP_X2;
     double dxc[3] = {spacing(b.g.nx) * pp.desc.grid_ratio, spacing(b.g.ny) * pp.desc.grid_ratio,
                      spacing(b.g.nz) * pp.desc.grid_ratio};
@@ -584,9 +606,9 @@ static int makeindex_t(Box &b, ParticleProfile &pp, int stage)
         while (bits < 32 && ((unsigned long long)ncell >> bits) != 0) bits++;
         for (int shift = 0; shift < bits; shift += 8) {
             hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(kSortBlock), 0, b.stream, kA, n, shift, chunk, hist, nblk);
-            hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, b.stream, hist, 256L * nblk);
+            hipLaunchKernelGGL(k_rs_scan_rows, dim3(256), dim3(256), 0, b.stream, hist, nblk, tot);
             hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(kSortBlock), 0, b.stream, kA, vA, kB, vB, n, shift,
-                               chunk, hist, nblk);
+                               chunk, hist, nblk, tot);
             LCS_HIP(hipGetLastError());
             std::swap(kA, kB);
             std::swap(vA, vB);
@@ -709,12 +731,12 @@ static int sort_pairs(Box &b, ParticleProfile &pp, long n, int bits, unsigned **
     chunk = ((chunk + kSortBlock - 1) / kSortBlock) * kSortBlock;
     if (chunk == 0) chunk = kSortBlock;
     unsigned *kA = *keys, *vA = *vals, *kB = vA + std::max(1L, n), *vB = kB + std::max(1L, n),
-             *hist = vB + std::max(1L, n);
+             *hist = vB + std::max(1L, n), *tot = hist + 256L * nblk;
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(kSortBlock), 0, b.stream, kA, n, shift, chunk, hist, nblk);
-        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, b.stream, hist, 256L * nblk);
+        hipLaunchKernelGGL(k_rs_scan_rows, dim3(256), dim3(256), 0, b.stream, hist, nblk, tot);
         hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(kSortBlock), 0, b.stream, kA, vA, kB, vB, n, shift, chunk,
-                           hist, nblk);
+                           hist, nblk, tot);
         LCS_HIP(hipGetLastError());
         std::swap(kA, kB);
         std::swap(vA, vB);
@@ -736,7 +758,7 @@ static int localize_t(Box &b, ParticleProfile &pp)
         b.bytes += 2 * sizeof(unsigned) * (size_t)nt;
     }
     int nblk = (int)std::min<long>(1024, std::max(1L, (n + 4 * kSortBlock - 1) / (4 * kSortBlock)));
-    size_t need = sizeof(unsigned) * (4 * (size_t)n + 256 * (size_t)nblk);
+    size_t need = sizeof(unsigned) * (4 * (size_t)n + 256 * (size_t)nblk + 256);
     if (need > pp.sort_tmp_bytes) {
         if (pp.sort_tmp) LCS_HIP(hipFree(pp.sort_tmp));
         pp.sort_tmp = nullptr;
