@@ -323,7 +323,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="use the size rule for tile shapes")
-    ap.add_argument("--localize-interval", type=int, default=10,
+    ap.add_argument("--localize-interval", type=int, default=30,
                     help="reorder particle storage by position every N steps (0: never; changes no result)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-sized device buffer once (a plain copy kernel of known size "
