@@ -122,17 +122,12 @@ static std::vector<E> xcd_order(const std::vector<E> &v)
 static int blocks_of(long n, int per = 256) { return (int)((n + per - 1) / per); }
 
 // Sweep tile shapes of a batch, one launch per shape (results do not depend
-// on them): the width that wastes fewer lanes over the array row, 8 rows.  In
-// a mixed ensemble, 16-row tiles for the 96^3 and 128^3 boxes measured 5-9 %
-// slower, and the boxes' own tuned single-box shapes 9 % slower.
+// on them).  Every box takes 32 x 8 tiles, so a sweep is one launch: in a
+// mixed 96-box ensemble this beat one launch of 64 x 8 tiles plus one of
+// 32 x 8 tiles chosen by lane waste per box (23.5 vs 24.4 ms a step), and
+// 16-row tiles were slower (25.2 ms).
 static const int kBatchSweepShapes[4][2] = {{64, 8}, {32, 8}, {64, 16}, {32, 16}};
-static int sweep_shape(const Box &b)
-{
-    const Geom &g = b.g;
-    const int X = g.nx + 2 * g.h;
-    const double e64 = (double)X / (64.0 * ((X + 63) / 64)), e32 = (double)X / (32.0 * ((X + 31) / 32));
-    return e32 > e64 + 0.05 ? 1 : 0;
-}
+static int sweep_shape(const Box &) { return 1; }
 
 template <typename T>
 static void fill_coefs(const Box &b, const lcs_hsmac_params &hs, BoxRec &r)
@@ -233,21 +228,25 @@ static int rebuild(Batch &B)
             fill_coefs<double>(b, B.hs, r);
         else
             fill_coefs<float>(b, B.hs, r);
-        // fused RHS tiles (kBatchRhsTX x batch_rhs_ty), z-chunks of about 64 planes
+        // fused RHS tiles (kBatchRhsTX x batch_rhs_ty), z-chunks of about 24 planes
+        // (each re-reads six halo planes; 64-plane chunks were 0.2 ms a step slower)
         const int rtx = kBatchRhsTX, rty = batch_rhs_ty(b.esize);
         const int rgx = (g.nx + rtx - 1) / rtx, rgy = (g.ny + rty - 1) / rty;
-        const int rgz = std::max(1, g.nz / 64);
+        const int rgz = std::max(1, g.nz / 24);
         r.rhs_zc = (g.nz + rgz - 1) / rgz;
         for (int z = 0; z * r.rhs_zc < g.nz; z++)
             for (int y = 0; y < rgy; y++)
                 for (int x = 0; x < rgx; x++) rtiles.push_back({i, x, y, z});
-        // fused HSMAC sweep tiles: TX x 8 over the whole array, TX chosen per size
+        // fused HSMAC sweep tiles over the whole array, in z-chunks of about 28
+        // planes: short chunks balance the launch across boxes of different
+        // sizes despite their five re-read halo planes (64-plane chunks: 25.1 ms
+        // a step, 12-plane chunks: 24.1 ms)
         const int zplanes = g.nz + 2 * g.h;
         const int shape = sweep_shape(b);
         r.hs_tx = kBatchSweepShapes[shape][0];
         r.hs_ty = kBatchSweepShapes[shape][1];
         const int hgx = (g.nx + 2 * g.h + r.hs_tx - 1) / r.hs_tx, hgy = (g.ny + 2 * g.h + r.hs_ty - 1) / r.hs_ty;
-        const int hgz = std::max(1, zplanes / 64);
+        const int hgz = std::max(1, zplanes / 28);
         r.hs_zc = (zplanes + hgz - 1) / hgz;
         int nz_used = (zplanes + r.hs_zc - 1) / r.hs_zc;
         r.hs_tiles = hgx * hgy * nz_used;
