@@ -106,18 +106,30 @@ __device__ inline void xcd_tile(int &bx, int &by, int &bz)
 }
 
 // Visits every interior cell once.  Each block walks one contiguous run of
-// rows (j, k) with x over its threads, so the index arithmetic per cell is one
-// add and the y/z neighbours a stencil reads were just read by the same block
-// or by a block on the same XCD (runs are handed out XCD by XCD).
+// rows (j, k) with x over its threads, so the y/z neighbours a stencil reads
+// were just read by the same block or by a block on the same XCD (runs are
+// handed out XCD by XCD).  Rows shorter than half the block go several at a
+// time (thread t takes row t / nx of each group, cell t % nx), so a 32-wide
+// row does not leave seven eighths of a 256-thread block idle.
 template <typename F>
 __device__ inline void for_rows(const Geom &g, int b, int nb, F &&f)
 {
     const int rows = g.ny * g.nz;
     const int per = (rows + nb - 1) / nb, r0 = b * per, r1 = min(rows, r0 + per);
+    const int nt = (int)blockDim.x, R = g.nx <= nt / 2 ? nt / g.nx : 1;
+    if (R > 1) {
+        const int dr = (int)threadIdx.x / g.nx, i = 1 + (int)threadIdx.x % g.nx;
+        if (dr >= R) return;
+        for (int r = r0 + dr; r < r1; r += R) {
+            const int j = r % g.ny + 1, k = r / g.ny + 1;
+            f(i, j, k, g.at(i, j, k));
+        }
+        return;
+    }
     int j = r0 % g.ny + 1, k = r0 / g.ny + 1;
     for (int r = r0; r < r1; r++) {
         const long base = g.at(0, j, k);
-        for (int i = 1 + (int)threadIdx.x; i <= g.nx; i += blockDim.x) f(i, j, k, base + i);
+        for (int i = 1 + (int)threadIdx.x; i <= g.nx; i += nt) f(i, j, k, base + i);
         if (++j > g.ny) {
             j = 1;
             k++;
