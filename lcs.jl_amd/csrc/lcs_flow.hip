@@ -1121,16 +1121,45 @@ __device__ inline int xcd_block()
     return x < rem ? x * (per_x + 1) + i0 : rem * (per_x + 1) + (x - rem) * per_x + i0;
 }
 
+// Columns of a 64 x 4 patch march up a chunk of zc planes, each keeping its
+// last four W values in registers, so the z-stencil reads W once instead of
+// four times (a row-walking block cannot keep four planes in L2 at 1024^2
+// cells a plane); U and V neighbours come from the patch and its y
+// neighbours, which run on the same XCD.  The maxima do not depend on the
+// order, so dt is the row-walking kernel's to the bit.
+constexpr int kCflPX = 64, kCflPY = 4;
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_cfl(Geom g, const T *__restrict__ U, const T *__restrict__ V,
-                                            const T *__restrict__ W, double *part)
+                                            const T *__restrict__ W, double *part, int zc)
 {
     __shared__ double lds[16];
-    double m[3];
-    cfl_block(g, U, V, W, xcd_block(), gridDim.x, m);
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int x = 1 + bx * kCflPX + (int)threadIdx.x % kCflPX, y = 1 + by * kCflPY + (int)threadIdx.x / kCflPX;
+    const int z0 = 1 + bz * zc, z1 = min(g.nz, z0 + zc - 1);
+    double m[3] = {0.0, 0.0, 0.0};
+    if (x <= g.nx && y <= g.ny && z0 <= z1) {
+        const long sy = g.sx, sz = g.sx * g.sy;
+        long p = g.at(x, y, z0);
+        T w0 = W[p - 2 * sz], w1 = W[p - sz], w2 = W[p];
+        for (int z = z0; z <= z1; z++, p += sz) {
+            const T w3 = W[p + sz];
+            const T uc = interp_o4(U[p - 2], U[p - 1], U[p], U[p + 1]);
+            const T vc = interp_o4(V[p - 2 * sy], V[p - sy], V[p], V[p + sy]);
+            const T wc = interp_o4(w0, w1, w2, w3);
+            m[0] = fmax(m[0], fabs((double)uc));
+            m[1] = fmax(m[1], fabs((double)vc));
+            m[2] = fmax(m[2], fabs((double)wc));
+            w0 = w1;
+            w1 = w2;
+            w2 = w3;
+        }
+    }
+    const int blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     for (int a = 0; a < 3; a++) {
         double s = block_max(m[a], lds);
-        if (threadIdx.x == 0) part[(long)a * kMaxPartBlocks + blockIdx.x] = s;
+        if (threadIdx.x == 0) part[(long)a * kMaxPartBlocks + blk] = s;
     }
 }
 
@@ -1169,14 +1198,20 @@ int launch_cfl(Box &b, double cfl, double dtmax)
     void *arr[3] = {b.f[0], b.f[1], b.f[2]};
     int rc = launch_ghost_fill(b, arr, 3, b.g.h, 0, nullptr);
     if (rc) return rc;
-    int blocks = grid_rows(b.g);
+    // at most 4096 workgroups; z-chunks of at least 8 planes
+    const int gx = (b.g.nx + kCflPX - 1) / kCflPX, gy = (b.g.ny + kCflPY - 1) / kCflPY;
+    int gz = std::max(1, std::min((b.g.nz + 7) / 8, 4096 / (gx * gy)));
+    const int zc = (b.g.nz + gz - 1) / gz;
+    gz = (b.g.nz + zc - 1) / zc;
+    const int blocks = gx * gy * gz;
+    if (blocks > kMaxPartBlocks) return fail(LCS_ERR_INVALID, "grid too large for the CFL reduction");
     ProfScope ps(b, "cfl", 3.0 * b.esize * b.g.cells());
     if (b.esize == 8)
-        hipLaunchKernelGGL(k_cfl<double>, dim3(blocks), dim3(256), 0, b.stream, b.g, (const double *)b.f[0],
-                           (const double *)b.f[1], (const double *)b.f[2], b.part);
+        hipLaunchKernelGGL(k_cfl<double>, dim3(gx, gy, gz), dim3(256), 0, b.stream, b.g, (const double *)b.f[0],
+                           (const double *)b.f[1], (const double *)b.f[2], b.part, zc);
     else
-        hipLaunchKernelGGL(k_cfl<float>, dim3(blocks), dim3(256), 0, b.stream, b.g, (const float *)b.f[0],
-                           (const float *)b.f[1], (const float *)b.f[2], b.part);
+        hipLaunchKernelGGL(k_cfl<float>, dim3(gx, gy, gz), dim3(256), 0, b.stream, b.g, (const float *)b.f[0],
+                           (const float *)b.f[1], (const float *)b.f[2], b.part, zc);
     LCS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(256), 0, b.stream, b.part, blocks, cfl, dtmax,
                        spacing(b.g.nx), spacing(b.g.ny), spacing(b.g.nz), b.Re(), b.S);
