@@ -897,11 +897,16 @@ int lcs_step(lcs_box *box, const lcs_step_config *cfg, lcs_step_log *log)
     for (int q = 0; q < np; q++)
         if ((rc = launch_p_stage(b, b.profiles[q], 2, dt2))) return rc;
     if ((rc = launch_rhs(b, 2, true, &b.f[LCS_FIELD_U], &b.f[LCS_FIELD_U], dt2))) return rc;
+    const bool lf_defer = cfg->forcing.kind == LCS_FORCING_LINEAR_POWER && cfg->hsmac.itrmax >= 1;
     switch (cfg->forcing.kind) {
     case LCS_FORCING_NONE:
         break;
     case LCS_FORCING_LINEAR_POWER:
-        if ((rc = launch_force_linear(b, cfg->forcing.power, dt2))) return rc;
+        // forcing! then couple!: the first HSMAC sweep applies the forcing as
+        // it loads the velocity, saving a read and write of U, V, W
+        if ((rc = lf_defer ? launch_force_linear_coef(b, cfg->forcing.power)
+                           : launch_force_linear(b, cfg->forcing.power, dt2)))
+            return rc;
         break;
     case LCS_FORCING_RC_ENERGY:
         if ((rc = launch_force_rcf(b, cfg->forcing.energy, cfg->forcing.kmin, cfg->forcing.kmax,
@@ -911,7 +916,7 @@ int lcs_step(lcs_box *box, const lcs_step_config *cfg, lcs_step_log *log)
     default:
         return fail(LCS_ERR_INVALID, "unknown forcing kind");
     }
-    if ((rc = launch_hsmac(b, &b.f[LCS_FIELD_U], dt2, cfg->hsmac, 1))) return rc;
+    if ((rc = launch_hsmac(b, &b.f[LCS_FIELD_U], dt2, cfg->hsmac, 1, lf_defer))) return rc;
     hipLaunchKernelGGL(k_advance_state, dim3(1), dim3(1), 0, b.stream, b.S);
     LCS_HIP(hipGetLastError());
     if (log) {

@@ -580,10 +580,11 @@ __global__ void __launch_bounds__(256) k_hsmac_pass(Geom g, BufSets<T> bs, Box3 
 // Fused red+black sweep over R, current -> other buffer set (the phase
 // functions and their schedule are in lcs_hsmac_tile.h).  Returns the
 // workgroup's div^2 sum (valid in thread 0).
-template <typename T, int COLORING, int TY, int TX>
-__device__ inline double hsmac_sweep_block(const HsArgs<T> &a, int bx, int by, int bz, T *lds, double *red_lds)
+template <typename T, int COLORING, int TY, int TX, bool FORCE = false>
+__device__ inline double hsmac_sweep_block(const HsArgs<T> &a, int bx, int by, int bz, T *lds, double *red_lds,
+                                           double *power = nullptr)
 {
-    using Tile = HsTile<T, COLORING, TY, TX>;
+    using Tile = HsTile<T, COLORING, TY, TX, FORCE>;
     typename Tile::Thread th;
     Tile::init(a, bx, by, bz, threadIdx.x, th);
     Tile::prologue(a, lds, th);
@@ -600,6 +601,7 @@ __device__ inline double hsmac_sweep_block(const HsArgs<T> &a, int bx, int by, i
         Tile::step_b(a, lds, q + 1, th, 1);
         __syncthreads();
     }
+    if (FORCE) *power = block_sum(th.pacc, red_lds);
     return block_sum(th.acc, red_lds);
 }
 
@@ -617,9 +619,13 @@ __device__ inline void hs_bind(HsArgs<T> &a, const BufSets<T> &bs, bool par)
     a.Pd = par ? bs.a[3] : bs.b[3];
 }
 
-template <typename T, int COLORING, int TY, int TX>
+// ppart != null: the linear forcing of S->lf is applied to U, V, W as they
+// are loaded, and each workgroup's forcing power goes to ppart (the last
+// partial slot of the box, which no other pass of a step uses).
+constexpr long kPowerSlot = (long)(kMaxSweepPasses + 3) * kMaxPartBlocks;
+template <typename T, int COLORING, int TY, int TX, bool FORCE = false>
 __global__ void __launch_bounds__(TX *TY) k_hsmac_sweep(Geom g, BufSets<T> bs, Box3 R, HsCoef<T> c, DtRef dt,
-                                                       DevScalars *S, double *part, int zc)
+                                                       DevScalars *S, double *part, int zc, double *ppart)
 {
     extern __shared__ __align__(16) unsigned char hs_lds_raw[];
     __shared__ double red_lds[16];
@@ -631,10 +637,22 @@ __global__ void __launch_bounds__(TX *TY) k_hsmac_sweep(Geom g, BufSets<T> bs, B
     a.c = c;
     a.dti = T(1.0 / (dt.factor * (*dt.src)));
     a.zc = zc;
+    if (FORCE) {  // the coefficients k_lf_apply uses (lf_apply_block)
+        a.xf = 1;
+        a.xA = T(S->lf[4]);
+        for (int q = 0; q < 3; q++) a.xm[q] = T(S->lf[1 + q]);
+        a.xdt = T(dt.factor * (*dt.src));
+    }
     int bx, by, bz;
     xcd_tile(bx, by, bz);
-    const double s = hsmac_sweep_block<T, COLORING, TY, TX>(a, bx, by, bz, reinterpret_cast<T *>(hs_lds_raw), red_lds);
-    if (threadIdx.x == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+    double pw = 0.0;
+    const double s = hsmac_sweep_block<T, COLORING, TY, TX, FORCE>(a, bx, by, bz, reinterpret_cast<T *>(hs_lds_raw),
+                                                                   red_lds, &pw);
+    const long blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    if (threadIdx.x == 0) {
+        part[blk] = s;
+        if (FORCE) ppart[blk] = pw;
+    }
 }
 
 // Two fused sweeps, RA then RB, current -> other buffer set (phase functions
@@ -753,6 +771,8 @@ __global__ void k_hsmac_end(DevScalars *S, int slot)
     }
 }
 
+__global__ void k_mean_to(const double *part, int nblk, double len, double *dst);
+
 template <typename T>
 struct HsmacRun {
     Box &b;
@@ -766,6 +786,7 @@ struct HsmacRun {
     std::vector<int> sweep_first;     // for each launch of sweep_bytes: index of its first sweep
     std::vector<const char *> sweep_name;
     int sweeps = 0;                   // sweeps enqueued so far in this call
+    bool force_next = false;          // apply the deferred linear forcing in the next fused sweep
 
     // one colour over boxes r0 and r1 (see k_hsmac_pass for when two boxes may share a launch)
     int pass(const Box3 &r0, const Box3 &r1, int color)
@@ -832,9 +853,23 @@ struct HsmacRun {
         sweep_bytes.push_back(8.0 * sizeof(T) * cells);
         sweep_first.push_back(sweeps++);
         sweep_name.push_back("hsmac_sweep");
-        hipLaunchKernelGGL((k_hsmac_sweep<T, COLORING, TY, TX>), dim3(gx, gy, nzc), dim3(Tile::NT), lds, b.stream, g,
-                           bs, r, c, dt, b.S, part, zc);
+        double *ppart = force_next ? b.part + kPowerSlot : nullptr;
+        if (force_next) {
+            rc = kernel_setup((const void *)k_hsmac_sweep<T, COLORING, TY, TX, true>, Tile::NT, lds, nullptr);
+            if (rc) return rc;
+            hipLaunchKernelGGL((k_hsmac_sweep<T, COLORING, TY, TX, true>), dim3(gx, gy, nzc), dim3(Tile::NT), lds,
+                               b.stream, g, bs, r, c, dt, b.S, part, zc, ppart);
+        } else {
+            hipLaunchKernelGGL((k_hsmac_sweep<T, COLORING, TY, TX>), dim3(gx, gy, nzc), dim3(Tile::NT), lds,
+                               b.stream, g, bs, r, c, dt, b.S, part, zc, ppart);
+        }
         LCS_HIP(hipGetLastError());
+        if (force_next) {  // state.p: mean cell power (lf.jl:103)
+            force_next = false;
+            hipLaunchKernelGGL(k_mean_to, dim3(1), dim3(256), 0, b.stream, ppart, (int)(cols * nzc),
+                               (double)g.cells(), &b.S->p);
+            LCS_HIP(hipGetLastError());
+        }
         ca.cnt[npass++] = (int)(cols * nzc);
         hipLaunchKernelGGL(k_hsmac_flip, dim3(1), dim3(1), 0, b.stream, b.S);
         LCS_HIP(hipGetLastError());
@@ -996,7 +1031,7 @@ struct HsmacRun {
 };
 
 template <typename T>
-static int hsmac_t(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int slot)
+static int hsmac_t(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int slot, bool lf_deferred)
 {
     const Geom &g = b.g;
     if (g.h < 3 && prm.time_blocking) return fail(LCS_ERR_INVALID, "time-blocked HSMAC needs halo_size >= 3");
@@ -1022,6 +1057,7 @@ static int hsmac_t(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int
     run.ca.dxmin = std::min(std::min(dx, dy), dz);
     run.ca.epsp = prm.epsp;
     run.ca.itrmin = prm.itrmin;
+    run.force_next = lf_deferred;  // the first sweep of the call applies the linear forcing
 
     hipLaunchKernelGGL(k_hsmac_begin, dim3(1), dim3(1), 0, b.stream, b.S);
     LCS_HIP(hipGetLastError());
@@ -1086,12 +1122,14 @@ static int hsmac_t(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int
     return LCS_OK;
 }
 
-int launch_hsmac(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int slot)
+int launch_hsmac(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int slot, bool lf_deferred)
 {
     if (prm.coloring != LCS_COLORING_RED_BLACK_FAST && prm.coloring != LCS_COLORING_RED_BLACK)
         return fail(LCS_ERR_INVALID, "unsupported coloring");
     if (prm.itrmax < 0 || prm.itrmin < 0) return fail(LCS_ERR_INVALID, "itrmin/itrmax must be >= 0");
-    return b.esize == 8 ? hsmac_t<double>(b, Us, dt, prm, slot) : hsmac_t<float>(b, Us, dt, prm, slot);
+    if (lf_deferred && prm.itrmax < 1) return fail(LCS_ERR_INVALID, "deferred forcing needs a sweep");
+    return b.esize == 8 ? hsmac_t<double>(b, Us, dt, prm, slot, lf_deferred)
+                        : hsmac_t<float>(b, Us, dt, prm, slot, lf_deferred);
 }
 
 // ---------------------------------------------------------------------------
@@ -1483,7 +1521,8 @@ int launch_flow_stat(Box &b, int integral_length, double *out_dev)
     return LCS_OK;
 }
 
-int launch_force_linear(Box &b, double power, DtRef dt)
+// The means and A of the linear forcing into S->lf (lf.jl:40-66).
+int launch_force_linear_coef(Box &b, double power)
 {
     void *arr[3] = {b.f[0], b.f[1], b.f[2]};
     int rc = launch_ghost_fill(b, arr, 3, b.g.h, 1, nullptr);
@@ -1502,6 +1541,15 @@ int launch_force_linear(Box &b, double power, DtRef dt)
     }
     hipLaunchKernelGGL(k_lf_finalize, dim3(1), dim3(256), 0, b.stream, b.part, blocks, len, power, b.S);
     LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+int launch_force_linear(Box &b, double power, DtRef dt)
+{
+    int rc = launch_force_linear_coef(b, power);
+    if (rc) return rc;
+    const int blocks = grid_rows(b.g);
+    const double len = (double)b.g.cells();
     {
     ProfScope ps(b, "lf_apply", 6.0 * b.esize * len);
     if (b.esize == 8)

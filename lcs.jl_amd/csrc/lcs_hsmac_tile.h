@@ -109,9 +109,18 @@ struct HsArgs {
     HsCoef<T> c;
     T dti;
     int zc;
+    // Linear forcing applied to U, V, W as they are loaded (src/flows/forcings/
+    // lf.jl:88-100): x <- x + (A (x - xmean)) dt, and the cell power
+    // u usrc + v vsrc + w wsrc summed over the interior cells the workgroup
+    // owns.  xf = 0: none.
+    int xf = 0;
+    T xA = T(0), xm[3] = {T(0), T(0), T(0)}, xdt = T(0);
 };
 
-template <typename T, int COLORING, int TY_, int TX_ = 64>
+// FORCE: the sweep applies the linear forcing of HsArgs::xf on load (the
+// first sweep of couple! after forcing!); a separate instantiation, so the
+// other sweeps carry none of its registers.
+template <typename T, int COLORING, int TY_, int TX_ = 64, bool FORCE = false>
 struct HsTile {
     static constexpr int TX = TX_, TY = TY_, NT = TX * TY;
     static constexpr int LX = TX + 4, LY = TY + 4, PLANE = LX * LY;  // fields: x in [X0-2, X1+2]
@@ -134,6 +143,8 @@ struct HsTile {
         T own[3][4];    // own cell's U, V, W at planes q-3..q
         T pP, pPn;      // P of the owned cell at planes q-3 and q-2
         double acc;
+        bool own_xy;    // the owned column is interior in x and y
+        double pacc;    // forcing power of the owned interior cells
     };
 
     __host__ __device__ static int m3(int q) { return ((q % 3) + 3) % 3; }
@@ -192,6 +203,11 @@ struct HsTile {
             for (int s = 0; s < 4; s++) th.own[f][s] = T(0);
         th.pP = th.pPn = T(0);
         th.acc = 0.0;
+        th.pacc = 0.0;
+        {
+            const int ox = th.X0 + tid % TX, oy = th.Y0 + tid / TX;
+            th.own_xy = ox >= 1 && ox <= g.nx && oy >= 1 && oy <= g.ny;
+        }
     }
 
     // loads of plane q into register slot s
@@ -209,9 +225,27 @@ struct HsTile {
         }
     }
 
-    // plane q (register slot s) into LDS and into the own-cell queue
-    __host__ __device__ static void store(T *lds, int q, Thread &th, int s)
+    // FORCE: the linear forcing on the elements of plane q in slot s (at the
+    // store, not the fetch, so the loads stay in flight for a whole step)
+    __host__ __device__ static void force(const HsArgs<T> &a, int q, Thread &th, int s)
     {
+        if (!plane_in(a.g, q)) return;
+        for (int e = 0; e <= NHS; e++) {
+            if ((e == 0 ? th.o_off : th.h_off[e - 1]) < 0) continue;
+            const T u = th.pf[s][0][e], v = th.pf[s][1][e], w = th.pf[s][2][e];
+            const T usrc = a.xA * (u - a.xm[0]), vsrc = a.xA * (v - a.xm[1]), wsrc = a.xA * (w - a.xm[2]);
+            if (e == 0 && th.own_xy && q >= th.z0 && q <= th.z1 && q >= 1 && q <= a.g.nz)
+                th.pacc += (double)((u * usrc + v * vsrc) + w * wsrc);
+            th.pf[s][0][e] = u + usrc * a.xdt;
+            th.pf[s][1][e] = v + vsrc * a.xdt;
+            th.pf[s][2][e] = w + wsrc * a.xdt;
+        }
+    }
+
+    // plane q (register slot s) into LDS and into the own-cell queue
+    __host__ __device__ static void store(const HsArgs<T> &a, T *lds, int q, Thread &th, int s)
+    {
+        if (FORCE) force(a, q, th, s);
         const int e = (th.tid / TX + 2) * LX + th.tid % TX + 2;
         Uf(lds, q, 0, e) = th.pf[s][0][0];
         Vf(lds, q, 0, e) = th.pf[s][1][0];
@@ -350,7 +384,7 @@ struct HsTile {
         if (q - 1 >= z0 && q - 1 <= z1 + 1) black(a, lds, q - 1, th);
         if (q - 3 >= z0) output(a, lds, q - 3, th);
         if (q + 1 <= z1 + 2)
-            store(lds, q + 1, th, s);
+            store(a, lds, q + 1, th, s);
         else
             shift(th, s);
         th.pP = th.pPn;
@@ -359,15 +393,16 @@ struct HsTile {
     __host__ __device__ static void prologue(const HsArgs<T> &a, T *lds, Thread &th)
     {
         fetch(a, th.z0 - 2, th, 0);
-        store(lds, th.z0 - 2, th, 0);
+        store(a, lds, th.z0 - 2, th, 0);
         fetch(a, th.z0 - 1, th, 0);
-        store(lds, th.z0 - 1, th, 0);
+        store(a, lds, th.z0 - 1, th, 0);
         fetch(a, th.z0, th, 0);
     }
 
     // The march of one workgroup for a host harness: the threads of each
     // part run one after the other.
-    static double emulate_block(const HsArgs<T> &a, T *lds, Thread *th, int bx, int by, int bz)
+    static double emulate_block(const HsArgs<T> &a, T *lds, Thread *th, int bx, int by, int bz,
+                                double *power = nullptr)
     {
         for (int t = 0; t < NT; t++) init(a, bx, by, bz, t, th[t]);
         const int z0 = th[0].z0, z1 = th[0].z1;
@@ -379,6 +414,8 @@ struct HsTile {
         }
         double sum = 0.0;
         for (int t = 0; t < NT; t++) sum += th[t].acc;
+        if (power)
+            for (int t = 0; t < NT; t++) *power += th[t].pacc;
         return sum;
     }
 };

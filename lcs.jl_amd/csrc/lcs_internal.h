@@ -201,8 +201,12 @@ int launch_update(Box &b, void *const out[3], void *const base[3], DtRef dt);
 int launch_cfl(Box &b, double cfl, double dtmax);  // writes S->umax, S->dt_flow
 // Us points at the box's three velocity pointers of the stage; when the
 // result ends up in the spare buffers these pointers (and P) are swapped.
-int launch_hsmac(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int log_slot);
+// lf_deferred: the linear forcing whose coefficients launch_force_linear_coef
+// left in S->lf is applied by the call's first sweep as it loads U, V, W
+// (the same values k_lf_apply writes), which also sets S->p.
+int launch_hsmac(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int log_slot, bool lf_deferred = false);
 int launch_force_linear(Box &b, double power, DtRef dt);  // writes S->lf, S->p
+int launch_force_linear_coef(Box &b, double power);       // writes S->lf only
 int launch_boxmean(Box &b, int c, double *coarse_dev);
 // Flows.stat raw values into out_dev[15] (device): U, V, W mean/max/min, means of
 // u2, s2, dudx^2..4, then the integral length when asked

@@ -8,11 +8,16 @@
 
 using namespace lcs;
 
-template <typename T, int COLORING, int TY, int TX>
-static double run(int nx, int ny, int nz, int h, const int *lo, const int *hi, double dt, double omega,
-                  void *const *src, void *const *dst, int zc)
+// linear forcing applied on load (coefficients as k_hsmac_sweep sets them):
+// null = none; else {A, mean U, mean V, mean W, dt}; power receives the sum
+static const double *g_force = nullptr;
+static double g_power = 0.0;
+
+template <typename T, int COLORING, int TY, int TX, bool FORCE>
+static double run_t(int nx, int ny, int nz, int h, const int *lo, const int *hi, double dt, double omega,
+                    void *const *src, void *const *dst, int zc)
 {
-    using Tile = HsTile<T, COLORING, TY, TX>;
+    using Tile = HsTile<T, COLORING, TY, TX, FORCE>;
     HsArgs<T> a;
     a.g = make_geom(nx, ny, nz, h);
     a.Us = (const T *)src[0];
@@ -35,6 +40,12 @@ static double run(int nx, int ny, int nz, int h, const int *lo, const int *hi, d
     a.c.omega = T(omega);
     a.dti = T(1.0 / dt);
     a.zc = zc;
+    if (g_force) {
+        a.xf = 1;
+        a.xA = T(g_force[0]);
+        for (int q = 0; q < 3; q++) a.xm[q] = T(g_force[1 + q]);
+        a.xdt = T(g_force[4]);
+    }
     std::vector<T> lds(Tile::LDS_ELEMS);
     std::vector<typename Tile::Thread> th(Tile::NT);
     const int gx = (nx + 2 * h + Tile::TX - 1) / Tile::TX, gy = (ny + 2 * h + TY - 1) / TY;
@@ -42,8 +53,17 @@ static double run(int nx, int ny, int nz, int h, const int *lo, const int *hi, d
     double s = 0.0;
     for (int bz = 0; bz < gz; bz++)
         for (int by = 0; by < gy; by++)
-            for (int bx = 0; bx < gx; bx++) s += Tile::emulate_block(a, lds.data(), th.data(), bx, by, bz);
+            for (int bx = 0; bx < gx; bx++)
+                s += Tile::emulate_block(a, lds.data(), th.data(), bx, by, bz, g_force ? &g_power : nullptr);
     return s;
+}
+
+template <typename T, int COLORING, int TY, int TX>
+static double run(int nx, int ny, int nz, int h, const int *lo, const int *hi, double dt, double omega,
+                  void *const *src, void *const *dst, int zc)
+{
+    return g_force ? run_t<T, COLORING, TY, TX, true>(nx, ny, nz, h, lo, hi, dt, omega, src, dst, zc)
+                   : run_t<T, COLORING, TY, TX, false>(nx, ny, nz, h, lo, hi, dt, omega, src, dst, zc);
 }
 
 extern "C" int hsmac_sweep_emul(int ty, int tx, int esize, int coloring, int nx, int ny, int nz, int h,
@@ -68,4 +88,18 @@ extern "C" int hsmac_sweep_emul(int ty, int tx, int esize, int coloring, int nx,
 #undef BY_SHAPE
 #undef RUN
     return 0;
+}
+
+// the same sweep with the linear forcing {A, mean U, V, W, dt} applied to
+// U, V, W on load; *power receives the forcing power summed over the interior
+extern "C" int hsmac_sweep_forced_emul(int ty, int tx, int esize, int coloring, int nx, int ny, int nz, int h,
+                                       const int *lo, const int *hi, double dt, double omega, void *const *src,
+                                       void *const *dst, int zc, const double *force, double *div2, double *power)
+{
+    g_force = force;
+    g_power = 0.0;
+    const int rc = hsmac_sweep_emul(ty, tx, esize, coloring, nx, ny, nz, h, lo, hi, dt, omega, src, dst, zc, div2);
+    *power = g_power;
+    g_force = nullptr;
+    return rc;
 }

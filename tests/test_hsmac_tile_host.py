@@ -74,3 +74,46 @@ def test_fused_sweep_equals_two_colour_passes(emul, dims, ty, tx, zc, box, color
     for name, a, b in zip("UVWP", dst, ref):
         np.testing.assert_array_equal(a, b, err_msg=name)
     assert s.value == pytest.approx(s_ref, rel=1e-12)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("dims,ty,tx,zc,box", [((20, 13, 11), 8, 64, 4, "d1"), ((40, 10, 9), 8, 32, 3, "core"),
+                                               ((70, 20, 6), 16, 64, 2, "d3")])
+def test_fused_sweep_with_linear_forcing_on_load(emul, dims, ty, tx, zc, box, dtype):
+    """The first sweep of couple! after the linear forcing applies it as it
+    loads U, V, W: the result equals x <- x + (A (x - mean)) dt on every
+    element (lf.jl:88-100; ghosts are copies, so transforming them equals
+    copying the transformed interior) followed by the two colour passes, and
+    the power sum is that of the cells (1:n)^3 (lf.jl:96,103)."""
+    h, dt, omega = 3, 0.01, 1.7
+    rng = np.random.default_rng(7)
+    src = [np.asfortranarray(rng.standard_normal(tuple(n + 2 * h for n in dims)).astype(dtype)) for _ in range(4)]
+    A, means, fdt = 0.37, [0.11, -0.07, 0.02], 0.004
+    T = np.dtype(dtype).type
+    ref = [a.copy(order="F") for a in src]
+    nx, ny, nz = dims
+    inner = (slice(h, h + nx), slice(h, h + ny), slice(h, h + nz))
+    power = 0.0
+    srcs = []
+    for c in range(3):
+        usrc = T(A) * (ref[c] - T(means[c]))
+        srcs.append(usrc)
+        ref[c] = ref[c] + usrc * T(fdt)
+    pw = (src[0] * srcs[0] + src[1] * srcs[1]) + src[2] * srcs[2]
+    power = pw[inner].astype(np.float64).sum()
+    lo, hi = _box(box, dims)
+    div2 = np.zeros_like(ref[0])
+    O.hsmac_pass(*ref, div2, h, lo, hi, 0, dt, omega, 0)
+    O.hsmac_pass(*ref, div2, h, lo, hi, 1, dt, omega, 0)
+    dst = [np.full_like(a, np.nan) for a in src]
+    sp = (C.c_void_p * 4)(*[a.ctypes.data for a in src])
+    dp = (C.c_void_p * 4)(*[a.ctypes.data for a in dst])
+    force = (C.c_double * 5)(A, *means, fdt)
+    s, pw_out = C.c_double(0), C.c_double(0)
+    rc = emul.hsmac_sweep_forced_emul(ty, tx, np.dtype(dtype).itemsize, 0, nx, ny, nz, h,
+                                      lo.ctypes.data_as(C.c_void_p), hi.ctypes.data_as(C.c_void_p), C.c_double(dt),
+                                      C.c_double(omega), sp, dp, zc, force, C.byref(s), C.byref(pw_out))
+    assert rc == 0
+    for name, a, b in zip("UVWP", dst, ref):
+        np.testing.assert_array_equal(a, b, err_msg=name)
+    assert pw_out.value == pytest.approx(power, rel=1e-10)
