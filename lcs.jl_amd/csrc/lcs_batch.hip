@@ -63,6 +63,9 @@ struct Batch {
     // id; captured on the second step after a membership change (the first
     // runs eagerly and sets the kernels' attributes) and replayed afterwards
     bool graphs = true, warm = false;
+    // the next fused sweep applies the deferred linear forcing of the
+    // linear-power boxes (set around stage 2's HSMAC only)
+    bool force_next = false;
     std::map<int, hipGraphExec_t> gexec;
 };
 
@@ -272,6 +275,7 @@ static int rebuild(Batch &B)
         // row-walking reductions
         r.row_blocks = std::min(g.ny * g.nz, kBatchRowBlocks);
         for (int k = 0; k < r.row_blocks; k++) rows.push_back({i, k});
+        r.lf_force = B.cfgs[i].forcing.kind == LCS_FORCING_LINEAR_POWER;
         if (B.cfgs[i].forcing.kind == LCS_FORCING_LINEAR_POWER) {
             lf_boxes.push_back(i);
             for (int k = 0; k < r.row_blocks; k++) lf_rows.push_back({i, k});
@@ -331,11 +335,14 @@ static int b_fill(Batch &B, int set, int narr, int width, int hsmac)
 static int b_sweep(Batch &B, int set, int kind, int slot, double dtfac)
 {
     int rc;
+    const bool force = B.force_next;
+    B.force_next = false;
     for (int q = 0; q < 4; q++)
         if ((rc = launch_b_hsmac_sweep(B.stream, B.esize, B.hs.coloring, kBatchSweepShapes[q][0],
                                        kBatchSweepShapes[q][1], B.tab.d, B.hs_tiles[q].d, B.hs_tiles[q].n, set, kind,
-                                       slot, dtfac)))
+                                       slot, dtfac, force)))
             return rc;
+    if (force && (rc = launch_b_lf_power(B.stream, B.tab.d, B.lf_boxes.d, B.lf_boxes.n))) return rc;
     return launch_b_hsmac_ctl(B.stream, B.tab.d, (int)B.boxes.size(), kHsFlip, 0, nullptr);
 }
 
@@ -484,6 +491,7 @@ static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
     // stage 2 up to forcing!: linear-power boxes batched in the segment,
     // rc-energy boxes one by one after it (their launches take the host's
     // current array pointers)
+    const bool lf_defer = B.lf_boxes.n > 0 && B.hs.itrmax >= 1;
     rc = segment(B, 2, [&]() {
         int r;
         if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 1, 4, h, 1, 0)) ||
@@ -492,7 +500,8 @@ static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
             return r;
         if (B.lf_boxes.n > 0) {
             if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 0, 3, h, 1, 0)) ||
-                (r = launch_b_lf(B.stream, B.esize, B.tab.d, B.lf_rows.d, B.lf_rows.n, B.lf_boxes.d, B.lf_boxes.n)))
+                (r = launch_b_lf(B.stream, B.esize, B.tab.d, B.lf_rows.d, B.lf_rows.n, B.lf_boxes.d, B.lf_boxes.n,
+                                 !lf_defer)))
                 return r;
         }
         return LCS_OK;
@@ -503,7 +512,12 @@ static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
         const lcs_forcing &f = B.cfgs[i].forcing;
         if ((rc = launch_force_rcf(b, f.energy, f.kmin, f.kmax, f.coarse_size, DtRef{&b.S->dt, 1.0}))) return rc;
     }
-    if ((rc = b_hsmac(B, 0, 1.0, 1, 20))) return rc;
+    // the first sweep of stage 2's couple! applies the linear forcing as it
+    // loads U, V, W (its coefficients were set above)
+    B.force_next = lf_defer;
+    rc = b_hsmac(B, 0, 1.0, 1, 20);
+    B.force_next = false;
+    if (rc) return rc;
     if ((rc = launch_b_state(B.stream, B.tab.d, n, logs ? B.sc_d : nullptr))) return rc;
     B.warm = true;
     if (logs) {

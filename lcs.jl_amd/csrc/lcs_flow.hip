@@ -619,10 +619,8 @@ __device__ inline void hs_bind(HsArgs<T> &a, const BufSets<T> &bs, bool par)
     a.Pd = par ? bs.a[3] : bs.b[3];
 }
 
-// ppart != null: the linear forcing of S->lf is applied to U, V, W as they
-// are loaded, and each workgroup's forcing power goes to ppart (the last
-// partial slot of the box, which no other pass of a step uses).
-constexpr long kPowerSlot = (long)(kMaxSweepPasses + 3) * kMaxPartBlocks;
+// FORCE: the linear forcing of S->lf is applied to U, V, W as they are
+// loaded, and each workgroup's forcing power goes to ppart.
 template <typename T, int COLORING, int TY, int TX, bool FORCE = false>
 __global__ void __launch_bounds__(TX *TY) k_hsmac_sweep(Geom g, BufSets<T> bs, Box3 R, HsCoef<T> c, DtRef dt,
                                                        DevScalars *S, double *part, int zc, double *ppart)
@@ -2190,6 +2188,26 @@ __global__ void __launch_bounds__(256) k_b_lf_apply(const BoxRec *tab, const Wor
     if (threadIdx.x == 0) B.part[w.blk] = v;
 }
 
+// state.p of the boxes whose first stage-2 sweep applied the forcing: the
+// mean of the per-tile power sums (lf.jl:103)
+__global__ void k_b_lf_power(const BoxRec *tab, const int *boxes)
+{
+    __shared__ double lds[16];
+    const BoxRec &B = tab[boxes[blockIdx.x]];
+    double v = 0.0;
+    for (int q = threadIdx.x; q < B.hs_tiles; q += blockDim.x) v += B.part[kPowerSlot + q];
+    v = block_sum(v, lds);
+    if (threadIdx.x == 0) B.S->p = v / (double)B.g.cells();
+}
+
+int launch_b_lf_power(hipStream_t s, const BoxRec *tab, const int *boxes, int nbox)
+{
+    if (nbox == 0) return LCS_OK;
+    hipLaunchKernelGGL(k_b_lf_power, dim3(nbox), dim3(256), 0, s, tab, boxes);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
 __global__ void k_b_lf_mean(const BoxRec *tab, const int *boxes)
 {
     __shared__ double lds[16];
@@ -2201,7 +2219,7 @@ __global__ void k_b_lf_mean(const BoxRec *tab, const int *boxes)
 }
 
 int launch_b_lf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *rows, int nrows, const int *boxes,
-                int nbox)
+                int nbox, bool apply)
 {
     if (nbox == 0) return LCS_OK;
     if (esize == 8)
@@ -2209,6 +2227,8 @@ int launch_b_lf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *row
     else
         hipLaunchKernelGGL(k_b_lf_means<float>, dim3(nrows), dim3(256), 0, s, tab, rows);
     hipLaunchKernelGGL(k_b_lf_fin, dim3(nbox), dim3(256), 0, s, tab, boxes);
+    LCS_HIP(hipGetLastError());
+    if (!apply) return LCS_OK;
     if (esize == 8)
         hipLaunchKernelGGL(k_b_lf_apply<double>, dim3(nrows), dim3(256), 0, s, tab, rows);
     else
@@ -2273,7 +2293,7 @@ __device__ inline void hs_coef(const BoxRec &B, HsCoef<T> &c)
     c.omega = T(B.hs_omega);
 }
 
-template <typename T, int COLORING, int TY, int TX>
+template <typename T, int COLORING, int TY, int TX, bool FORCE = false>
 __global__ void __launch_bounds__(TX *TY) k_b_hsmac_sweep(const BoxRec *tab, const TileItem *tiles, int set, int kind,
                                                          int slot, double dtfac)
 {
@@ -2294,52 +2314,74 @@ __global__ void __launch_bounds__(TX *TY) k_b_hsmac_sweep(const BoxRec *tab, con
     hs_coef(B, a.c);
     a.dti = T(1.0 / (dtfac * B.S->dt));
     a.zc = B.hs_zc;
-    const double s =
-        hsmac_sweep_block<T, COLORING, TY, TX>(a, t.bx, t.by, t.bz, reinterpret_cast<T *>(hs_lds_raw), red_lds);
+    if (FORCE && B.lf_force) {  // the coefficients k_b_lf_apply uses (lf_apply_block)
+        a.xf = 1;
+        a.xA = T(B.S->lf[4]);
+        for (int q = 0; q < 3; q++) a.xm[q] = T(B.S->lf[1 + q]);
+        a.xdt = T(dtfac * B.S->dt);
+    }
+    double pw = 0.0;
+    const double s = hsmac_sweep_block<T, COLORING, TY, TX, FORCE>(a, t.bx, t.by, t.bz,
+                                                                   reinterpret_cast<T *>(hs_lds_raw), red_lds, &pw);
     const int gx = (B.g.nx + 2 * B.g.h + TX - 1) / TX, gy = (B.g.ny + 2 * B.g.h + TY - 1) / TY;
-    if (threadIdx.x == 0) B.part[(long)slot * kMaxPartBlocks + t.bx + gx * (t.by + gy * t.bz)] = s;
+    const long tix = t.bx + gx * (t.by + gy * t.bz);
+    if (threadIdx.x == 0) {
+        B.part[(long)slot * kMaxPartBlocks + tix] = s;
+        if (FORCE && B.lf_force) B.part[kPowerSlot + tix] = pw;
+    }
 }
 
-template <typename T, int COLORING, int TX, int TY>
-static int b_sweep_t(hipStream_t s, const BoxRec *tab, const TileItem *tiles, int ntiles, int set, int kind, int slot,
+template <typename T, int COLORING, int TX, int TY, bool FORCE>
+static int b_sweep_f(hipStream_t s, const BoxRec *tab, const TileItem *tiles, int ntiles, int set, int kind, int slot,
                      double dtfac)
 {
-    using Tile = HsTile<T, COLORING, TY, TX>;
+    using Tile = HsTile<T, COLORING, TY, TX, FORCE>;
     const size_t lds = Tile::LDS_ELEMS * sizeof(T);
-    int rc = kernel_setup((const void *)k_b_hsmac_sweep<T, COLORING, TY, TX>, Tile::NT, lds, nullptr);
+    int rc = kernel_setup((const void *)k_b_hsmac_sweep<T, COLORING, TY, TX, FORCE>, Tile::NT, lds, nullptr);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_b_hsmac_sweep<T, COLORING, TY, TX>), dim3(ntiles), dim3(Tile::NT), lds, s, tab, tiles, set,
-                       kind, slot, dtfac);
+    hipLaunchKernelGGL((k_b_hsmac_sweep<T, COLORING, TY, TX, FORCE>), dim3(ntiles), dim3(Tile::NT), lds, s, tab, tiles,
+                       set, kind, slot, dtfac);
     LCS_HIP(hipGetLastError());
     return LCS_OK;
 }
 
+template <typename T, int COLORING, int TX, int TY>
+static int b_sweep_t(hipStream_t s, const BoxRec *tab, const TileItem *tiles, int ntiles, int set, int kind, int slot,
+                     double dtfac, bool force)
+{
+    return force ? b_sweep_f<T, COLORING, TX, TY, true>(s, tab, tiles, ntiles, set, kind, slot, dtfac)
+                 : b_sweep_f<T, COLORING, TX, TY, false>(s, tab, tiles, ntiles, set, kind, slot, dtfac);
+}
+
 template <typename T, int TX, int TY>
 static int b_sweep_c(hipStream_t s, int coloring, const BoxRec *tab, const TileItem *tiles, int ntiles, int set,
-                     int kind, int slot, double dtfac)
+                     int kind, int slot, double dtfac, bool force)
 {
     return coloring == LCS_COLORING_RED_BLACK
-               ? b_sweep_t<T, LCS_COLORING_RED_BLACK, TX, TY>(s, tab, tiles, ntiles, set, kind, slot, dtfac)
-               : b_sweep_t<T, LCS_COLORING_RED_BLACK_FAST, TX, TY>(s, tab, tiles, ntiles, set, kind, slot, dtfac);
+               ? b_sweep_t<T, LCS_COLORING_RED_BLACK, TX, TY>(s, tab, tiles, ntiles, set, kind, slot, dtfac, force)
+               : b_sweep_t<T, LCS_COLORING_RED_BLACK_FAST, TX, TY>(s, tab, tiles, ntiles, set, kind, slot, dtfac,
+                                                                    force);
 }
 
 template <typename T>
 static int b_sweep_shape(hipStream_t s, int coloring, int tx, int ty, const BoxRec *tab, const TileItem *tiles,
-                         int ntiles, int set, int kind, int slot, double dtfac)
+                         int ntiles, int set, int kind, int slot, double dtfac, bool force)
 {
-    if (tx == 64 && ty == 8) return b_sweep_c<T, 64, 8>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac);
-    if (tx == 32 && ty == 8) return b_sweep_c<T, 32, 8>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac);
-    if (tx == 64 && ty == 16) return b_sweep_c<T, 64, 16>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac);
-    if (tx == 32 && ty == 16) return b_sweep_c<T, 32, 16>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac);
+    if (tx == 64 && ty == 8) return b_sweep_c<T, 64, 8>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac, force);
+    if (tx == 32 && ty == 8) return b_sweep_c<T, 32, 8>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac, force);
+    if (tx == 64 && ty == 16)
+        return b_sweep_c<T, 64, 16>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac, force);
+    if (tx == 32 && ty == 16)
+        return b_sweep_c<T, 32, 16>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac, force);
     return fail(LCS_ERR_INVALID, "unsupported sweep tile shape");
 }
 
 int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, int tx, int ty, const BoxRec *tab,
-                         const TileItem *tiles, int ntiles, int set, int kind, int slot, double dtfac)
+                         const TileItem *tiles, int ntiles, int set, int kind, int slot, double dtfac, bool force)
 {
     if (ntiles == 0) return LCS_OK;
-    return esize == 8 ? b_sweep_shape<double>(s, coloring, tx, ty, tab, tiles, ntiles, set, kind, slot, dtfac)
-                      : b_sweep_shape<float>(s, coloring, tx, ty, tab, tiles, ntiles, set, kind, slot, dtfac);
+    return esize == 8 ? b_sweep_shape<double>(s, coloring, tx, ty, tab, tiles, ntiles, set, kind, slot, dtfac, force)
+                      : b_sweep_shape<float>(s, coloring, tx, ty, tab, tiles, ntiles, set, kind, slot, dtfac, force);
 }
 
 // one colour over both face slabs of one axis, in place on the current set

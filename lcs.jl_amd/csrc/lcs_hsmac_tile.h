@@ -245,7 +245,7 @@ struct HsTile {
     // plane q (register slot s) into LDS and into the own-cell queue
     __host__ __device__ static void store(const HsArgs<T> &a, T *lds, int q, Thread &th, int s)
     {
-        if (FORCE) force(a, q, th, s);
+        if (FORCE && a.xf) force(a, q, th, s);
         const int e = (th.tid / TX + 2) * LX + th.tid % TX + 2;
         Uf(lds, q, 0, e) = th.pf[s][0][0];
         Vf(lds, q, 0, e) = th.pf[s][1][0];

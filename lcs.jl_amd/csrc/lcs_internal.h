@@ -81,6 +81,9 @@ constexpr int kMaxProfiles = 8;
 constexpr int kMaxPartBlocks = 16384;    // blocks per reducing pass
 constexpr int kMaxSweepPasses = 14;      // core 2 + 6 faces x 2 colours
 constexpr long kPartSlots = (long)kMaxPartBlocks * (kMaxSweepPasses + 4);
+// the last partial slot: per-workgroup linear-forcing power of the sweep that
+// applies the forcing on load (no other pass of a step uses it)
+constexpr long kPowerSlot = (long)(kMaxSweepPasses + 3) * kMaxPartBlocks;
 
 struct ParticleProfile {
     lcs_particle_desc desc;
@@ -264,6 +267,7 @@ struct BoxRec {
     int hs_tiles;        // workgroups of one fused sweep
     int face_blocks[3];  // blocks of one paired face pass, axes z, y, x
     int row_blocks;      // blocks of the row-walking reductions (CFL, forcing)
+    int lf_force;        // linear-power forcing: applied by the first stage-2 sweep
     int nprof;
     DevProfile prof[kMaxProfiles];
 };
@@ -289,12 +293,18 @@ int launch_b_ghost_fill(hipStream_t s, int esize, const BoxRec *tab, const WorkI
                         int narr, int r, int full, int hsmac);
 int launch_b_cfl(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *rows, int nrows, int nbox,
                  double cfl, double dtmax);
+// apply = false: the means and A only (the first stage-2 sweep applies the
+// forcing, launch_b_hsmac_sweep with force, then launch_b_lf_power sets p)
 int launch_b_lf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *rows, int nrows, const int *boxes,
-                int nbox);
+                int nbox, bool apply = true);
+int launch_b_lf_power(hipStream_t s, const BoxRec *tab, const int *boxes, int nbox);
 int launch_b_rhs(hipStream_t s, int esize, const BoxRec *tab, const TileItem *tiles, int ntiles, int stage);
 // tx: tile width of the fused sweep (32 or 64); the tiles listed must all be of that width
+// force: boxes with lf_force apply the linear forcing of S->lf on load and
+// leave their per-tile power in part[kPowerSlot]
 int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, int tx, int ty, const BoxRec *tab,
-                         const TileItem *tiles, int ntiles, int set, int kind, int slot, double dtfac);
+                         const TileItem *tiles, int ntiles, int set, int kind, int slot, double dtfac,
+                         bool force = false);
 int launch_b_hsmac_faces(hipStream_t s, int esize, int coloring, const BoxRec *tab, const WorkItem *work,
                          int nwork, int set, int domain, int axis, int color, int slot, double dtfac);
 // kinds of the passes since the last check: 0 = fused sweep, 1..3 = face pass of axis z, y, x
