@@ -70,11 +70,20 @@ struct Hid {
     operator hid_t() const { return id; }
 };
 
-void quiet()
-{
-    // errors are reported through lcs_io_last_error, not printed
-    H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
-}
+// Errors are reported through lcs_io_last_error, not printed: the default
+// error stack's printing is off for the duration of a call and restored after.
+struct Quiet {
+    H5E_auto2_t func = nullptr;
+    void *data = nullptr;
+    Quiet()
+    {
+        H5Eget_auto2(H5E_DEFAULT, &func, &data);
+        H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+    }
+    ~Quiet() { H5Eset_auto2(H5E_DEFAULT, func, data); }
+    Quiet(const Quiet &) = delete;
+    Quiet &operator=(const Quiet &) = delete;
+};
 
 bool file_exists(const char *file) { return access(file, F_OK) == 0; }
 
@@ -199,7 +208,7 @@ const char *lcs_io_last_error(void) { return g_err.c_str(); }
 int lcs_save_restart(lcs_box *box, const char *file)
 {
     if (!box || !file) return fail(LCS_ERR_INVALID, "null argument");
-    quiet();
+    Quiet quiet;
     lcs_box_desc d;
     int nprof = 0, rc;
     if ((rc = check_desc(box, &d, &nprof))) return rc;
@@ -265,7 +274,7 @@ int lcs_save_restart(lcs_box *box, const char *file)
 int lcs_load_flow_fields(lcs_box *box, const char *file)
 {
     if (!box || !file) return fail(LCS_ERR_INVALID, "null argument");
-    quiet();
+    Quiet quiet;
     lcs_box_desc d;
     int rc;
     if ((rc = check_desc(box, &d, nullptr))) return rc;
@@ -306,7 +315,7 @@ int lcs_load_flow_fields(lcs_box *box, const char *file)
 int lcs_load_particles(lcs_box *box, const char *file, int profile)
 {
     if (!box || !file) return fail(LCS_ERR_INVALID, "null argument");
-    quiet();
+    Quiet quiet;
     lcs_box_desc d;
     int nprof = 0, rc;
     if ((rc = check_desc(box, &d, &nprof))) return rc;
@@ -440,7 +449,7 @@ int lcs_save_stat(const char *file, int64_t step, const lcs_h5_entry *entries, i
     if (!file) return fail(LCS_ERR_INVALID, "null argument");
     int rc;
     if ((rc = check_entries(entries, n))) return rc;
-    quiet();
+    Quiet quiet;
     Hid f;
     if ((rc = open_cw(file, &f))) return rc;
     const std::string key = std::to_string((long long)step);
@@ -459,7 +468,7 @@ int lcs_h5_write(const char *file, const lcs_h5_entry *entries, int n)
     if (!file) return fail(LCS_ERR_INVALID, "null argument");
     int rc;
     if ((rc = check_entries(entries, n))) return rc;
-    quiet();
+    Quiet quiet;
     Hid f;
     if ((rc = open_cw(file, &f))) return rc;
     if ((rc = write_entries(f, "", entries, n))) return rc;
@@ -470,7 +479,7 @@ int lcs_h5_write(const char *file, const lcs_h5_entry *entries, int n)
 int lcs_h5_info(const char *file, const char *path, int *kind, int *type, int *rank, int64_t *dims)
 {
     if (!file || !path || !kind) return fail(LCS_ERR_INVALID, "null argument");
-    quiet();
+    Quiet quiet;
     Hid f;
     int rc;
     if ((rc = open_r(file, &f))) return rc;
@@ -512,7 +521,7 @@ int lcs_h5_read(const char *file, const char *path, int type, void *out, int64_t
 {
     if (!file || !path || !out) return fail(LCS_ERR_INVALID, "null argument");
     if (type < LCS_H5_I64 || type > LCS_H5_STR) return fail(LCS_ERR_INVALID, "unknown type");
-    quiet();
+    Quiet quiet;
     Hid f;
     int rc;
     if ((rc = open_r(file, &f))) return rc;
@@ -574,7 +583,7 @@ herr_t key_cb(hid_t, const char *name, const H5L_info_t *, void *data)
 int lcs_h5_keys(const char *file, const char *path, char *names, size_t len, int *n, size_t *needed)
 {
     if (!file || !path || !n || (len && !names)) return fail(LCS_ERR_INVALID, "null argument");
-    quiet();
+    Quiet quiet;
     Hid f;
     int rc;
     if ((rc = open_r(file, &f))) return rc;
