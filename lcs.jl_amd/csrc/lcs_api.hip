@@ -875,18 +875,21 @@ int lcs_step(lcs_box *box, const lcs_step_config *cfg, lcs_step_log *log)
         for (int q = 0; q < np; q++)
             if ((rc = launch_p_localize(b, b.profiles[q]))) return rc;
     b.steps_host++;
+    // stage 1's FullHalo fill of U, V, W, P (evolve.jl:74-81) comes first: it
+    // writes every ghost the CFL's FaceHalo fill would, with the same values,
+    // and nothing writes the velocity in between, so the CFL reads it instead
+    // HSMAC may swap field pointers with its ping-pong partners: read b.f at use
+    void *UsP[4] = {b.f[LCS_FIELD_U], b.f[LCS_FIELD_V], b.f[LCS_FIELD_W], b.f[LCS_FIELD_P]};
+    if ((rc = launch_ghost_fill(b, UsP, 4, b.g.h, 1, nullptr))) return rc;
     // timestep!: src/timestep.jl:152-173
-    if ((rc = launch_cfl(b, cfg->cfl, cfg->dtmax))) return rc;
+    if ((rc = launch_cfl(b, cfg->cfl, cfg->dtmax, false))) return rc;
     for (int q = 0; q < np; q++)
         if ((rc = launch_p_timestep(b, b.profiles[q], cfg->dtmax, &b.S->dtp[q]))) return rc;
     if ((rc = launch_finalize_dt(b, np, cfg->dtmax))) return rc;
 
     // evolve!: src/evolve.jl:63-146 (FlowMode :15-61 when there are no particles)
     const DtRef dt1{&b.S->dt, 0.5}, dt2{&b.S->dt, 1.0};
-    // HSMAC may swap field pointers with its ping-pong partners: read b.f at use
-    void *UsP[4] = {b.f[LCS_FIELD_U], b.f[LCS_FIELD_V], b.f[LCS_FIELD_W], b.f[LCS_FIELD_P]};
-    // stage 1
-    if ((rc = launch_ghost_fill(b, UsP, 4, b.g.h, 1, nullptr))) return rc;
+    // stage 1 (ghosts filled above)
     for (int q = 0; q < np; q++)
         if ((rc = launch_p_stage(b, b.profiles[q], 1, dt1))) return rc;
     if ((rc = launch_rhs(b, 1, true, &b.f[LCS_FIELD_U2], &b.f[LCS_FIELD_U], dt1))) return rc;

@@ -52,7 +52,7 @@ struct Batch {
     // device state
     std::vector<BoxRec> tab_h;
     DevVec<BoxRec> tab;
-    DevVec<WorkItem> ghost_face, ghost_full, ghost_one, rows, lf_rows, parts, diams, faces[3];
+    DevVec<WorkItem> ghost_full, ghost_one, rows, lf_rows, parts, diams, faces[3];
     DevVec<int> lf_boxes;            // boxes with linear-power forcing
     std::vector<int> rcf_boxes;      // boxes with rc-energy forcing (forced one by one)
     DevVec<TileItem> rhs_tiles, hs_tiles[4];  // sweep tiles by shape: kBatchSweepShapes
@@ -208,7 +208,7 @@ static int rebuild(Batch &B)
         LCS_HIP(hipHostMalloc((void **)&B.sc_h, sizeof(DevScalars) * (size_t)B.cap_boxes, hipHostMallocDefault));
     }
     B.tab_h.assign(n, BoxRec{});
-    std::vector<WorkItem> gface, gfull, gone, rows, lf_rows, parts, diams, faces[3];
+    std::vector<WorkItem> gfull, gone, rows, lf_rows, parts, diams, faces[3];
     std::vector<int> lf_boxes;
     B.rcf_boxes.clear();
     std::vector<TileItem> rtiles, htiles[4];
@@ -269,7 +269,6 @@ static int rebuild(Batch &B)
             const long tot = 2L * rr * X * Y + 2L * rr * g.nz * X + 2L * rr * g.ny * g.nz;
             for (int k = 0; k < blocks_of(tot); k++) w.push_back({i, k});
         };
-        ghost(g.h, 0, gface);
         ghost(g.h, 1, gfull);
         ghost(1, 1, gone);
         // row-walking reductions
@@ -310,7 +309,7 @@ static int rebuild(Batch &B)
     }
     int rc;
     if ((rc = upload_table(B))) return rc;
-    if ((rc = B.ghost_face.assign(xcd_order(gface), B.stream)) || (rc = B.ghost_full.assign(xcd_order(gfull), B.stream)) ||
+    if ((rc = B.ghost_full.assign(xcd_order(gfull), B.stream)) ||
         (rc = B.ghost_one.assign(xcd_order(gone), B.stream)) || (rc = B.rows.assign(xcd_order(rows), B.stream)) ||
         (rc = B.lf_rows.assign(xcd_order(lf_rows), B.stream)) || (rc = B.lf_boxes.assign(lf_boxes, B.stream)) ||
         (rc = B.parts.assign(xcd_order(parts), B.stream)) || (rc = B.diams.assign(xcd_order(diams), B.stream)) ||
@@ -475,13 +474,14 @@ static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
     if (moved && (rc = upload_table(B))) return rc;
     const lcs_step_config &c0 = B.cfgs[0];
     const int h = B.boxes[0]->g.h;
-    // timestep! (CFL after a FaceHalo fill, particle dt) and stage 1 up to couple!
+    // timestep! (CFL, particle dt) and stage 1 up to couple!; stage 1's
+    // FullHalo fill comes first and serves the CFL too (it writes every ghost
+    // a FaceHalo fill would, with the same values)
     rc = segment(B, 1, [&]() {
         int r;
-        if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_face.d, B.ghost_face.n, 0, 3, h, 0, 0)) ||
+        if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 0, 4, h, 1, 0)) ||
             (r = launch_b_cfl(B.stream, B.esize, B.tab.d, B.rows.d, B.rows.n, n, c0.cfl, c0.dtmax)) ||
             (r = launch_b_particle_dt(B.stream, B.esize, B.tab.d, B.diams.d, B.diams.n, n, c0.dtmax)) ||
-            (r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 0, 4, h, 1, 0)) ||
             (r = launch_b_particles(B.stream, B.esize, B.tab.d, B.parts.d, B.parts.n, 1)) ||
             (r = launch_b_rhs(B.stream, B.esize, B.tab.d, B.rhs_tiles.d, B.rhs_tiles.n, 1)))
             return r;
@@ -603,8 +603,8 @@ int lcs_batch_destroy(lcs_batch *batch)
     (void)hipSetDevice(B->ctx->device);
     drop_graphs(*B);
     while (!B->boxes.empty()) (void)detach(*B, (int)B->boxes.size() - 1);
-    DevVec<WorkItem> *w[] = {&B->ghost_face, &B->ghost_full, &B->ghost_one, &B->rows,    &B->lf_rows,
-                             &B->parts,      &B->diams,      &B->faces[0],  &B->faces[1], &B->faces[2]};
+    DevVec<WorkItem> *w[] = {&B->ghost_full, &B->ghost_one, &B->rows,     &B->lf_rows,  &B->parts,
+                             &B->diams,      &B->faces[0],  &B->faces[1], &B->faces[2]};
     for (auto *v : w) v->release();
     B->lf_boxes.release();
     B->rhs_tiles.release();

@@ -1229,10 +1229,10 @@ __global__ void k_cfl_finalize(const double *part, int nblk, double cfl, double 
     if (threadIdx.x == 0) cfl_dt(S, mm, cfl, dtmax, dx, dy, dz, Re);
 }
 
-int launch_cfl(Box &b, double cfl, double dtmax)
+int launch_cfl(Box &b, double cfl, double dtmax, bool fill)
 {
     void *arr[3] = {b.f[0], b.f[1], b.f[2]};
-    int rc = launch_ghost_fill(b, arr, 3, b.g.h, 0, nullptr);
+    int rc = fill ? launch_ghost_fill(b, arr, 3, b.g.h, 0, nullptr) : LCS_OK;
     if (rc) return rc;
     // at most 4096 workgroups; z-chunks of at least 8 planes
     const int gx = (b.g.nx + kCflPX - 1) / kCflPX, gy = (b.g.ny + kCflPY - 1) / kCflPY;

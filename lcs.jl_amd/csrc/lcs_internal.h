@@ -201,7 +201,9 @@ int launch_ghost_fill(Box &b, void *const *arrays, int narrays, int width, int f
                       const int *skip_flag);
 int launch_rhs(Box &b, int stage_in, bool fused, void *const out[3], void *const base[3], DtRef dt);
 int launch_update(Box &b, void *const out[3], void *const base[3], DtRef dt);
-int launch_cfl(Box &b, double cfl, double dtmax);  // writes S->umax, S->dt_flow
+// fill = false: the ghosts of U, V, W are already periodic copies (a FullHalo
+// fill since the last write), so the FaceHalo fill is skipped
+int launch_cfl(Box &b, double cfl, double dtmax, bool fill = true);  // writes S->umax, S->dt_flow
 // Us points at the box's three velocity pointers of the stage; when the
 // result ends up in the spare buffers these pointers (and P) are swapped.
 // lf_deferred: the linear forcing whose coefficients launch_force_linear_coef
