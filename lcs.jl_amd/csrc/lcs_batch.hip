@@ -66,6 +66,8 @@ struct Batch {
     // the next fused sweep applies the deferred linear forcing of the
     // linear-power boxes (set around stage 2's HSMAC only)
     bool force_next = false;
+    bool wrap_next = false;  // the next sweep loads ghost U, V, W from their images
+    bool wrap_ok = false;    // every box has n >= h on each axis
     std::map<int, hipGraphExec_t> gexec;
 };
 
@@ -208,6 +210,9 @@ static int rebuild(Batch &B)
         LCS_HIP(hipHostMalloc((void **)&B.sc_h, sizeof(DevScalars) * (size_t)B.cap_boxes, hipHostMallocDefault));
     }
     B.tab_h.assign(n, BoxRec{});
+    B.wrap_ok = true;
+    for (const Box *b : B.boxes)
+        if (b->g.nx < b->g.h || b->g.ny < b->g.h || b->g.nz < b->g.h) B.wrap_ok = false;
     std::vector<WorkItem> gfull, gone, rows, lf_rows, parts, diams, faces[3];
     std::vector<int> lf_boxes;
     B.rcf_boxes.clear();
@@ -334,12 +339,12 @@ static int b_fill(Batch &B, int set, int narr, int width, int hsmac)
 static int b_sweep(Batch &B, int set, int kind, int slot, double dtfac)
 {
     int rc;
-    const bool force = B.force_next;
-    B.force_next = false;
+    const bool force = B.force_next, wrap = B.wrap_next;
+    B.force_next = B.wrap_next = false;
     for (int q = 0; q < 4; q++)
         if ((rc = launch_b_hsmac_sweep(B.stream, B.esize, B.hs.coloring, kBatchSweepShapes[q][0],
                                        kBatchSweepShapes[q][1], B.tab.d, B.hs_tiles[q].d, B.hs_tiles[q].n, set, kind,
-                                       slot, dtfac, force)))
+                                       slot, dtfac, force, wrap)))
             return rc;
     if (force && (rc = launch_b_lf_power(B.stream, B.tab.d, B.lf_boxes.d, B.lf_boxes.n))) return rc;
     return launch_b_hsmac_ctl(B.stream, B.tab.d, (int)B.boxes.size(), kHsFlip, 0, nullptr);
@@ -398,7 +403,12 @@ static int b_group(Batch &B, int set, double dtfac)
         }
         return b_check(B, kinds);
     }
-    if ((rc = b_fill(B, set, 3, B.boxes[0]->g.h, 1))) return rc;
+    // the FullHalo fill of U, V, W folds into the loads of the group's first
+    // sweep where every box's ghosts have one image each
+    if (B.wrap_ok)
+        B.wrap_next = true;
+    else if ((rc = b_fill(B, set, 3, B.boxes[0]->g.h, 1)))
+        return rc;
     if (p.overlap) {
         if ((rc = b_core_faces(B, set, 1, dtfac, kinds))) return rc;
     } else {

@@ -623,7 +623,7 @@ __device__ inline void hs_bind(HsArgs<T> &a, const BufSets<T> &bs, bool par)
 // loaded, and each workgroup's forcing power goes to ppart.
 template <typename T, int COLORING, int TY, int TX, bool FORCE = false>
 __global__ void __launch_bounds__(TX *TY) k_hsmac_sweep(Geom g, BufSets<T> bs, Box3 R, HsCoef<T> c, DtRef dt,
-                                                       DevScalars *S, double *part, int zc, double *ppart)
+                                                       DevScalars *S, double *part, int zc, double *ppart, int wrap)
 {
     extern __shared__ __align__(16) unsigned char hs_lds_raw[];
     __shared__ double red_lds[16];
@@ -635,6 +635,7 @@ __global__ void __launch_bounds__(TX *TY) k_hsmac_sweep(Geom g, BufSets<T> bs, B
     a.c = c;
     a.dti = T(1.0 / (dt.factor * (*dt.src)));
     a.zc = zc;
+    a.wrap = wrap;
     if (FORCE) {  // the coefficients k_lf_apply uses (lf_apply_block)
         a.xf = 1;
         a.xA = T(S->lf[4]);
@@ -785,6 +786,7 @@ struct HsmacRun {
     std::vector<const char *> sweep_name;
     int sweeps = 0;                   // sweeps enqueued so far in this call
     bool force_next = false;          // apply the deferred linear forcing in the next fused sweep
+    bool wrap_next = false;           // the next fused sweep loads ghost U, V, W from their images (in place of a fill)
 
     // one colour over boxes r0 and r1 (see k_hsmac_pass for when two boxes may share a launch)
     int pass(const Box3 &r0, const Box3 &r1, int color)
@@ -852,14 +854,16 @@ struct HsmacRun {
         sweep_first.push_back(sweeps++);
         sweep_name.push_back("hsmac_sweep");
         double *ppart = force_next ? b.part + kPowerSlot : nullptr;
+        const int wrap = wrap_next ? 1 : 0;
+        wrap_next = false;
         if (force_next) {
             rc = kernel_setup((const void *)k_hsmac_sweep<T, COLORING, TY, TX, true>, Tile::NT, lds, nullptr);
             if (rc) return rc;
             hipLaunchKernelGGL((k_hsmac_sweep<T, COLORING, TY, TX, true>), dim3(gx, gy, nzc), dim3(Tile::NT), lds,
-                               b.stream, g, bs, r, c, dt, b.S, part, zc, ppart);
+                               b.stream, g, bs, r, c, dt, b.S, part, zc, ppart, wrap);
         } else {
             hipLaunchKernelGGL((k_hsmac_sweep<T, COLORING, TY, TX>), dim3(gx, gy, nzc), dim3(Tile::NT), lds,
-                               b.stream, g, bs, r, c, dt, b.S, part, zc, ppart);
+                               b.stream, g, bs, r, c, dt, b.S, part, zc, ppart, wrap);
         }
         LCS_HIP(hipGetLastError());
         if (force_next) {  // state.p: mean cell power (lf.jl:103)
@@ -1079,8 +1083,14 @@ static int hsmac_t(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int
         // _couple_time_blocking!: hsmac.jl:131-201
         const Box3 d1 = {{-1, -1, -1}, {g.nx + 3, g.ny + 3, g.nz + 3}};
         const Box3 d2 = {{0, 0, 0}, {g.nx + 2, g.ny + 2, g.nz + 2}};
+        // The group's FullHalo fill of U, V, W folds into the loads of its
+        // first sweep (HsArgs::wrap) where every ghost has one image.
+        const bool wrap = g.nx >= g.h && g.ny >= g.h && g.nz >= g.h;
         for (int grp = 0; 3 * grp < prm.itrmax && !done; grp++) {
-            if ((rc = run.fill(g.h, false))) return rc;
+            if (wrap)
+                run.wrap_next = true;
+            else if ((rc = run.fill(g.h, false)))
+                return rc;
             rc = prm.overlap ? run.sweep_core_faces(d1) : run.fused(d1);
             if (rc) return rc;
             if ((rc = run.check())) return rc;
@@ -2295,7 +2305,7 @@ __device__ inline void hs_coef(const BoxRec &B, HsCoef<T> &c)
 
 template <typename T, int COLORING, int TY, int TX, bool FORCE = false>
 __global__ void __launch_bounds__(TX *TY) k_b_hsmac_sweep(const BoxRec *tab, const TileItem *tiles, int set, int kind,
-                                                         int slot, double dtfac)
+                                                         int slot, double dtfac, int wrap)
 {
     extern __shared__ __align__(16) unsigned char hs_lds_raw[];
     __shared__ double red_lds[16];
@@ -2314,6 +2324,7 @@ __global__ void __launch_bounds__(TX *TY) k_b_hsmac_sweep(const BoxRec *tab, con
     hs_coef(B, a.c);
     a.dti = T(1.0 / (dtfac * B.S->dt));
     a.zc = B.hs_zc;
+    a.wrap = wrap;
     if (FORCE && B.lf_force) {  // the coefficients k_b_lf_apply uses (lf_apply_block)
         a.xf = 1;
         a.xA = T(B.S->lf[4]);
@@ -2333,55 +2344,60 @@ __global__ void __launch_bounds__(TX *TY) k_b_hsmac_sweep(const BoxRec *tab, con
 
 template <typename T, int COLORING, int TX, int TY, bool FORCE>
 static int b_sweep_f(hipStream_t s, const BoxRec *tab, const TileItem *tiles, int ntiles, int set, int kind, int slot,
-                     double dtfac)
+                     double dtfac, int wrap)
 {
     using Tile = HsTile<T, COLORING, TY, TX, FORCE>;
     const size_t lds = Tile::LDS_ELEMS * sizeof(T);
     int rc = kernel_setup((const void *)k_b_hsmac_sweep<T, COLORING, TY, TX, FORCE>, Tile::NT, lds, nullptr);
     if (rc) return rc;
     hipLaunchKernelGGL((k_b_hsmac_sweep<T, COLORING, TY, TX, FORCE>), dim3(ntiles), dim3(Tile::NT), lds, s, tab, tiles,
-                       set, kind, slot, dtfac);
+                       set, kind, slot, dtfac, wrap);
     LCS_HIP(hipGetLastError());
     return LCS_OK;
 }
 
 template <typename T, int COLORING, int TX, int TY>
 static int b_sweep_t(hipStream_t s, const BoxRec *tab, const TileItem *tiles, int ntiles, int set, int kind, int slot,
-                     double dtfac, bool force)
+                     double dtfac, bool force, int wrap)
 {
-    return force ? b_sweep_f<T, COLORING, TX, TY, true>(s, tab, tiles, ntiles, set, kind, slot, dtfac)
-                 : b_sweep_f<T, COLORING, TX, TY, false>(s, tab, tiles, ntiles, set, kind, slot, dtfac);
+    return force ? b_sweep_f<T, COLORING, TX, TY, true>(s, tab, tiles, ntiles, set, kind, slot, dtfac, wrap)
+                 : b_sweep_f<T, COLORING, TX, TY, false>(s, tab, tiles, ntiles, set, kind, slot, dtfac, wrap);
 }
 
 template <typename T, int TX, int TY>
 static int b_sweep_c(hipStream_t s, int coloring, const BoxRec *tab, const TileItem *tiles, int ntiles, int set,
-                     int kind, int slot, double dtfac, bool force)
+                     int kind, int slot, double dtfac, bool force, int wrap)
 {
     return coloring == LCS_COLORING_RED_BLACK
-               ? b_sweep_t<T, LCS_COLORING_RED_BLACK, TX, TY>(s, tab, tiles, ntiles, set, kind, slot, dtfac, force)
+               ? b_sweep_t<T, LCS_COLORING_RED_BLACK, TX, TY>(s, tab, tiles, ntiles, set, kind, slot, dtfac, force,
+                                                              wrap)
                : b_sweep_t<T, LCS_COLORING_RED_BLACK_FAST, TX, TY>(s, tab, tiles, ntiles, set, kind, slot, dtfac,
-                                                                    force);
+                                                                    force, wrap);
 }
 
 template <typename T>
 static int b_sweep_shape(hipStream_t s, int coloring, int tx, int ty, const BoxRec *tab, const TileItem *tiles,
-                         int ntiles, int set, int kind, int slot, double dtfac, bool force)
+                         int ntiles, int set, int kind, int slot, double dtfac, bool force, int wrap)
 {
-    if (tx == 64 && ty == 8) return b_sweep_c<T, 64, 8>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac, force);
-    if (tx == 32 && ty == 8) return b_sweep_c<T, 32, 8>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac, force);
-    if (tx == 64 && ty == 16)
-        return b_sweep_c<T, 64, 16>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac, force);
-    if (tx == 32 && ty == 16)
-        return b_sweep_c<T, 32, 16>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac, force);
+#define B_SHAPE(X, Y) \
+    if (tx == X && ty == Y) return b_sweep_c<T, X, Y>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac, force, wrap)
+    B_SHAPE(64, 8);
+    B_SHAPE(32, 8);
+    B_SHAPE(64, 16);
+    B_SHAPE(32, 16);
+#undef B_SHAPE
     return fail(LCS_ERR_INVALID, "unsupported sweep tile shape");
 }
 
 int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, int tx, int ty, const BoxRec *tab,
-                         const TileItem *tiles, int ntiles, int set, int kind, int slot, double dtfac, bool force)
+                         const TileItem *tiles, int ntiles, int set, int kind, int slot, double dtfac, bool force,
+                         bool wrap)
 {
     if (ntiles == 0) return LCS_OK;
-    return esize == 8 ? b_sweep_shape<double>(s, coloring, tx, ty, tab, tiles, ntiles, set, kind, slot, dtfac, force)
-                      : b_sweep_shape<float>(s, coloring, tx, ty, tab, tiles, ntiles, set, kind, slot, dtfac, force);
+    const int w = wrap ? 1 : 0;
+    return esize == 8
+               ? b_sweep_shape<double>(s, coloring, tx, ty, tab, tiles, ntiles, set, kind, slot, dtfac, force, w)
+               : b_sweep_shape<float>(s, coloring, tx, ty, tab, tiles, ntiles, set, kind, slot, dtfac, force, w);
 }
 
 // one colour over both face slabs of one axis, in place on the current set

@@ -115,6 +115,11 @@ struct HsArgs {
     // owns.  xf = 0: none.
     int xf = 0;
     T xA = T(0), xm[3] = {T(0), T(0), T(0)}, xdt = T(0);
+    // U, V, W of a ghost cell are loaded from its periodic image
+    // A[mod1(i, n)]: the sweep reads what a FullHalo fill of width h of
+    // U, V, W (lib/topologies/src/halo.jl:138-159) would have left, without
+    // the fill.  P is loaded as it is.  wrap = 0: plain loads.
+    int wrap = 0;
 };
 
 // FORCE: the sweep applies the linear forcing of HsArgs::xf on load (the
@@ -138,6 +143,7 @@ struct HsTile {
         int h_lds[NHS];   // LDS offsets of this thread's halo elements, -1 if none
         long h_off[NHS];  // their in-plane offsets, -1 outside the array
         long o_off;     // in-plane offset of the owned cell, -1 outside the array
+        long u_off;     // where its U, V, W are loaded from (o_off, or its image when wrapping)
         int r_row, r_k, b_row, b_k;
         T pf[2][3][1 + NHS];  // two fetched planes in flight: own cell, halo elements
         T own[3][4];    // own cell's U, V, W at planes q-3..q
@@ -167,6 +173,16 @@ struct HsTile {
         const bool ok = x >= 1 - g.h && x <= g.nx + g.h && y >= 1 - g.h && y <= g.ny + g.h;
         return ok ? (long)(x - 1 + g.h) + g.sx * (long)(y - 1 + g.h) : -1;
     }
+    __host__ __device__ static int mod1(int i, int n) { return ((i - 1) % n + n) % n + 1; }
+    // in_plane of the periodic image of (x, y); -1 outside the array
+    __host__ __device__ static long in_plane_img(const Geom &g, int x, int y)
+    {
+        return in_plane(g, x, y) < 0 ? -1 : in_plane(g, mod1(x, g.nx), mod1(y, g.ny));
+    }
+    __host__ __device__ static long load_plane(const HsArgs<T> &a, int q)
+    {
+        return plane_base(a.g, a.wrap ? mod1(q, a.g.nz) : q);
+    }
 
     __host__ __device__ static void init(const HsArgs<T> &a, int bx, int by, int bz, int tid, Thread &th)
     {
@@ -177,6 +193,7 @@ struct HsTile {
         th.z0 = 1 - g.h + a.zc * bz;
         th.z1 = th.z0 + a.zc - 1 < g.nz + g.h ? th.z0 + a.zc - 1 : g.nz + g.h;
         th.o_off = in_plane(g, th.X0 + tid % TX, th.Y0 + tid / TX);
+        th.u_off = a.wrap ? in_plane_img(g, th.X0 + tid % TX, th.Y0 + tid / TX) : th.o_off;
         for (int s = 0; s < NHS; s++) {
             const int e = tid + s * NT;
             th.h_lds[s] = -1;
@@ -193,7 +210,8 @@ struct HsTile {
                 lx = c < 2 ? c : TX + c;
             }
             th.h_lds[s] = ly * LX + lx;
-            th.h_off[s] = in_plane(g, th.X0 - 2 + lx, th.Y0 - 2 + ly);
+            const int hx = th.X0 - 2 + lx, hy = th.Y0 - 2 + ly;
+            th.h_off[s] = a.wrap ? in_plane_img(g, hx, hy) : in_plane(g, hx, hy);
         }
         th.r_row = tid / RPR;
         th.r_k = tid % RPR;
@@ -214,9 +232,9 @@ struct HsTile {
     __host__ __device__ static void fetch(const HsArgs<T> &a, int q, Thread &th, int s)
     {
         const bool zin = plane_in(a.g, q);
-        const long base = plane_base(a.g, q);
+        const long base = load_plane(a, q);
         for (int e = 0; e <= NHS; e++) {
-            const long off = e == 0 ? th.o_off : th.h_off[e - 1];
+            const long off = e == 0 ? th.u_off : th.h_off[e - 1];
             const bool ok = zin && off >= 0;
             const long p = ok ? base + off : 0;
             th.pf[s][0][e] = ok ? a.Us[p] : T(0);

@@ -304,9 +304,11 @@ int launch_b_rhs(hipStream_t s, int esize, const BoxRec *tab, const TileItem *ti
 // tx: tile width of the fused sweep (32 or 64); the tiles listed must all be of that width
 // force: boxes with lf_force apply the linear forcing of S->lf on load and
 // leave their per-tile power in part[kPowerSlot]
+// wrap: U, V, W of ghost cells are loaded from their periodic images (the
+// sweep stands in for a FullHalo fill of U, V, W of width h before it)
 int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, int tx, int ty, const BoxRec *tab,
                          const TileItem *tiles, int ntiles, int set, int kind, int slot, double dtfac,
-                         bool force = false);
+                         bool force = false, bool wrap = false);
 int launch_b_hsmac_faces(hipStream_t s, int esize, int coloring, const BoxRec *tab, const WorkItem *work,
                          int nwork, int set, int domain, int axis, int color, int slot, double dtfac);
 // kinds of the passes since the last check: 0 = fused sweep, 1..3 = face pass of axis z, y, x

@@ -12,6 +12,8 @@ using namespace lcs;
 // null = none; else {A, mean U, mean V, mean W, dt}; power receives the sum
 static const double *g_force = nullptr;
 static double g_power = 0.0;
+// U, V, W of ghost cells loaded from their periodic images (HsArgs::wrap)
+static int g_wrap = 0;
 
 template <typename T, int COLORING, int TY, int TX, bool FORCE>
 static double run_t(int nx, int ny, int nz, int h, const int *lo, const int *hi, double dt, double omega,
@@ -40,6 +42,7 @@ static double run_t(int nx, int ny, int nz, int h, const int *lo, const int *hi,
     a.c.omega = T(omega);
     a.dti = T(1.0 / dt);
     a.zc = zc;
+    a.wrap = g_wrap;
     if (g_force) {
         a.xf = 1;
         a.xA = T(g_force[0]);
@@ -88,6 +91,22 @@ extern "C" int hsmac_sweep_emul(int ty, int tx, int esize, int coloring, int nx,
 #undef BY_SHAPE
 #undef RUN
     return 0;
+}
+
+// the same sweep loading U, V, W of ghost cells from their periodic images,
+// optionally with the linear forcing (force null: none)
+extern "C" int hsmac_sweep_wrap_emul(int ty, int tx, int esize, int coloring, int nx, int ny, int nz, int h,
+                                     const int *lo, const int *hi, double dt, double omega, void *const *src,
+                                     void *const *dst, int zc, const double *force, double *div2, double *power)
+{
+    g_wrap = 1;
+    g_force = force;
+    g_power = 0.0;
+    const int rc = hsmac_sweep_emul(ty, tx, esize, coloring, nx, ny, nz, h, lo, hi, dt, omega, src, dst, zc, div2);
+    *power = g_power;
+    g_force = nullptr;
+    g_wrap = 0;
+    return rc;
 }
 
 // the same sweep with the linear forcing {A, mean U, V, W, dt} applied to

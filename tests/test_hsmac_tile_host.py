@@ -117,3 +117,55 @@ def test_fused_sweep_with_linear_forcing_on_load(emul, dims, ty, tx, zc, box, dt
     for name, a, b in zip("UVWP", dst, ref):
         np.testing.assert_array_equal(a, b, err_msg=name)
     assert pw_out.value == pytest.approx(power, rel=1e-10)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("forced", [False, True])
+@pytest.mark.parametrize("dims,ty,tx,zc,box,coloring", [((20, 13, 11), 8, 64, 4, "d1", 0),
+                                                        ((16, 12, 10), 4, 64, 5, "core", 1),
+                                                        ((5, 3, 4), 4, 64, 2, "d1", 0),
+                                                        ((40, 10, 9), 8, 32, 3, "d1", 1),
+                                                        ((70, 20, 6), 16, 64, 2, "core", 0)])
+def test_fused_sweep_with_fill_on_load(emul, dims, ty, tx, zc, box, coloring, forced, dtype):
+    """The first sweep of a time-blocked group loads U, V, W of ghost cells
+    from their periodic images: equal bit for bit, ghosts included, to the
+    FullHalo fill of width h of U, V, W that precedes the group
+    (hsmac.jl:131-201, halo.jl:138-159) followed by the two colour passes.
+    P's ghosts are used as they are (the fill does not touch P).  With the
+    linear forcing on load as well, the forcing applies to the filled
+    values."""
+    h, dt, omega = 3, 0.01, 1.7
+    rng = np.random.default_rng(abs(hash((dims, ty, tx, zc, box, coloring, forced))) % 2**32)
+    src = [np.asfortranarray(rng.standard_normal(tuple(n + 2 * h for n in dims)).astype(dtype)) for _ in range(4)]
+    T = np.dtype(dtype).type
+    A, means, fdt = 0.37, [0.11, -0.07, 0.02], 0.004
+    ref = [a.copy(order="F") for a in src]
+    for c in range(3):
+        O.ghost_fill(ref[c], h, h, True)
+    nx, ny, nz = dims
+    inner = (slice(h, h + nx), slice(h, h + ny), slice(h, h + nz))
+    power = 0.0
+    if forced:
+        srcs = [T(A) * (ref[c] - T(means[c])) for c in range(3)]
+        power = ((ref[0] * srcs[0] + ref[1] * srcs[1]) + ref[2] * srcs[2])[inner].astype(np.float64).sum()
+        for c in range(3):
+            ref[c] = ref[c] + srcs[c] * T(fdt)
+    lo, hi = _box(box, dims)
+    div2 = np.zeros_like(ref[0])
+    O.hsmac_pass(*ref, div2, h, lo, hi, 0, dt, omega, coloring)
+    O.hsmac_pass(*ref, div2, h, lo, hi, 1, dt, omega, coloring)
+    s_ref = div2[h:h + nx + 1, h:h + ny + 1, h:h + nz + 1].astype(np.float64).sum()
+    dst = [np.full_like(a, np.nan) for a in src]
+    sp = (C.c_void_p * 4)(*[a.ctypes.data for a in src])
+    dp = (C.c_void_p * 4)(*[a.ctypes.data for a in dst])
+    force = (C.c_double * 5)(A, *means, fdt) if forced else None
+    s, pw_out = C.c_double(0), C.c_double(0)
+    rc = emul.hsmac_sweep_wrap_emul(ty, tx, np.dtype(dtype).itemsize, coloring, nx, ny, nz, h,
+                                    lo.ctypes.data_as(C.c_void_p), hi.ctypes.data_as(C.c_void_p), C.c_double(dt),
+                                    C.c_double(omega), sp, dp, zc, force, C.byref(s), C.byref(pw_out))
+    assert rc == 0
+    for name, a, b in zip("UVWP", dst, ref):
+        np.testing.assert_array_equal(a, b, err_msg=name)
+    assert s.value == pytest.approx(s_ref, rel=1e-12)
+    if forced:
+        assert pw_out.value == pytest.approx(power, rel=1e-10)
