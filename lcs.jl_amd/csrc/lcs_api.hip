@@ -252,6 +252,7 @@ int lcs_box_destroy(lcs_box *box)
     }
     if (b->S) (void)hipFree(b->S);
     if (b->part) (void)hipFree(b->part);
+    if (b->xf) (void)hipFree(b->xf);
     if (b->rcf) (void)hipFree(b->rcf);
     if (b->S_host) (void)hipHostFree(b->S_host);
     prof_collect(*b);

@@ -16,6 +16,7 @@
 #include "lcs_internal.h"
 #include "lcs_hsmac_tile.h"
 #include "lcs_hsmac_tile2.h"
+#include "lcs_hsmac_xface.h"
 #include "lcs_rhs_tile.h"
 
 namespace lcs {
@@ -577,6 +578,63 @@ __global__ void __launch_bounds__(256) k_hsmac_pass(Geom g, BufSets<T> bs, Box3 
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// Both colours over the two x face slabs (lcs_hsmac_xface.h), launch 1:
+// patches of slab rows, final values into the compact buffer.  The grid
+// strides over the patches; each workgroup leaves its div^2 sum in part.
+template <typename T, int COLORING>
+__global__ void __launch_bounds__(XfPatch<T>::N *XfPatch<T>::N)
+    k_hsmac_xface(Geom g, BufSets<T> bs, Box3 r0, Box3 r1, HsCoef<T> c, DtRef dt, const DevScalars *S, T *buf,
+                  double *part)
+{
+    using X = XFaceOf<T, COLORING>;
+    extern __shared__ __align__(16) unsigned char xf_lds_raw[];
+    T *lds = reinterpret_cast<T *>(xf_lds_raw);
+    __shared__ double red_lds[16];
+    if (S->hs_done) return;
+    const bool alt = S->hs_par != 0;
+    XfArgs<T> a;
+    a.g = g;
+    a.U = alt ? bs.b[0] : bs.a[0];
+    a.V = alt ? bs.b[1] : bs.a[1];
+    a.W = alt ? bs.b[2] : bs.a[2];
+    a.P = alt ? bs.b[3] : bs.a[3];
+    a.R[0] = r0;
+    a.R[1] = r1;
+    a.c = c;
+    a.dti = T(1.0 / (dt.factor * (*dt.src)));
+    a.buf = buf;
+    double acc = 0.0;
+    const int nblk = X::blocks(g);
+    for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        typename X::Patch p;
+        X::patch(a, blk, p);
+        X::load(a, p, lds, threadIdx.x);
+        __syncthreads();
+        X::red(a, p, lds, threadIdx.x, acc);
+        __syncthreads();
+        X::black(a, p, lds, threadIdx.x, acc);
+        __syncthreads();
+        X::finals(a, p, lds, threadIdx.x);
+        __syncthreads();
+    }
+    const double s = block_sum(acc, red_lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// launch 2: the compact buffer into U, V, W, P, one thread per slab row
+template <typename T>
+__global__ void __launch_bounds__(256) k_hsmac_xface_commit(Geom g, BufSets<T> bs, Box3 r0, Box3 r1,
+                                                           const DevScalars *S, const T *buf)
+{
+    if (S->hs_done) return;
+    const bool alt = S->hs_par != 0;
+    T *U = alt ? bs.b[0] : bs.a[0], *V = alt ? bs.b[1] : bs.a[1];
+    T *W = alt ? bs.b[2] : bs.a[2], *P = alt ? bs.b[3] : bs.a[3];
+    const long rows = xf_rows(g);
+    for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (long)gridDim.x * blockDim.x)
+        XFaceOf<T, LCS_COLORING_RED_BLACK>::commit_row(g, r0, r1, U, V, W, P, buf, r);
+}
+
 // Fused red+black sweep over R, current -> other buffer set (the phase
 // functions and their schedule are in lcs_hsmac_tile.h).  Returns the
 // workgroup's div^2 sum (valid in thread 0).
@@ -815,6 +873,51 @@ struct HsmacRun {
         ca.cnt[npass++] = blocks;
         return LCS_OK;
     }
+    // both colours over the x- and x+ slabs, through the compact buffer
+    int xface(const Box3 &r0, const Box3 &r1)
+    {
+        using X = XFaceOf<T, LCS_COLORING_RED_BLACK>;
+        const Geom &g = b.g;
+        const size_t need = (size_t)kXfVals * xf_rows(g) * sizeof(T);
+        if (b.xf_bytes < need) {
+            if (b.xf) {
+                LCS_HIP(hipStreamSynchronize(b.stream));
+                LCS_HIP(hipFree(b.xf));
+                b.bytes -= b.xf_bytes;
+                b.xf = nullptr;
+                b.xf_bytes = 0;
+            }
+            LCS_HIP(hipMalloc(&b.xf, need));
+            b.xf_bytes = need;
+            b.bytes += need;
+        }
+        const int blocks = std::min(X::blocks(g), kMaxPartBlocks);
+        double *part = b.part + (long)npass * kMaxPartBlocks;
+        const double cells = (double)(r0.hi[0] - r0.lo[0] + r1.hi[0] - r1.lo[0] + 2) * (g.ny - 1) * (g.nz - 1);
+        const size_t lds = X::LDS_ELEMS * sizeof(T);
+        static_assert(X::LDS_ELEMS * sizeof(T) + 128 <= 160 * 1024, "x-face patch exceeds the LDS of a CU");
+        {
+            ProfScope ps(b, "hsmac_xface", 4.0 * sizeof(T) * cells);
+            const bool rb = coloring == LCS_COLORING_RED_BLACK;
+            const void *k = rb ? (const void *)k_hsmac_xface<T, LCS_COLORING_RED_BLACK>
+                               : (const void *)k_hsmac_xface<T, LCS_COLORING_RED_BLACK_FAST>;
+            int rc = kernel_setup(k, X::NT, lds, nullptr);
+            if (rc) return rc;
+            if (rb)
+                hipLaunchKernelGGL((k_hsmac_xface<T, LCS_COLORING_RED_BLACK>), dim3(blocks), dim3(X::NT), lds,
+                                   b.stream, g, bs, r0, r1, c, dt, b.S, (T *)b.xf, part);
+            else
+                hipLaunchKernelGGL((k_hsmac_xface<T, LCS_COLORING_RED_BLACK_FAST>), dim3(blocks), dim3(X::NT), lds,
+                                   b.stream, g, bs, r0, r1, c, dt, b.S, (T *)b.xf, part);
+            LCS_HIP(hipGetLastError());
+        }
+        ca.cnt[npass++] = blocks;
+        ProfScope ps(b, "hsmac_xface_commit", 4.0 * sizeof(T) * cells);
+        hipLaunchKernelGGL(k_hsmac_xface_commit<T>, dim3(grid_for(xf_rows(g), 256, 8192)), dim3(256), 0, b.stream, g,
+                           bs, r0, r1, b.S, (const T *)b.xf);
+        LCS_HIP(hipGetLastError());
+        return LCS_OK;
+    }
     // one fused red+black sweep over a full box, then the parity flip
     template <int COLORING, int TY, int TX>
     int fused_launch(const Box3 &r)
@@ -913,7 +1016,9 @@ struct HsmacRun {
         // the same result as (z- and z+ red), (z- and z+ black), (y- ...).
         const bool pair = g.nx >= 2 && g.ny >= 2 && g.nz >= 2;
         for (int q = 0; q < 6; q += 2) {
-            if (pair) {
+            if (q == 4 && xface_fits(g, f + 4)) {
+                if ((rc = xface(f[4], f[5]))) return rc;
+            } else if (pair) {
                 if ((rc = pass(f[q], f[q + 1], 0)) || (rc = pass(f[q], f[q + 1], 1))) return rc;
             } else {
                 const Box3 none = {{1, 1, 1}, {0, 0, 0}};

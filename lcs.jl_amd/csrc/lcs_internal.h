@@ -154,6 +154,8 @@ struct Box {
     DevScalars *S = nullptr;  // device scalar block
     DevScalars *S_host = nullptr;  // pinned mirror for readback
     double *part = nullptr;  // reduction partials (kPartSlots)
+    void *xf = nullptr;      // compact buffer of the x-slab HSMAC passes (lazily allocated)
+    size_t xf_bytes = 0;
     double *rcf = nullptr;   // coarse grids + cropped spectra
     size_t rcf_bytes = 0;
     std::vector<ParticleProfile> profiles;

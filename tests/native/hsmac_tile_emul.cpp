@@ -122,3 +122,69 @@ extern "C" int hsmac_sweep_forced_emul(int ty, int tx, int esize, int coloring, 
     g_force = nullptr;
     return rc;
 }
+
+// ---------------------------------------------------------------------------
+// Both colour passes over the two x face slabs as k_hsmac_xface and
+// k_hsmac_xface_commit run them (lcs_hsmac_xface.h): every workgroup of the
+// compute launch, then every row of the commit launch.  lo/hi hold the two
+// slabs (x- then x+), three entries each.  Returns -1 for slabs the form
+// does not handle.
+// ---------------------------------------------------------------------------
+#include "lcs_hsmac_xface.h"
+
+template <typename T, int COLORING, int PY>
+static int xface_t(int nx, int ny, int nz, int h, const int *lo, const int *hi, double dt, double omega,
+                   void *const *f, double *div2)
+{
+    using X = XFace<T, COLORING, PY, PY>;
+    XfArgs<T> a;
+    a.g = make_geom(nx, ny, nz, h);
+    T *F[4];
+    for (int q = 0; q < 4; q++) F[q] = (T *)f[q];
+    a.U = F[0];
+    a.V = F[1];
+    a.W = F[2];
+    a.P = F[3];
+    for (int s = 0; s < 2; s++)
+        for (int q = 0; q < 3; q++) {
+            a.R[s].lo[q] = lo[3 * s + q];
+            a.R[s].hi[q] = hi[3 * s + q];
+        }
+    if (!xface_fits(a.g, a.R)) return -1;
+    const T dxi = T(1.0 / spacing(nx)), dyi = T(1.0 / spacing(ny)), dzi = T(1.0 / spacing(nz));
+    a.c.dxi = dxi;
+    a.c.dyi = dyi;
+    a.c.dzi = dzi;
+    a.c.dl = T(1) / (T(2) * ((dxi * dxi + dyi * dyi) + dzi * dzi));
+    a.c.omega = T(omega);
+    a.dti = T(1.0 / dt);
+    const long rows = xf_rows(a.g);
+    std::vector<T> buf(kXfVals * rows);
+    a.buf = buf.data();
+    std::vector<T> lds(X::LDS_ELEMS);
+    double s = 0.0;
+    for (int blk = 0; blk < X::blocks(a.g); blk++) s += X::emulate_block(a, lds.data(), blk);
+    for (long r = 0; r < rows; r++) X::commit_row(a.g, a.R[0], a.R[1], F[0], F[1], F[2], F[3], (const T *)buf.data(), r);
+    *div2 = s;
+    return 0;
+}
+
+extern "C" int hsmac_xface_emul(int patch, int esize, int coloring, int nx, int ny, int nz, int h, const int *lo,
+                                const int *hi, double dt, double omega, void *const *f, double *div2)
+{
+#define RUN(T, C, Y) return xface_t<T, C, Y>(nx, ny, nz, h, lo, hi, dt, omega, f, div2)
+#define BY_PATCH(T, C)          \
+    if (patch == 32) RUN(T, C, 32); \
+    if (patch == 16) RUN(T, C, 16); \
+    if (patch == 4) RUN(T, C, 4);   \
+    return -2;
+    if (esize == 4) {
+        if (coloring == LCS_COLORING_RED_BLACK) { BY_PATCH(float, LCS_COLORING_RED_BLACK) }
+        else { BY_PATCH(float, LCS_COLORING_RED_BLACK_FAST) }
+    } else {
+        if (coloring == LCS_COLORING_RED_BLACK) { BY_PATCH(double, LCS_COLORING_RED_BLACK) }
+        else { BY_PATCH(double, LCS_COLORING_RED_BLACK_FAST) }
+    }
+#undef BY_PATCH
+#undef RUN
+}

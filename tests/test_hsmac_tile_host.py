@@ -169,3 +169,50 @@ def test_fused_sweep_with_fill_on_load(emul, dims, ty, tx, zc, box, coloring, fo
     assert s.value == pytest.approx(s_ref, rel=1e-12)
     if forced:
         assert pw_out.value == pytest.approx(power, rel=1e-10)
+
+
+def _x_slabs(box, n):
+    """The x- and x+ face slabs of region `box` in the overlap order
+    (lib/topologies/src/halo.jl:349-366): rows (y, z) in [2, n]."""
+    lo, hi = _box(box, n)
+    nx, ny, nz = n
+    return (np.array([lo[0], 2, 2, nx + 1, 2, 2], dtype=np.int32),
+            np.array([1, ny, nz, hi[0], ny, nz], dtype=np.int32))
+
+
+XCASES = [((16, 12, 20), 16, "d1"), ((40, 37, 35), 16, "d1"), ((9, 13, 10), 4, "d1"), ((9, 13, 10), 4, "d2"),
+          ((11, 6, 14), 4, "d3"), ((2, 5, 7), 4, "d1"), ((3, 2, 2), 4, "d1"), ((20, 18, 17), 16, "d3"),
+          ((6, 9, 4), 4, "d2"), ((40, 70, 45), 32, "d1"), ((9, 33, 36), 32, "d3")]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("coloring", [0, 1])
+@pytest.mark.parametrize("dims,patch,box", XCASES)
+def test_x_slab_pair_equals_four_colour_passes(emul, dims, patch, box, coloring, dtype):
+    """Both colours over the x- and x+ slabs by patches of slab rows, final
+    values through a compact buffer: equal bit for bit, ghosts included, to
+    the reference's order x- red, x- black, x+ red, x+ black
+    (hsmac.jl:1-35, foraxes.jl:73-107), and the div^2 sum over the slab
+    cells in (1:n+1)^3 equal to the oracle's."""
+    h, dt, omega = 3, 0.01, 1.7
+    rng = np.random.default_rng(abs(hash((dims, patch, box, coloring))) % 2**32)
+    src = [np.asfortranarray(rng.standard_normal(tuple(n + 2 * h for n in dims)).astype(dtype)) for _ in range(4)]
+    lo, hi = _x_slabs(box, dims)
+    ref = [a.copy(order="F") for a in src]
+    div2 = np.zeros_like(ref[0])
+    for s in range(2):
+        for color in range(2):
+            O.hsmac_pass(*ref, div2, h, lo[3 * s:3 * s + 3].copy(), hi[3 * s:3 * s + 3].copy(), color, dt, omega,
+                         coloring)
+    nx, ny, nz = dims
+    s_ref = div2[h:h + nx + 1, h:h + ny + 1, h:h + nz + 1].astype(np.float64).sum()
+    out = [a.copy(order="F") for a in src]
+    fp = (C.c_void_p * 4)(*[a.ctypes.data for a in out])
+    s = C.c_double(0)
+    rc = emul.hsmac_xface_emul(patch, np.dtype(dtype).itemsize, coloring, nx, ny, nz, h,
+                               lo.ctypes.data_as(C.c_void_p), hi.ctypes.data_as(C.c_void_p), C.c_double(dt),
+                               C.c_double(omega), fp, C.byref(s))
+    assert rc == 0
+    for name, a, b in zip("UVWP", out, ref):
+        np.testing.assert_array_equal(a, b, err_msg=name)
+    assert s.value == pytest.approx(s_ref, rel=1e-12)
