@@ -12,6 +12,7 @@
 
 #include "lcs_internal.h"
 #include "lcs_hsmac_tile.h"
+#include "lcs_hsmac_xface.h"
 #include "lcs_rhs_tile.h"
 
 namespace lcs {
@@ -53,6 +54,8 @@ struct Batch {
     std::vector<BoxRec> tab_h;
     DevVec<BoxRec> tab;
     DevVec<WorkItem> ghost_full, ghost_one, rows, lf_rows, parts, diams, faces[3];
+    DevVec<WorkItem> xf_work, xf_commit;  // x-slab pair: patches, then 256-row blocks
+    bool xface = false;                   // every box runs its x slabs as the pair
     DevVec<int> lf_boxes;            // boxes with linear-power forcing
     std::vector<int> rcf_boxes;      // boxes with rc-energy forcing (forced one by one)
     DevVec<TileItem> rhs_tiles, hs_tiles[4];  // sweep tiles by shape: kBatchSweepShapes
@@ -213,11 +216,17 @@ static int rebuild(Batch &B)
     B.wrap_ok = true;
     for (const Box *b : B.boxes)
         if (b->g.nx < b->g.h || b->g.ny < b->g.h || b->g.nz < b->g.h) B.wrap_ok = false;
-    std::vector<WorkItem> gfull, gone, rows, lf_rows, parts, diams, faces[3];
+    std::vector<WorkItem> gfull, gone, rows, lf_rows, parts, diams, faces[3], xfw, xfc;
     std::vector<int> lf_boxes;
     B.rcf_boxes.clear();
     std::vector<TileItem> rtiles, htiles[4];
     const int domain = B.hs.time_blocking ? 1 : 3;
+    B.xface = n > 0;
+    for (int i = 0; i < n; i++) {
+        Box3 xs[2];
+        hs_faces(B.boxes[i]->g, domain, 2, xs[0], xs[1]);
+        if (!xface_fits(B.boxes[i]->g, xs)) B.xface = false;
+    }
     for (int i = 0; i < n; i++) {
         Box &b = *B.boxes[i];
         const Geom &g = b.g;
@@ -261,12 +270,34 @@ static int rebuild(Batch &B)
         for (int z = 0; z < nz_used; z++)
             for (int y = 0; y < hgy; y++)
                 for (int x = 0; x < hgx; x++) htiles[shape].push_back({i, x, y, z});
-        // face passes of the overlap order
+        // face passes of the overlap order; the x slabs as the pair when every box can
         for (int a = 0; a < 3; a++) {
             Box3 f0, f1;
             hs_faces(g, domain, a, f0, f1);
             r.face_blocks[a] = std::max(1, blocks_of(pass_count(f0) + pass_count(f1)));
+            if (a == 2 && B.xface) continue;
             for (int k = 0; k < r.face_blocks[a]; k++) faces[a].push_back({i, k});
+        }
+        r.xf = nullptr;
+        if (B.xface) {
+            const size_t need = (size_t)kXfVals * xf_rows(g) * b.esize;
+            if (b.xf_bytes < need) {
+                if (b.xf) {
+                    LCS_HIP(hipFree(b.xf));
+                    b.bytes -= b.xf_bytes;
+                }
+                b.xf = nullptr;
+                b.xf_bytes = 0;
+                LCS_HIP(hipMalloc(&b.xf, need));
+                b.xf_bytes = need;
+                b.bytes += need;
+            }
+            r.xf = b.xf;
+            const int pn = b.esize == 8 ? XfPatch<double>::N : XfPatch<float>::N;
+            r.face_blocks[2] = 2 * ((g.ny - 1 + pn - 1) / pn) * ((g.nz - 1 + pn - 1) / pn);
+            for (int k = 0; k < r.face_blocks[2]; k++) xfw.push_back({i, k});
+            const long cb = (xf_rows(g) + 255) / 256;
+            for (long k = 0; k < cb; k++) xfc.push_back({i, (int)k});
         }
         // ghost fills
         auto ghost = [&](int rr, int full, std::vector<WorkItem> &w) {
@@ -324,6 +355,8 @@ static int rebuild(Batch &B)
         if ((rc = B.hs_tiles[q].assign(xcd_order(htiles[q]), B.stream))) return rc;
     for (int a = 0; a < 3; a++)
         if ((rc = B.faces[a].assign(xcd_order(faces[a]), B.stream))) return rc;
+    if ((rc = B.xf_work.assign(xcd_order(xfw), B.stream)) || (rc = B.xf_commit.assign(xcd_order(xfc), B.stream)))
+        return rc;
     LCS_HIP(hipStreamSynchronize(B.stream));  // host vectors above are temporaries
     B.dirty = false;
     return LCS_OK;
@@ -356,7 +389,17 @@ static int b_core_faces(Batch &B, int set, int domain, double dtfac, std::vector
     int rc = b_sweep(B, set, 0, 0, dtfac);
     if (rc) return rc;
     kinds.assign(1, 0);
-    for (int a = 0; a < 3; a++)
+    for (int a = 0; a < 3; a++) {
+        if (a == 2 && B.xface) {
+            const int slot = (int)kinds.size();
+            rc = launch_b_hsmac_xface(B.stream, B.esize, B.hs.coloring, B.tab.d, B.xf_work.d, B.xf_work.n, set, domain,
+                                      slot, dtfac);
+            if (rc || (rc = launch_b_hsmac_xface_commit(B.stream, B.esize, B.tab.d, B.xf_commit.d, B.xf_commit.n, set,
+                                                        domain)))
+                return rc;
+            kinds.push_back(1 + a);
+            continue;
+        }
         for (int c = 0; c < 2; c++) {
             const int slot = (int)kinds.size();
             rc = launch_b_hsmac_faces(B.stream, B.esize, B.hs.coloring, B.tab.d, B.faces[a].d, B.faces[a].n, set,
@@ -364,6 +407,7 @@ static int b_core_faces(Batch &B, int set, int domain, double dtfac, std::vector
             if (rc) return rc;
             kinds.push_back(1 + a);
         }
+    }
     return LCS_OK;
 }
 
@@ -614,7 +658,8 @@ int lcs_batch_destroy(lcs_batch *batch)
     drop_graphs(*B);
     while (!B->boxes.empty()) (void)detach(*B, (int)B->boxes.size() - 1);
     DevVec<WorkItem> *w[] = {&B->ghost_full, &B->ghost_one, &B->rows,     &B->lf_rows,  &B->parts,
-                             &B->diams,      &B->faces[0],  &B->faces[1], &B->faces[2]};
+                             &B->diams,      &B->faces[0],  &B->faces[1], &B->faces[2], &B->xf_work,
+                             &B->xf_commit};
     for (auto *v : w) v->release();
     B->lf_boxes.release();
     B->rhs_tiles.release();

@@ -873,6 +873,12 @@ struct HsmacRun {
         ca.cnt[npass++] = blocks;
         return LCS_OK;
     }
+    // A box alone runs its x slabs as the pair from kXfMinRows rows per slab:
+    // below that a slab has a few patches, whose rows then all go through the
+    // few CUs that run them, and the two colour passes spread over many CUs
+    // are faster (20-25 us a group at 32^3-128^3; 256^3 and up gain).  In a
+    // batch the patches of all boxes fill the device, so every box uses it.
+    static constexpr long kXfMinRows = 1L << 15;
     // both colours over the x- and x+ slabs, through the compact buffer
     int xface(const Box3 &r0, const Box3 &r1)
     {
@@ -1016,7 +1022,7 @@ struct HsmacRun {
         // the same result as (z- and z+ red), (z- and z+ black), (y- ...).
         const bool pair = g.nx >= 2 && g.ny >= 2 && g.nz >= 2;
         for (int q = 0; q < 6; q += 2) {
-            if (q == 4 && xface_fits(g, f + 4)) {
+            if (q == 4 && xface_fits(g, f + 4) && (long)(g.ny - 1) * (g.nz - 1) >= kXfMinRows) {
                 if ((rc = xface(f[4], f[5]))) return rc;
             } else if (pair) {
                 if ((rc = pass(f[q], f[q + 1], 0)) || (rc = pass(f[q], f[q + 1], 1))) return rc;
@@ -2578,6 +2584,100 @@ int launch_b_hsmac_faces(hipStream_t s, int esize, int coloring, const BoxRec *t
             B_FACES(float, LCS_COLORING_RED_BLACK_FAST);
     }
 #undef B_FACES
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+// both colours over the x face slabs of one box per work item (one patch)
+template <typename T, int COLORING>
+__global__ void __launch_bounds__(XfPatch<T>::N *XfPatch<T>::N)
+    k_b_hsmac_xface(const BoxRec *tab, const WorkItem *work, int set, int domain, int slot, double dtfac)
+{
+    using X = XFaceOf<T, COLORING>;
+    extern __shared__ __align__(16) unsigned char xf_lds_raw[];
+    T *lds = reinterpret_cast<T *>(xf_lds_raw);
+    __shared__ double red_lds[16];
+    const WorkItem w = work[blockIdx.x];
+    const BoxRec &B = tab[w.box];
+    const DevScalars *S = B.S;
+    if (S->hs_done) return;
+    const bool alt = S->hs_par != 0;
+    XfArgs<T> a;
+    a.g = B.g;
+    a.U = (T *)(alt ? B.spare[0] : set_array(B, set, 0));
+    a.V = (T *)(alt ? B.spare[1] : set_array(B, set, 1));
+    a.W = (T *)(alt ? B.spare[2] : set_array(B, set, 2));
+    a.P = (T *)(alt ? B.spare[3] : set_array(B, set, 3));
+    hs_faces(a.g, domain, 2, a.R[0], a.R[1]);
+    hs_coef(B, a.c);
+    a.dti = T(1.0 / (dtfac * S->dt));
+    a.buf = (T *)B.xf;
+    typename X::Patch p;
+    X::patch(a, w.blk, p);
+    double acc = 0.0;
+    X::load(a, p, lds, threadIdx.x);
+    __syncthreads();
+    X::red(a, p, lds, threadIdx.x, acc);
+    __syncthreads();
+    X::black(a, p, lds, threadIdx.x, acc);
+    __syncthreads();
+    X::finals(a, p, lds, threadIdx.x);
+    const double s = block_sum(acc, red_lds);
+    if (threadIdx.x == 0) B.part[(long)slot * kMaxPartBlocks + w.blk] = s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_b_hsmac_xface_commit(const BoxRec *tab, const WorkItem *work, int set, int domain)
+{
+    const WorkItem w = work[blockIdx.x];
+    const BoxRec &B = tab[w.box];
+    const DevScalars *S = B.S;
+    if (S->hs_done) return;
+    const bool alt = S->hs_par != 0;
+    T *U = (T *)(alt ? B.spare[0] : set_array(B, set, 0)), *V = (T *)(alt ? B.spare[1] : set_array(B, set, 1));
+    T *W = (T *)(alt ? B.spare[2] : set_array(B, set, 2)), *P = (T *)(alt ? B.spare[3] : set_array(B, set, 3));
+    Box3 r0, r1;
+    hs_faces(B.g, domain, 2, r0, r1);
+    const long r = (long)w.blk * blockDim.x + threadIdx.x;
+    if (r < xf_rows(B.g))
+        XFaceOf<T, LCS_COLORING_RED_BLACK>::commit_row(B.g, r0, r1, U, V, W, P, (const T *)B.xf, r);
+}
+
+template <typename T, int COLORING>
+static int b_xface_t(hipStream_t s, const BoxRec *tab, const WorkItem *work, int nwork, int set, int domain, int slot,
+                     double dtfac)
+{
+    using X = XFaceOf<T, COLORING>;
+    const size_t lds = X::LDS_ELEMS * sizeof(T);
+    int rc = kernel_setup((const void *)k_b_hsmac_xface<T, COLORING>, X::NT, lds, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_b_hsmac_xface<T, COLORING>), dim3(nwork), dim3(X::NT), lds, s, tab, work, set, domain, slot,
+                       dtfac);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+int launch_b_hsmac_xface(hipStream_t s, int esize, int coloring, const BoxRec *tab, const WorkItem *work, int nwork,
+                         int set, int domain, int slot, double dtfac)
+{
+    if (nwork == 0) return LCS_OK;
+    const bool rb = coloring == LCS_COLORING_RED_BLACK;
+    if (esize == 8)
+        return rb ? b_xface_t<double, LCS_COLORING_RED_BLACK>(s, tab, work, nwork, set, domain, slot, dtfac)
+                  : b_xface_t<double, LCS_COLORING_RED_BLACK_FAST>(s, tab, work, nwork, set, domain, slot, dtfac);
+    return rb ? b_xface_t<float, LCS_COLORING_RED_BLACK>(s, tab, work, nwork, set, domain, slot, dtfac)
+              : b_xface_t<float, LCS_COLORING_RED_BLACK_FAST>(s, tab, work, nwork, set, domain, slot, dtfac);
+}
+
+int launch_b_hsmac_xface_commit(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *work, int nwork, int set,
+                                int domain)
+{
+    if (nwork == 0) return LCS_OK;
+    if (esize == 8)
+        hipLaunchKernelGGL(k_b_hsmac_xface_commit<double>, dim3(nwork), dim3(256), 0, s, tab, work, set, domain);
+    else
+        hipLaunchKernelGGL(k_b_hsmac_xface_commit<float>, dim3(nwork), dim3(256), 0, s, tab, work, set, domain);
     LCS_HIP(hipGetLastError());
     return LCS_OK;
 }

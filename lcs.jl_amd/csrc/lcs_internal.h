@@ -270,6 +270,7 @@ struct BoxRec {
     int rhs_zc, hs_zc, hs_tx, hs_ty;
     int hs_tiles;        // workgroups of one fused sweep
     int face_blocks[3];  // blocks of one paired face pass, axes z, y, x
+    void *xf;            // compact buffer of the x-slab pair (Box::xf), when the batch uses it
     int row_blocks;      // blocks of the row-walking reductions (CFL, forcing)
     int lf_force;        // linear-power forcing: applied by the first stage-2 sweep
     int nprof;
@@ -311,6 +312,13 @@ int launch_b_rhs(hipStream_t s, int esize, const BoxRec *tab, const TileItem *ti
 int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, int tx, int ty, const BoxRec *tab,
                          const TileItem *tiles, int ntiles, int set, int kind, int slot, double dtfac,
                          bool force = false, bool wrap = false);
+// both colours over the x face slabs of every box (lcs_hsmac_xface.h): one
+// work item per patch, partials of a box in part[slot][patch]; then one work
+// item per 256 slab rows for the commit
+int launch_b_hsmac_xface(hipStream_t s, int esize, int coloring, const BoxRec *tab, const WorkItem *work, int nwork,
+                         int set, int domain, int slot, double dtfac);
+int launch_b_hsmac_xface_commit(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *work, int nwork, int set,
+                                int domain);
 int launch_b_hsmac_faces(hipStream_t s, int esize, int coloring, const BoxRec *tab, const WorkItem *work,
                          int nwork, int set, int domain, int axis, int color, int slot, double dtfac);
 // kinds of the passes since the last check: 0 = fused sweep, 1..3 = face pass of axis z, y, x

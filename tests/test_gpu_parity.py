@@ -154,6 +154,28 @@ def test_hsmac_multi_tile_non_cubic(rng, tb, ov, coloring, dtype):
         assert rel_l2(interior(box.download(name), h), interior(e, h)) < tol(dtype)
 
 
+@pytest.mark.parametrize("tb", [True, False])
+@pytest.mark.parametrize("coloring", ["red-black-fast", "red-black"])
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_hsmac_x_slab_pair(rng, tb, coloring, dtype):
+    """A box alone with at least 2^15 rows per x face slab runs both colours
+    over its x slabs as one patch launch plus a commit (overlap order,
+    halo.jl:349-366): 189 x 179 slab rows make ragged 32- and 16-row
+    patches."""
+    dims, h, dt = (6, 190, 180), 3, 0.02
+    L = lcs()
+    A = [rand_halo(rng, dims, h, dtype) for _ in range(4)]
+    box = make_box(dims, h, dtype)
+    upload_all(box, {"U": A[0], "V": A[1], "W": A[2], "P": A[3]})
+    hs = L.HSMAC(tb, True, coloring, 1.7, 1e-3, 4, 4)
+    res = L.couple(box, hs, 2, dt)
+    itr, div = O.hsmac(*A, h, dt, tb, True, 0 if coloring == "red-black-fast" else 1, 1.7, 1e-3, 4, 4)
+    assert res["itr"] == itr
+    assert abs(res["div"] - div) <= tol(dtype) * 10 * div
+    for name, e in zip(("U", "V", "W", "P"), A):
+        assert rel_l2(interior(box.download(name), h), interior(e, h)) < tol(dtype)
+
+
 @pytest.mark.parametrize("tb,ov", HSMAC_CASES)
 @pytest.mark.parametrize("coloring", ["red-black-fast", "red-black"])
 def test_hsmac_converging_smooth_field(tb, ov, coloring):
