@@ -1,8 +1,10 @@
-// Both colour passes over the two x face slabs of the overlap order, in two
-// launches instead of one per colour.  The slabs are three cells (d1) or one
-// cell (d3) thick, so a colour pass touches a few elements of every array row
-// it visits and moves whole memory segments for them; here each segment is
-// read once and written once.
+// Both colour passes over the two x face slabs of the overlap order, as one
+// launch over patches of slab rows plus a small commit launch, instead of one
+// launch per colour.  The slabs are three cells (d1) or one cell (d3) thick,
+// so a colour pass touches a few elements of every array row it visits and
+// moves whole memory segments for them, and two passes move them twice.  Here
+// a row's segments are read about once (patch windows overlap by two rows)
+// and written once.
 //
 // Semantics (lib/topologies/src/halo.jl:349-366, src/flows/couples/hsmac.jl:
 // 1-35, lib/parallel/src/foraxes.jl:73-107): over slab R, every visited red
