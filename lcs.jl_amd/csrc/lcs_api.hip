@@ -908,7 +908,7 @@ int lcs_step(lcs_box *box, const lcs_step_config *cfg, lcs_step_log *log)
     case LCS_FORCING_LINEAR_POWER:
         // forcing! then couple!: the first HSMAC sweep applies the forcing as
         // it loads the velocity, saving a read and write of U, V, W
-        if ((rc = lf_defer ? launch_force_linear_coef(b, cfg->forcing.power)
+        if ((rc = lf_defer ? launch_force_linear_coef(b, cfg->forcing.power, hsmac_group_wraps(b.g, cfg->hsmac))
                            : launch_force_linear(b, cfg->forcing.power, dt2)))
             return rc;
         break;

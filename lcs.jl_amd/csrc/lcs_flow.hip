@@ -1151,6 +1151,11 @@ struct HsmacRun {
     }
 };
 
+bool hsmac_group_wraps(const Geom &g, const lcs_hsmac_params &p)
+{
+    return p.time_blocking && g.nx >= g.h && g.ny >= g.h && g.nz >= g.h;
+}
+
 template <typename T>
 static int hsmac_t(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int slot, bool lf_deferred)
 {
@@ -1204,7 +1209,7 @@ static int hsmac_t(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int
         const Box3 d2 = {{0, 0, 0}, {g.nx + 2, g.ny + 2, g.nz + 2}};
         // The group's FullHalo fill of U, V, W folds into the loads of its
         // first sweep (HsArgs::wrap) where every ghost has one image.
-        const bool wrap = g.nx >= g.h && g.ny >= g.h && g.nz >= g.h;
+        const bool wrap = hsmac_group_wraps(g, prm);
         for (int grp = 0; 3 * grp < prm.itrmax && !done; grp++) {
             if (wrap)
                 run.wrap_next = true;
@@ -1649,10 +1654,10 @@ int launch_flow_stat(Box &b, int integral_length, double *out_dev)
 }
 
 // The means and A of the linear forcing into S->lf (lf.jl:40-66).
-int launch_force_linear_coef(Box &b, double power)
+int launch_force_linear_coef(Box &b, double power, bool face_only)
 {
     void *arr[3] = {b.f[0], b.f[1], b.f[2]};
-    int rc = launch_ghost_fill(b, arr, 3, b.g.h, 1, nullptr);
+    int rc = face_only ? launch_ghost_fill(b, arr, 3, 1, 0, nullptr) : launch_ghost_fill(b, arr, 3, b.g.h, 1, nullptr);
     if (rc) return rc;
     int blocks = grid_rows(b.g);
     double len = (double)b.g.cells();

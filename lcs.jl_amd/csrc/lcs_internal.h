@@ -213,7 +213,13 @@ int launch_cfl(Box &b, double cfl, double dtmax, bool fill = true);  // writes S
 // (the same values k_lf_apply writes), which also sets S->p.
 int launch_hsmac(Box &b, void **Us, DtRef dt, const lcs_hsmac_params &prm, int log_slot, bool lf_deferred = false);
 int launch_force_linear(Box &b, double power, DtRef dt);  // writes S->lf, S->p
-int launch_force_linear_coef(Box &b, double power);       // writes S->lf only
+// writes S->lf only; face_only: the means read the x = 0, y = 0, z = 0
+// ghosts alone, so a width-1 FaceHalo fill of U, V, W serves them where the
+// next sweep loads every other ghost from its periodic image
+int launch_force_linear_coef(Box &b, double power, bool face_only = false);
+// a time-blocked HSMAC group's first sweep loads ghost U, V, W from their
+// periodic images instead of a FullHalo fill (every ghost has one image)
+bool hsmac_group_wraps(const Geom &g, const lcs_hsmac_params &p);
 int launch_boxmean(Box &b, int c, double *coarse_dev);
 // Flows.stat raw values into out_dev[15] (device): U, V, W mean/max/min, means of
 // u2, s2, dudx^2..4, then the integral length when asked
