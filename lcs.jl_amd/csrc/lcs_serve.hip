@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "lcs_internal.h"
+#include "lcs_hsmac_xface.h"
 
 namespace lcs {
 
@@ -44,13 +45,14 @@ struct Server {
 };
 
 // Device bytes a request will hold: 11 halo arrays (Us, Us2, P and the four
-// HSMAC partners), the reduction partials, and per particle 17 properties plus
-// staging, map and sort buffers.
+// HSMAC partners), the compact buffer of the x-slab HSMAC pair, the reduction
+// partials, and per particle 17 properties plus staging, map and sort buffers.
 static size_t footprint(const lcs_request &r)
 {
     const size_t es = r.dtype == LCS_F64 ? 8 : 4;
     const size_t m = (size_t)(r.n + 6);
     size_t bytes = 11 * m * m * m * es + sizeof(double) * (size_t)kPartSlots + (1u << 20);
+    bytes += (size_t)kXfVals * 2 * (size_t)(r.n - 1) * (size_t)(r.n - 1) * es;
     const size_t np = (size_t)std::max<int64_t>(r.particles, 0);
     bytes += np * (16 * es + 8 + 8 + es + 8 + 4 + 16 + 8);
     return bytes;
