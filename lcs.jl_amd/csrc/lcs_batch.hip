@@ -283,6 +283,7 @@ static int rebuild(Batch &B)
             const size_t need = (size_t)kXfVals * xf_rows(g) * b.esize;
             if (b.xf_bytes < need) {
                 if (b.xf) {
+                    LCS_HIP(hipStreamSynchronize(b.stream));
                     LCS_HIP(hipFree(b.xf));
                     b.bytes -= b.xf_bytes;
                 }
