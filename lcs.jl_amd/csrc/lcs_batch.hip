@@ -554,7 +554,11 @@ static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
             (r = launch_b_rhs(B.stream, B.esize, B.tab.d, B.rhs_tiles.d, B.rhs_tiles.n, 2)))
             return r;
         if (B.lf_boxes.n > 0) {
-            if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 0, 3, h, 1, 0)) ||
+            // the means read the x = 0, y = 0, z = 0 ghosts alone; where the
+            // first sweep loads every ghost from its image, width 1 serves them
+            const bool one = lf_defer && B.hs.time_blocking && B.wrap_ok;
+            const DevVec<WorkItem> &gw = one ? B.ghost_one : B.ghost_full;
+            if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, gw.d, gw.n, 0, 3, one ? 1 : h, 1, 0)) ||
                 (r = launch_b_lf(B.stream, B.esize, B.tab.d, B.lf_rows.d, B.lf_rows.n, B.lf_boxes.d, B.lf_boxes.n,
                                  !lf_defer)))
                 return r;
