@@ -223,9 +223,12 @@ static int rebuild(Batch &B)
     const int domain = B.hs.time_blocking ? 1 : 3;
     B.xface = n > 0;
     for (int i = 0; i < n; i++) {
+        const Geom &g = B.boxes[i]->g;
         Box3 xs[2];
-        hs_faces(B.boxes[i]->g, domain, 2, xs[0], xs[1]);
-        if (!xface_fits(B.boxes[i]->g, xs)) B.xface = false;
+        hs_faces(g, domain, 2, xs[0], xs[1]);
+        const long pn = B.boxes[i]->esize == 8 ? XfPatch<double>::N : XfPatch<float>::N;
+        const long patches = 2 * ((g.ny - 1 + pn - 1) / pn) * ((g.nz - 1 + pn - 1) / pn);
+        if (!xface_fits(g, xs) || patches > kMaxPartBlocks) B.xface = false;  // one partial per patch
     }
     for (int i = 0; i < n; i++) {
         Box &b = *B.boxes[i];
