@@ -330,8 +330,8 @@ def main():
                          "for scaling hardware traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the ensemble and single-box "
-              f"paths shard by box across GPUs, which is not part of this build", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only (asked for {args.gpus}); ensembles would shard by box across "
+              f"GPUs and a single box by slabs with halo exchange, neither of which is implemented", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "c4":

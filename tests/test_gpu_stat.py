@@ -6,7 +6,7 @@ src/flows/stat.jl:53-182).
   included, on a non-cubic box;
 * one sine mode uploaded to the device gives the closed forms (flatness 3/2,
   skewness 0, k = A^2/4, integral length from (n/2) cos(theta r));
-* long runs: a 32^3 box stepped 1000 times (t = 10, forced
+* long runs (kinetic energy and dissipation within 0.5 %): a 32^3 box stepped 1000 times (t = 10, forced
   to a stationary state) keeps kinetic energy and dissipation within 0.5 % of
   the FP64 oracle's at every 100th step, for FP64 and for FP32 boxes.
 """

@@ -2,7 +2,7 @@
 step) against the oracle's phase-by-phase restatement, from identical initial
 states (the device initialises; the oracle starts from the downloaded copy).
 
-Bar (BASELINE north star): after 100 steps at 64^3 in FP64, U, V, W, P agree
+Bar: after 100 steps at 64^3 in FP64, U, V, W, P agree
 within relative L2 1e-5; HSMAC sweep counts agree step by step; particle cell
 indices agree bit-exactly for particles whose positions agree.
 """
