@@ -438,7 +438,7 @@ static int time_shapes(Box &t, const TileShapes &s, int coloring, float &sweep_m
     for (auto &x : e) LCS_HIP(hipEventCreate(&x));
     int rc = LCS_OK;
     sweep_ms = rhs_ms = 1e30f;
-    for (int rep = 0; rep < 4 && rc == LCS_OK; rep++) {  // round 0 sets kernel attributes; best of the rest
+    for (int rep = 0; rep < 4 && rc == LCS_OK; rep++) {  // rep 0 sets kernel attributes; best of the rest
         if (hipEventRecord(e[0], t.stream) != hipSuccess) rc = fail(LCS_ERR_HIP, "hipEventRecord failed");
         if (rc == LCS_OK) rc = launch_hsmac(t, &t.f[LCS_FIELD_U], DtRef{&t.S->dt_phase, 1.0}, p, -1);
         if (rc == LCS_OK && hipEventRecord(e[1], t.stream) != hipSuccess) rc = fail(LCS_ERR_HIP, "hipEventRecord failed");
