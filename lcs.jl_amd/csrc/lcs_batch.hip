@@ -283,7 +283,9 @@ static int rebuild(Batch &B)
         }
         r.xf = nullptr;
         if (B.xface) {
-            const size_t need = (size_t)kXfVals * xf_rows(g) * b.esize;
+            Box3 xs[2];
+            hs_faces(g, domain, 2, xs[0], xs[1]);
+            const size_t need = (size_t)kXfVals * SlabAxes<0>::rows(xs[0]) * b.esize;
             if (b.xf_bytes < need) {
                 if (b.xf) {
                     LCS_HIP(hipStreamSynchronize(b.stream));
@@ -300,7 +302,7 @@ static int rebuild(Batch &B)
             const int pn = b.esize == 8 ? XfPatch<double>::N : XfPatch<float>::N;
             r.face_blocks[2] = 2 * ((g.ny - 1 + pn - 1) / pn) * ((g.nz - 1 + pn - 1) / pn);
             for (int k = 0; k < r.face_blocks[2]; k++) xfw.push_back({i, k});
-            const long cb = (xf_rows(g) + 255) / 256;
+            const long cb = (SlabAxes<0>::rows(xs[0]) + 255) / 256;
             for (long k = 0; k < cb; k++) xfc.push_back({i, (int)k});
         }
         // ghost fills

@@ -604,7 +604,7 @@ __global__ void __launch_bounds__(XfPatch<T>::N *XfPatch<T>::N)
     a.dti = T(1.0 / (dt.factor * (*dt.src)));
     a.buf = buf;
     double acc = 0.0;
-    const int nblk = X::blocks(g);
+    const int nblk = X::blocks(r0);
     for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
         typename X::Patch p;
         X::patch(a, blk, p);
@@ -630,7 +630,7 @@ __global__ void __launch_bounds__(256) k_hsmac_xface_commit(Geom g, BufSets<T> b
     const bool alt = S->hs_par != 0;
     T *U = alt ? bs.b[0] : bs.a[0], *V = alt ? bs.b[1] : bs.a[1];
     T *W = alt ? bs.b[2] : bs.a[2], *P = alt ? bs.b[3] : bs.a[3];
-    const long rows = xf_rows(g);
+    const long rows = SlabAxes<0>::rows(r0);
     for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (long)gridDim.x * blockDim.x)
         XFaceOf<T, LCS_COLORING_RED_BLACK>::commit_row(g, r0, r1, U, V, W, P, buf, r);
 }
@@ -884,7 +884,7 @@ struct HsmacRun {
     {
         using X = XFaceOf<T, LCS_COLORING_RED_BLACK>;
         const Geom &g = b.g;
-        const size_t need = (size_t)kXfVals * xf_rows(g) * sizeof(T);
+        const size_t need = (size_t)kXfVals * SlabAxes<0>::rows(r0) * sizeof(T);
         if (b.xf_bytes < need) {
             if (b.xf) {
                 LCS_HIP(hipStreamSynchronize(b.stream));
@@ -897,7 +897,7 @@ struct HsmacRun {
             b.xf_bytes = need;
             b.bytes += need;
         }
-        const int blocks = std::min(X::blocks(g), kMaxPartBlocks);
+        const int blocks = std::min(X::blocks(r0), kMaxPartBlocks);
         double *part = b.part + (long)npass * kMaxPartBlocks;
         const double cells = (double)(r0.hi[0] - r0.lo[0] + r1.hi[0] - r1.lo[0] + 2) * (g.ny - 1) * (g.nz - 1);
         const size_t lds = X::LDS_ELEMS * sizeof(T);
@@ -919,7 +919,7 @@ struct HsmacRun {
         }
         ca.cnt[npass++] = blocks;
         ProfScope ps(b, "hsmac_xface_commit", 4.0 * sizeof(T) * cells);
-        hipLaunchKernelGGL(k_hsmac_xface_commit<T>, dim3(grid_for(xf_rows(g), 256, 8192)), dim3(256), 0, b.stream, g,
+        hipLaunchKernelGGL(k_hsmac_xface_commit<T>, dim3(grid_for(SlabAxes<0>::rows(r0), 256, 8192)), dim3(256), 0, b.stream, g,
                            bs, r0, r1, b.S, (const T *)b.xf);
         LCS_HIP(hipGetLastError());
         return LCS_OK;
@@ -2653,7 +2653,7 @@ __global__ void __launch_bounds__(256)
     Box3 r0, r1;
     hs_faces(B.g, domain, 2, r0, r1);
     const long r = (long)w.blk * blockDim.x + threadIdx.x;
-    if (r < xf_rows(B.g))
+    if (r < SlabAxes<0>::rows(r0))
         XFaceOf<T, LCS_COLORING_RED_BLACK>::commit_row(B.g, r0, r1, U, V, W, P, (const T *)B.xf, r);
 }
 

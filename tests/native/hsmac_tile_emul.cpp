@@ -132,11 +132,11 @@ extern "C" int hsmac_sweep_forced_emul(int ty, int tx, int esize, int coloring, 
 // ---------------------------------------------------------------------------
 #include "lcs_hsmac_xface.h"
 
-template <typename T, int COLORING, int PY>
+template <typename T, int COLORING, int PY, int TI = 0>
 static int xface_t(int nx, int ny, int nz, int h, const int *lo, const int *hi, double dt, double omega,
                    void *const *f, double *div2)
 {
-    using X = XFace<T, COLORING, PY, PY>;
+    using X = XFace<T, COLORING, PY, PY, TI>;
     XfArgs<T> a;
     a.g = make_geom(nx, ny, nz, h);
     T *F[4];
@@ -150,7 +150,7 @@ static int xface_t(int nx, int ny, int nz, int h, const int *lo, const int *hi, 
             a.R[s].lo[q] = lo[3 * s + q];
             a.R[s].hi[q] = hi[3 * s + q];
         }
-    if (!xface_fits(a.g, a.R)) return -1;
+    if (!SlabAxes<TI>::fits(a.g, a.R)) return -1;
     const T dxi = T(1.0 / spacing(nx)), dyi = T(1.0 / spacing(ny)), dzi = T(1.0 / spacing(nz));
     a.c.dxi = dxi;
     a.c.dyi = dyi;
@@ -158,12 +158,12 @@ static int xface_t(int nx, int ny, int nz, int h, const int *lo, const int *hi, 
     a.c.dl = T(1) / (T(2) * ((dxi * dxi + dyi * dyi) + dzi * dzi));
     a.c.omega = T(omega);
     a.dti = T(1.0 / dt);
-    const long rows = xf_rows(a.g);
+    const long rows = SlabAxes<TI>::rows(a.R[0]);
     std::vector<T> buf(kXfVals * rows);
     a.buf = buf.data();
     std::vector<T> lds(X::LDS_ELEMS);
     double s = 0.0;
-    for (int blk = 0; blk < X::blocks(a.g); blk++) s += X::emulate_block(a, lds.data(), blk);
+    for (int blk = 0; blk < X::blocks(a.R[0]); blk++) s += X::emulate_block(a, lds.data(), blk);
     for (long r = 0; r < rows; r++) X::commit_row(a.g, a.R[0], a.R[1], F[0], F[1], F[2], F[3], (const T *)buf.data(), r);
     *div2 = s;
     return 0;
@@ -186,5 +186,31 @@ extern "C" int hsmac_xface_emul(int patch, int esize, int coloring, int nx, int 
         else { BY_PATCH(double, LCS_COLORING_RED_BLACK_FAST) }
     }
 #undef BY_PATCH
+#undef RUN
+}
+
+// the slab pair across axis ti (0 x, 1 y, 2 z) with patch edge 16 or 4
+extern "C" int hsmac_slab_pair_emul(int ti, int patch, int esize, int coloring, int nx, int ny, int nz, int h,
+                                    const int *lo, const int *hi, double dt, double omega, void *const *f,
+                                    double *div2)
+{
+#define RUN(T, C, Y, A) return xface_t<T, C, Y, A>(nx, ny, nz, h, lo, hi, dt, omega, f, div2)
+#define BY_AXIS(T, C, Y)          \
+    if (ti == 0) RUN(T, C, Y, 0); \
+    if (ti == 1) RUN(T, C, Y, 1); \
+    if (ti == 2) RUN(T, C, Y, 2);
+#define BY_PATCH(T, C)                      \
+    if (patch == 16) { BY_AXIS(T, C, 16) }  \
+    if (patch == 4) { BY_AXIS(T, C, 4) }    \
+    return -2;
+    if (esize == 4) {
+        if (coloring == LCS_COLORING_RED_BLACK) { BY_PATCH(float, LCS_COLORING_RED_BLACK) }
+        else { BY_PATCH(float, LCS_COLORING_RED_BLACK_FAST) }
+    } else {
+        if (coloring == LCS_COLORING_RED_BLACK) { BY_PATCH(double, LCS_COLORING_RED_BLACK) }
+        else { BY_PATCH(double, LCS_COLORING_RED_BLACK_FAST) }
+    }
+#undef BY_PATCH
+#undef BY_AXIS
 #undef RUN
 }

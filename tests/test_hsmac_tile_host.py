@@ -216,3 +216,54 @@ def test_x_slab_pair_equals_four_colour_passes(emul, dims, patch, box, coloring,
     for name, a, b in zip("UVWP", out, ref):
         np.testing.assert_array_equal(a, b, err_msg=name)
     assert s.value == pytest.approx(s_ref, rel=1e-12)
+
+
+def _slabs(ti, box, n):
+    """The two face slabs across axis ti (0 x, 1 y, 2 z) of region `box` in
+    the overlap order (lib/topologies/src/halo.jl:349-366)."""
+    lo, hi = _box(box, n)
+    nx, ny, nz = n
+    if ti == 0:
+        return _x_slabs(box, n)
+    if ti == 1:
+        return (np.array([lo[0], lo[1], 2, lo[0], ny + 1, 2], dtype=np.int32),
+                np.array([hi[0], 1, nz, hi[0], hi[1], nz], dtype=np.int32))
+    return (np.array([lo[0], lo[1], lo[2], lo[0], lo[1], nz + 1], dtype=np.int32),
+            np.array([hi[0], hi[1], 1, hi[0], hi[1], hi[2]], dtype=np.int32))
+
+
+YZCASES = [(1, (16, 12, 20), 16, "d1"), (1, (9, 13, 10), 4, "d2"), (1, (11, 6, 14), 4, "d3"),
+           (1, (20, 18, 17), 16, "d3"), (2, (16, 12, 20), 16, "d1"), (2, (9, 13, 10), 4, "d2"),
+           (2, (11, 6, 14), 4, "d3"), (2, (20, 18, 17), 16, "d1"), (1, (2, 5, 7), 4, "d1"),
+           (2, (5, 3, 2), 4, "d1"), (0, (16, 12, 20), 4, "d2")]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("coloring", [0, 1])
+@pytest.mark.parametrize("ti,dims,patch,box", YZCASES)
+def test_y_and_z_slab_pairs_equal_four_colour_passes(emul, ti, dims, patch, box, coloring, dtype):
+    """The same patch pair across y and z: rows (x, z) and (x, y) that reach
+    into the ghosts; equal bit for bit to the reference's lower red, lower
+    black, upper red, upper black passes, and the div^2 sum to the oracle's."""
+    h, dt, omega = 3, 0.01, 1.7
+    rng = np.random.default_rng(abs(hash((ti, dims, patch, box, coloring))) % 2**32)
+    src = [np.asfortranarray(rng.standard_normal(tuple(n + 2 * h for n in dims)).astype(dtype)) for _ in range(4)]
+    lo, hi = _slabs(ti, box, dims)
+    ref = [a.copy(order="F") for a in src]
+    div2 = np.zeros_like(ref[0])
+    for s in range(2):
+        for color in range(2):
+            O.hsmac_pass(*ref, div2, h, lo[3 * s:3 * s + 3].copy(), hi[3 * s:3 * s + 3].copy(), color, dt, omega,
+                         coloring)
+    nx, ny, nz = dims
+    s_ref = div2[h:h + nx + 1, h:h + ny + 1, h:h + nz + 1].astype(np.float64).sum()
+    out = [a.copy(order="F") for a in src]
+    fp = (C.c_void_p * 4)(*[a.ctypes.data for a in out])
+    s = C.c_double(0)
+    rc = emul.hsmac_slab_pair_emul(ti, patch, np.dtype(dtype).itemsize, coloring, nx, ny, nz, h,
+                                   lo.ctypes.data_as(C.c_void_p), hi.ctypes.data_as(C.c_void_p), C.c_double(dt),
+                                   C.c_double(omega), fp, C.byref(s))
+    assert rc == 0
+    for name, a, b in zip("UVWP", out, ref):
+        np.testing.assert_array_equal(a, b, err_msg=name)
+    assert s.value == pytest.approx(s_ref, rel=1e-12)
