@@ -489,10 +489,18 @@ static int b_hsmac(Batch &B, int set, double dtfac, int slot, int seg_id)
         return r;
     });
     if (rc) return rc;
+    // The deferred linear forcing belongs to the first sweep that runs.  A
+    // replayed segment does not pass through b_sweep, so the host flag is
+    // cleared here for captured and eager runs alike.  If no group precedes
+    // the first poll (small itrmin), the forced group is a segment of its
+    // own: the group segment that later polls replay must never force.
+    if (pre > 0) B.force_next = false;
     bool done = false, polled_last = false;
     for (int g = pre; g < ngroups && !done; g++) {
-        if ((rc = segment(B, seg_id + 1, [&]() { return b_group(B, set, dtfac); }))) return rc;
-        if ((rc = b_poll(B, done))) return rc;
+        const int id = B.force_next ? seg_id + 2 : seg_id + 1;
+        rc = segment(B, id, [&]() { return b_group(B, set, dtfac); });
+        B.force_next = false;
+        if (rc || (rc = b_poll(B, done))) return rc;
         polled_last = true;
     }
     if ((rc = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsEnd, slot, nullptr))) return rc;

@@ -18,7 +18,9 @@ from test_gpu_step import c1_config, oracle_from_device
 pytestmark = pytest.mark.gpu
 
 
-def mixed_configs(dtype="f64"):
+def mixed_configs(dtype="f64", hsmac=None):
+    import dataclasses
+
     from lcs_mi355x import config as C
     cfgs = []
     for dims, npart, forcing, profiles in [((24, 24, 24), 500, C.ConstantPowerLF(0.5), 1),
@@ -26,6 +28,8 @@ def mixed_configs(dtype="f64"):
                                            ((16, 16, 16), 0, C.ConstantPowerLF(0.7), 0),
                                            ((24, 24, 24), 400, C.EnergyPreserveRCF(0.649, 1, 2, 6), 1)]:
         cfg = c1_config(dims=dims, nparticles=max(npart, 1), forcing=forcing, dtype=dtype, profiles=profiles)
+        if hsmac is not None:
+            cfg = dataclasses.replace(cfg, flow=dataclasses.replace(cfg.flow, couple=hsmac))
         cfgs.append(cfg)
     return cfgs
 
@@ -101,13 +105,17 @@ def test_batch_membership_changes():
     batch.close()
 
 
-@pytest.mark.parametrize("dtype", ["f32", "f64"])
-def test_graph_replays_match_eager_steps(dtype):
+@pytest.mark.parametrize("dtype,tb,itrmin", [("f32", True, 6), ("f64", True, 6), ("f64", True, 2), ("f32", True, 0),
+                                             ("f64", False, 0), ("f64", False, 3)])
+def test_graph_replays_match_eager_steps(dtype, tb, itrmin):
     """Every replayed step equals the eager step bit for bit, with eager work
     (downloads, a membership change that drops and re-captures the graphs)
-    between replays."""
+    between replays.  Small itrmin puts the group that applies the deferred
+    linear forcing right before the first convergence poll; that group must
+    not be the one the later polled groups replay."""
     import lcs_mi355x as L
-    cfgs = mixed_configs(dtype)
+    from lcs_mi355x import config as C
+    cfgs = mixed_configs(dtype, C.HSMAC(tb, True, "red-black-fast", 1.7, 1e-3, itrmin, 300))
     ga = [L.build(c) for c in cfgs]
     ea = [L.build(c) for c in cfgs]
     bg, be = L.Batch(), L.Batch()
