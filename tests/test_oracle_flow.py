@@ -497,6 +497,98 @@ def test_rcf_rescales_shell_energy_to_target(rng):
         assert np.isclose((1 + r) ** 2 * e, en0, rtol=1e-10)
 
 
+def _rcf_spectrum_numpy(coarse3, kmin, kmax, e10):
+    """rcf.jl:45-67 with FFTW's rfft replaced by numpy's rfftn (same sign
+    convention, exp(-2 pi i k x / n), unnormalised), then FourierTools'
+    rfftcoeff normalisation (FourierTools.jl:30-41), enespe (:103-120) and
+    crop (:142-157), visiting wavevectors in the reference's loop order so
+    that aliased entries (kmax = c/2) are scaled as often as it scales them."""
+    c = coarse3[0].shape[0]
+    # axes=(2, 1, 0): the real transform halves axis 0 (x), as FFTW's rfft of a
+    # column-major array does; the result is indexed [kx, ky, kz]
+    hats = [np.fft.rfftn(np.asarray(A, dtype=np.float64), axes=(2, 1, 0)) for A in coarse3]
+    norm = (2 * (c // 2 + 1) - 2) * c * c
+    fi = lambda k, n: k if k >= 0 else n + k  # noqa: E731  (fftindex, 0-based)
+    ens = []
+    for k0 in range(kmin, kmax + 1):
+        lo, hi = (k0 - 0.5) ** 2, (k0 + 0.5) ** 2
+        e = 0.0
+        for kz in range(-k0, k0 + 1):
+            for ky in range(-k0, k0 + 1):
+                for kx in range(0, k0 + 1):
+                    if lo <= kx * kx + ky * ky + kz * kz < hi:
+                        w = 1 if kx == 0 else 2
+                        e += w * sum(abs(H[kx, fi(ky, c), fi(kz, c)] / norm) ** 2 for H in hats)
+        en = e / 2
+        ens.append(en)
+        r = np.sqrt(O.enespe0(k0, e10) / max(en, 1e-8)) - 1.0
+        for kz in range(-kmax, kmax + 1):
+            for ky in range(-kmax, kmax + 1):
+                for kx in range(0, kmax + 1):
+                    if lo <= kx * kx + ky * ky + kz * kz < hi:
+                        for H in hats:
+                            H[kx, fi(ky, c), fi(kz, c)] *= r
+    nk = 2 * kmax + 1
+    crop = np.zeros((3, nk, nk, kmax + 1), dtype=complex)  # [a, kz, ky, kx]
+    for kz in range(-kmax, kmax + 1):
+        for ky in range(-kmax, kmax + 1):
+            for kx in range(0, kmax + 1):
+                for a, H in enumerate(hats):
+                    crop[a, fi(kz, nk), fi(ky, nk), kx] = H[kx, fi(ky, c), fi(kz, c)] / norm
+    return crop, np.array(ens)
+
+
+@pytest.mark.parametrize("c,kmin,kmax,e10", [(8, 1, 3, 0.649), (8, 1, 4, 0.649), (6, 1, 2, 3.0), (10, 2, 4, 0.1)])
+def test_rcf_forward_dft_matches_numpy_rfftn(c, kmin, kmax, e10):
+    """Pins the oracle's forward real DFT (its stand-in for FFTW's rfft,
+    rcf.jl:45-47) against numpy's rfftn on asymmetric random coarse grids, one
+    different grid per component: every cropped coefficient kx >= 0 (all of
+    them when kmax = c/2, aliased entries included), re and im separately, and
+    the shell energies en.  A conjugated transform, a swapped axis or a wrong
+    normalisation changes the imaginary parts or their placement and fails."""
+    rng = np.random.default_rng(1000 + c + 10 * kmax)
+    coarse = [np.asfortranarray(rng.standard_normal((c, c, c)) + 0.3 * a) for a in range(3)]
+    crop, en = O.rcf_spectrum(coarse, kmin, kmax, e10)
+    nk = 2 * kmax + 1
+    got = (crop[0::2] + 1j * crop[1::2]).reshape(3, nk, nk, kmax + 1)
+    want, en_want = _rcf_spectrum_numpy(coarse, kmin, kmax, e10)
+    scale = np.abs(want).max()
+    assert np.abs(want.imag).max() > 0.1 * scale  # the imaginary parts carry the sign
+    np.testing.assert_allclose(got.real, want.real, rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(got.imag, want.imag, rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(en, en_want, rtol=1e-12)
+    # the same fields in FP32: the DFT runs in double on the float32 values
+    c32 = [np.asfortranarray(A.astype(np.float32)) for A in coarse]
+    crop32, _ = O.rcf_spectrum(c32, kmin, kmax, e10)
+    want32, _ = _rcf_spectrum_numpy(c32, kmin, kmax, e10)
+    got32 = (crop32[0::2] + 1j * crop32[1::2]).reshape(3, nk, nk, kmax + 1)
+    np.testing.assert_allclose(got32, want32, rtol=0, atol=1e-12 * np.abs(want32).max())
+
+
+@pytest.mark.parametrize("e10,sign", [(50.0, 1), (1e-4, -1)])
+def test_rcf_forcing_moves_shell_energy_toward_target(e10, sign):
+    """End-to-end sign check of forcing!(::EnergyPreserveRCF) (rcf.jl:21-100).
+    The source is r times the shell-filtered velocity, so the power p has the
+    sign of r (r > 0 when the shells hold less energy than en0), and the box
+    mean of U + src moves every forced shell's energy toward en0 without
+    overshooting (box averaging attenuates each mode by less than 1).  A
+    mirrored source (a conjugated forward or inverse transform) gives p a
+    random sign and, for r < 0, raises the energy on average instead."""
+    rng = np.random.default_rng(7)
+    n, h, c, kmin, kmax, dt = 16, 3, 8, 1, 2, 0.01
+    Us = [np.asfortranarray(rng.standard_normal((n + 2 * h,) * 3)) for _ in range(3)]
+    before = O.rcf_spectrum([O.boxmean(A, h, (c,) * 3) for A in Us], kmin, kmax, e10)[1]
+    en0 = np.array([O.enespe0(k, e10) for k in range(kmin, kmax + 1)])
+    assert np.all(np.sign(en0 - before) == sign)
+    p, e10_out = O.force_rcf(*Us, h, e10, kmin, kmax, c, dt)
+    assert e10_out == e10
+    assert np.sign(p) == sign
+    after = O.rcf_spectrum([O.boxmean(A, h, (c,) * 3) for A in Us], kmin, kmax, e10)[1]
+    for b, a, t in zip(before, after, en0):
+        assert abs(a - t) < 0.5 * abs(b - t), (b, a, t)
+        assert min(b, t) <= a <= max(b, t), (b, a, t)
+
+
 def test_rcf_rejects_kmax_beyond_nyquist():
     c = 4
     z = [np.zeros((c, c, c), order="F")] * 3
