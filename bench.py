@@ -28,34 +28,11 @@ METRIC = "cell-steps/s at 1024³ (%HBM peak); ensemble box-steps/s + p99 vs seri
 
 
 def workload(name, n_override=None, particles_override=None):
-    from lcs_mi355x import config as C
-    if name == "c3":
-        n, npart, dtype, Re, forcing, st = 1024, 64_000_000, "f32", 1333.3, C.ConstantPowerLF(0.5), (1.0, 1.0)
-        desc = "single periodic box 1024^3 FP32, linear-power P=0.5, Re=1333, 6.4e7 particles St=1"
-    elif name == "c2":
-        n, npart, dtype, Re, forcing, st = 256, 2_000_000, "f32", 360.0, C.EnergyPreserveRCF(0.649, 1, 2, 8), (0.1, 10.0)
-        desc = "single periodic box 256^3 FP32, rc-energy (coarse 8), Re=360, 2e6 particles St log-U[0.1,10]"
-    elif name == "c1":
-        n, npart, dtype, Re, forcing, st = 64, 10_000, "f64", 53.3, C.ConstantPowerLF(0.5), (1.0, 1.0)
-        desc = "single periodic box 64^3 FP64, linear-power P=0.5, Re=53.3, 1e4 particles St=1"
-    else:
-        raise SystemExit(f"unknown workload {name}")
-    if n_override:
-        npart = int(round(npart * (n_override / n) ** 3))
-        n = n_override
-        desc += f" (sampled at {n}^3 with {npart} particles)"
-    if particles_override is not None:
-        npart = particles_override
-    part = ()
-    if npart:
-        part = (C.ParticleConfig(C.Population(npart), C.ParticleParams(0.0, 843.0), 4, "linear",
-                                 C.ParticleInit(seed=7, velocity="zero-slip", size="stokes", st_min=st[0],
-                                                st_max=st[1], eps=0.5)),)
-    cfg = C.Config(grid=C.Grid((n, n, n), 3), timestep=C.TimeStep(0.3, 0.01), simulate=C.SimulateConfig(0),
-                   flow=C.FlowConfig(C.FlowParams(1.0, 1.0, 1.0 / Re), C.SpectrumInit(1, 4.0, 1.0),
-                                     C.HSMAC(True, True, "red-black-fast", 1.7, 1e-3, 6, 300), forcing),
-                   particles=part, dtype=dtype)
-    return cfg, desc
+    from lcs_mi355x.workloads import single_box
+    try:
+        return single_box(name, n_override, particles_override)
+    except ValueError as e:
+        raise SystemExit(str(e))
 
 
 def copy_calibration(nbytes):
@@ -79,28 +56,8 @@ def copy_calibration(nbytes):
 
 
 def ensemble_configs(nbox, seed=12345):
-    """Batched-ensemble workload: nbox FP32 periodic boxes, N uniform over
-    {32, 48, 64, 96, 128}, particle counts log-uniform in [1e3, 1e6] with Stokes
-    numbers log-uniform in [0.1, 10], seeded spectra with k0 from {2, 4, 6},
-    linear-power forcing P = 0.5 (Re scaled with N as Re = 53.3 (N/64)^(4/3))."""
-    import numpy as np
-    from lcs_mi355x import config as C
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(nbox):
-        n = int(rng.choice([32, 48, 64, 96, 128]))
-        npart = int(round(10 ** rng.uniform(3.0, 6.0)))
-        k0 = float(rng.choice([2.0, 4.0, 6.0]))
-        Re = 53.3 * (n / 64.0) ** (4.0 / 3.0)
-        part = (C.ParticleConfig(C.Population(npart), C.ParticleParams(0.0, 843.0), 4, "linear",
-                                 C.ParticleInit(seed=2000 + i, velocity="zero-slip", size="stokes", st_min=0.1,
-                                                st_max=10.0, eps=0.5)),)
-        out.append(C.Config(grid=C.Grid((n, n, n), 3), timestep=C.TimeStep(0.3, 0.01), simulate=C.SimulateConfig(0),
-                            flow=C.FlowConfig(C.FlowParams(1.0, 1.0, 1.0 / Re), C.SpectrumInit(1000 + i, k0, 1.0),
-                                              C.HSMAC(True, True, "red-black-fast", 1.7, 1e-3, 6, 300),
-                                              C.ConstantPowerLF(0.5)),
-                            particles=part, dtype="f32"))
-    return out
+    from lcs_mi355x.workloads import ensemble_configs as make
+    return make(nbox, seed)
 
 
 def bench_ensemble(args):
@@ -169,22 +126,8 @@ def bench_ensemble(args):
 
 
 def serve_requests(nreq, seed=4242):
-    """Served-stream workload: N in {32, 48, ..., 128}, particle count
-    log-uniform in [1e3, 2e6], one Stokes number per run log-uniform in
-    [0.1, 10], forcing linear-power (P = 0.5) or rc-energy (energy 0.649),
-    step count uniform in [50, 500], a seed per request."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(nreq):
-        n = int(rng.choice(np.arange(32, 129, 16)))
-        st = float(10 ** rng.uniform(-1.0, 1.0))
-        rc = bool(rng.integers(0, 2))
-        out.append(dict(n=n, particles=int(round(10 ** rng.uniform(3.0, np.log10(2e6)))), st=(st, st),
-                        forcing="rc-energy" if rc else "linear-power", forcing_param=0.649 if rc else 0.5,
-                        steps=int(rng.integers(50, 501)), seed=10_000 + i, k0=4.0,
-                        Re=53.3 * (n / 64.0) ** (4.0 / 3.0), dtype="f32"))
-    return out
+    from lcs_mi355x.workloads import serve_requests as make
+    return make(nreq, seed)
 
 
 def bench_serve(args):

@@ -3,8 +3,9 @@ step) against the oracle's phase-by-phase restatement, from identical initial
 states (the device initialises; the oracle starts from the downloaded copy).
 
 Bar: after 100 steps at 64^3 in FP64, U, V, W, P agree
-within relative L2 1e-5; HSMAC sweep counts agree step by step; particle cell
-indices agree bit-exactly for particles whose positions agree.
+within relative L2 1e-5; HSMAC sweep counts agree step by step; the device's
+cell indices of all 10^4 particles equal the oracle's makeindex of the same
+positions bit for bit (so they agree wherever positions agree).
 """
 import numpy as np
 import pytest
@@ -73,9 +74,11 @@ def test_c1_64cubed_fp64_100_steps():
     print("rel L2 after 100 steps:", errs, flush=True)
     assert all(e <= 1e-5 for e in errs.values()), errs
     assert box.profiles and logs[-1][0]["step"] == 100
-    # particles: compare by id; cell indices bit-exact where positions agree
+    # particles: positions compared by id; then the device re-binning of the
+    # device positions against the oracle's makeindex of the same positions
     ps = sim.particles[0]
     nv = ps.nvalid
+    assert nv == 10000
     ids = box.download_prop(0, "ids")[:nv]
     order = np.argsort(ids)
     gx = [box.download_prop(0, k)[:nv][order] for k in ("x", "y", "z")]
@@ -84,16 +87,25 @@ def test_c1_64cubed_fp64_100_steps():
     for a, b in zip(gx, ox):
         assert rel_l2(a, b) < 1e-6
     same = np.all([a == b for a, b in zip(gx, ox)], axis=0)
-    hg = O.hash_cells(cfg.grid.dims, 4, [a[same].copy() for a in gx], int(same.sum()))
     import lcs_mi355x as L
     L.makeindex(box, 0, 1)
-    dh, _, _ = box.cellindex(0)
+    dh, dstarts, dstops = box.cellindex(0)
+    dx = [box.download_prop(0, k)[:nv] for k in ("x", "y", "z")]
     dids = box.download_prop(0, "ids")[:nv]
+    # every particle: the device hash of its device position is the oracle's
+    # hash of that position, bit for bit, and the cell index agrees
+    oh, _, ostarts, ostops = O.makeindex(cfg.grid.dims, 4, [a.copy() for a in dx], dids.copy(), [], nv)
+    assert len(dh) == nv
+    np.testing.assert_array_equal(dh, oh)
+    np.testing.assert_array_equal(dstarts, ostarts)
+    np.testing.assert_array_equal(dstops, ostops)
+    # where the two runs left a particle at the same position, its cell is the
+    # oracle's cell for its own run's position (the same value, so the same hash)
     pos = {int(i): k for k, i in enumerate(dids)}
-    sel_ids = np.sort(ids)[same]
-    dev_hash = np.array([dh[pos[int(i)]] for i in sel_ids])
-    assert np.array_equal(dev_hash, hg)
-    print("particles with bit-identical positions:", int(same.sum()), "of", nv, flush=True)
+    sel = np.sort(ids)[same]
+    ohash_same = O.hash_cells(cfg.grid.dims, 4, [a[same].copy() for a in ox], int(same.sum()))
+    np.testing.assert_array_equal(np.array([dh[pos[int(i)]] for i in sel], dtype=np.int64), ohash_same)
+    print("particles re-binned:", nv, "; with bit-identical positions after 100 steps:", int(same.sum()), flush=True)
 
 
 def test_precompile_like_rc_energy_two_profiles():
