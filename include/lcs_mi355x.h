@@ -190,7 +190,7 @@ int lcs_rhs_update(lcs_box *box, int stage, double dt_stage);
 int lcs_hsmac(lcs_box *box, int stage, double dt_stage, const lcs_hsmac_params *params, int *itr,
               double *div);
 
-/* forcing!(::ConstantPowerLF) (src/flows/forcings/lf.jl:6-192) on Us */
+/* forcing!(::ConstantPowerLF) (src/flows/forcings/lf.jl:6-106) on Us */
 int lcs_force_linear(lcs_box *box, double power, double dt, double *p);
 /* forcing!(::EnergyPreserveRCF) (src/flows/forcings/rcf.jl:1-191) on Us;
  * box mean, spectrum rescaling and sparse inverse DFT all run on the device */
@@ -335,7 +335,7 @@ int lcs_init_particles(lcs_box *box, int profile, uint64_t seed, double st_min, 
 
 /* ---- whole step ------------------------------------------------------------ */
 /* LCS.timestep! + LCS.evolve!(::FlowParticleMode / ::FlowMode)
- * (src/timestep.jl:152-173, src/evolve.jl:15-146).  All phases are enqueued
+ * (src/timestep.jl:1-27, src/evolve.jl:15-146).  All phases are enqueued
  * on the box stream; only the log is read back.  log may be NULL, in which
  * case the call does not synchronise. */
 int lcs_step(lcs_box *box, const lcs_step_config *cfg, lcs_step_log *log);
@@ -375,14 +375,18 @@ int lcs_batch_set_graphs(lcs_batch *batch, int on);
 
 /* ---- serving ---------------------------------------------------------------- */
 /* No counterpart in the reference.  A request loop over one batch: runs are
- * submitted as requests, admitted in arrival order while a slot (max_live)
- * and their device-memory footprint (memory_budget) fit, given their own box
- * (seeded spectrum, one particle profile with log-uniform Stokes numbers,
- * zero slip), and advanced together by lcs_server_step, one batched step per
- * call.  A run that reaches its step count leaves the batch at once and its
- * slot goes to the next request; its box stays readable (lcs_server_box)
- * until lcs_server_release.  Times are seconds on the server's monotonic
- * clock since lcs_server_create. */
+ * submitted as requests and admitted while a slot (max_live) is free and
+ * their device-memory footprint fits what the budget (memory_budget) has
+ * left.  Admission follows arrival order, except that a request waiting for
+ * memory lets later requests that fit go first, until it has been passed
+ * over max_live times.  A request larger than the whole budget is refused at
+ * submit with LCS_ERR_CAPACITY.  An admitted run gets its own box (seeded
+ * spectrum, one particle profile with log-uniform Stokes numbers, zero
+ * slip), and lcs_server_step advances all live runs together, one batched
+ * step per call.  A run that reaches its step count leaves the batch at once
+ * and its slot goes to the next request; its box stays readable
+ * (lcs_server_box) until lcs_server_release.  Times are seconds on the
+ * server's monotonic clock since lcs_server_create. */
 typedef struct lcs_server lcs_server;
 #define LCS_RUN_QUEUED 0
 #define LCS_RUN_LIVE 1
@@ -401,7 +405,8 @@ typedef struct {
     double Re;            /* U0 L0 / nu with U0 = L0 = 1 */
 } lcs_request;
 typedef struct {
-    size_t memory_budget; /* bytes of device memory the live runs may hold */
+    size_t memory_budget; /* bytes of device memory the live runs may hold; 0: nine tenths of the
+                             device's free memory at lcs_server_create, at most 288e9 */
     int max_live;         /* runs in the batch at once */
     double cfl, dtmax;
     lcs_hsmac_params hsmac;

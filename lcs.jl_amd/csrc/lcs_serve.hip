@@ -11,6 +11,8 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <string>
+#include <algorithm>
 #include <map>
 #include <vector>
 
@@ -26,7 +28,11 @@ struct Run {
     int status = LCS_RUN_QUEUED;
     double submit_s = 0, start_s = 0, finish_s = 0;
     size_t bytes = 0;
+    int bypassed = 0;  // admissions of later requests while this one waited for memory
 };
+
+// one MI355X's HBM, the most a server's memory budget defaults to
+static constexpr size_t kMaxServeBudget = size_t(288) * 1000 * 1000 * 1000;
 
 struct Server {
     lcs_ctx *ctx = nullptr;
@@ -113,14 +119,24 @@ static lcs_step_config run_config(const Server &S, const lcs_request &r)
     return c;
 }
 
-// admits queued requests in arrival order while a slot and memory allow
+// Admits queued requests while a slot is free, in arrival order, except that
+// a request whose footprint does not fit the memory left lets later requests
+// that do fit go first.  Once it has been passed over max_live times nothing
+// passes it any more, so a large run waits at most about one generation of
+// the batch.  Every queued request fits the budget on its own (submit checks
+// it), so an empty server always admits the head.
 static int admit(Server &S)
 {
-    while (!S.queue.empty() && (int)S.live.size() < S.cfg.max_live) {
-        Run &run = S.runs[S.queue.front()];
+    for (auto it = S.queue.begin(); it != S.queue.end() && (int)S.live.size() < S.cfg.max_live;) {
+        Run &run = S.runs[*it];
         const size_t need = footprint(run.req);
-        if (S.mem_used + need > S.cfg.memory_budget && !S.live.empty()) break;  // wait for memory to free up
-        S.queue.pop_front();
+        if (S.mem_used + need > S.cfg.memory_budget) {
+            if (run.bypassed >= S.cfg.max_live) break;
+            ++it;
+            continue;
+        }
+        for (auto jt = S.queue.begin(); jt != it; ++jt) S.runs[*jt].bypassed++;
+        it = S.queue.erase(it);
         int rc = materialise(S, run);
         if (rc == LCS_OK) {
             const lcs_step_config c = run_config(S, run.req);
@@ -153,10 +169,18 @@ int lcs_server_create(lcs_ctx *ctx, const lcs_server_config *cfg, lcs_server **o
 {
     if (!ctx || !cfg || !out) return fail(LCS_ERR_INVALID, "null argument");
     if (cfg->max_live < 1) return fail(LCS_ERR_INVALID, "max_live must be >= 1");
-    if (cfg->memory_budget == 0) return fail(LCS_ERR_INVALID, "memory_budget must be > 0");
+    lcs_server_config c = *cfg;
+    if (c.memory_budget == 0) {
+        // the device's free memory now, less a tenth for the batch's own tables,
+        // graphs and the runtime, and at most one MI355X's 288 GB
+        size_t free_b = 0, total_b = 0;
+        LCS_HIP(hipSetDevice(reinterpret_cast<Ctx *>(ctx)->device));
+        LCS_HIP(hipMemGetInfo(&free_b, &total_b));
+        c.memory_budget = std::min<size_t>(free_b / 10 * 9, kMaxServeBudget);
+    }
     Server *S = new Server;
     S->ctx = ctx;
-    S->cfg = *cfg;
+    S->cfg = c;
     int rc = lcs_batch_create(ctx, &S->batch);
     if (rc) {
         delete S;
@@ -183,6 +207,10 @@ int lcs_server_submit(lcs_server *server, const lcs_request *req, int64_t *id)
     int rc = check_request(*req);
     if (rc) return rc;
     Server &S = *SV(server);
+    if (footprint(*req) > S.cfg.memory_budget)
+        return fail(LCS_ERR_CAPACITY, "request needs " + std::to_string(footprint(*req)) +
+                                          " bytes of device memory, more than the server's budget of " +
+                                          std::to_string(S.cfg.memory_budget));
     Run run;
     run.id = S.next_id++;
     run.req = *req;

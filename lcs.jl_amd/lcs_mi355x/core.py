@@ -449,9 +449,11 @@ class Server:
     """Serving loop over one batch (lcs_server_*): runs are submitted as
     requests, admitted by slot and device-memory footprint, advanced together
     one batched step per ``step()``, and retired when they reach their step
-    count.  No counterpart in the reference."""
+    count.  No counterpart in the reference.  memory_budget 0 takes nine
+    tenths of the device's free memory (at most 288e9 bytes); a request larger
+    than the budget is refused with LCSError (capacity)."""
 
-    def __init__(self, memory_budget=200 * 2**30, max_live=128, cfl=0.3, dtmax=0.01, hsmac=None,
+    def __init__(self, memory_budget=0, max_live=128, cfl=0.3, dtmax=0.01, hsmac=None,
                  localize_interval=10, device=0):
         self.L = _lib.load()
         hs = hsmac or HSMAC(True, True, "red-black-fast", 1.7, 1e-3, 6, 300)

@@ -99,6 +99,34 @@ def test_admission_by_slots_and_memory():
     S.close()
 
 
+def test_requests_that_fit_pass_one_waiting_for_memory():
+    """A request waiting for memory lets a later one that fits go first; one
+    larger than the whole budget is refused at submit (LCS_ERR_CAPACITY)."""
+    import lcs_mi355x as L
+    from lcs_mi355x._lib import LCS_ERR_CAPACITY, LCSError
+    big = dict(OTHERS[0], steps=6)
+    small = dict(OTHERS[1], steps=3)
+    S = L.Server(max_live=8, memory_budget=20 * 2**20, localize_interval=0)
+    ids = [S.submit(dtype="f32", **big), S.submit(dtype="f32", **dict(big, seed=9)), S.submit(dtype="f32", **small)]
+    live, _ = S.step()
+    assert live == 2  # the first big run and the small one; the second big run waits
+    assert S.status(ids[1])["status"] == "queued" and S.status(ids[2])["status"] == "live"
+    for i in ids:
+        run_to_end(S, i)
+    assert S.status(ids[0])["finish_s"] <= S.status(ids[1])["start_s"]
+    assert S.status(ids[2])["start_s"] <= S.status(ids[1])["start_s"]
+    S.close()
+    S = L.Server(max_live=8, memory_budget=4 * 2**20, localize_interval=0)
+    with pytest.raises(LCSError) as e:
+        S.submit(dtype="f32", **big)
+    assert e.value.code == LCS_ERR_CAPACITY
+    S.close()
+    S = L.Server(max_live=8)  # the default budget: nine tenths of the free device memory
+    S.submit(dtype="f32", **small)
+    assert S.step()[0] == 1
+    S.close()
+
+
 def test_invalid_requests_are_rejected():
     import lcs_mi355x as L
     S = L.Server(max_live=2)
