@@ -70,7 +70,7 @@ int lcs_save_restart(lcs_box *box, const char *file);
 int lcs_load_flow_fields(lcs_box *box, const char *file);
 
 /* LCSIO.load_particle_counts! + load_particle_{id,position,velocity,size}!
- * (src/LCSIO/load.jl:44-120) for profile (0-based; group particles/<profile+1>).
+ * (src/LCSIO/load.jl:44-118) for profile (0-based; group particles/<profile+1>).
  * counts must have one entry (one process, load.jl:60) and it must equal the
  * profile's nvalid: a profile's population is fixed when it is added. */
 int lcs_load_particles(lcs_box *box, const char *file, int profile);

@@ -882,7 +882,7 @@ int lcs_step(lcs_box *box, const lcs_step_config *cfg, lcs_step_log *log)
     // HSMAC may swap field pointers with its ping-pong partners: read b.f at use
     void *UsP[4] = {b.f[LCS_FIELD_U], b.f[LCS_FIELD_V], b.f[LCS_FIELD_W], b.f[LCS_FIELD_P]};
     if ((rc = launch_ghost_fill(b, UsP, 4, b.g.h, 1, nullptr))) return rc;
-    // timestep!: src/timestep.jl:152-173
+    // timestep!: src/timestep.jl:1-27
     if ((rc = launch_cfl(b, cfg->cfl, cfg->dtmax, false))) return rc;
     for (int q = 0; q < np; q++)
         if ((rc = launch_p_timestep(b, b.profiles[q], cfg->dtmax, &b.S->dtp[q]))) return rc;

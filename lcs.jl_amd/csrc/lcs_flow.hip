@@ -292,10 +292,10 @@ int launch_ghost_fill(Box &b, void *const *arrays, int narrays, int width, int f
 }
 
 // ---------------------------------------------------------------------------
-// RHS (src/flows/rhs.jl:59-291) fused over the three components and tiled
+// RHS (src/flows/rhs.jl:44-252) fused over the three components and tiled
 // through LDS (lcs_rhs_tile.h): each flux J1/J3 is formed once instead of being
 // stored in six scratch arrays.  FUSED writes out = base + dUdt*dt (update!,
-// src/flows/update.jl:255-268), otherwise out = dUdt (interior layout).
+// src/flows/update.jl:37-51), otherwise out = dUdt (interior layout).
 // ---------------------------------------------------------------------------
 template <typename T>
 struct RhsCoef {
@@ -464,7 +464,7 @@ int launch_rhs(Box &b, int stage_in, bool fused, void *const out[3], void *const
                         : rhs_t<float>(b, stage_in, fused, out, base, dt);
 }
 
-// update! alone: out = base + dUdt*dt (src/flows/update.jl:255-268)
+// update! alone: out = base + dUdt*dt (src/flows/update.jl:37-51)
 template <typename T>
 __global__ void k_update(Geom g, T *o0, T *o1, T *o2, const T *b0, const T *b1, const T *b2,
                          const T *__restrict__ d0, const T *__restrict__ d1, const T *__restrict__ d2,
@@ -1390,7 +1390,7 @@ int launch_cfl(Box &b, double cfl, double dtmax, bool fill)
 }
 
 // ---------------------------------------------------------------------------
-// Constant-power linear forcing (src/flows/forcings/lf.jl:113-192)
+// Constant-power linear forcing (src/flows/forcings/lf.jl:27-106)
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ inline void lf_means_block(const Geom &g, const T *__restrict__ U, const T *__restrict__ V,
@@ -1423,7 +1423,7 @@ __global__ void __launch_bounds__(256) k_lf_means(Geom g, const T *__restrict__ 
     }
 }
 
-// e and A = P0/(2e) from the four means (src/flows/forcings/lf.jl:120-160)
+// e and A = P0/(2e) from the four means (src/flows/forcings/lf.jl:56-64)
 __device__ inline void lf_coef(DevScalars *S, const double m[4], double len, double power)
 {
     double u2 = m[0] / len, u = m[1] / len, v = m[2] / len, w = m[3] / len;

@@ -345,7 +345,7 @@ int lcs_load_particles(lcs_box *box, const char *file, int profile)
                                          " holds " + std::to_string((long long)nvalid));
     if (nvalid == 0) return LCS_OK;  // read_property skips empty properties (hdf5.jl:52)
 
-    // load_particle_{id,position,velocity,size}! (load.jl:44-71,92-120): the
+    // load_particle_{id,position,velocity,size}! (load.jl:44-66,88-118): the
     // first nvalid elements; the invalid tail keeps what the profile holds
     const size_t es = d.dtype == LCS_F32 ? 4 : 8;
     std::vector<unsigned char> host((size_t)pd.ntotal * 8);

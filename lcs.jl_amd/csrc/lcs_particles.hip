@@ -123,7 +123,7 @@ struct MotionCoef {
     int drag;
 };
 
-// motion! per particle (src/particles/motion.jl:188-206)
+// motion! per particle (src/particles/motion.jl:62-81)
 template <typename T>
 __device__ inline void motion_one(const Geom &g, const T *U, const T *V, const T *W, const MotionCoef<T> &c, T x,
                                   T y, T z, T up, T vp, T wp, T diam, T &du, T &dv, T &dw)
@@ -159,7 +159,7 @@ __global__ void k_p_motion(Geom g, const T *U, const T *V, const T *W, MotionCoe
     dwdts[p] = dw;
 }
 
-// update! (src/particles/update.jl:258-272)
+// update! (src/particles/update.jl:48-62)
 template <typename T>
 __global__ void k_p_update(long n, DtRef dt, T *xs, T *ys, T *zs, const T *xs0, const T *ys0, const T *zs0, T *us,
                            T *vs, T *ws, const T *us0, const T *vs0, const T *ws0, const T *dudts, const T *dvdts,
@@ -334,7 +334,7 @@ int launch_p_stage(Box &b, ParticleProfile &pp, int stage, DtRef dt)
 
 // ---------------------------------------------------------------------------
 // Particle dt (src/particles/timestep.jl:15-25): max diameter over all ntotal
-// slots, then min(0.5 taup, dtmax); flow/particle minimum (src/timestep.jl:160-164).
+// slots, then min(0.5 taup, dtmax); flow/particle minimum (src/timestep.jl:14-18).
 // ---------------------------------------------------------------------------
 __device__ inline double block_max_d(double v, double *lds)
 {
@@ -411,7 +411,7 @@ int launch_finalize_dt(Box &b, int nprofiles, double)
 // Cell re-binning: makeindex! (src/particles/makeindex.jl:29-67)
 // ---------------------------------------------------------------------------
 // hash = encode(hasher, floor((x - x0)/dxc) + 1) over axes 0:nc+1
-// (src/utils/hasher.jl:82-105); keys are < 2^32 (hasher.jl:74-75).
+// (src/utils/hasher.jl:15-38); keys are < 2^32 (hasher.jl:7-8).
 template <typename T>
 __global__ void k_hash(long n, const T *xs, const T *ys, const T *zs, T dxc0, T dxc1, T dxc2, int nc0, int nc1,
                        unsigned *keys, unsigned *vals)
@@ -550,7 +550,7 @@ __global__ void k_store_hashes(long n, const unsigned *keys, int64_t *hashes)
 }
 
 // starts[x] = searchsortedfirst(hashes, x), stops[x] = searchsortedlast(hashes, x)
-// for every hash x (src/utils/fast.jl:160-188), 1-based.
+// for every hash x (src/utils/fast.jl:8-40), 1-based.
 __global__ void k_starts_stops(const int64_t *v, long n, long ncell, int64_t *starts, int64_t *stops)
 {
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -615,7 +615,7 @@ static int makeindex_t(Box &b, ParticleProfile &pp, int stage)
         }
         hipLaunchKernelGGL(k_store_hashes, dim3(pgrid(n)), dim3(256), 0, b.stream, n, kA, (int64_t *)pp.hashes);
         LCS_HIP(hipGetLastError());
-        // sortbyperm! of all 17 properties (src/utils/sort.jl:133-148): permute
+        // sortbyperm! of all 17 properties (src/utils/sort.jl:7-22): permute
         // into the staging buffer, then swap buffers
         for (int q = 0; q < LCS_N_PROPERTIES; q++) {
             if (q == LCS_PROP_ID)

@@ -1,4 +1,4 @@
-// Tiled RHS (+ RK update) of src/flows/rhs.jl:59-291 and src/flows/update.jl:255-268.
+// Tiled RHS (+ RK update) of src/flows/rhs.jl:44-252 and src/flows/update.jl:37-51.
 //
 // A workgroup owns a TX x TY column of interior cells and a chunk of z-planes
 // and marches up in z.  Each thread keeps its own column's U, V, W for planes

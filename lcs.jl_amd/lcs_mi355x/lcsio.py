@@ -308,7 +308,7 @@ def load_flow_fields(file, box):
 
 def load_particles(file, box, profile):
     """LCSIO.load_particle_counts! then load_particle_{id,position,velocity,size}!
-    (load.jl:44-120) for profile (0-based; the file's group is profile+1)."""
+    (load.jl:44-118) for profile (0-based; the file's group is profile+1)."""
     _check(load_io().lcs_load_particles(box.handle, _b(file), int(profile)))
 
 

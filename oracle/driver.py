@@ -1,7 +1,7 @@
 """Whole-step CPU oracle (test infrastructure only).
 
 Composes the C restatement's phases in the reference's per-step order:
-``LCS.timestep!`` (``src/timestep.jl:160-173``: flow CFL dt, particle dt, min)
+``LCS.timestep!`` (``src/timestep.jl:14-27``: flow CFL dt, particle dt, min)
 followed by ``LCS.evolve!(::FlowParticleMode)`` (``src/evolve.jl:63-146``).
 The config argument is duck-typed: it needs the attributes of the reference's
 ``LCS.Config`` (grid, timestep, flow.params/couple/forcing, particles).
@@ -55,7 +55,7 @@ class OracleSim:
         self.dt = 0.0
         self.Re = _Re(config.flow.params)
 
-    # src/timestep.jl:160-173
+    # src/timestep.jl:14-27
     def timestep(self):
         c = self.config
         dt, _ = O.flow_timestep(*self.Us, self.h, c.timestep.cfl, c.timestep.dtmax, self.Re)

@@ -108,7 +108,7 @@ void o_ghost_fill(real *A, int nx, int ny, int nz, int h, int r, int full)
 }
 
 /* ------------------------------------------------------------------------- */
-/* Flow RHS: src/flows/rhs.jl:59-291.                                          */
+/* Flow RHS: src/flows/rhs.jl:44-252.                                          */
 /* Component c (0=x,1=y,2=z) of velocity Q.  First pass over domain1 =        */
 /* (-1:n+1)^3 writes dQdt (domain2 only) and the six flux scratches; second   */
 /* pass over domain2 = (1:n)^3 subtracts the flux divergence.                 */
@@ -190,7 +190,7 @@ void o_rhs(int nx, int ny, int nz, int h, double Re, const real *U, const real *
     for (int a = 0; a < 6; a++) free(J[a]);
 }
 
-/* update!: src/flows/update.jl:241-272.  U = U0 + dUdt*dt on (1:n)^3. */
+/* update!: src/flows/update.jl:23-54.  U = U0 + dUdt*dt on (1:n)^3. */
 void o_update(int nx, int ny, int nz, int h, double dt, real *U, real *V, real *W, const real *U0,
               const real *V0, const real *W0, const real *dU, const real *dV, const real *dW)
 {
@@ -462,7 +462,7 @@ double o_flow_timestep(int nx, int ny, int nz, int h, double cfl, double dtmax, 
 }
 
 /* ------------------------------------------------------------------------- */
-/* Constant-power linear forcing: src/flows/forcings/lf.jl:113-192.           */
+/* Constant-power linear forcing: src/flows/forcings/lf.jl:27-106.           */
 /* FullHalo(h) on U,V,W in place; returns mean power p.                       */
 /* ------------------------------------------------------------------------- */
 double o_force_linear(int nx, int ny, int nz, int h, double power, double dt, real *U, real *V,
@@ -508,7 +508,7 @@ double o_force_linear(int nx, int ny, int nz, int h, double power, double dt, re
         means5[3] = wm;
         means5[4] = A;
     }
-    /* _linear_forcing!: lf.jl:158-192 */
+    /* _linear_forcing!: lf.jl:72-106 */
     real rA = R(A), rum = R(um), rvm = R(vm), rwm = R(wm), rdt = R(dt);
 #pragma omp parallel for schedule(static)
     for (int k = 1; k <= nz; k++) {
@@ -1157,7 +1157,7 @@ void o_p_fluid_velocity(int nx, int ny, int nz, int h, const real *U, const real
     }
 }
 
-/* motion!: src/particles/motion.jl:164-210; coeff: src/particles/config.jl:128,139;
+/* motion!: src/particles/motion.jl:38-84; coeff: src/particles/config.jl:128,139;
  * taup: config.jl:41.  drag 0 = linear, 1 = nonlinear (Schiller-Naumann). */
 void o_p_motion(int nx, int ny, int nz, int h, double Re, double gravity, double density_ratio,
                 int drag, const real *U, const real *V, const real *W, long n, const real *xs,
@@ -1188,7 +1188,7 @@ void o_p_motion(int nx, int ny, int nz, int h, double Re, double gravity, double
     }
 }
 
-/* update!: src/particles/update.jl:242-275 */
+/* update!: src/particles/update.jl:32-65 */
 void o_p_update(long n, double dt, real *xs, real *ys, real *zs, const real *xs0, const real *ys0,
                 const real *zs0, real *us, real *vs, real *ws, const real *us0, const real *vs0,
                 const real *ws0, const real *dudts, const real *dvdts, const real *dwdts)
@@ -1218,7 +1218,7 @@ double o_p_timestep(long ntotal, const real *diams, double Re, double density_ra
 }
 
 /* makeindex!: src/particles/makeindex.jl:29-67; encode/sub2ind
- * src/utils/hasher.jl:82-105; searchsortedfirst/last src/utils/fast.jl:8-40.
+ * src/utils/hasher.jl:15-38; searchsortedfirst/last src/utils/fast.jl:8-40.
  * props: array of 16 real property pointers (diams, xss, xss2, uss, uss2,
  * dudtss) permuted together with ids (int64) and hashes.  The sort is stable
  * (ties keep input order).  starts/stops are 1-based, length (nc+2)^3. */
@@ -1283,7 +1283,7 @@ void o_makeindex(int nx, int ny, int nz, int grid_ratio, long n, real *xs, real 
     }
     for (long p = 0; p < n; p++) perm[count[hashes[p]]++] = p + 1; /* 1-based like sortperm */
     free(count);
-    /* sortbyperm!: src/utils/sort.jl:133-148 */
+    /* sortbyperm!: src/utils/sort.jl:7-22 */
     void *tmp = malloc((size_t)n * 8);
     int64_t *ti = (int64_t *)tmp;
     for (long p = 0; p < n; p++) ti[p] = hashes[perm[p] - 1];

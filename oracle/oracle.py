@@ -4,7 +4,7 @@ Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg may
 import this module; it is the checker, never the product path.  The C
 restatement lives in ``lcs_oracle.c`` (built by ``make -C oracle``); this module
 only marshals numpy arrays and composes the phases in the order the reference
-calls them (``src/evolve.jl:63-146``, ``src/timestep.jl:160-173``).
+calls them (``src/evolve.jl:63-146``, ``src/timestep.jl:14-27``).
 
 Arrays use the reference layout: Fortran order (x fastest) with halo ``h`` on
 every side, shape ``(nx+2h, ny+2h, nz+2h)``; dU/dt arrays are interior-only.
@@ -124,7 +124,7 @@ def rhs(U, V, W, P_, h, Re):
 
 
 def update(Us, Us0, dUs, h, dt):
-    """Flows.update!: src/flows/update.jl:241-272 (in place on Us)."""
+    """Flows.update!: src/flows/update.jl:23-54 (in place on Us)."""
     nx, ny, nz = dims_of(Us[0], h)
     lib(Us[0].dtype).o_update(nx, ny, nz, h, dt, *[_p(a) for a in Us], *[_p(a) for a in Us0], *[_p(a) for a in dUs])
 
@@ -166,7 +166,7 @@ def flow_timestep(U, V, W, h, cfl, dtmax, Re):
 
 
 def force_linear(U, V, W, h, power, dt):
-    """forcing!(::ConstantPowerLF): src/flows/forcings/lf.jl:113-192. Returns p."""
+    """forcing!(::ConstantPowerLF): src/flows/forcings/lf.jl:27-106. Returns p."""
     nx, ny, nz = dims_of(U, h)
     m = np.zeros(5)
     p = lib(U.dtype).o_force_linear(nx, ny, nz, h, power, dt, _p(U), _p(V), _p(W), _p(m))

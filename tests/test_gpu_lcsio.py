@@ -1,5 +1,5 @@
 """Restart files of device boxes (lcs_save_restart, lcs_load_flow_fields,
-lcs_load_particles: src/LCSIO/save.jl:40-167, load.jl:23-120) and the
+lcs_load_particles: src/LCSIO/save.jl:40-167, load.jl:23-118) and the
 simulate loop's stat/restart/resume cadence (src/simulate.jl:54-88), after
 test/test_integration_mpi.jl:68-237.
 

@@ -6,7 +6,7 @@ order, over the same ``RhsTile`` code the kernel compiles (lcs_rhs_tile.h).
 Without FMA contraction the result must equal the oracle's ``rhs``
 (src/flows/rhs.jl:59-81) bit for bit, for full and ragged tiles, several
 z-chunkings and both element types; the fused form must equal
-``base + dUdt*dt`` (src/flows/update.jl:255-268).
+``base + dUdt*dt`` (src/flows/update.jl:37-51).
 """
 import ctypes as C
 import os
