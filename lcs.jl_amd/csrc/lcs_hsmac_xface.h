@@ -400,8 +400,9 @@ struct XFace {
 };
 
 // Patch edge by element type: 32 x 32 rows (1024 threads, 98 KB of LDS) for
-// FP32, 16 x 16 (256 threads, 51 KB) for FP64.  Bigger patches re-read fewer
-// window rows and send fewer rows through the compact buffer.
+// FP32, 16 x 16 (256 threads, 61 KB, two workgroups per CU) for FP64; LDS
+// holds 19 values for each of the (N + 4)^2 window rows.  Bigger patches
+// re-read fewer window rows and send fewer rows through the compact buffer.
 template <typename T>
 struct XfPatch {
     static constexpr int N = sizeof(T) == 4 ? 32 : 16;

@@ -14,6 +14,7 @@ z-chunkings, both colourings, both element types.
 import ctypes as C
 import os
 import subprocess
+import zlib
 
 import numpy as np
 import pytest
@@ -53,7 +54,7 @@ CASES = [((8, 8, 8), 4, 64, 14, "d1"), ((16, 12, 10), 4, 64, 5, "d2"), ((70, 9, 
 @pytest.mark.parametrize("dims,ty,tx,zc,box", CASES)
 def test_fused_sweep_equals_two_colour_passes(emul, dims, ty, tx, zc, box, coloring, dtype):
     h, dt, omega = 3, 0.01, 1.7
-    rng = np.random.default_rng(abs(hash((dims, ty, tx, zc, box, coloring))) % 2**32)
+    rng = np.random.default_rng(zlib.crc32(repr((dims, ty, tx, zc, box, coloring)).encode()))
     src = [np.asfortranarray(rng.standard_normal(tuple(n + 2 * h for n in dims)).astype(dtype)) for _ in range(4)]
     lo, hi = _box(box, dims)
     ref = [a.copy(order="F") for a in src]
@@ -135,7 +136,7 @@ def test_fused_sweep_with_fill_on_load(emul, dims, ty, tx, zc, box, coloring, fo
     linear forcing on load as well, the forcing applies to the filled
     values."""
     h, dt, omega = 3, 0.01, 1.7
-    rng = np.random.default_rng(abs(hash((dims, ty, tx, zc, box, coloring, forced))) % 2**32)
+    rng = np.random.default_rng(zlib.crc32(repr((dims, ty, tx, zc, box, coloring, forced)).encode()))
     src = [np.asfortranarray(rng.standard_normal(tuple(n + 2 * h for n in dims)).astype(dtype)) for _ in range(4)]
     T = np.dtype(dtype).type
     A, means, fdt = 0.37, [0.11, -0.07, 0.02], 0.004
@@ -195,7 +196,7 @@ def test_x_slab_pair_equals_four_colour_passes(emul, dims, patch, box, coloring,
     (hsmac.jl:1-35, foraxes.jl:73-107), and the div^2 sum over the slab
     cells in (1:n+1)^3 equal to the oracle's."""
     h, dt, omega = 3, 0.01, 1.7
-    rng = np.random.default_rng(abs(hash((dims, patch, box, coloring))) % 2**32)
+    rng = np.random.default_rng(zlib.crc32(repr((dims, patch, box, coloring)).encode()))
     src = [np.asfortranarray(rng.standard_normal(tuple(n + 2 * h for n in dims)).astype(dtype)) for _ in range(4)]
     lo, hi = _x_slabs(box, dims)
     ref = [a.copy(order="F") for a in src]
@@ -246,7 +247,7 @@ def test_y_and_z_slab_pairs_equal_four_colour_passes(emul, ti, dims, patch, box,
     into the ghosts; equal bit for bit to the reference's lower red, lower
     black, upper red, upper black passes, and the div^2 sum to the oracle's."""
     h, dt, omega = 3, 0.01, 1.7
-    rng = np.random.default_rng(abs(hash((ti, dims, patch, box, coloring))) % 2**32)
+    rng = np.random.default_rng(zlib.crc32(repr((ti, dims, patch, box, coloring)).encode()))
     src = [np.asfortranarray(rng.standard_normal(tuple(n + 2 * h for n in dims)).astype(dtype)) for _ in range(4)]
     lo, hi = _slabs(ti, box, dims)
     ref = [a.copy(order="F") for a in src]

@@ -11,6 +11,7 @@ z-chunkings and both element types; the fused form must equal
 import ctypes as C
 import os
 import subprocess
+import zlib
 
 import numpy as np
 import pytest
@@ -56,7 +57,7 @@ CASES = [((16, 16, 16), 1, 16), ((20, 13, 11), 1, 4), ((70, 9, 7), 0, 3), ((17, 
 @pytest.mark.parametrize("dims,shape,zc", CASES)
 def test_tile_rhs_equals_oracle(emul, dims, shape, zc, dtype):
     h = 3
-    rng = np.random.default_rng(hash((dims, shape, zc)) % 2**32)
+    rng = np.random.default_rng(zlib.crc32(repr((dims, shape, zc)).encode()))
     A = [np.asfortranarray(rng.standard_normal(tuple(n + 2 * h for n in dims)).astype(dtype)) for _ in range(4)]
     for a in A:
         O.ghost_fill(a, h, h, True)
