@@ -55,6 +55,29 @@ def copy_calibration(nbytes):
     return nbytes
 
 
+def hbm_ceiling(nbytes=4 << 30, reps=5):
+    """Best rates of hand-written streaming kernels (read, write, copy,
+    triad; liblcs_stream.so) over nbytes-sized arrays on this GPU: what a pure
+    stream reaches, for reading the kernels' fractions of the 8 TB/s peak."""
+    import ctypes as C
+    L = C.CDLL(os.path.join(ROOT, "lcs.jl_amd", "lcs_mi355x", "liblcs_stream.so"))
+    L.lcs_stream_probe.argtypes = [C.c_int, C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]
+    nv = L.lcs_stream_variants()
+    names = [f"U{u}" for u in (1, 2, 4, 8)] + [f"U{u}-nt" for u in (1, 2, 4, 8)]
+    out = {"unit": "GB/s", "bytes_per_array": nbytes, "peak": PEAK_HBM_GBS}
+    for kind, name in enumerate(("read", "write", "copy", "triad")):
+        g, m = (C.c_double * nv)(), (C.c_double * nv)()
+        rc = L.lcs_stream_probe(0, kind, nbytes, reps, g, m)
+        if rc != 0:
+            raise RuntimeError(f"stream probe {name}: hip error {rc}")
+        best = max(range(nv), key=lambda v: g[v])
+        out[name] = {"GBs": g[best], "frac": g[best] / PEAK_HBM_GBS, "variant": names[best],
+                     "all": {names[v]: round(g[v], 1) for v in range(nv)}}
+        print(f"# ceiling {name:5s} {g[best]:7.0f} GB/s ({names[best]})", file=sys.stderr, flush=True)
+    return out
+
+
 def ensemble_configs(nbox, seed=12345):
     from lcs_mi355x.workloads import ensemble_configs as make
     return make(nbox, seed)
@@ -122,7 +145,7 @@ def bench_ensemble(args):
     batch.close()
     for b in boxes:
         b.close()
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def serve_requests(nreq, seed=4242):
@@ -216,11 +239,38 @@ def bench_serve(args):
                                                                       f"{args.serial_steps} steps each, one lcs_step at a time"},
         "speedup_vs_serial": rate / serial,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
-def cpu_baseline(name, n, steps):
-    """Time the CPU oracle on a bounded sample (n^3 box, same parameters)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(name, n, steps, n1, steps1):
+    """Time the CPU oracle on a bounded sample (n^3 box, same parameters) with
+    every OpenMP thread, and on a smaller sample (n1^3) with one thread."""
+    from oracle import oracle as O
+    threads = O.set_threads(0)
+    out = oracle_rate(name, n, steps)
+    out.update({"cores": threads, "kind": "port", "cpu_model": cpu_model(),
+                "affinity_cores": len(os.sched_getaffinity(0))})
+    O.set_threads(1)
+    try:
+        one = oracle_rate(name, n1, steps1)
+    finally:
+        O.set_threads(threads)
+    out["one_core"] = {"value": one["value"], "unit": one["unit"], "cores": 1, "sample": one["sample"]}
+    return out
+
+
+def oracle_rate(name, n, steps):
     import numpy as np
     import lcs_mi355x as L
     from oracle.driver import OracleSim, ParticleSet
@@ -243,10 +293,9 @@ def cpu_baseline(name, n, steps):
         sim.timestep()
         sim.evolve()
     el = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": n ** 3 * steps / el, "unit": "cell-steps/s", "cores": cores, "kind": "port",
+    return {"value": n ** 3 * steps / el, "unit": "cell-steps/s",
             "sample": f"{steps} timed steps (after 1 warm-up) of the same pipeline at {n}^3 ({desc}); "
-                      f"{el:.1f} s of CPU time"}
+                      f"{el:.1f} s wall"}
 
 
 def main():
@@ -264,7 +313,17 @@ def main():
     ap.add_argument("--size", type=int, default=None, help="override the grid size (testing only)")
     ap.add_argument("--cpu-size", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--cpu1-size", type=int, default=128, help="grid of the one-thread CPU sample")
+    ap.add_argument("--cpu1-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ceiling", action="store_true", help="skip the streaming-kernel HBM ceiling")
+    ap.add_argument("--no-ensemble", action="store_true", help="default run: skip the 96-box ensemble sub-run")
+    ap.add_argument("--ensemble-steps", type=int, default=30, help="timed batch steps of the ensemble sub-run")
+    ap.add_argument("--no-serve", action="store_true", help="default run: skip the served-stream sub-run")
+    ap.add_argument("--serve-requests", type=int, default=150, help="requests of the served-stream sub-run")
+    ap.add_argument("--serve-rate", type=float, default=4.5,
+                    help="arrival rate of the served-stream sub-run, requests/s (about 3/4 of the stream's "
+                         "measured capacity, so latencies measure service, not a growing queue)")
     ap.add_argument("--no-autotune", action="store_true", help="use the size rule for tile shapes")
     ap.add_argument("--localize-interval", type=int, default=30,
                     help="reorder particle storage by position every N steps (0: never; changes no result)")
@@ -278,9 +337,12 @@ def main():
         sys.exit(2)
 
     if args.workload == "c4":
-        return bench_ensemble(args)
+        print(json.dumps(bench_ensemble(args)), flush=True)
+        return
     if args.workload == "c5":
-        return bench_serve(args)
+        print(json.dumps(bench_serve(args)), flush=True)
+        return
+    ceiling = None if args.no_ceiling else hbm_ceiling()
     import lcs_mi355x as L
     cfg, desc = workload(args.workload, n_override=args.size)
     t0 = time.perf_counter()
@@ -346,11 +408,39 @@ def main():
                      "ms_per_launch": ms / max(launches, 1), "launches": launches},
         "kernels": kernels,
     }
+    if args.localize_interval > 0 and box.profiles:
+        # the timed steps fall between two reorders of particle storage
+        # (steps 0 and 30), so none of them pays for one: time one here and
+        # report the per-step share it would add
+        box.sync()
+        t0 = time.perf_counter()
+        for q in range(len(box.profiles)):
+            L.localize(box, q)
+        box.sync()
+        loc_ms = (time.perf_counter() - t0) * 1e3
+        per_step = el / args.steps + loc_ms * 1e-3 / args.localize_interval
+        out["particle_localize"] = {"ms": loc_ms, "interval_steps": args.localize_interval,
+                                    "amortized_ms_per_step": loc_ms / args.localize_interval,
+                                    "value_with_localize": n3 / per_step}
     box.close()
     if args.copy_calibration:
         out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration << 20)
+    if ceiling:
+        out["hbm_ceiling"] = ceiling
+        for k in ("roofline", "step_hbm"):
+            a = out[k]["achieved"] if k == "roofline" else out[k]["achieved_GBs"]
+            out[k]["frac_of_copy_ceiling"] = a / ceiling["copy"]["GBs"]
+    if args.workload == "c3" and args.size is None and not args.no_ensemble:
+        ea = argparse.Namespace(**vars(args))
+        ea.steps, ea.warmup, ea.serial_steps = args.ensemble_steps, 3, 3
+        out["ensemble"] = bench_ensemble(ea)
+    if args.workload == "c3" and args.size is None and not args.no_serve:
+        sa = argparse.Namespace(**vars(args))
+        sa.requests, sa.rate = args.serve_requests, args.serve_rate
+        out["served"] = bench_serve(sa)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_size, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_size, args.cpu_steps, args.cpu1_size,
+                                           args.cpu1_steps)
     print(json.dumps(out), flush=True)
 
 

@@ -43,6 +43,11 @@ def lib(dtype):
     return _libs[key]
 
 
+def set_threads(n):
+    """Sets the OpenMP thread count of both builds (0 keeps it); returns it."""
+    return min(lib(dt).o_set_threads(int(n)) for dt in (np.float64, np.float32))
+
+
 P = C.c_void_p
 I = C.c_int
 D = C.c_double
@@ -52,6 +57,7 @@ U64 = C.c_uint64
 
 def _declare(L):
     sig = {
+        "o_set_threads": (I, [I]),
         "o_ghost_fill": (None, [P, I, I, I, I, I, I]),
         "o_rhs": (None, [I, I, I, I, D, P, P, P, P, P, P, P]),
         "o_update": (None, [I, I, I, I, D, P, P, P, P, P, P, P, P, P]),

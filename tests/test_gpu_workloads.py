@@ -85,20 +85,26 @@ def test_c2_256cubed_rc_energy_against_oracle_then_fp64():
     box = L.build(cfg)
     sim = oracle_from_device(cfg, box)
     scfg = L.step_config(cfg, localize_interval=30)
+    p_scale = None
     for s in range(3):
         d = L.step(box, scfg)
         sim.timestep()
         o = sim.evolve()
+        dp, op = d["flow"]["forcing"]["p"], o["flow"]["forcing"]["p"]
+        errs = {n: rel_l2(interior(box.download(n), 3), interior(r, 3)) for n, r in zip("UVWP", (*sim.Us, sim.P))}
         say(f"step {s}: itr {d['flow']['couple1']['itr']}/{d['flow']['couple2']['itr']} oracle "
-            f"{o['flow']['couple1']['itr']}/{o['flow']['couple2']['itr']} p {d['flow']['forcing']['p']:.6g} "
-            f"oracle {o['flow']['forcing']['p']:.6g}")
+            f"{o['flow']['couple1']['itr']}/{o['flow']['couple2']['itr']} p {dp:.6g} oracle {op:.6g}; rel L2 {errs}")
         assert d["flow"]["couple1"]["itr"] == o["flow"]["couple1"]["itr"], s
         assert d["flow"]["couple2"]["itr"] == o["flow"]["couple2"]["itr"], s
         assert np.isclose(d["dt"], sim.dt, rtol=1e-5), s
-        assert np.isclose(d["flow"]["forcing"]["p"], o["flow"]["forcing"]["p"], rtol=1e-4), s
-    errs = {n: rel_l2(interior(box.download(n), 3), interior(r, 3)) for n, r in zip("UVWP", (*sim.Us, sim.P))}
-    say("C2 rel L2 after 3 steps:", errs)
-    assert all(e <= 1e-5 for e in errs.values()), errs
+        assert all(e <= 1e-5 for e in errs.values()), (s, errs)
+        # p = <u.src>/dt is a mean of products of both signs.  The first step
+        # injects the most power (the shells start far below en0); later steps
+        # inject a small difference of large terms, so FP32 field differences
+        # of ~1e-7 reach p through that cancellation.  The bound is relative to
+        # the forcing's own scale, the first step's power.
+        p_scale = abs(op) if p_scale is None else p_scale
+        assert abs(dp - op) <= 1e-5 * p_scale, (s, dp, op, p_scale)
     compare_particles_by_id(box, sim, 0, 1e-5)
     nv = check_cellindex_against_oracle(box, 0, cfg.grid.dims, 4)
     say("C2 cell indices equal the oracle's for", nv, "particles")
