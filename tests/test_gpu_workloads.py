@@ -1,11 +1,12 @@
 """The benchmark workloads themselves on the GPU, at their full sizes.
 
-* C2 (256^3 FP32, rc-energy forcing, 2e6 particles, St log-uniform in
-  [0.1, 10]): three steps against the FP32 oracle (HSMAC sweep counts per
-  stage equal, fields within relative L2 1e-5, particle positions by id within
-  1e-5, cell indices of the device positions equal to the oracle's bit for
-  bit); then 200 steps in FP32 against the same run in FP64 (the reference's
-  float type): kinetic energy and dissipation within 0.5 %.
+* C2 (256^3, rc-energy forcing, 2e6 particles, St log-uniform in [0.1, 10]):
+  three steps in FP64 (the reference's float type) against the FP64 oracle
+  within 1e-9, three in FP32 against the FP32 oracle (HSMAC sweep counts per
+  stage equal, velocities within relative L2 1e-5, pressure within 1e-3),
+  particle positions by id, cell indices of all device positions equal to the
+  oracle's bit for bit; then 200 steps in FP32 against the same run in FP64:
+  kinetic energy and dissipation within 0.5 %.
 * C3 (1024^3 FP32, linear-power, 6.4e7 particles): size-independent
   properties after two steps: velocity means conserved, the RMS divergence at
   HSMAC exit below epsp/dxmin (from the log and recomputed on the host from
@@ -75,46 +76,68 @@ def check_cellindex_against_oracle(box, q, dims, grid_ratio, full_oracle=True):
 
 
 # ---------------------------------------------------------------------------
-# C2: 256^3 FP32, rc-energy, 2e6 particles
+# C2: 256^3, rc-energy, 2e6 particles
 # ---------------------------------------------------------------------------
-def test_c2_256cubed_rc_energy_against_oracle_then_fp64():
+def _step_against(box, sim, scfg, nsteps, tol_uvw, tol_p, tol_power, tag):
+    """Steps box and oracle side by side; per step: equal HSMAC sweep counts,
+    equal dt, fields and forcing power within the given bounds."""
     import lcs_mi355x as L
-    from lcs_mi355x.workloads import single_box
-    cfg, desc = single_box("c2")
-    say("C2:", desc)
-    box = L.build(cfg)
-    sim = oracle_from_device(cfg, box)
-    scfg = L.step_config(cfg, localize_interval=30)
     p_scale = None
-    for s in range(3):
+    for s in range(nsteps):
         d = L.step(box, scfg)
         sim.timestep()
         o = sim.evolve()
         dp, op = d["flow"]["forcing"]["p"], o["flow"]["forcing"]["p"]
         errs = {n: rel_l2(interior(box.download(n), 3), interior(r, 3)) for n, r in zip("UVWP", (*sim.Us, sim.P))}
-        say(f"step {s}: itr {d['flow']['couple1']['itr']}/{d['flow']['couple2']['itr']} oracle "
-            f"{o['flow']['couple1']['itr']}/{o['flow']['couple2']['itr']} p {dp:.6g} oracle {op:.6g}; rel L2 {errs}")
+        say(f"{tag} step {s}: itr {d['flow']['couple1']['itr']}/{d['flow']['couple2']['itr']} oracle "
+            f"{o['flow']['couple1']['itr']}/{o['flow']['couple2']['itr']} p {dp:.9g} oracle {op:.9g}; rel L2 {errs}")
         assert d["flow"]["couple1"]["itr"] == o["flow"]["couple1"]["itr"], s
         assert d["flow"]["couple2"]["itr"] == o["flow"]["couple2"]["itr"], s
-        assert np.isclose(d["dt"], sim.dt, rtol=1e-5), s
-        assert all(e <= 1e-5 for e in errs.values()), (s, errs)
+        assert np.isclose(d["dt"], sim.dt, rtol=tol_uvw), s
+        assert all(errs[n] <= tol_uvw for n in "UVW"), (s, errs)
+        assert errs["P"] <= tol_p, (s, errs)
         # p = <u.src>/dt is a mean of products of both signs.  The first step
         # injects the most power (the shells start far below en0); later steps
-        # inject a small difference of large terms, so FP32 field differences
-        # of ~1e-7 reach p through that cancellation.  The bound is relative to
-        # the forcing's own scale, the first step's power.
+        # inject a small difference of large terms, so field differences reach
+        # p through that cancellation.  The bound is relative to the forcing's
+        # own scale, the first step's power.
         p_scale = abs(op) if p_scale is None else p_scale
-        assert abs(dp - op) <= 1e-5 * p_scale, (s, dp, op, p_scale)
-    compare_particles_by_id(box, sim, 0, 1e-5)
-    nv = check_cellindex_against_oracle(box, 0, cfg.grid.dims, 4)
-    say("C2 cell indices equal the oracle's for", nv, "particles")
-    del sim
-    # 200 steps in all: FP32 against the same workload in FP64
+        assert abs(dp - op) <= tol_power * p_scale, (s, dp, op, p_scale)
+
+
+def test_c2_256cubed_rc_energy_against_oracle_then_fp64():
+    """C2 at its full size.  In FP64, the reference's float type, three steps
+    follow the FP64 oracle to 1e-9 (fields, particles, all 2e6 cell indices
+    bit for bit): this pins the algorithm on the workload itself, including
+    its first step, whose rc-energy forcing injects far more energy than
+    HSMAC removes in itrmax = 300 sweeps.  In FP32 the same three steps follow
+    the FP32 oracle with equal sweep counts and velocities within 1e-5; the
+    pressure sums the corrections of every sweep (336 in the first step, 300
+    of them without convergence) and the device contracts multiply-adds the
+    oracle keeps apart, so its bound is 1e-3.  Then 200 steps in all, FP32
+    against FP64: kinetic energy and dissipation within 0.5 %."""
+    import lcs_mi355x as L
+    from lcs_mi355x.workloads import single_box
     cfg64, _ = single_box("c2", dtype="f64")
+    cfg, desc = single_box("c2")
+    say("C2:", desc)
     b64 = L.build(cfg64)
     s64 = L.step_config(cfg64, localize_interval=30)
-    for _ in range(3):
-        L.step(b64, s64, want_log=False)
+    sim = oracle_from_device(cfg64, b64)
+    _step_against(b64, sim, s64, 3, 1e-9, 1e-9, 1e-9, "C2 FP64")
+    compare_particles_by_id(b64, sim, 0, 1e-9)
+    nv = check_cellindex_against_oracle(b64, 0, cfg64.grid.dims, 4)
+    say("C2 FP64 cell indices equal the oracle's for", nv, "particles")
+    del sim
+    box = L.build(cfg)
+    scfg = L.step_config(cfg, localize_interval=30)
+    sim = oracle_from_device(cfg, box)
+    _step_against(box, sim, scfg, 3, 1e-5, 1e-3, 1e-5, "C2 FP32")
+    compare_particles_by_id(box, sim, 0, 1e-5)
+    nv = check_cellindex_against_oracle(box, 0, cfg.grid.dims, 4)
+    say("C2 FP32 cell indices equal the oracle's for", nv, "particles")
+    del sim
+    # 200 steps in all: FP32 against the same workload in FP64
     for _ in range(197):
         L.step(box, scfg, want_log=False)
         L.step(b64, s64, want_log=False)
