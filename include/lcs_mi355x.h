@@ -196,6 +196,18 @@ int lcs_force_linear(lcs_box *box, double power, double dt, double *p);
  * box mean, spectrum rescaling and sparse inverse DFT all run on the device */
 int lcs_force_rcf(lcs_box *box, double energy, int kmin, int kmax, int coarse_size, double dt,
                   double *p);
+/* lcs_force_rcf without any device allocation.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops).  Same arguments, kernels and results as
+ * lcs_force_rcf, but the coarse grids, spectra and crop live in one fixed
+ * 1 MiB device array per GPU (131072 doubles: 3*c^3 + 12*(kmax+1)*(2kmax+1)^2
+ * must fit, e.g. coarse_size <= 32 with kmax <= 8) instead of the box's own
+ * buffer; larger requests fail with LCS_ERR_CAPACITY.  Every call on a GPU
+ * shares that array, so calls that overlap on one GPU (boxes on different
+ * streams) race; the array keeps the last call's contents, and every element
+ * a call reads is written by that call first. */
+int lcs_force_rcf_noalloc(lcs_box *box, double energy, int kmin, int kmax, int coarse_size, double dt,
+                          double *p);
 /* box-mean filter only (Flows.boxmean_l!, ext/AMDGPUExt.jl:18-41): coarse3
  * receives 3*c^3 values (U then V then W), column-major, as doubles */
 int lcs_boxmean(lcs_box *box, int coarse_size, double *coarse3);

@@ -24,6 +24,32 @@ int fail(int code, const std::string &msg)
 
 __global__ void k_set_double(double *dst, double v) { *dst = v; }
 
+// ---- scratch of the allocation-free rc-energy forcing ------------------------
+// The coarse grids, spectra and crop of lcs_force_rcf_noalloc live in this
+// file-scope device array: every device the library runs on holds one copy
+// until the process ends.  In host code `scratch` is only a host-side stand-in
+// for the symbol; its device address comes from hipGetSymbolAddress, once per
+// device (lcs_ctx_create, with that device current), and only that address is
+// ever handed to kernels.
+constexpr size_t kRcfScratchDoubles = size_t(1) << 17;  // 1 MiB: c <= 32 with kmax <= 8
+static __device__ __align__(256) double scratch[kRcfScratchDoubles];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kMaxDevices] = {};
+
+static int register_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(LCS_ERR_INVALID, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return LCS_OK;
+    double *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(LCS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = ptr;
+    return LCS_OK;
+}
+
 __global__ void k_set_state(DevScalars *S, int64_t step, double t)
 {
     S->step = step;
@@ -171,6 +197,8 @@ int lcs_ctx_create(int device, lcs_ctx **out)
     LCS_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(LCS_ERR_INVALID, "no such device");
     LCS_HIP(hipSetDevice(device));
+    int rc = register_scratch(device);
+    if (rc) return rc;
     Ctx *c = new Ctx;
     c->device = device;
     *out = reinterpret_cast<lcs_ctx *>(c);
@@ -408,6 +436,27 @@ int lcs_force_rcf(lcs_box *box, double energy, int kmin, int kmax, int c, double
     Box &b = *B(box);
     int rc = set_phase_dt(b, dt);
     if (rc || (rc = launch_force_rcf(b, energy, kmin, kmax, c, DtRef{&b.S->dt_phase, 1.0})) ||
+        (rc = read_scalars(b)))
+        return rc;
+    if (p) *p = b.S_host->p;
+    return LCS_OK;
+}
+
+int lcs_force_rcf_noalloc(lcs_box *box, double energy, int kmin, int kmax, int c, double dt, double *p)
+{
+    if (check_box(box)) return LCS_ERR_INVALID;
+    Box &b = *B(box);
+    if (c < 1 || kmax < 0) return fail(LCS_ERR_INVALID, "coarse_size must be >= 1 and kmax >= 0");
+    const size_t need = rcf_scratch_doubles(c, kmax);
+    if (need > kRcfScratchDoubles)
+        return fail(LCS_ERR_CAPACITY, "lcs_force_rcf_noalloc: coarse_size " + std::to_string(c) + " and kmax " +
+                                          std::to_string(kmax) + " need " + std::to_string(need) +
+                                          " scratch doubles, more than the fixed " +
+                                          std::to_string(kRcfScratchDoubles) + " (use lcs_force_rcf)");
+    double *ptr = g_scratch_dev[b.ctx->device];
+    if (!ptr) return fail(LCS_ERR_INVALID, "scratch of this device was never registered");
+    int rc = set_phase_dt(b, dt);
+    if (rc || (rc = launch_force_rcf(b, energy, kmin, kmax, c, DtRef{&b.S->dt_phase, 1.0}, ptr)) ||
         (rc = read_scalars(b)))
         return rc;
     if (p) *p = b.S_host->p;

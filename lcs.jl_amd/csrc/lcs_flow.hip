@@ -1937,10 +1937,15 @@ static int kset_host(int k0, signed char (*out)[3], int cap)
     return n;
 }
 
+size_t rcf_scratch_doubles(int c, int kmax)
+{
+    size_t nk = 2 * (size_t)kmax + 1, ncrop = ((size_t)kmax + 1) * nk * nk;
+    return (size_t)3 * c * c * c + 6 * ncrop + 6 * ncrop;
+}
+
 static int rcf_buffers(Box &b, int c, int kmax)
 {
-    int nk = 2 * kmax + 1, ncrop = (kmax + 1) * nk * nk;
-    size_t need = sizeof(double) * ((size_t)3 * c * c * c + 6 * (size_t)ncrop + 6 * (size_t)ncrop);
+    size_t need = sizeof(double) * rcf_scratch_doubles(c, kmax);
     if (need > b.rcf_bytes) {
         if (b.rcf) LCS_HIP(hipFree(b.rcf));
         b.rcf = nullptr;
@@ -1968,12 +1973,12 @@ int launch_boxmean(Box &b, int c, double *coarse_dev)
     return LCS_OK;
 }
 
-int launch_force_rcf(Box &b, double energy, int kmin, int kmax, int c, DtRef dt)
+int launch_force_rcf(Box &b, double energy, int kmin, int kmax, int c, DtRef dt, double *scratch)
 {
     if (kmin < 1 || kmax < kmin) return fail(LCS_ERR_INVALID, "rc-energy needs 1 <= kmin <= kmax");
     if (kmax > c / 2) return fail(LCS_ERR_INVALID, "rc-energy needs kmax <= coarse_size/2");
     if (kmax - kmin + 1 > 8) return fail(LCS_ERR_INVALID, "at most 8 forced shells");
-    int rc = rcf_buffers(b, c, kmax);
+    int rc = scratch ? LCS_OK : rcf_buffers(b, c, kmax);
     if (rc) return rc;
     RcfModes md;
     memset(&md, 0, sizeof(md));
@@ -1986,7 +1991,7 @@ int launch_force_rcf(Box &b, double energy, int kmin, int kmax, int c, DtRef dt)
         md.n += n;
     }
     int nk = 2 * kmax + 1, ncrop = (kmax + 1) * nk * nk;
-    double *coarse = b.rcf;
+    double *coarse = scratch ? scratch : b.rcf;
     double *hat = coarse + 3L * c * c * c;
     double *crop = hat + 6L * ncrop;
     if ((rc = launch_boxmean(b, c, coarse))) return rc;

@@ -228,7 +228,10 @@ int launch_flow_stat(Box &b, int integral_length, double *out_dev);
 int particle_stat(Box &b, ParticleProfile &pp, const lcs_particle_stat_params &p, double tau_eta,
                   lcs_particle_stats &o, double *edges, int64_t *npairs, double *gr, double *wavenumbers, double *Enp);
 int particle_density(Box &b, ParticleProfile &pp, double *out);
-int launch_force_rcf(Box &b, double energy, int kmin, int kmax, int c, DtRef dt);
+// scratch: coarse grids, spectra and crop (rcf_scratch_doubles(c, kmax)
+// doubles); nullptr uses the box's own buffer, allocated on first use
+int launch_force_rcf(Box &b, double energy, int kmin, int kmax, int c, DtRef dt, double *scratch = nullptr);
+size_t rcf_scratch_doubles(int c, int kmax);
 int launch_init_spectrum(Box &b, uint64_t seed, double k0, double uprime);
 
 // ---- launchers (lcs_particles.hip) -----------------------------------------

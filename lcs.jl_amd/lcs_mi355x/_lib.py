@@ -136,6 +136,7 @@ SIGNATURES = {
     "lcs_hsmac": [P, I, D, C.POINTER(HsmacParams), C.POINTER(I), C.POINTER(D)],
     "lcs_force_linear": [P, D, D, C.POINTER(D)],
     "lcs_force_rcf": [P, D, I, I, I, D, C.POINTER(D)],
+    "lcs_force_rcf_noalloc": [P, D, I, I, I, D, C.POINTER(D)],
     "lcs_boxmean": [P, I, P],
     "lcs_flow_stat": [P, I, C.POINTER(FlowStat)],
     "lcs_autotune": [P, C.POINTER(TileShapes)],
