@@ -64,7 +64,7 @@ def hbm_ceiling(nbytes=4 << 30, reps=5):
     L.lcs_stream_probe.argtypes = [C.c_int, C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]
     nv = L.lcs_stream_variants()
-    names = [f"U{u}" for u in (1, 2, 4, 8)] + [f"U{u}-nt" for u in (1, 2, 4, 8)]
+    names = [f"U{u}" for u in (1, 2, 4, 8)] + [f"U{u}-nt" for u in (1, 2, 4, 8)] + ["dword", "dword-nt", "dword-U4", "dword-U16"]
     out = {"unit": "GB/s", "bytes_per_array": nbytes, "peak": PEAK_HBM_GBS}
     for kind, name in enumerate(("read", "write", "copy", "triad")):
         g, m = (C.c_double * nv)(), (C.c_double * nv)()
@@ -73,7 +73,7 @@ def hbm_ceiling(nbytes=4 << 30, reps=5):
             raise RuntimeError(f"stream probe {name}: hip error {rc}")
         best = max(range(nv), key=lambda v: g[v])
         out[name] = {"GBs": g[best], "frac": g[best] / PEAK_HBM_GBS, "variant": names[best],
-                     "all": {names[v]: round(g[v], 1) for v in range(nv)}}
+                     "all": {names[v]: round(g[v], 1) for v in range(nv) if g[v] > 0}}
         print(f"# ceiling {name:5s} {g[best]:7.0f} GB/s ({names[best]})", file=sys.stderr, flush=True)
     return out
 
@@ -115,6 +115,8 @@ def bench_ensemble(args):
     serial = len(boxes) * args.serial_steps / serial_s
     print(f"# serial: {serial:.1f} box-steps/s", file=sys.stderr, flush=True)
     batch = L.Batch()
+    if args.no_graphs:
+        batch.set_graphs(False)
     for b, s in zip(boxes, scfgs):
         batch.add(b, s)
     for _ in range(args.warmup):
@@ -136,7 +138,7 @@ def bench_ensemble(args):
                                f"[1e3, 1e6], St log-uniform [0.1, 10], k0 in {{2, 4, 6}}, linear-power P=0.5",
                    "boxes": len(boxes), "cells": cells,
                    "particles": sum(c.particles[0].population.valid for c in cfgs),
-                   "particle_localize_interval": args.localize_interval},
+                   "particle_localize_interval": args.localize_interval, "graphs": not args.no_graphs},
         "cell_steps_per_s": cells * args.steps / el,
         "serial": {"value": serial, "unit": "box-steps/s", "steps_per_box": args.serial_steps,
                    "how": "the same boxes, one lcs_step call per box at a time, same GPU"},
@@ -153,16 +155,12 @@ def serve_requests(nreq, seed=4242):
     return make(nreq, seed)
 
 
-def bench_serve(args):
-    """Continuous batching of a Poisson request stream: completion latency
-    percentiles and box-steps/s, beside the same requests' serial rate
-    (lcs_step one run at a time) measured on a sample."""
+def _serve_stream(L, reqs, arrivals, max_live, localize_interval, tag):
+    """Feeds reqs to a server at the given arrival times (seconds from the
+    start) and steps it until every run has finished.  Returns (box-steps,
+    wall seconds, completion latencies)."""
     import numpy as np
-    import lcs_mi355x as L
-    reqs = serve_requests(args.requests)
-    rng = np.random.default_rng(99)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, len(reqs)))
-    S = L.Server(max_live=args.max_live, localize_interval=args.localize_interval)
+    S = L.Server(max_live=max_live, localize_interval=localize_interval)
     done, lat, steps_total, live_ids = 0, [], 0, set()
     nxt = 0
     t0 = time.perf_counter()
@@ -188,26 +186,40 @@ def bench_serve(args):
         if time.perf_counter() - last_report > 20:
             last_report = time.perf_counter()
             c = S.counts()
-            print(f"# t={last_report - t0:.0f}s submitted {nxt} done {done} live {c['live']} queued {c['queued']}",
-                  file=sys.stderr, flush=True)
+            print(f"# {tag} t={last_report - t0:.0f}s submitted {nxt} done {done} live {c['live']} "
+                  f"queued {c['queued']}", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     S.close()
-    rate = steps_total / wall
+    return steps_total, wall, np.array(lat)
+
+
+def bench_serve(args):
+    """Continuous batching of a request stream, in two runs of the same
+    requests: every request submitted at once (the server's capacity in
+    box-steps/s, against the same requests' serial rate), then Poisson
+    arrivals at a rate the server sustains (completion latency percentiles
+    that measure service rather than a growing queue).  --rate fixes the
+    arrival rate; by default it is --load times the capacity."""
+    import numpy as np
+    import lcs_mi355x as L
+    reqs = serve_requests(args.requests)
+    mean_steps = float(np.mean([r["steps"] for r in reqs]))
+    cap_steps, cap_wall, cap_lat = _serve_stream(L, reqs, np.zeros(len(reqs)), args.max_live,
+                                                 args.localize_interval, "capacity")
+    capacity = cap_steps / cap_wall
+    print(f"# served capacity: {capacity:.1f} box-steps/s ({capacity / mean_steps:.2f} requests/s)", file=sys.stderr,
+          flush=True)
+    rate = args.rate if args.rate else args.load * capacity / mean_steps
+    rng = np.random.default_rng(99)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+    steps_total, wall, lat = _serve_stream(L, reqs, arrivals, args.max_live, args.localize_interval, "poisson")
+    delivered = steps_total / wall
     # serial reference on a sample: the first few requests, a few steps each, one at a time
     sample = reqs[: args.serial_sample]
     boxes = []
-    for q, r in enumerate(sample):
-        from lcs_mi355x import config as C
-        cfg = C.Config(grid=C.Grid((r["n"],) * 3, 3), timestep=C.TimeStep(0.3, 0.01), simulate=C.SimulateConfig(0),
-                       flow=C.FlowConfig(C.FlowParams(1.0, 1.0, 1.0 / r["Re"]), C.SpectrumInit(r["seed"], r["k0"], 1.0),
-                                         C.HSMAC(True, True, "red-black-fast", 1.7, 1e-3, 6, 300),
-                                         C.ConstantPowerLF(0.5) if r["forcing"] == "linear-power"
-                                         else C.EnergyPreserveRCF(0.649, 1, 2, 8)),
-                       particles=(C.ParticleConfig(C.Population(r["particles"]), C.ParticleParams(0.0, 843.0), 4,
-                                                   "linear", C.ParticleInit(seed=r["seed"], velocity="zero-slip",
-                                                                            size="stokes", st_min=r["st"][0],
-                                                                            st_max=r["st"][1], eps=0.5)),),
-                       dtype="f32")
+    from lcs_mi355x.workloads import request_config
+    for r in sample:
+        cfg = request_config(r)
         b = L.build(cfg)
         if not args.no_autotune:
             L.autotune(b)
@@ -223,21 +235,26 @@ def bench_serve(args):
     serial = len(boxes) * args.serial_steps / (time.perf_counter() - ts)
     for b, _ in boxes:
         b.close()
-    lat = np.array(lat)
     out = {
-        "metric": METRIC, "value": rate, "unit": "box-steps/s", "n_gpus": 1, "steps": int(steps_total),
-        "warmup": 0, "ms_per_step": 1e3 / rate, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "metric": METRIC, "value": capacity, "unit": "box-steps/s", "n_gpus": 1, "steps": int(cap_steps),
+        "warmup": 0, "ms_per_step": 1e3 / capacity, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (per-request seeds)",
-        "config": {"workload": f"served stream of {len(reqs)} Poisson requests at {args.rate}/s, N in 32..128 "
-                               f"step 16, particles log-uniform [1e3, 2e6], St log-uniform [0.1, 10], forcing "
-                               f"linear-power or rc-energy, steps uniform [50, 500]",
-                   "max_live": args.max_live, "particle_localize_interval": args.localize_interval},
-        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                      "mean": float(lat.mean()), "max": float(lat.max())},
-        "wall_s": wall, "completed": int(len(lat)),
+        "config": {"workload": f"served stream of {len(reqs)} requests, N in 32..128 step 16, particles "
+                               f"log-uniform [1e3, 2e6], St log-uniform [0.1, 10], forcing linear-power or "
+                               f"rc-energy, steps uniform [50, 500]",
+                   "max_live": args.max_live, "particle_localize_interval": args.localize_interval,
+                   "mean_steps_per_request": mean_steps},
+        "capacity": {"box_steps_per_s": capacity, "wall_s": cap_wall, "how": "every request submitted at once; "
+                     "box-steps of the whole stream over its makespan (drain included)",
+                     "latency_s": {"p50": float(np.percentile(cap_lat, 50)),
+                                   "p99": float(np.percentile(cap_lat, 99))}},
+        "poisson": {"rate_requests_per_s": rate, "load": rate * mean_steps / capacity,
+                    "delivered_box_steps_per_s": delivered, "wall_s": wall, "completed": int(len(lat)),
+                    "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                  "mean": float(lat.mean()), "max": float(lat.max())}},
         "serial": {"value": serial, "unit": "box-steps/s", "sample": f"first {len(boxes)} requests, "
                                                                       f"{args.serial_steps} steps each, one lcs_step at a time"},
-        "speedup_vs_serial": rate / serial,
+        "speedup_vs_serial": capacity / serial,
     }
     return out
 
@@ -305,7 +322,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--requests", type=int, default=2000, help="served requests (workload c5)")
-    ap.add_argument("--rate", type=float, default=8.0, help="Poisson arrival rate, requests/s (c5)")
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="Poisson arrival rate, requests/s (c5); 0: --load times the measured capacity")
+    ap.add_argument("--load", type=float, default=0.75, help="arrival rate as a fraction of capacity (c5)")
     ap.add_argument("--max-live", type=int, default=128, help="runs in the batch at once (c5)")
     ap.add_argument("--serial-sample", type=int, default=16, help="requests of the serial reference (c5)")
     ap.add_argument("--boxes", type=int, default=96, help="ensemble size (workload c4)")
@@ -320,11 +339,11 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="default run: skip the 96-box ensemble sub-run")
     ap.add_argument("--ensemble-steps", type=int, default=30, help="timed batch steps of the ensemble sub-run")
     ap.add_argument("--no-serve", action="store_true", help="default run: skip the served-stream sub-run")
-    ap.add_argument("--serve-requests", type=int, default=150, help="requests of the served-stream sub-run")
-    ap.add_argument("--serve-rate", type=float, default=4.5,
-                    help="arrival rate of the served-stream sub-run, requests/s (about 3/4 of the stream's "
-                         "measured capacity, so latencies measure service, not a growing queue)")
+    ap.add_argument("--serve-requests", type=int, default=200, help="requests of the served-stream sub-run")
     ap.add_argument("--no-autotune", action="store_true", help="use the size rule for tile shapes")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="c4: launch the batched step eagerly instead of replaying hipGraphs (kernel work is the "
+                         "same; used for profiling, where rocprofv3 ended graph replays with a host fault)")
     ap.add_argument("--localize-interval", type=int, default=30,
                     help="reorder particle storage by position every N steps (0: never; changes no result)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -436,7 +455,7 @@ def main():
         out["ensemble"] = bench_ensemble(ea)
     if args.workload == "c3" and args.size is None and not args.no_serve:
         sa = argparse.Namespace(**vars(args))
-        sa.requests, sa.rate = args.serve_requests, args.serve_rate
+        sa.requests = args.serve_requests
         out["served"] = bench_serve(sa)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_size, args.cpu_steps, args.cpu1_size,

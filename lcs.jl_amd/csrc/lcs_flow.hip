@@ -373,9 +373,11 @@ static int rhs_tile_launch(Box &b, const RhsArgs<T> &a0, DtRef dt)
     return LCS_OK;
 }
 
-// Tile shapes (tx, ty) the fused kernels are built for.  Sweep: 64 or 32
-// wide, 16 or 8 tall.  RHS: FP32 up to 64 x 16; FP64 up to 64 x 8 (LDS).
-static const int kSweepShapes[][2] = {{64, 16}, {64, 8}, {32, 16}, {32, 8}};
+// Tile shapes (tx, ty) the fused kernels are built for (the LDS of one CU
+// bounds them).  Sweep: 64 or 32 wide and 16 or 8 tall, and in FP32 also
+// 128 x 8.  RHS: FP32 up to 64 x 16; FP64 up to 64 x 8.
+static const int kSweepShapesF[][2] = {{64, 16}, {64, 8}, {32, 16}, {32, 8}, {128, 8}};
+static const int kSweepShapesD[][2] = {{64, 16}, {64, 8}, {32, 16}, {32, 8}};
 static const int kRhsShapesF[][2] = {{64, 16}, {64, 8}, {32, 16}, {32, 8}};
 static const int kRhsShapesD[][2] = {{64, 8}, {32, 8}, {32, 4}};
 // two-sweep pass (FP32 only; (0, 0) = two single sweeps)
@@ -383,8 +385,9 @@ static const int kSweep2Shapes[][2] = {{32, 24}, {64, 12}, {32, 16}, {0, 0}};
 
 int tile_candidates(int esize, int kind, int out[][2], int cap)
 {
-    const int(*src)[2] = kind == 0 ? kSweepShapes : kind == 2 ? kSweep2Shapes : (esize == 8 ? kRhsShapesD : kRhsShapesF);
-    const int n = kind == 0 ? 4 : kind == 2 ? (esize == 8 ? 0 : 4) : (esize == 8 ? 3 : 4);
+    const int(*src)[2] = kind == 0 ? (esize == 8 ? kSweepShapesD : kSweepShapesF)
+                                   : kind == 2 ? kSweep2Shapes : (esize == 8 ? kRhsShapesD : kRhsShapesF);
+    const int n = kind == 0 ? (esize == 8 ? 4 : 5) : kind == 2 ? (esize == 8 ? 0 : 4) : (esize == 8 ? 3 : 4);
     for (int i = 0; i < n && i < cap; i++) {
         out[i][0] = src[i][0];
         out[i][1] = src[i][1];
@@ -647,16 +650,21 @@ __device__ inline double hsmac_sweep_block(const HsArgs<T> &a, int bx, int by, i
     Tile::init(a, bx, by, bz, threadIdx.x, th);
     Tile::prologue(a, lds, th);
     __syncthreads();
-    const int q1 = th.z1 + 3;
-    for (int q = th.z0 - 1; q <= q1; q += 2) {
-        Tile::step_a(a, lds, q, th, 0);
+    const int q1 = Tile::last_step(th);
+    for (int q = Tile::first_step(th); q <= q1; q += 3) {  // unrolled by the three register slots
+        Tile::step_a(a, lds, q, th, 2);
         __syncthreads();
         Tile::step_b(a, lds, q, th, 0);
         __syncthreads();
         if (q + 1 > q1) break;
-        Tile::step_a(a, lds, q + 1, th, 1);
+        Tile::step_a(a, lds, q + 1, th, 0);
         __syncthreads();
         Tile::step_b(a, lds, q + 1, th, 1);
+        __syncthreads();
+        if (q + 2 > q1) break;
+        Tile::step_a(a, lds, q + 2, th, 1);
+        __syncthreads();
+        Tile::step_b(a, lds, q + 2, th, 2);
         __syncthreads();
     }
     if (FORCE) *power = block_sum(th.pacc, red_lds);
@@ -1001,6 +1009,8 @@ struct HsmacRun {
         if (tx == 64 && ty == 8) return fused_launch<COLORING, 8, 64>(r);
         if (tx == 32 && ty == 16) return fused_launch<COLORING, 16, 32>(r);
         if (tx == 32 && ty == 8) return fused_launch<COLORING, 8, 32>(r);
+        if constexpr (sizeof(T) == 4)
+            if (tx == 128 && ty == 8) return fused_launch<COLORING, 8, 128>(r);
         return fail(LCS_ERR_INVALID, "unsupported sweep tile shape");
     }
     // one fused sweep with the box's tile shape

@@ -10,25 +10,35 @@
 //
 // A workgroup owns a TX x TY column of cells and a chunk of planes [z0, z1]
 // and marches up in z.  At step q:
-//   phase A  issue the loads of plane q+2 into registers (two planes in
-//            flight hide the memory latency); red dp on plane q over x in
+//   phase A  issue the loads of plane q+3 (U, V, W over the column and its
+//            ring) and of P at plane q (owned cells) into registers: three
+//            planes stay in flight; red dp on plane q over x in
 //            [X0-1, X1+2], y in [Y0-1, Y1+2] (the red neighbours of the black
 //            cells below);
 //   phase B  black dp on plane q-1 over x in [X0, X1+1], y in [Y0, Y1+1];
 //            final U, V, W, P of plane q-3 for the owned cells; plane q+1
-//            (loaded during the previous step) goes into LDS.
+//            (loaded two steps earlier) goes into LDS.
 // LDS holds U and V for planes q-1..q+1, W for q-2..q+1 over x in
 // [X0-2, X1+2], y in [Y0-2, Y1+2], and one dp plane per z for q-3..q in
 // which red cells hold red dp and black cells black dp.  A thread keeps its
 // own cell's U, V, W for planes q-3..q in registers (the only reader of plane
-// q-3).  Two barriers per plane.  Every cell (x, y, z) owned by the workgroup
-// writes P and its upper faces U(x), V(y), W(z); the grid covers the whole
-// array, so elements outside R are carried over to the other set.
+// q-3).  Two barriers per plane.  (A march with one barrier per plane, whose
+// larger rings let red dp of plane q, black dp of q-2 and the output of q-4
+// share a phase, was bit-exact and 9 % slower at 1024^3.)  Every cell
+// (x, y, z) owned by the workgroup writes P and its upper faces U(x), V(y),
+// W(z); the grid covers the whole array, so elements outside R are carried
+// over to the other set.
+//
+// Every global load and store is a range-checked plane access (lcs_bufio.h):
+// elements outside the array read as 0 and stores outside it are dropped, so
+// the march has no branch around a memory operation and the plane loads
+// issued three steps ahead stay in flight until the step that stores them.
 //
 // All of it is __host__ __device__ so a host harness can run the phases
 // thread by thread against the oracle.
 #pragma once
 
+#include "lcs_bufio.h"
 #include "lcs_internal.h"
 
 namespace lcs {
@@ -136,6 +146,9 @@ struct HsTile {
     static constexpr int NHS = (NHALO + NT - 1) / NT;              // halo elements per thread
     static constexpr int RPR = (DXN + 1) / 2, BPR = (TX + 2) / 2;  // cells of one colour per row
     static constexpr int BYN = TY + 1;
+    // register slots of fetched planes: plane p lives in slot (p - z0) mod 3
+    // from the step that fetches it (p - 3) to the step that stores it (p - 1)
+    static constexpr int NSLOT = 3;
     static_assert(RPR * DYN <= NT && BPR * BYN <= NT, "one red and one black slot per thread");
 
     struct Thread {
@@ -144,10 +157,12 @@ struct HsTile {
         long h_off[NHS];  // their in-plane offsets, -1 outside the array
         long o_off;     // in-plane offset of the owned cell, -1 outside the array
         long u_off;     // where its U, V, W are loaded from (o_off, or its image when wrapping)
+        uint32_t o_vo, u_vo;     // byte offsets of o_off and u_off (kOff0 outside the array)
+        uint32_t h_vo[NHS];      // byte offsets of h_off (kOff0: none)
         int r_row, r_k, b_row, b_k;
-        T pf[2][3][1 + NHS];  // two fetched planes in flight: own cell, halo elements
+        T pf[NSLOT][3][1 + NHS];  // fetched planes in flight: own cell, halo elements
         T own[3][4];    // own cell's U, V, W at planes q-3..q
-        T pP, pPn;      // P of the owned cell at planes q-3 and q-2
+        T pq[4];        // P of the owned cell at planes q-3..q (the last just fetched)
         double acc;
         bool own_xy;    // the owned column is interior in x and y
         double pacc;    // forcing power of the owned interior cells
@@ -183,6 +198,13 @@ struct HsTile {
     {
         return plane_base(a.g, a.wrap ? mod1(q, a.g.nz) : q);
     }
+    __host__ __device__ static uint32_t plane_bytes(const Geom &g) { return (uint32_t)(g.sx * g.sy * sizeof(T)); }
+    __host__ __device__ static uint32_t vo(long off) { return off >= 0 ? (uint32_t)(off * (long)sizeof(T)) : kOff0; }
+    // plane q of A, or an empty plane (every access out of range) when !ok
+    __host__ __device__ static Plane plane_of(const T *A, const Geom &g, long base, bool ok)
+    {
+        return make_plane(ok ? A + base : A, ok ? plane_bytes(g) : 0u);
+    }
 
     __host__ __device__ static void init(const HsArgs<T> &a, int bx, int by, int bz, int tid, Thread &th)
     {
@@ -213,13 +235,16 @@ struct HsTile {
             const int hx = th.X0 - 2 + lx, hy = th.Y0 - 2 + ly;
             th.h_off[s] = a.wrap ? in_plane_img(g, hx, hy) : in_plane(g, hx, hy);
         }
+        th.o_vo = vo(th.o_off);
+        th.u_vo = vo(th.u_off);
+        for (int s = 0; s < NHS; s++) th.h_vo[s] = vo(th.h_off[s]);
         th.r_row = tid / RPR;
         th.r_k = tid % RPR;
         th.b_row = tid / BPR;
         th.b_k = tid % BPR;
         for (int f = 0; f < 3; f++)
             for (int s = 0; s < 4; s++) th.own[f][s] = T(0);
-        th.pP = th.pPn = T(0);
+        for (int k = 0; k < 4; k++) th.pq[k] = T(0);
         th.acc = 0.0;
         th.pacc = 0.0;
         {
@@ -228,18 +253,19 @@ struct HsTile {
         }
     }
 
-    // loads of plane q into register slot s
+    // loads of plane q into register slot s (planes past z1 + 2 are not
+    // needed: their loads read as 0 without touching memory)
     __host__ __device__ static void fetch(const HsArgs<T> &a, int q, Thread &th, int s)
     {
-        const bool zin = plane_in(a.g, q);
+        const bool zin = plane_in(a.g, q) && q <= th.z1 + 2;
         const long base = load_plane(a, q);
+        const Plane pu = plane_of(a.Us, a.g, base, zin), pv = plane_of(a.Vs, a.g, base, zin),
+                    pw = plane_of(a.Ws, a.g, base, zin);
         for (int e = 0; e <= NHS; e++) {
-            const long off = e == 0 ? th.u_off : th.h_off[e - 1];
-            const bool ok = zin && off >= 0;
-            const long p = ok ? base + off : 0;
-            th.pf[s][0][e] = ok ? a.Us[p] : T(0);
-            th.pf[s][1][e] = ok ? a.Vs[p] : T(0);
-            th.pf[s][2][e] = ok ? a.Ws[p] : T(0);
+            const uint32_t off = e == 0 ? th.u_vo : th.h_vo[e - 1];
+            th.pf[s][0][e] = pload<T>(pu, off);
+            th.pf[s][1][e] = pload<T>(pv, off);
+            th.pf[s][2][e] = pload<T>(pw, off);
         }
     }
 
@@ -289,9 +315,11 @@ struct HsTile {
         }
     }
 
+    // P of the owned cell at plane o into the top of the P queue (planes
+    // outside [z0, z1] read as 0)
     __host__ __device__ static void fetch_p(const HsArgs<T> &a, int o, Thread &th)
     {
-        if (th.o_off >= 0) th.pPn = a.Ps[plane_base(a.g, o) + th.o_off];
+        th.pq[3] = pload<T>(plane_of(a.Ps, a.g, plane_base(a.g, o), o >= th.z0 && o <= th.z1), th.o_vo);
     }
 
     __host__ __device__ static bool owned(const Thread &th, int x, int y, int z)
@@ -349,10 +377,10 @@ struct HsTile {
     }
 
     // phase B, second part: final values of the owned cell on plane o
-    // (its U, V, W are own[.][0], its P is pP)
+    // (its U, V, W are own[.][0], its P is pq[0]).  Planes below z0 and cells
+    // outside the array compute values that are not stored.
     __host__ __device__ static void output(const HsArgs<T> &a, T *lds, int o, Thread &th)
     {
-        if (th.o_off < 0) return;
         const int tx = th.tid % TX, ty = th.tid / TX;
         const int x = th.X0 + tx, y = th.Y0 + ty, dx = tx + 1, dy = ty + 1;
         const HsCoef<T> &c = a.c;
@@ -360,7 +388,7 @@ struct HsTile {
         // correction of this cell and of its +x, +y, +z neighbours (0 outside R)
         const T d0 = D(lds, o, dy, dx), dx1 = D(lds, o, dy, dx + 1), dy1 = D(lds, o, dy + 1, dx);
         const T dz1 = D(lds, o + 1, dy, dx);
-        T pv = th.pP;
+        T pv = th.pq[0];
         if (visited<COLORING>(a.R, x, y, o)) pv += d0 * a.dti;
         T u = th.own[0][0], v = th.own[1][0], w = th.own[2][0];
         if (isred) {
@@ -378,43 +406,51 @@ struct HsTile {
             w -= dz1 * c.dzi;
             w += d0 * c.dzi;
         }
-        const long gi = plane_base(a.g, o) + th.o_off;
-        a.Pd[gi] = pv;
-        a.Ud[gi] = u;
-        a.Vd[gi] = v;
-        a.Wd[gi] = w;
+        const long base = plane_base(a.g, o);
+        const bool zin = o >= th.z0;
+        pstore<T>(plane_of(a.Pd, a.g, base, zin), th.o_vo, pv);
+        pstore<T>(plane_of(a.Ud, a.g, base, zin), th.o_vo, u);
+        pstore<T>(plane_of(a.Vd, a.g, base, zin), th.o_vo, v);
+        pstore<T>(plane_of(a.Wd, a.g, base, zin), th.o_vo, w);
     }
 
-    // One step of the march (the kernel runs it with s alternating 0, 1 so
-    // the register slots are compile-time constants).  SYNC is a workgroup
-    // barrier in the kernel; the host harness runs every thread's part A,
-    // then every thread's part B.
-    __host__ __device__ static void step_a(const HsArgs<T> &a, T *lds, int q, Thread &th, int s)
+    // One step of the march.  Step q = z0 - 1 + u fetches plane q + 3 into
+    // slot (u + 2) mod 3 and stores plane q + 1 from slot u mod 3; the kernel
+    // unrolls the march by three so both are compile-time constants.  SYNC is
+    // a workgroup barrier in the kernel; the host harness runs every thread's
+    // part A, then every thread's part B.
+    __host__ __device__ static void step_a(const HsArgs<T> &a, T *lds, int q, Thread &th, int sf)
     {
-        const int z0 = th.z0, z1 = th.z1;
-        if (q + 2 <= z1 + 2) fetch(a, q + 2, th, s ^ 1);
-        if (q - 2 >= z0 && q - 2 <= z1) fetch_p(a, q - 2, th);
-        if (q <= z1 + 2) red(a, lds, q, th);
+        fetch(a, q + 3, th, sf);
+        fetch_p(a, q, th);
+        if (q <= th.z1 + 2) red(a, lds, q, th);
     }
-    __host__ __device__ static void step_b(const HsArgs<T> &a, T *lds, int q, Thread &th, int s)
+    __host__ __device__ static void step_b(const HsArgs<T> &a, T *lds, int q, Thread &th, int ss)
     {
         const int z0 = th.z0, z1 = th.z1;
         if (q - 1 >= z0 && q - 1 <= z1 + 1) black(a, lds, q - 1, th);
-        if (q - 3 >= z0) output(a, lds, q - 3, th);
+        output(a, lds, q - 3, th);
         if (q + 1 <= z1 + 2)
-            store(a, lds, q + 1, th, s);
+            store(a, lds, q + 1, th, ss);
         else
-            shift(th, s);
-        th.pP = th.pPn;
+            shift(th, ss);
+        th.pq[0] = th.pq[1];
+        th.pq[1] = th.pq[2];
+        th.pq[2] = th.pq[3];
     }
-    // before the first step: planes z0-2 and z0-1 in LDS, plane z0 in flight (slot 0)
+    // the first and last steps of a chunk's march
+    __host__ __device__ static int first_step(const Thread &th) { return th.z0 - 1; }
+    __host__ __device__ static int last_step(const Thread &th) { return th.z1 + 3; }
+    // before the first step: planes z0-2 and z0-1 in LDS, planes z0 and z0+1
+    // in flight (slots 0 and 1)
     __host__ __device__ static void prologue(const HsArgs<T> &a, T *lds, Thread &th)
     {
-        fetch(a, th.z0 - 2, th, 0);
-        store(a, lds, th.z0 - 2, th, 0);
-        fetch(a, th.z0 - 1, th, 0);
-        store(a, lds, th.z0 - 1, th, 0);
+        fetch(a, th.z0 - 2, th, 2);
+        store(a, lds, th.z0 - 2, th, 2);
+        fetch(a, th.z0 - 1, th, 2);
+        store(a, lds, th.z0 - 1, th, 2);
         fetch(a, th.z0, th, 0);
+        fetch(a, th.z0 + 1, th, 1);
     }
 
     // The march of one workgroup for a host harness: the threads of each
@@ -423,12 +459,10 @@ struct HsTile {
                                 double *power = nullptr)
     {
         for (int t = 0; t < NT; t++) init(a, bx, by, bz, t, th[t]);
-        const int z0 = th[0].z0, z1 = th[0].z1;
         for (int t = 0; t < NT; t++) prologue(a, lds, th[t]);
-        int s = 0;
-        for (int q = z0 - 1; q <= z1 + 3; q++, s ^= 1) {
-            for (int t = 0; t < NT; t++) step_a(a, lds, q, th[t], s);
-            for (int t = 0; t < NT; t++) step_b(a, lds, q, th[t], s);
+        for (int q = first_step(th[0]), u = 0; q <= last_step(th[0]); q++, u++) {
+            for (int t = 0; t < NT; t++) step_a(a, lds, q, th[t], (u + 2) % NSLOT);
+            for (int t = 0; t < NT; t++) step_b(a, lds, q, th[t], u % NSLOT);
         }
         double sum = 0.0;
         for (int t = 0; t < NT; t++) sum += th[t].acc;

@@ -126,8 +126,50 @@ hipError_t launch(int kind, float4 *a, float4 *b, float4 *c, float *part, long n
     return hipGetLastError();
 }
 
+// b = a one float per thread: the width of the stencil kernels' own loads
+template <bool NT>
+__global__ void __launch_bounds__(kThreads) k_copy_dword(const float *__restrict__ a, float *__restrict__ b, long n)
+{
+    const long i = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    if constexpr (NT)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(a + i), b + i);
+    else
+        b[i] = a[i];
+}
+
+// the same with U floats per thread at a stride of one workgroup: as many
+// workgroups (U = 4) as the 16-byte copy with one vector per thread, so the
+// two differ only in the width of each load and store
+template <int U>
+__global__ void __launch_bounds__(kThreads) k_copy_dwords(const float *__restrict__ a, float *__restrict__ b, long n)
+{
+    const long i0 = (long)blockIdx.x * (kThreads * U) + threadIdx.x;
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if (i0 + (long)u * kThreads < n) v[u] = a[i0 + (long)u * kThreads];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if (i0 + (long)u * kThreads < n) b[i0 + (long)u * kThreads] = v[u];
+}
+
 hipError_t launch_variant(int variant, int kind, float4 *a, float4 *b, float4 *c, float *part, long n, hipStream_t s)
 {
+    if (variant >= 8) {  // dword copies (kind 2 only)
+        const long m = 4 * n;
+        const int per = variant == 10 ? 4 : (variant == 11 ? 16 : 1);
+        const unsigned blocks = (unsigned)((m + (long)kThreads * per - 1) / ((long)kThreads * per));
+        if (variant == 8)
+            hipLaunchKernelGGL(k_copy_dword<false>, dim3(blocks), dim3(kThreads), 0, s, (const float *)a, (float *)b, m);
+        else if (variant == 9)
+            hipLaunchKernelGGL(k_copy_dword<true>, dim3(blocks), dim3(kThreads), 0, s, (const float *)a, (float *)b, m);
+        else if (variant == 10)
+            hipLaunchKernelGGL(k_copy_dwords<4>, dim3(blocks), dim3(kThreads), 0, s, (const float *)a, (float *)b, m);
+        else
+            hipLaunchKernelGGL(k_copy_dwords<16>, dim3(blocks), dim3(kThreads), 0, s, (const float *)a, (float *)b, m);
+        return hipGetLastError();
+    }
     switch (variant) {
     case 0: return launch<1, false>(kind, a, b, c, part, n, s);
     case 1: return launch<2, false>(kind, a, b, c, part, n, s);
@@ -144,15 +186,16 @@ hipError_t launch_variant(int variant, int kind, float4 *a, float4 *b, float4 *c
 
 extern "C" {
 
-// Number of kernel variants lcs_stream_probe tries per kind.
-int lcs_stream_variants(void) { return 8; }
+// Number of kernel variants lcs_stream_probe tries per kind: 16-byte vectors
+// (U = 1, 2, 4, 8 per thread; plain, then nontemporal), then for copies one
+// float per thread (plain, nontemporal) and 4 and 16 floats per thread.
+int lcs_stream_variants(void) { return 12; }
 
 // Times kind (0 read, 1 write, 2 copy b = a, 3 triad a = b + s c) over arrays
-// of `bytes` each on `device`, variant by variant (U = 1, 2, 4, 8 vectors per
-// thread; plain then nontemporal), `reps` launches each after one warm-up.
-// gbs[v] receives variant v's best rate in GB/s (bytes read + written per
-// launch / fastest launch); ms[v] that launch's time.  Returns 0 or a
-// hipError_t.
+// of `bytes` each on `device`, variant by variant, `reps` launches each after
+// one warm-up.  gbs[v] (12 entries) receives variant v's best rate in GB/s
+// (bytes read + written per launch / fastest launch), ms[v] that launch's
+// time; variants a kind does not have stay 0.  Returns 0 or a hipError_t.
 int lcs_stream_probe(int device, int kind, size_t bytes, int reps, double *gbs, double *ms)
 {
     hipError_t e = hipSetDevice(device);
@@ -172,7 +215,9 @@ int lcs_stream_probe(int device, int kind, size_t bytes, int reps, double *gbs, 
     if (e == hipSuccess) e = hipEventCreate(&e1);
     for (int q = 0; q < narr && e == hipSuccess; q++) e = hipMemsetAsync(arr[q], 0, (size_t)n * sizeof(float4), s);
     const double moved = (double)n * sizeof(float4) * (kind == 3 ? 3.0 : (kind == 2 ? 2.0 : 1.0));
-    for (int v = 0; v < 8 && e == hipSuccess; v++) {
+    const int nvar = kind == 2 ? 12 : 8;
+    for (int v = 0; v < 12; v++) gbs[v] = ms[v] = 0.0;
+    for (int v = 0; v < nvar && e == hipSuccess; v++) {
         float best = 1e30f;
         for (int r = 0; r <= reps && e == hipSuccess; r++) {
             e = hipEventRecord(e0, s);
