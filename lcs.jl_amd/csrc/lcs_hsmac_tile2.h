@@ -22,6 +22,9 @@
 // Cells of E outside the array, and planes outside it, hold 0 -- what the
 // second of two separate sweeps would load there.
 //
+// Global loads and stores are range-checked plane accesses (lcs_bufio.h), with
+// no branch around them, as in the single sweep.
+//
 // The div^2 sums of both sweeps are kept apart (one per check of the
 // reference).  Every element of the output set is written by exactly one
 // workgroup, so the result equals two launches of the single sweep bit for bit.
@@ -66,6 +69,9 @@ struct HsTile2 {
         int tid, X0, Y0, z0, z1, za0, za1;
         int a_lds[NA];    // A-input ring index (ay * AX + ax) of element tid + k NT, -1 if none
         long a_off[NA];   // its in-plane offset, -1 outside the array
+        uint32_t a_vo[NA];  // byte offsets of a_off (kOff0 outside the array)
+        uint32_t p_vo[NE];  // byte offsets of the owned E cells' P (kOff0 for the others)
+        uint32_t o_vo;      // byte offset of o_off (kOff0 outside the array)
         int e_own[NE];    // E slot k: index of the owned cell it is, -1 if it is not one
         long o_off;       // the thread's owned cell (sweep B), -1 outside the array
         T pf[2][3][NA];   // two A-input planes in flight
@@ -131,6 +137,8 @@ struct HsTile2 {
             th.a_lds[k] = ay * AX + ax;
             th.a_off[k] = One::in_plane(g, th.X0 - 4 + ax, th.Y0 - 4 + ay);
         }
+        for (int k = 0; k < NA; k++) th.a_vo[k] = One::vo(th.a_off[k]);
+        th.o_vo = One::vo(th.o_off);
         for (int k = 0; k < NE; k++) {
             const int e = tid + k * NT;
             th.e_own[k] = -1;
@@ -141,6 +149,7 @@ struct HsTile2 {
             const int ex = e % EX, ey = e / EX;
             if (ex >= 2 && ex < TX + 2 && ey >= 2 && ey < TY + 2) th.e_own[k] = (ey - 2) * TX + (ex - 2);
         }
+        for (int k = 0; k < NE; k++) th.p_vo[k] = th.e_own[k] >= 0 && k < NA ? th.a_vo[k] : kOff0;
         th.r_row = tid / A_RPR;
         th.r_k = tid % A_RPR;
         th.b_row = tid / A_BPR;
@@ -156,16 +165,18 @@ struct HsTile2 {
     }
 
     // ---- sweep A ---------------------------------------------------------------
+    // A's input plane q into register slot s (planes past za1 + 2 read as 0
+    // without touching memory)
     __host__ __device__ static void fetch(const Hs2Args<T> &a, int q, Thread &th, int s)
     {
-        const bool zin = One::plane_in(a.g, q);
+        const bool zin = One::plane_in(a.g, q) && q <= th.za1 + 2;
         const long base = One::plane_base(a.g, q);
+        const Plane pu = One::plane_of(a.Us, a.g, base, zin), pv = One::plane_of(a.Vs, a.g, base, zin),
+                    pw = One::plane_of(a.Ws, a.g, base, zin);
         for (int k = 0; k < NA; k++) {
-            const bool ok = zin && th.a_off[k] >= 0;
-            const long p = ok ? base + th.a_off[k] : 0;
-            th.pf[s][0][k] = ok ? a.Us[p] : T(0);
-            th.pf[s][1][k] = ok ? a.Vs[p] : T(0);
-            th.pf[s][2][k] = ok ? a.Ws[p] : T(0);
+            th.pf[s][0][k] = pload<T>(pu, th.a_vo[k]);
+            th.pf[s][1][k] = pload<T>(pv, th.a_vo[k]);
+            th.pf[s][2][k] = pload<T>(pw, th.a_vo[k]);
         }
     }
     // the E cells' queues advance by one plane (the new top comes from slot s)
@@ -190,12 +201,13 @@ struct HsTile2 {
         }
         shift(th, s);
     }
+    // A's P input of the owned E cells at plane o (A's planes reach two beyond
+    // the chunk, and so past the array: those read as 0 and are not used)
     __host__ __device__ static void fetch_p(const Hs2Args<T> &a, int o, Thread &th)
     {
-        if (!One::plane_in(a.g, o)) return;  // A's planes reach two beyond the chunk, and so past the array
-        const long base = One::plane_base(a.g, o);
-        for (int k = 0; k < NE; k++)
-            if (th.e_own[k] >= 0 && th.a_off[k] >= 0) th.apPn[k] = a.Ps[base + th.a_off[k]];
+        const bool zin = One::plane_in(a.g, o) && o >= th.za0 && o <= th.za1;
+        const Plane pp = One::plane_of(a.Ps, a.g, One::plane_base(a.g, o), zin);
+        for (int k = 0; k < NE; k++) th.apPn[k] = pload<T>(pp, th.p_vo[k]);
     }
     __host__ __device__ static void red_a(const Hs2Args<T> &a, T *lds, int q, Thread &th)
     {
@@ -329,9 +341,10 @@ struct HsTile2 {
         }
         BD(lds, p, dy, dx) = dp;
     }
+    // final values of the owned cell on plane o; stored only for o in [z0, z1]
+    // and cells inside the array
     __host__ __device__ static void output_b(const Hs2Args<T> &a, T *lds, int o, Thread &th)
     {
-        if (th.o_off < 0) return;
         const int tx = th.tid % TX, ty = th.tid / TX;
         const int x = th.X0 + tx, y = th.Y0 + ty, dx = tx + 1, dy = ty + 1;
         const HsCoef<T> &c = a.c;
@@ -356,11 +369,12 @@ struct HsTile2 {
             w -= dz1 * c.dzi;
             w += d0 * c.dzi;
         }
-        const long gi = One::plane_base(a.g, o) + th.o_off;
-        a.Pd[gi] = pv;
-        a.Ud[gi] = u;
-        a.Vd[gi] = v;
-        a.Wd[gi] = w;
+        const long base = One::plane_base(a.g, o);
+        const bool zin = o >= th.z0 && o <= th.z1;
+        pstore<T>(One::plane_of(a.Pd, a.g, base, zin), th.o_vo, pv);
+        pstore<T>(One::plane_of(a.Ud, a.g, base, zin), th.o_vo, u);
+        pstore<T>(One::plane_of(a.Vd, a.g, base, zin), th.o_vo, v);
+        pstore<T>(One::plane_of(a.Wd, a.g, base, zin), th.o_vo, w);
     }
     // B's own-cell queues take plane r of A's result
     __host__ __device__ static void take_b(T *lds, int r, Thread &th)
@@ -382,8 +396,8 @@ struct HsTile2 {
     // ---- the march -----------------------------------------------------------------
     __host__ __device__ static void step_1(const Hs2Args<T> &a, T *lds, int q, Thread &th, int s)
     {
-        if (q + 2 <= th.za1 + 2) fetch(a, q + 2, th, s ^ 1);
-        if (q - 2 >= th.za0 && q - 2 <= th.za1) fetch_p(a, q - 2, th);
+        fetch(a, q + 2, th, s ^ 1);
+        fetch_p(a, q - 2, th);
         if (q <= th.za1 + 2) red_a(a, lds, q, th);
         const int qb = q - 5;
         if (qb >= th.z0 - 1 && qb <= th.z1 + 2) red_b(a, lds, qb, th);
@@ -399,7 +413,7 @@ struct HsTile2 {
         for (int k = 0; k < NE; k++) th.apP[k] = th.apPn[k];
         const int qb = q - 5;
         if (qb - 1 >= th.z0 && qb - 1 <= th.z1 + 1) black_b(a, lds, qb - 1, th);
-        if (qb - 3 >= th.z0 && qb - 3 <= th.z1) output_b(a, lds, qb - 3, th);
+        output_b(a, lds, qb - 3, th);
         take_b(lds, q - 4, th);
     }
     // before the first step: A's planes za0-2 and za0-1 in LDS, za0 in flight (slot 0)
