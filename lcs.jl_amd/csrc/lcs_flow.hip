@@ -1301,22 +1301,26 @@ __device__ inline int xcd_block()
     return x < rem ? x * (per_x + 1) + i0 : rem * (per_x + 1) + (x - rem) * per_x + i0;
 }
 
-// Columns of a 64 x 4 patch march up a chunk of zc planes, each keeping its
+// Columns of a 64 x PY patch march up a chunk of zc planes, each keeping its
 // last four W values in registers, so the z-stencil reads W once instead of
 // four times (a row-walking block cannot keep four planes in L2 at 1024^2
-// cells a plane); U and V neighbours come from the patch and its y
-// neighbours, which run on the same XCD.  The maxima do not depend on the
-// order, so dt is the row-walking kernel's to the bit.
-constexpr int kCflPX = 64, kCflPY = 4;
+// cells a plane).  U and V neighbours are loads of the same workgroup (L1
+// and L2 hits) except the patch's own ring: with 16-row patches V reads 19
+// rows per 16 from HBM (4-row patches read 7 per 4, and the counters showed
+// the y-neighbour patch's rows did not stay in L2 for them: 1.67x the three
+// arrays at 1024^3).  The maxima do not depend on the order, so dt is the
+// row-walking kernel's to the bit.
+constexpr int kCflPX = 64;
+inline int cfl_patch_rows(const Geom &g) { return g.ny >= 64 ? 16 : 4; }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_cfl(Geom g, const T *__restrict__ U, const T *__restrict__ V,
-                                            const T *__restrict__ W, double *part, int zc)
+template <typename T, int PY>
+__global__ void __launch_bounds__(kCflPX *PY) k_cfl(Geom g, const T *__restrict__ U, const T *__restrict__ V,
+                                                   const T *__restrict__ W, double *part, int zc)
 {
     __shared__ double lds[16];
     int bx, by, bz;
     xcd_tile(bx, by, bz);
-    const int x = 1 + bx * kCflPX + (int)threadIdx.x % kCflPX, y = 1 + by * kCflPY + (int)threadIdx.x / kCflPX;
+    const int x = 1 + bx * kCflPX + (int)threadIdx.x % kCflPX, y = 1 + by * PY + (int)threadIdx.x / kCflPX;
     const int z0 = 1 + bz * zc, z1 = min(g.nz, z0 + zc - 1);
     double m[3] = {0.0, 0.0, 0.0};
     if (x <= g.nx && y <= g.ny && z0 <= z1) {
@@ -1341,6 +1345,24 @@ __global__ void __launch_bounds__(256) k_cfl(Geom g, const T *__restrict__ U, co
         double s = block_max(m[a], lds);
         if (threadIdx.x == 0) part[(long)a * kMaxPartBlocks + blk] = s;
     }
+}
+
+// grid of the column-marching row kernels (CFL, forcing means): at most
+// 4096 workgroups, z-chunks of at least 8 planes
+struct ColGrid {
+    int gx, gy, gz, zc, py;
+    int blocks() const { return gx * gy * gz; }
+};
+inline ColGrid col_grid(const Geom &g)
+{
+    ColGrid c;
+    c.py = cfl_patch_rows(g);
+    c.gx = (g.nx + kCflPX - 1) / kCflPX;
+    c.gy = (g.ny + c.py - 1) / c.py;
+    c.gz = std::max(1, std::min((g.nz + 7) / 8, 4096 / (c.gx * c.gy)));
+    c.zc = (g.nz + c.gz - 1) / c.gz;
+    c.gz = (g.nz + c.zc - 1) / c.zc;
+    return c;
 }
 
 // dt from the per-axis maxima (src/flows/timestep.jl:56-64)
@@ -1378,20 +1400,20 @@ int launch_cfl(Box &b, double cfl, double dtmax, bool fill)
     void *arr[3] = {b.f[0], b.f[1], b.f[2]};
     int rc = fill ? launch_ghost_fill(b, arr, 3, b.g.h, 0, nullptr) : LCS_OK;
     if (rc) return rc;
-    // at most 4096 workgroups; z-chunks of at least 8 planes
-    const int gx = (b.g.nx + kCflPX - 1) / kCflPX, gy = (b.g.ny + kCflPY - 1) / kCflPY;
-    int gz = std::max(1, std::min((b.g.nz + 7) / 8, 4096 / (gx * gy)));
-    const int zc = (b.g.nz + gz - 1) / gz;
-    gz = (b.g.nz + zc - 1) / zc;
-    const int blocks = gx * gy * gz;
+    const ColGrid c = col_grid(b.g);
+    const int blocks = c.blocks();
     if (blocks > kMaxPartBlocks) return fail(LCS_ERR_INVALID, "grid too large for the CFL reduction");
     ProfScope ps(b, "cfl", 3.0 * b.esize * b.g.cells());
-    if (b.esize == 8)
-        hipLaunchKernelGGL(k_cfl<double>, dim3(gx, gy, gz), dim3(256), 0, b.stream, b.g, (const double *)b.f[0],
-                           (const double *)b.f[1], (const double *)b.f[2], b.part, zc);
-    else
-        hipLaunchKernelGGL(k_cfl<float>, dim3(gx, gy, gz), dim3(256), 0, b.stream, b.g, (const float *)b.f[0],
-                           (const float *)b.f[1], (const float *)b.f[2], b.part, zc);
+    const dim3 grid(c.gx, c.gy, c.gz);
+#define CFL_LAUNCH(T, PY)                                                                                      \
+    hipLaunchKernelGGL((k_cfl<T, PY>), grid, dim3(kCflPX * PY), 0, b.stream, b.g, (const T *)b.f[0],          \
+                       (const T *)b.f[1], (const T *)b.f[2], b.part, c.zc)
+    if (b.esize == 8) {
+        if (c.py == 16) CFL_LAUNCH(double, 16); else CFL_LAUNCH(double, 4);
+    } else {
+        if (c.py == 16) CFL_LAUNCH(float, 16); else CFL_LAUNCH(float, 4);
+    }
+#undef CFL_LAUNCH
     LCS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(256), 0, b.stream, b.part, blocks, cfl, dtmax,
                        spacing(b.g.nx), spacing(b.g.ny), spacing(b.g.nz), b.Re(), b.S);
@@ -1420,16 +1442,42 @@ __device__ inline void lf_means_block(const Geom &g, const T *__restrict__ U, co
     });
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_lf_means(Geom g, const T *__restrict__ U, const T *__restrict__ V,
-                                                 const T *__restrict__ W, double *part)
+// The means of one box march columns of a 64 x PY patch up a chunk of
+// planes like k_cfl (W of the plane below stays in a register), so each of
+// U, V, W is read about once from HBM; a row-walking pass read V and W twice
+// (the row below and the plane below were no longer in L2).  Sums are per
+// workgroup in a fixed order, combined in block order by k_lf_finalize.
+template <typename T, int PY>
+__global__ void __launch_bounds__(kCflPX *PY) k_lf_means(Geom g, const T *__restrict__ U, const T *__restrict__ V,
+                                                        const T *__restrict__ W, double *part, int zc)
 {
     __shared__ double lds[16];
-    double s[4];
-    lf_means_block(g, U, V, W, xcd_block(), gridDim.x, s);
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int x = 1 + bx * kCflPX + (int)threadIdx.x % kCflPX, y = 1 + by * PY + (int)threadIdx.x / kCflPX;
+    const int z0 = 1 + bz * zc, z1 = min(g.nz, z0 + zc - 1);
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    if (x <= g.nx && y <= g.ny && z0 <= z1) {
+        const long sy = g.sx, sz = g.sx * g.sy;
+        long p = g.at(x, y, z0);
+        T wm = W[p - sz];
+        for (int z = z0; z <= z1; z++, p += sz) {
+            const T u0 = U[p], v0 = V[p], w0 = W[p];
+            const T u = interp_o2(U[p - 1], u0);
+            const T v = interp_o2(V[p - sy], v0);
+            const T w = interp_o2(wm, w0);
+            const T u2 = (u * u + v * v) + w * w;
+            s[0] += (double)u2;
+            s[1] += (double)u0;
+            s[2] += (double)v0;
+            s[3] += (double)w0;
+            wm = w0;
+        }
+    }
+    const int blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     for (int a = 0; a < 4; a++) {
         double r = block_sum(s[a], lds);
-        if (threadIdx.x == 0) part[(long)a * kMaxPartBlocks + blockIdx.x] = r;
+        if (threadIdx.x == 0) part[(long)a * kMaxPartBlocks + blk] = r;
     }
 }
 
@@ -1669,16 +1717,22 @@ int launch_force_linear_coef(Box &b, double power, bool face_only)
     void *arr[3] = {b.f[0], b.f[1], b.f[2]};
     int rc = face_only ? launch_ghost_fill(b, arr, 3, 1, 0, nullptr) : launch_ghost_fill(b, arr, 3, b.g.h, 1, nullptr);
     if (rc) return rc;
-    int blocks = grid_rows(b.g);
+    const ColGrid c = col_grid(b.g);
+    const int blocks = c.blocks();
+    if (blocks > kMaxPartBlocks) return fail(LCS_ERR_INVALID, "grid too large for the forcing reduction");
     double len = (double)b.g.cells();
     {
     ProfScope ps(b, "lf_means", 3.0 * b.esize * len);
-    if (b.esize == 8)
-        hipLaunchKernelGGL(k_lf_means<double>, dim3(blocks), dim3(256), 0, b.stream, b.g, (const double *)b.f[0],
-                           (const double *)b.f[1], (const double *)b.f[2], b.part);
-    else
-        hipLaunchKernelGGL(k_lf_means<float>, dim3(blocks), dim3(256), 0, b.stream, b.g, (const float *)b.f[0],
-                           (const float *)b.f[1], (const float *)b.f[2], b.part);
+    const dim3 grid(c.gx, c.gy, c.gz);
+#define LF_LAUNCH(T, PY)                                                                                       \
+    hipLaunchKernelGGL((k_lf_means<T, PY>), grid, dim3(kCflPX * PY), 0, b.stream, b.g, (const T *)b.f[0],     \
+                       (const T *)b.f[1], (const T *)b.f[2], b.part, c.zc)
+    if (b.esize == 8) {
+        if (c.py == 16) LF_LAUNCH(double, 16); else LF_LAUNCH(double, 4);
+    } else {
+        if (c.py == 16) LF_LAUNCH(float, 16); else LF_LAUNCH(float, 4);
+    }
+#undef LF_LAUNCH
     LCS_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_lf_finalize, dim3(1), dim3(256), 0, b.stream, b.part, blocks, len, power, b.S);
