@@ -651,21 +651,35 @@ __device__ inline double hsmac_sweep_block(const HsArgs<T> &a, int bx, int by, i
     Tile::prologue(a, lds, th);
     __syncthreads();
     const int q1 = Tile::last_step(th);
-    for (int q = Tile::first_step(th); q <= q1; q += 3) {  // unrolled by the three register slots
-        Tile::step_a(a, lds, q, th, 2);
-        __syncthreads();
-        Tile::step_b(a, lds, q, th, 0);
-        __syncthreads();
-        if (q + 1 > q1) break;
-        Tile::step_a(a, lds, q + 1, th, 0);
-        __syncthreads();
-        Tile::step_b(a, lds, q + 1, th, 1);
-        __syncthreads();
-        if (q + 2 > q1) break;
-        Tile::step_a(a, lds, q + 2, th, 1);
-        __syncthreads();
-        Tile::step_b(a, lds, q + 2, th, 2);
-        __syncthreads();
+    if constexpr (Tile::NSLOT == 3) {
+        for (int q = Tile::first_step(th); q <= q1; q += 3) {  // unrolled by the three register slots
+            Tile::step_a(a, lds, q, th, 2);
+            __syncthreads();
+            Tile::step_b(a, lds, q, th, 0);
+            __syncthreads();
+            if (q + 1 > q1) break;
+            Tile::step_a(a, lds, q + 1, th, 0);
+            __syncthreads();
+            Tile::step_b(a, lds, q + 1, th, 1);
+            __syncthreads();
+            if (q + 2 > q1) break;
+            Tile::step_a(a, lds, q + 2, th, 1);
+            __syncthreads();
+            Tile::step_b(a, lds, q + 2, th, 2);
+            __syncthreads();
+        }
+    } else {
+        for (int q = Tile::first_step(th); q <= q1; q += 2) {  // unrolled by the two register slots
+            Tile::step_a(a, lds, q, th, 1);
+            __syncthreads();
+            Tile::step_b(a, lds, q, th, 0);
+            __syncthreads();
+            if (q + 1 > q1) break;
+            Tile::step_a(a, lds, q + 1, th, 0);
+            __syncthreads();
+            Tile::step_b(a, lds, q + 1, th, 1);
+            __syncthreads();
+        }
     }
     if (FORCE) *power = block_sum(th.pacc, red_lds);
     return block_sum(th.acc, red_lds);

@@ -29,10 +29,15 @@
 // W(z); the grid covers the whole array, so elements outside R are carried
 // over to the other set.
 //
-// Every global load and store is a range-checked plane access (lcs_bufio.h):
-// elements outside the array read as 0 and stores outside it are dropped, so
-// the march has no branch around a memory operation and the plane loads
-// issued three steps ahead stay in flight until the step that stores them.
+// In 1024-thread tiles every global load and store is a range-checked plane
+// access (lcs_bufio.h): elements outside the array read as 0 and stores
+// outside it are dropped, so the march has no branch around a memory
+// operation and the plane loads issued three steps ahead stay in flight until
+// the step that stores them.  Smaller tiles, which run several workgroups per
+// CU over short chunks, use exec-masked global loads and stores and fetch two
+// planes ahead: the plane resources' scalar work per step cost them more than
+// the exact waits gained (96-box ensemble 4,530 against 4,160 box-steps/s),
+// while the 1024^3 sweep gains 2 % from them.
 //
 // All of it is __host__ __device__ so a host harness can run the phases
 // thread by thread against the oracle.
@@ -146,9 +151,12 @@ struct HsTile {
     static constexpr int NHS = (NHALO + NT - 1) / NT;              // halo elements per thread
     static constexpr int RPR = (DXN + 1) / 2, BPR = (TX + 2) / 2;  // cells of one colour per row
     static constexpr int BYN = TY + 1;
-    // register slots of fetched planes: plane p lives in slot (p - z0) mod 3
-    // from the step that fetches it (p - 3) to the step that stores it (p - 1)
-    static constexpr int NSLOT = 3;
+    // plane accesses (lcs_bufio.h) for 1024-thread tiles
+    static constexpr bool PLANES = NT >= 1024;
+    // register slots of fetched planes: plane p lives in slot (p - z0) mod
+    // NSLOT from the step that fetches it (p - NSLOT) to the step that stores
+    // it (p - 1)
+    static constexpr int NSLOT = PLANES ? 3 : 2;
     static_assert(RPR * DYN <= NT && BPR * BYN <= NT, "one red and one black slot per thread");
 
     struct Thread {
@@ -259,13 +267,24 @@ struct HsTile {
     {
         const bool zin = plane_in(a.g, q) && q <= th.z1 + 2;
         const long base = load_plane(a, q);
-        const Plane pu = plane_of(a.Us, a.g, base, zin), pv = plane_of(a.Vs, a.g, base, zin),
-                    pw = plane_of(a.Ws, a.g, base, zin);
-        for (int e = 0; e <= NHS; e++) {
-            const uint32_t off = e == 0 ? th.u_vo : th.h_vo[e - 1];
-            th.pf[s][0][e] = pload<T>(pu, off);
-            th.pf[s][1][e] = pload<T>(pv, off);
-            th.pf[s][2][e] = pload<T>(pw, off);
+        if constexpr (PLANES) {
+            const Plane pu = plane_of(a.Us, a.g, base, zin), pv = plane_of(a.Vs, a.g, base, zin),
+                        pw = plane_of(a.Ws, a.g, base, zin);
+            for (int e = 0; e <= NHS; e++) {
+                const uint32_t off = e == 0 ? th.u_vo : th.h_vo[e - 1];
+                th.pf[s][0][e] = pload<T>(pu, off);
+                th.pf[s][1][e] = pload<T>(pv, off);
+                th.pf[s][2][e] = pload<T>(pw, off);
+            }
+        } else {
+            for (int e = 0; e <= NHS; e++) {
+                const long off = e == 0 ? th.u_off : th.h_off[e - 1];
+                const bool ok = zin && off >= 0;
+                const long p = ok ? base + off : 0;
+                th.pf[s][0][e] = ok ? a.Us[p] : T(0);
+                th.pf[s][1][e] = ok ? a.Vs[p] : T(0);
+                th.pf[s][2][e] = ok ? a.Ws[p] : T(0);
+            }
         }
     }
 
@@ -319,7 +338,10 @@ struct HsTile {
     // outside [z0, z1] read as 0)
     __host__ __device__ static void fetch_p(const HsArgs<T> &a, int o, Thread &th)
     {
-        th.pq[3] = pload<T>(plane_of(a.Ps, a.g, plane_base(a.g, o), o >= th.z0 && o <= th.z1), th.o_vo);
+        if constexpr (PLANES)
+            th.pq[3] = pload<T>(plane_of(a.Ps, a.g, plane_base(a.g, o), o >= th.z0 && o <= th.z1), th.o_vo);
+        else if (th.o_off >= 0)
+            th.pq[3] = a.Ps[plane_base(a.g, o) + th.o_off];
     }
 
     __host__ __device__ static bool owned(const Thread &th, int x, int y, int z)
@@ -381,6 +403,7 @@ struct HsTile {
     // outside the array compute values that are not stored.
     __host__ __device__ static void output(const HsArgs<T> &a, T *lds, int o, Thread &th)
     {
+        if (!PLANES && th.o_off < 0) return;
         const int tx = th.tid % TX, ty = th.tid / TX;
         const int x = th.X0 + tx, y = th.Y0 + ty, dx = tx + 1, dy = ty + 1;
         const HsCoef<T> &c = a.c;
@@ -407,50 +430,67 @@ struct HsTile {
             w += d0 * c.dzi;
         }
         const long base = plane_base(a.g, o);
-        const bool zin = o >= th.z0;
-        pstore<T>(plane_of(a.Pd, a.g, base, zin), th.o_vo, pv);
-        pstore<T>(plane_of(a.Ud, a.g, base, zin), th.o_vo, u);
-        pstore<T>(plane_of(a.Vd, a.g, base, zin), th.o_vo, v);
-        pstore<T>(plane_of(a.Wd, a.g, base, zin), th.o_vo, w);
+        if constexpr (PLANES) {
+            const bool zin = o >= th.z0;
+            pstore<T>(plane_of(a.Pd, a.g, base, zin), th.o_vo, pv);
+            pstore<T>(plane_of(a.Ud, a.g, base, zin), th.o_vo, u);
+            pstore<T>(plane_of(a.Vd, a.g, base, zin), th.o_vo, v);
+            pstore<T>(plane_of(a.Wd, a.g, base, zin), th.o_vo, w);
+        } else {
+            const long gi = base + th.o_off;
+            a.Pd[gi] = pv;
+            a.Ud[gi] = u;
+            a.Vd[gi] = v;
+            a.Wd[gi] = w;
+        }
     }
 
-    // One step of the march.  Step q = z0 - 1 + u fetches plane q + 3 into
-    // slot (u + 2) mod 3 and stores plane q + 1 from slot u mod 3; the kernel
-    // unrolls the march by three so both are compile-time constants.  SYNC is
-    // a workgroup barrier in the kernel; the host harness runs every thread's
-    // part A, then every thread's part B.
+    // One step of the march.  Step q = z0 - 1 + u fetches plane q + NSLOT into
+    // slot sf = (u + NSLOT - 1) mod NSLOT and stores plane q + 1 from slot
+    // ss = u mod NSLOT; the kernel unrolls the march by NSLOT so both are
+    // compile-time constants.  P of plane q (plane accesses) or q - 2 enters
+    // the top of the P queue.  SYNC is a workgroup barrier in the kernel; the
+    // host harness runs every thread's part A, then every thread's part B.
     __host__ __device__ static void step_a(const HsArgs<T> &a, T *lds, int q, Thread &th, int sf)
     {
-        fetch(a, q + 3, th, sf);
-        fetch_p(a, q, th);
+        if constexpr (PLANES) {
+            fetch(a, q + 3, th, sf);
+            fetch_p(a, q, th);
+        } else {
+            if (q + 2 <= th.z1 + 2) fetch(a, q + 2, th, sf);
+            if (q - 2 >= th.z0 && q - 2 <= th.z1) fetch_p(a, q - 2, th);
+        }
         if (q <= th.z1 + 2) red(a, lds, q, th);
     }
     __host__ __device__ static void step_b(const HsArgs<T> &a, T *lds, int q, Thread &th, int ss)
     {
         const int z0 = th.z0, z1 = th.z1;
         if (q - 1 >= z0 && q - 1 <= z1 + 1) black(a, lds, q - 1, th);
-        output(a, lds, q - 3, th);
+        if (PLANES || q - 3 >= z0) output(a, lds, q - 3, th);
         if (q + 1 <= z1 + 2)
             store(a, lds, q + 1, th, ss);
         else
             shift(th, ss);
-        th.pq[0] = th.pq[1];
-        th.pq[1] = th.pq[2];
-        th.pq[2] = th.pq[3];
+        if constexpr (PLANES) {
+            th.pq[0] = th.pq[1];
+            th.pq[1] = th.pq[2];
+            th.pq[2] = th.pq[3];
+        } else {
+            th.pq[0] = th.pq[3];
+        }
     }
     // the first and last steps of a chunk's march
     __host__ __device__ static int first_step(const Thread &th) { return th.z0 - 1; }
     __host__ __device__ static int last_step(const Thread &th) { return th.z1 + 3; }
-    // before the first step: planes z0-2 and z0-1 in LDS, planes z0 and z0+1
-    // in flight (slots 0 and 1)
+    // before the first step: planes z0-2 and z0-1 in LDS, planes z0 .. z0 +
+    // NSLOT - 2 in flight (slots 0 ..)
     __host__ __device__ static void prologue(const HsArgs<T> &a, T *lds, Thread &th)
     {
-        fetch(a, th.z0 - 2, th, 2);
-        store(a, lds, th.z0 - 2, th, 2);
-        fetch(a, th.z0 - 1, th, 2);
-        store(a, lds, th.z0 - 1, th, 2);
-        fetch(a, th.z0, th, 0);
-        fetch(a, th.z0 + 1, th, 1);
+        fetch(a, th.z0 - 2, th, NSLOT - 1);
+        store(a, lds, th.z0 - 2, th, NSLOT - 1);
+        fetch(a, th.z0 - 1, th, NSLOT - 1);
+        store(a, lds, th.z0 - 1, th, NSLOT - 1);
+        for (int k = 0; k < NSLOT - 1; k++) fetch(a, th.z0 + k, th, k);
     }
 
     // The march of one workgroup for a host harness: the threads of each
@@ -461,7 +501,7 @@ struct HsTile {
         for (int t = 0; t < NT; t++) init(a, bx, by, bz, t, th[t]);
         for (int t = 0; t < NT; t++) prologue(a, lds, th[t]);
         for (int q = first_step(th[0]), u = 0; q <= last_step(th[0]); q++, u++) {
-            for (int t = 0; t < NT; t++) step_a(a, lds, q, th[t], (u + 2) % NSLOT);
+            for (int t = 0; t < NT; t++) step_a(a, lds, q, th[t], (u + NSLOT - 1) % NSLOT);
             for (int t = 0; t < NT; t++) step_b(a, lds, q, th[t], u % NSLOT);
         }
         double sum = 0.0;
