@@ -122,6 +122,15 @@ def bench_ensemble(args):
     for _ in range(args.warmup):
         batch.step(want_log=False)
     batch.sync()
+    tiling = None
+    if not args.no_autotune:  # the batch's sweep tiling per size class, timed on the boxes as they are
+        t0 = time.perf_counter()
+        tiles = batch.autotune()
+        tiling = {"x".join(map(str, c.grid.dims)): f"{t[0]}x{t[1]}, z-chunks of {t[2]}" for c, t in zip(cfgs, tiles)}
+        print(f"# batch tiling ({time.perf_counter() - t0:.1f} s): {tiling}", file=sys.stderr, flush=True)
+        for _ in range(max(1, args.warmup)):  # tables rebuilt, graphs captured again
+            batch.step(want_log=False)
+        batch.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         batch.step(want_log=False)
@@ -138,7 +147,8 @@ def bench_ensemble(args):
                                f"[1e3, 1e6], St log-uniform [0.1, 10], k0 in {{2, 4, 6}}, linear-power P=0.5",
                    "boxes": len(boxes), "cells": cells,
                    "particles": sum(c.particles[0].population.valid for c in cfgs),
-                   "particle_localize_interval": args.localize_interval, "graphs": not args.no_graphs},
+                   "particle_localize_interval": args.localize_interval, "graphs": not args.no_graphs,
+                   "sweep_tiling": tiling or "32x8, z-chunks of about 28 planes (default)"},
         "cell_steps_per_s": cells * args.steps / el,
         "serial": {"value": serial, "unit": "box-steps/s", "steps_per_box": args.serial_steps,
                    "how": "the same boxes, one lcs_step call per box at a time, same GPU"},

@@ -378,12 +378,39 @@ int lcs_batch_size(lcs_batch *batch, int *nboxes);
  * per box in the order of addition (at most nlogs) */
 int lcs_batch_step(lcs_batch *batch, lcs_step_log *logs, int nlogs);
 int lcs_batch_sync(lcs_batch *batch);
-/* Each step's runs of launches between convergence polls (stage prefix up to
- * couple!, the HSMAC groups before itrmin, one HSMAC group, stage 2 up to the
- * linear-power forcing) are captured as hipGraphs on the second step after a
- * membership change and replayed from then on; on by default.  Results are
- * the same either way. */
+/* hipGraphs of the batched step; results are the same in every mode.
+ * on = 1 (default): the first step after a membership change runs eagerly;
+ * the second captures its runs of launches between convergence polls (stage
+ * prefix up to couple!, the HSMAC groups before itrmin, one HSMAC group,
+ * stage 2 up to the linear-power forcing) and replays them; from the third
+ * step on, each step is ONE graph: both stages with as many HSMAC groups per
+ * stage as any of the last 8 steps needed, a device-side gate after them that
+ * stops the rest of the graph (every kernel returns at once) if a box still
+ * needs groups, and the copies of both stages' convergence state to the host.
+ * A stopped step goes on from the gate with polled groups.  Batches with
+ * rc-energy boxes stay on the segments (their forcing takes host pointers).
+ * on = 2: segments only.  on = 3: one graph per step holding no groups beyond
+ * those before itrmin (every step that needs more goes on from the gate; for
+ * tests).  on = 0: eager launches. */
 int lcs_batch_set_graphs(lcs_batch *batch, int on);
+/* No counterpart in the reference (its kernels have fixed workgroups).  The
+ * batched HSMAC sweep tiles each size class of boxes (equal nx, ny, nz) with
+ * one shape, 64x8, 32x8, 64x16 or 32x16, in z-chunks of a target length; by
+ * default 32x8 and about 28 planes for every class.  lcs_batch_autotune times
+ * the sweeps of one HSMAC group over the whole batch for every shape and
+ * chunk target of 16, 28 and 48 planes, one class at a time with the others
+ * at their current choice, in two rounds, and keeps the fastest per class.
+ * The timed sweeps read the boxes' current fields and write only their
+ * ping-pong partners, so no box changes.  Fields and sweep counts do not
+ * depend on the tiling; the logged divergence and forcing power are sums of
+ * per-tile partials, so their last bits do.  Synchronises; the next step
+ * rebuilds the batch's tables and graphs. */
+int lcs_batch_autotune(lcs_batch *batch);
+/* Sets the tiling of the size class of box `index` (order of addition);
+ * LCS_ERR_INVALID for another shape or zchunk < 1. */
+int lcs_batch_set_tiles(lcs_batch *batch, int index, int tx, int ty, int zchunk);
+/* The sweep tile of box `index` and its z-chunk length in planes. */
+int lcs_batch_get_tiles(lcs_batch *batch, int index, int *tx, int *ty, int *zchunk);
 
 /* ---- serving ---------------------------------------------------------------- */
 /* No counterpart in the reference.  A request loop over one batch: runs are

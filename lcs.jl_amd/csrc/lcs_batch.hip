@@ -6,7 +6,9 @@
 // single-box kernels' own device code, so a box advanced in a batch gives the
 // same result as the same box advanced alone with lcs_step.
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <deque>
 #include <map>
 #include <vector>
 
@@ -64,20 +66,35 @@ struct Batch {
     int cap_boxes = 0;
     // hipGraphs of the runs of launches between host polls, keyed by segment
     // id; captured on the second step after a membership change (the first
-    // runs eagerly and sets the kernels' attributes) and replayed afterwards
+    // runs eagerly and sets the kernels' attributes) and replayed afterwards.
+    // From the third step on, each step is one graph (step_graphs, keyed by
+    // the post groups it holds per stage) unless graph_mode is 2.
     bool graphs = true, warm = false;
+    int graph_mode = 1;  // lcs_batch_set_graphs: 1 whole steps, 2 segments only, 3 whole steps, fewest groups
+    std::map<std::array<int, 2>, hipGraphExec_t> step_graphs;
+    std::array<int, 2> post_seen = {0, 0};        // post groups the last call of each stage needed
+    std::deque<std::array<int, 2>> post_hist;     // ... over the last kPostWindow steps
+    int *halt_d = nullptr, *halt_h = nullptr;     // stop flag of the step graphs (BoxRec::halt)
     // the next fused sweep applies the deferred linear forcing of the
     // linear-power boxes (set around stage 2's HSMAC only)
     bool force_next = false;
     bool wrap_next = false;  // the next sweep loads ghost U, V, W from their images
     bool wrap_ok = false;    // every box has n >= h on each axis
     std::map<int, hipGraphExec_t> gexec;
+    // sweep tiling per size class (nx, ny, nz): index into kBatchSweepShapes
+    // and z-chunk target in planes (lcs_batch_autotune, lcs_batch_set_tiles)
+    std::map<std::array<int, 3>, std::array<int, 2>> sweep_tune;
 };
+
+constexpr int kPostWindow = 8;
 
 static void drop_graphs(Batch &B)
 {
     for (auto &kv : B.gexec) (void)hipGraphExecDestroy(kv.second);
     B.gexec.clear();
+    for (auto &kv : B.step_graphs) (void)hipGraphExecDestroy(kv.second);
+    B.step_graphs.clear();
+    B.post_hist.clear();
     B.warm = false;
 }
 
@@ -129,13 +146,23 @@ static std::vector<E> xcd_order(const std::vector<E> &v)
 
 static int blocks_of(long n, int per = 256) { return (int)((n + per - 1) / per); }
 
-// Sweep tile shapes of a batch, one launch per shape (results do not depend
-// on them).  Every box takes 32 x 8 tiles, so a sweep is one launch: in a
-// mixed 96-box ensemble this beat one launch of 64 x 8 tiles plus one of
-// 32 x 8 tiles chosen by lane waste per box (23.5 vs 24.4 ms a step), and
-// 16-row tiles were slower (25.2 ms).
+// Sweep tile shapes of a batch, one launch per shape in use (fields do not
+// depend on them; the per-tile partial sums of the divergence do).  Until
+// lcs_batch_autotune or lcs_batch_set_tiles chooses per size class, every box
+// takes 32 x 8 tiles in z-chunks of about 28 planes, so a sweep is one
+// launch: in a mixed 96-box ensemble this beat one
+// launch of 64 x 8 tiles plus one of 32 x 8 tiles chosen by lane waste per
+// box (23.5 vs 24.4 ms a step), and 16-row tiles were slower (25.2 ms).
 static const int kBatchSweepShapes[4][2] = {{64, 8}, {32, 8}, {64, 16}, {32, 16}};
-static int sweep_shape(const Box &) { return 1; }
+constexpr int kBatchSweepChunk = 28;
+// z-chunk targets the autotuner tries
+static const int kBatchSweepChunks[3] = {16, 28, 48};
+
+static std::array<int, 2> sweep_tiling(const Batch &B, const Geom &g)
+{
+    const auto it = B.sweep_tune.find({g.nx, g.ny, g.nz});
+    return it == B.sweep_tune.end() ? std::array<int, 2>{1, kBatchSweepChunk} : it->second;
+}
 
 template <typename T>
 static void fill_coefs(const Box &b, const lcs_hsmac_params &hs, BoxRec &r)
@@ -207,8 +234,9 @@ static int rebuild(Batch &B)
         if (B.sc_d) LCS_HIP(hipFree(B.sc_d));
         if (B.sc_h) LCS_HIP(hipHostFree(B.sc_h));
         B.cap_boxes = std::max(n, 2 * B.cap_boxes);
-        LCS_HIP(hipMalloc(&B.poll_d, sizeof(HsPoll) * (size_t)B.cap_boxes));
-        LCS_HIP(hipHostMalloc((void **)&B.poll_h, sizeof(HsPoll) * (size_t)B.cap_boxes, hipHostMallocDefault));
+        // two polls per box: stage 1 and stage 2 of a step graph
+        LCS_HIP(hipMalloc(&B.poll_d, sizeof(HsPoll) * 2 * (size_t)B.cap_boxes));
+        LCS_HIP(hipHostMalloc((void **)&B.poll_h, sizeof(HsPoll) * 2 * (size_t)B.cap_boxes, hipHostMallocDefault));
         LCS_HIP(hipMalloc(&B.sc_d, sizeof(DevScalars) * (size_t)B.cap_boxes));
         LCS_HIP(hipHostMalloc((void **)&B.sc_h, sizeof(DevScalars) * (size_t)B.cap_boxes, hipHostMallocDefault));
     }
@@ -241,6 +269,7 @@ static int rebuild(Batch &B)
         BoxRec &r = B.tab_h[i];
         r.g = g;
         r.S = b.S;
+        r.halt = B.halt_d;
         r.part = b.part;
         r.Re = b.Re();
         r.power = B.cfgs[i].forcing.kind == LCS_FORCING_LINEAR_POWER ? B.cfgs[i].forcing.power : 0.0;
@@ -257,16 +286,17 @@ static int rebuild(Batch &B)
         for (int z = 0; z * r.rhs_zc < g.nz; z++)
             for (int y = 0; y < rgy; y++)
                 for (int x = 0; x < rgx; x++) rtiles.push_back({i, x, y, z});
-        // fused HSMAC sweep tiles over the whole array, in z-chunks of about 28
-        // planes: short chunks balance the launch across boxes of different
-        // sizes despite their five re-read halo planes (64-plane chunks: 25.1 ms
-        // a step, 12-plane chunks: 24.1 ms)
+        // fused HSMAC sweep tiles over the whole array, in z-chunks (by default
+        // of about 28 planes: short chunks balance the launch across boxes of
+        // different sizes despite their five re-read halo planes; 64-plane
+        // chunks: 25.1 ms a step, 12-plane chunks: 24.1 ms)
         const int zplanes = g.nz + 2 * g.h;
-        const int shape = sweep_shape(b);
+        const std::array<int, 2> tiling = sweep_tiling(B, g);
+        const int shape = tiling[0];
         r.hs_tx = kBatchSweepShapes[shape][0];
         r.hs_ty = kBatchSweepShapes[shape][1];
         const int hgx = (g.nx + 2 * g.h + r.hs_tx - 1) / r.hs_tx, hgy = (g.ny + 2 * g.h + r.hs_ty - 1) / r.hs_ty;
-        const int hgz = std::max(1, zplanes / 28);
+        const int hgz = std::max(1, zplanes / tiling[1]);
         r.hs_zc = (zplanes + hgz - 1) / hgz;
         int nz_used = (zplanes + r.hs_zc - 1) / r.hs_zc;
         r.hs_tiles = hgx * hgy * nz_used;
@@ -472,31 +502,65 @@ static int b_group(Batch &B, int set, double dtfac)
     return LCS_OK;
 }
 
-// set 1 acts on (U2, V2, W2, P) (stage 1), set 0 on (U, V, W, P) (stage 2).
-// Groups before the first one that may end the iteration (itrmin) run
-// unconditionally with the begin kernel in one segment; after that each group
-// is one segment followed by a poll of every box's convergence flag.
-static int b_hsmac(Batch &B, int set, double dtfac, int slot, int seg_id)
+// HSMAC groups of a call: all of them, and those before the first one that
+// may end the iteration (itrmin), which run without a poll after them
+static int hs_groups(const Batch &B) { return B.hs.time_blocking ? (B.hs.itrmax + 2) / 3 : B.hs.itrmax; }
+static int hs_pre_groups(const Batch &B)
+{
+    const int ngroups = hs_groups(B);
+    int pre = 0;
+    while (pre < ngroups && (B.hs.time_blocking ? 3 * pre + 2 : pre) < B.hs.itrmin) pre++;
+    return pre;
+}
+
+// The groups after the pre groups that a finished HSMAC call needed, from the
+// poll of every box: a box's final itr counts its checks that did not
+// converge, so it converged in group itr / 3 (time-blocked: three checks a
+// group) or itr (naive); a box that never converged ran every group.
+static int hs_post_needed(const Batch &B, const HsPoll *poll)
+{
+    const int ngroups = hs_groups(B), n = (int)B.boxes.size();
+    int need = 0;
+    for (int i = 0; i < n; i++) {
+        const int g = poll[i].done ? (B.hs.time_blocking ? poll[i].itr / 3 : poll[i].itr) + 1 : ngroups;
+        need = std::max(need, std::min(g, ngroups));
+    }
+    return std::max(0, need - hs_pre_groups(B));
+}
+
+// begin + the pre groups of a call on `set`
+static int enqueue_hsmac_head(Batch &B, int set, double dtfac)
+{
+    const int pre = hs_pre_groups(B);
+    int r = launch_b_hsmac_ctl(B.stream, B.tab.d, (int)B.boxes.size(), kHsBegin, 0, nullptr);
+    for (int g = 0; g < pre && r == LCS_OK; g++) r = b_group(B, set, dtfac);
+    return r;
+}
+
+// swaps the host mirror of the arrays of `set` for every box whose result
+// lives in the spares (the device table was swapped by the commit kernel)
+static void mirror_commit(Batch &B, int set, const HsPoll *poll)
+{
+    for (int i = 0; i < (int)B.boxes.size(); i++)
+        if (poll[i].par) {
+            Box &b = *B.boxes[i];
+            for (int q = 0; q < 4; q++) {
+                const int f = q < 3 ? 3 * set + q : LCS_FIELD_P;
+                std::swap(b.f[f], b.spare[q]);
+            }
+            refresh_record(B, i);
+        }
+}
+
+// Groups g0.. of a call on `set`, each one segment followed by a poll of every
+// box's convergence flag, then the end, commit and host mirror swap.
+static int b_hsmac_from(Batch &B, int set, double dtfac, int slot, int seg_id, int g0)
 {
     const int n = (int)B.boxes.size();
-    const lcs_hsmac_params &p = B.hs;
-    const int ngroups = p.time_blocking ? (p.itrmax + 2) / 3 : p.itrmax;
-    int pre = 0;  // groups with no poll after them
-    while (pre < ngroups && (p.time_blocking ? 3 * pre + 2 : pre) < p.itrmin) pre++;
-    int rc = segment(B, seg_id, [&]() {
-        int r = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsBegin, 0, nullptr);
-        for (int g = 0; g < pre && r == LCS_OK; g++) r = b_group(B, set, dtfac);
-        return r;
-    });
-    if (rc) return rc;
-    // The deferred linear forcing belongs to the first sweep that runs.  A
-    // replayed segment does not pass through b_sweep, so the host flag is
-    // cleared here for captured and eager runs alike.  If no group precedes
-    // the first poll (small itrmin), the forced group is a segment of its
-    // own: the group segment that later polls replay must never force.
-    if (pre > 0) B.force_next = false;
+    const int ngroups = hs_groups(B);
+    int rc;
     bool done = false, polled_last = false;
-    for (int g = pre; g < ngroups && !done; g++) {
+    for (int g = g0; g < ngroups && !done; g++) {
         const int id = B.force_next ? seg_id + 2 : seg_id + 1;
         rc = segment(B, id, [&]() { return b_group(B, set, dtfac); });
         B.force_next = false;
@@ -508,16 +572,193 @@ static int b_hsmac(Batch &B, int set, double dtfac, int slot, int seg_id)
     // the final parity decides where each box's result lives: swap on the
     // device (commit) and in the host mirror
     if ((rc = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsCommit, set, nullptr))) return rc;
-    for (int i = 0; i < n; i++)
-        if (B.poll_h[i].par) {
-            Box &b = *B.boxes[i];
-            for (int q = 0; q < 4; q++) {
-                const int f = q < 3 ? 3 * set + q : LCS_FIELD_P;
-                std::swap(b.f[f], b.spare[q]);
-            }
-            refresh_record(B, i);
-        }
+    mirror_commit(B, set, B.poll_h);
+    B.post_seen[set == 1 ? 0 : 1] = hs_post_needed(B, B.poll_h);
     return LCS_OK;
+}
+
+// set 1 acts on (U2, V2, W2, P) (stage 1), set 0 on (U, V, W, P) (stage 2).
+// Groups before the first one that may end the iteration (itrmin) run
+// unconditionally with the begin kernel in one segment; after that each group
+// is one segment followed by a poll of every box's convergence flag.
+static int b_hsmac(Batch &B, int set, double dtfac, int slot, int seg_id)
+{
+    const int pre = hs_pre_groups(B);
+    int rc = segment(B, seg_id, [&]() { return enqueue_hsmac_head(B, set, dtfac); });
+    if (rc) return rc;
+    // The deferred linear forcing belongs to the first sweep that runs.  A
+    // replayed segment does not pass through b_sweep, so the host flag is
+    // cleared here for captured and eager runs alike.  If no group precedes
+    // the first poll (small itrmin), the forced group is a segment of its
+    // own: the group segment that later polls replay must never force.
+    if (pre > 0) B.force_next = false;
+    return b_hsmac_from(B, set, dtfac, slot, seg_id, pre);
+}
+
+// timestep! (CFL, particle dt) and stage 1 up to couple!; stage 1's FullHalo
+// fill comes first and serves the CFL too (it writes every ghost a FaceHalo
+// fill would, with the same values)
+static int enqueue_stage1_prefix(Batch &B)
+{
+    const int n = (int)B.boxes.size(), h = B.boxes[0]->g.h;
+    const lcs_step_config &c0 = B.cfgs[0];
+    int r;
+    if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 0, 4, h, 1, 0)) ||
+        (r = launch_b_cfl(B.stream, B.esize, B.tab.d, B.rows.d, B.rows.n, n, c0.cfl, c0.dtmax)) ||
+        (r = launch_b_particle_dt(B.stream, B.esize, B.tab.d, B.diams.d, B.diams.n, n, c0.dtmax)) ||
+        (r = launch_b_particles(B.stream, B.esize, B.tab.d, B.parts.d, B.parts.n, 1)) ||
+        (r = launch_b_rhs(B.stream, B.esize, B.tab.d, B.rhs_tiles.d, B.rhs_tiles.n, 1)))
+        return r;
+    return LCS_OK;
+}
+
+// stage 2 up to forcing!: linear-power boxes batched (rc-energy boxes are
+// forced one by one after it, with the host's current array pointers)
+static bool lf_deferred(const Batch &B) { return B.lf_boxes.n > 0 && B.hs.itrmax >= 1; }
+static int enqueue_stage2_prefix(Batch &B)
+{
+    const int h = B.boxes[0]->g.h;
+    const bool lf_defer = lf_deferred(B);
+    int r;
+    if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 1, 4, h, 1, 0)) ||
+        (r = launch_b_particles(B.stream, B.esize, B.tab.d, B.parts.d, B.parts.n, 2)) ||
+        (r = launch_b_rhs(B.stream, B.esize, B.tab.d, B.rhs_tiles.d, B.rhs_tiles.n, 2)))
+        return r;
+    if (B.lf_boxes.n > 0) {
+        // the means read the x = 0, y = 0, z = 0 ghosts alone; where the
+        // first sweep loads every ghost from its image, width 1 serves them
+        const bool one = lf_defer && B.hs.time_blocking && B.wrap_ok;
+        const DevVec<WorkItem> &gw = one ? B.ghost_one : B.ghost_full;
+        if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, gw.d, gw.n, 0, 3, one ? 1 : h, 1, 0)) ||
+            (r = launch_b_lf(B.stream, B.esize, B.tab.d, B.lf_rows.d, B.lf_rows.n, B.lf_boxes.d, B.lf_boxes.n,
+                             !lf_defer)))
+            return r;
+    }
+    return LCS_OK;
+}
+
+static void fill_logs(Batch &B, lcs_step_log *logs, int nlogs)
+{
+    for (int i = 0; i < (int)B.boxes.size() && i < nlogs; i++) {
+        const DevScalars &s = B.sc_h[i];
+        lcs_step_log &l = logs[i];
+        l.step = s.step;
+        l.t = s.t;
+        l.dt = s.dt;
+        l.itr1 = s.itr[0];
+        l.itr2 = s.itr[1];
+        l.div1 = s.div[0];
+        l.div2 = s.div[1];
+        l.p = s.p;
+        l.e10 = s.e10;
+    }
+}
+
+// Stage 2 from its prefix to the end of the step (the segmented path).
+static int step_from_stage2(Batch &B, lcs_step_log *logs, int nlogs)
+{
+    const int n = (int)B.boxes.size();
+    int rc = segment(B, 2, [&]() { return enqueue_stage2_prefix(B); });
+    if (rc) return rc;
+    for (int i : B.rcf_boxes) {
+        Box &b = *B.boxes[i];
+        const lcs_forcing &f = B.cfgs[i].forcing;
+        if ((rc = launch_force_rcf(b, f.energy, f.kmin, f.kmax, f.coarse_size, DtRef{&b.S->dt, 1.0}))) return rc;
+    }
+    // the first sweep of stage 2's couple! applies the linear forcing as it
+    // loads U, V, W (its coefficients were set above)
+    B.force_next = lf_deferred(B);
+    rc = b_hsmac(B, 0, 1.0, 1, 20);
+    B.force_next = false;
+    if (rc) return rc;
+    if ((rc = launch_b_state(B.stream, B.tab.d, n, logs ? B.sc_d : nullptr))) return rc;
+    if (logs) {
+        LCS_HIP(hipMemcpyAsync(B.sc_h, B.sc_d, sizeof(DevScalars) * (size_t)n, hipMemcpyDeviceToHost, B.stream));
+        LCS_HIP(hipStreamSynchronize(B.stream));
+        fill_logs(B, logs, nlogs);
+    }
+    return LCS_OK;
+}
+
+// Post groups a step graph holds per stage: the most any of the last
+// kPostWindow calls of that stage needed, and at least one where no group
+// precedes the first check (so the forced first sweep of stage 2 is always
+// inside the graph).  Mode 3 holds the least.
+static std::array<int, 2> step_graph_groups(const Batch &B)
+{
+    const int pre = hs_pre_groups(B), room = hs_groups(B) - pre;
+    std::array<int, 2> G = {0, 0};
+    for (int s = 0; s < 2; s++) {
+        if (B.graph_mode != 3)
+            for (const auto &h : B.post_hist) G[s] = std::max(G[s], h[s]);
+        G[s] = std::min(std::max(G[s], pre == 0 ? 1 : 0), room);
+    }
+    return G;
+}
+
+// One step as one captured graph: stage 1 (prefix, begin, pre groups, G1
+// more groups), a gate that stops the graph if a box still needs groups,
+// stage 1's end, commit and poll, stage 2 likewise, the state update, and the
+// copies of both polls, the scalars and the stop flag to the host.  Returns
+// the stop flag: 0, or the stage whose gate stopped the graph.
+static int step_graph(Batch &B, const std::array<int, 2> &G, int &halted)
+{
+    const int n = (int)B.boxes.size();
+    const int pre = hs_pre_groups(B), ngroups = hs_groups(B);
+    auto it = B.step_graphs.find(G);
+    if (it == B.step_graphs.end()) {
+        LCS_HIP(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+        auto enqueue = [&]() {
+            int r;
+            const std::array<int, 2> sets = {1, 0};
+            for (int s = 0; s < 2; s++) {
+                const int set = sets[s];
+                const double dtfac = s == 0 ? 0.5 : 1.0;
+                if ((r = s == 0 ? enqueue_stage1_prefix(B) : enqueue_stage2_prefix(B))) return r;
+                B.force_next = s == 1 && lf_deferred(B);
+                if ((r = enqueue_hsmac_head(B, set, dtfac))) return r;
+                for (int g = 0; g < G[s]; g++)
+                    if ((r = b_group(B, set, dtfac))) return r;
+                B.force_next = false;
+                if (pre + G[s] < ngroups && (r = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsGate, s + 1, nullptr)))
+                    return r;
+                if ((r = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsEnd, s, nullptr)) ||
+                    (r = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsCommit, set, nullptr)) ||
+                    (r = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsPoll, 0, B.poll_d + (long)s * n)))
+                    return r;
+            }
+            if ((r = launch_b_state(B.stream, B.tab.d, n, B.sc_d))) return r;
+            LCS_HIP(hipMemcpyAsync(B.poll_h, B.poll_d, sizeof(HsPoll) * 2 * (size_t)n, hipMemcpyDeviceToHost,
+                                   B.stream));
+            LCS_HIP(hipMemcpyAsync(B.sc_h, B.sc_d, sizeof(DevScalars) * (size_t)n, hipMemcpyDeviceToHost, B.stream));
+            LCS_HIP(hipMemcpyAsync(B.halt_h, B.halt_d, sizeof(int), hipMemcpyDeviceToHost, B.stream));
+            return LCS_OK;
+        };
+        const int rc = enqueue();
+        B.force_next = B.wrap_next = false;
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(B.stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(LCS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) return fail(LCS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        it = B.step_graphs.emplace(G, x).first;
+    }
+    LCS_HIP(hipGraphLaunch(it->second, B.stream));
+    LCS_HIP(hipStreamSynchronize(B.stream));
+    halted = *B.halt_h;
+    return LCS_OK;
+}
+
+static void record_post(Batch &B)
+{
+    B.post_hist.push_back(B.post_seen);
+    while ((int)B.post_hist.size() > kPostWindow) B.post_hist.pop_front();
 }
 
 static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
@@ -540,76 +781,145 @@ static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
         b.steps_host++;
     }
     if (moved && (rc = upload_table(B))) return rc;
-    const lcs_step_config &c0 = B.cfgs[0];
-    const int h = B.boxes[0]->g.h;
-    // timestep! (CFL, particle dt) and stage 1 up to couple!; stage 1's
-    // FullHalo fill comes first and serves the CFL too (it writes every ghost
-    // a FaceHalo fill would, with the same values)
-    rc = segment(B, 1, [&]() {
-        int r;
-        if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 0, 4, h, 1, 0)) ||
-            (r = launch_b_cfl(B.stream, B.esize, B.tab.d, B.rows.d, B.rows.n, n, c0.cfl, c0.dtmax)) ||
-            (r = launch_b_particle_dt(B.stream, B.esize, B.tab.d, B.diams.d, B.diams.n, n, c0.dtmax)) ||
-            (r = launch_b_particles(B.stream, B.esize, B.tab.d, B.parts.d, B.parts.n, 1)) ||
-            (r = launch_b_rhs(B.stream, B.esize, B.tab.d, B.rhs_tiles.d, B.rhs_tiles.n, 1)))
-            return r;
-        return LCS_OK;
-    });
-    if (rc || (rc = b_hsmac(B, 1, 0.5, 0, 10))) return rc;
-    // stage 2 up to forcing!: linear-power boxes batched in the segment,
-    // rc-energy boxes one by one after it (their launches take the host's
-    // current array pointers)
-    const bool lf_defer = B.lf_boxes.n > 0 && B.hs.itrmax >= 1;
-    rc = segment(B, 2, [&]() {
-        int r;
-        if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 1, 4, h, 1, 0)) ||
-            (r = launch_b_particles(B.stream, B.esize, B.tab.d, B.parts.d, B.parts.n, 2)) ||
-            (r = launch_b_rhs(B.stream, B.esize, B.tab.d, B.rhs_tiles.d, B.rhs_tiles.n, 2)))
-            return r;
-        if (B.lf_boxes.n > 0) {
-            // the means read the x = 0, y = 0, z = 0 ghosts alone; where the
-            // first sweep loads every ghost from its image, width 1 serves them
-            const bool one = lf_defer && B.hs.time_blocking && B.wrap_ok;
-            const DevVec<WorkItem> &gw = one ? B.ghost_one : B.ghost_full;
-            if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, gw.d, gw.n, 0, 3, one ? 1 : h, 1, 0)) ||
-                (r = launch_b_lf(B.stream, B.esize, B.tab.d, B.lf_rows.d, B.lf_rows.n, B.lf_boxes.d, B.lf_boxes.n,
-                                 !lf_defer)))
-                return r;
+    const int pre = hs_pre_groups(B);
+    // the whole step as one graph once the batch is warm and the post-group
+    // counts of a step are known (not with rc-energy boxes, which are forced
+    // through the host's pointers)
+    if (B.graphs && B.warm && B.graph_mode != 2 && B.rcf_boxes.empty() && !B.post_hist.empty()) {
+        const std::array<int, 2> G = step_graph_groups(B);
+        int halted = 0;
+        if ((rc = step_graph(B, G, halted))) return rc;
+        if (!halted) {
+            mirror_commit(B, 1, B.poll_h);
+            mirror_commit(B, 0, B.poll_h + n);
+            B.post_seen = {hs_post_needed(B, B.poll_h), hs_post_needed(B, B.poll_h + n)};
+            record_post(B);
+            if (logs) fill_logs(B, logs, nlogs);
+            return LCS_OK;
         }
+        // a stage needs more groups than the graph held: every kernel after
+        // its gate returned at once; go on from there with polled groups
+        if ((rc = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsUnhalt, 0, nullptr))) return rc;
+        if (halted == 1) {
+            if ((rc = b_hsmac_from(B, 1, 0.5, 0, 10, pre + G[0])) || (rc = step_from_stage2(B, logs, nlogs)))
+                return rc;
+        } else {
+            std::vector<HsPoll> p1(B.poll_h, B.poll_h + n);  // stage 1 finished inside the graph
+            mirror_commit(B, 1, p1.data());
+            B.post_seen[0] = hs_post_needed(B, p1.data());
+            B.force_next = false;  // the graph's first stage-2 sweep applied it
+            if ((rc = b_hsmac_from(B, 0, 1.0, 1, 20, pre + G[1]))) return rc;
+            if ((rc = launch_b_state(B.stream, B.tab.d, n, logs ? B.sc_d : nullptr))) return rc;
+            if (logs) {
+                LCS_HIP(hipMemcpyAsync(B.sc_h, B.sc_d, sizeof(DevScalars) * (size_t)n, hipMemcpyDeviceToHost,
+                                       B.stream));
+                LCS_HIP(hipStreamSynchronize(B.stream));
+                fill_logs(B, logs, nlogs);
+            }
+        }
+        record_post(B);
         return LCS_OK;
-    });
-    if (rc) return rc;
-    for (int i : B.rcf_boxes) {
-        Box &b = *B.boxes[i];
-        const lcs_forcing &f = B.cfgs[i].forcing;
-        if ((rc = launch_force_rcf(b, f.energy, f.kmin, f.kmax, f.coarse_size, DtRef{&b.S->dt, 1.0}))) return rc;
     }
-    // the first sweep of stage 2's couple! applies the linear forcing as it
-    // loads U, V, W (its coefficients were set above)
-    B.force_next = lf_defer;
-    rc = b_hsmac(B, 0, 1.0, 1, 20);
-    B.force_next = false;
-    if (rc) return rc;
-    if ((rc = launch_b_state(B.stream, B.tab.d, n, logs ? B.sc_d : nullptr))) return rc;
+    rc = segment(B, 1, [&]() { return enqueue_stage1_prefix(B); });
+    if (rc || (rc = b_hsmac(B, 1, 0.5, 0, 10))) return rc;
+    if ((rc = step_from_stage2(B, logs, nlogs))) return rc;
+    if (B.warm) record_post(B);
     B.warm = true;
-    if (logs) {
-        LCS_HIP(hipMemcpyAsync(B.sc_h, B.sc_d, sizeof(DevScalars) * (size_t)n, hipMemcpyDeviceToHost, B.stream));
-        LCS_HIP(hipStreamSynchronize(B.stream));
-        for (int i = 0; i < n && i < nlogs; i++) {
-            const DevScalars &s = B.sc_h[i];
-            lcs_step_log &l = logs[i];
-            l.step = s.step;
-            l.t = s.t;
-            l.dt = s.dt;
-            l.itr1 = s.itr[0];
-            l.itr2 = s.itr[1];
-            l.div1 = s.div[0];
-            l.div2 = s.div[1];
-            l.p = s.p;
-            l.e10 = s.e10;
-        }
+    return LCS_OK;
+}
+
+// ---- sweep tiling per size class ----------------------------------------------
+// Fastest of `reps` runs of the sweeps of one HSMAC group (time-blocked: the
+// group's three sweeps; naive: one) over the whole batch with the current
+// tiling.  Begin sets every box's parity to 0 and nothing flips it, so each
+// sweep reads the boxes' current arrays and writes only their ping-pong
+// partners, which hold no state between HSMAC calls: no box changes.
+static int time_group_sweeps(Batch &B, int reps, hipEvent_t e0, hipEvent_t e1, float &best)
+{
+    int rc;
+    if ((rc = rebuild(B))) return rc;
+    const int n = (int)B.boxes.size();
+    std::vector<int> kinds;
+    if (B.hs.time_blocking)
+        kinds = {B.hs.overlap ? 0 : 1, 2, 3};
+    else
+        kinds = {B.hs.overlap ? 0 : 3};
+    auto run = [&]() {
+        int r = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsBegin, 0, nullptr);
+        for (int k : kinds)
+            for (int q = 0; q < 4 && r == LCS_OK; q++)
+                r = launch_b_hsmac_sweep(B.stream, B.esize, B.hs.coloring, kBatchSweepShapes[q][0],
+                                         kBatchSweepShapes[q][1], B.tab.d, B.hs_tiles[q].d, B.hs_tiles[q].n, 0, k, 0,
+                                         1.0, false, false);
+        return r;
+    };
+    if ((rc = run())) return rc;
+    best = 1e30f;
+    for (int r = 0; r < reps; r++) {
+        LCS_HIP(hipEventRecord(e0, B.stream));
+        if ((rc = run())) return rc;
+        LCS_HIP(hipEventRecord(e1, B.stream));
+        LCS_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        LCS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, ms);
     }
     return LCS_OK;
+}
+
+// Coordinate descent over size classes: each class in turn tries every shape
+// and chunk target with the other classes at their current choice, and keeps
+// a candidate only if it beats the current one by more than 1 %.
+static int batch_autotune(Batch &B)
+{
+    if (B.boxes.empty()) return LCS_OK;
+    LCS_HIP(hipStreamSynchronize(B.stream));
+    std::vector<std::array<int, 3>> classes;
+    for (const Box *b : B.boxes) {
+        const std::array<int, 3> c = {b->g.nx, b->g.ny, b->g.nz};
+        if (std::find(classes.begin(), classes.end(), c) == classes.end()) classes.push_back(c);
+        if (!B.sweep_tune.count(c)) B.sweep_tune[c] = {1, kBatchSweepChunk};
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return fail(LCS_ERR_HIP, "hipEventCreate failed");
+    }
+    int rc = LCS_OK;
+    const int reps = 5;
+    for (int round = 0; round < 2 && rc == LCS_OK; round++)
+        for (const auto &c : classes) {
+            std::array<int, 2> best = B.sweep_tune[c];
+            float best_ms = 0;
+            B.dirty = true;
+            if ((rc = time_group_sweeps(B, reps, e0, e1, best_ms))) break;
+            for (int q = 0; q < 4 && rc == LCS_OK; q++)
+                for (int z : kBatchSweepChunks) {
+                    const std::array<int, 2> cand = {q, z};
+                    if (cand == best) continue;
+                    B.sweep_tune[c] = cand;
+                    B.dirty = true;
+                    float ms = 0;
+                    // a tiling whose per-box tile count does not fit a partial
+                    // slot is skipped, not an error
+                    rc = time_group_sweeps(B, reps, e0, e1, ms);
+                    if (rc == LCS_ERR_CAPACITY) {
+                        rc = LCS_OK;
+                        continue;
+                    }
+                    if (rc) break;
+                    if (ms < 0.99f * best_ms) {
+                        best = cand;
+                        best_ms = ms;
+                    }
+                }
+            B.sweep_tune[c] = best;
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    B.dirty = true;  // the next step rebuilds the tables (and drops the graphs)
+    if (rc == LCS_OK && (rc = rebuild(B))) return rc;
+    return rc;
 }
 
 static bool same_hsmac(const lcs_hsmac_params &a, const lcs_hsmac_params &b)
@@ -664,6 +974,16 @@ int lcs_batch_create(lcs_ctx *ctx, lcs_batch **out)
         delete B;
         return fail(LCS_ERR_HIP, "hipStreamCreate failed");
     }
+    if (hipMalloc(&B->halt_d, sizeof(int)) != hipSuccess ||
+        hipHostMalloc((void **)&B->halt_h, sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipMemset(B->halt_d, 0, sizeof(int)) != hipSuccess) {
+        if (B->halt_d) (void)hipFree(B->halt_d);
+        if (B->halt_h) (void)hipHostFree(B->halt_h);
+        (void)hipStreamDestroy(B->stream);
+        delete B;
+        return fail(LCS_ERR_HIP, "stop flag allocation failed");
+    }
+    *B->halt_h = 0;
     *out = reinterpret_cast<lcs_batch *>(B);
     return LCS_OK;
 }
@@ -687,6 +1007,8 @@ int lcs_batch_destroy(lcs_batch *batch)
     if (B->poll_h) (void)hipHostFree(B->poll_h);
     if (B->sc_d) (void)hipFree(B->sc_d);
     if (B->sc_h) (void)hipHostFree(B->sc_h);
+    if (B->halt_d) (void)hipFree(B->halt_d);
+    if (B->halt_h) (void)hipHostFree(B->halt_h);
     if (B->stream) (void)hipStreamDestroy(B->stream);
     delete B;
     return LCS_OK;
@@ -755,9 +1077,51 @@ int lcs_batch_set_graphs(lcs_batch *batch, int on)
 {
     if (!batch) return fail(LCS_ERR_INVALID, "null argument");
     Batch &B = *BT(batch);
+    if (on < 0 || on > 3) return fail(LCS_ERR_INVALID, "graph mode must be 0, 1, 2 or 3");
     LCS_HIP(hipStreamSynchronize(B.stream));
     drop_graphs(B);
     B.graphs = on != 0;
+    if (on) B.graph_mode = on;
+    return LCS_OK;
+}
+
+int lcs_batch_autotune(lcs_batch *batch)
+{
+    if (!batch) return fail(LCS_ERR_INVALID, "null argument");
+    Batch &B = *BT(batch);
+    LCS_HIP(hipSetDevice(B.ctx->device));
+    return batch_autotune(B);
+}
+
+int lcs_batch_set_tiles(lcs_batch *batch, int index, int tx, int ty, int zchunk)
+{
+    if (!batch) return fail(LCS_ERR_INVALID, "null argument");
+    Batch &B = *BT(batch);
+    if (index < 0 || index >= (int)B.boxes.size()) return fail(LCS_ERR_INVALID, "box index out of range");
+    int q = -1;
+    for (int k = 0; k < 4; k++)
+        if (kBatchSweepShapes[k][0] == tx && kBatchSweepShapes[k][1] == ty) q = k;
+    if (q < 0) return fail(LCS_ERR_INVALID, "sweep tile shape must be 64x8, 32x8, 64x16 or 32x16");
+    if (zchunk < 1) return fail(LCS_ERR_INVALID, "zchunk must be >= 1");
+    const Geom &g = B.boxes[index]->g;
+    LCS_HIP(hipStreamSynchronize(B.stream));
+    B.sweep_tune[{g.nx, g.ny, g.nz}] = {q, zchunk};
+    B.dirty = true;
+    return LCS_OK;
+}
+
+int lcs_batch_get_tiles(lcs_batch *batch, int index, int *tx, int *ty, int *zchunk)
+{
+    if (!batch || !tx || !ty || !zchunk) return fail(LCS_ERR_INVALID, "null argument");
+    Batch &B = *BT(batch);
+    if (index < 0 || index >= (int)B.boxes.size()) return fail(LCS_ERR_INVALID, "box index out of range");
+    LCS_HIP(hipSetDevice(B.ctx->device));
+    int rc;
+    if (B.dirty && (rc = rebuild(B))) return rc;
+    const BoxRec &r = B.tab_h[index];
+    *tx = r.hs_tx;
+    *ty = r.hs_ty;
+    *zchunk = r.hs_zc;
     return LCS_OK;
 }
 

@@ -2298,6 +2298,7 @@ __global__ void __launch_bounds__(256) k_b_ghost_fill(const BoxRec *tab, const W
 {
     const WorkItem w = work[blockIdx.x];
     const BoxRec &B = tab[w.box];
+    if (b_halted(B)) return;
     if (hsmac && B.S->hs_done) return;
     const bool alt = hsmac && B.S->hs_par;
     const Geom g = B.g;
@@ -2332,6 +2333,7 @@ __global__ void __launch_bounds__(256) k_b_cfl(const BoxRec *tab, const WorkItem
     __shared__ double lds[16];
     const WorkItem w = work[blockIdx.x];
     const BoxRec &B = tab[w.box];
+    if (b_halted(B)) return;
     double m[3];
     cfl_block(B.g, (const T *)B.f[0], (const T *)B.f[1], (const T *)B.f[2], w.blk, B.row_blocks, m);
     for (int a = 0; a < 3; a++) {
@@ -2344,6 +2346,7 @@ __global__ void k_b_cfl_fin(const BoxRec *tab, double cfl, double dtmax)
 {
     __shared__ double lds[16];
     const BoxRec &B = tab[blockIdx.x];
+    if (b_halted(B)) return;
     double mm[3];
     for (int a = 0; a < 3; a++) {
         double m = 0.0;
@@ -2372,6 +2375,7 @@ __global__ void __launch_bounds__(256) k_b_lf_means(const BoxRec *tab, const Wor
     __shared__ double lds[16];
     const WorkItem w = work[blockIdx.x];
     const BoxRec &B = tab[w.box];
+    if (b_halted(B)) return;
     double s4[4];
     lf_means_block(B.g, (const T *)B.f[0], (const T *)B.f[1], (const T *)B.f[2], w.blk, B.row_blocks, s4);
     for (int a = 0; a < 4; a++) {
@@ -2384,6 +2388,7 @@ __global__ void k_b_lf_fin(const BoxRec *tab, const int *boxes)
 {
     __shared__ double lds[16];
     const BoxRec &B = tab[boxes[blockIdx.x]];
+    if (b_halted(B)) return;
     double m[4];
     for (int a = 0; a < 4; a++) {
         double v = 0.0;
@@ -2399,6 +2404,7 @@ __global__ void __launch_bounds__(256) k_b_lf_apply(const BoxRec *tab, const Wor
     __shared__ double lds[16];
     const WorkItem w = work[blockIdx.x];
     const BoxRec &B = tab[w.box];
+    if (b_halted(B)) return;
     const T rdt = T(B.S->dt);  // stage-2 dt
     double v = block_sum(lf_apply_block(B.g, (T *)B.f[0], (T *)B.f[1], (T *)B.f[2], B.S, rdt, w.blk, B.row_blocks),
                          lds);
@@ -2411,6 +2417,7 @@ __global__ void k_b_lf_power(const BoxRec *tab, const int *boxes)
 {
     __shared__ double lds[16];
     const BoxRec &B = tab[boxes[blockIdx.x]];
+    if (b_halted(B)) return;
     double v = 0.0;
     for (int q = threadIdx.x; q < B.hs_tiles; q += blockDim.x) v += B.part[kPowerSlot + q];
     v = block_sum(v, lds);
@@ -2429,6 +2436,7 @@ __global__ void k_b_lf_mean(const BoxRec *tab, const int *boxes)
 {
     __shared__ double lds[16];
     const BoxRec &B = tab[boxes[blockIdx.x]];
+    if (b_halted(B)) return;
     double v = 0.0;
     for (int q = threadIdx.x; q < B.row_blocks; q += blockDim.x) v += B.part[q];
     v = block_sum(v, lds);
@@ -2462,6 +2470,7 @@ __global__ void __launch_bounds__(TX *TY) k_b_rhs_tile(const BoxRec *tab, const 
     extern __shared__ __align__(16) unsigned char rhs_lds_raw[];
     const TileItem t = tiles[blockIdx.x];
     const BoxRec &B = tab[t.box];
+    if (b_halted(B)) return;
     RhsArgs<T> a;
     a.g = B.g;
     const int in = stage == 1 ? 0 : 3;
@@ -2518,6 +2527,7 @@ __global__ void __launch_bounds__(TX *TY) k_b_hsmac_sweep(const BoxRec *tab, con
     __shared__ double red_lds[16];
     const TileItem t = tiles[blockIdx.x];
     const BoxRec &B = tab[t.box];
+    if (b_halted(B)) return;
     if (B.S->hs_done) return;
     BufSets<T> bs;
     for (int q = 0; q < 4; q++) {
@@ -2615,6 +2625,7 @@ __global__ void __launch_bounds__(256) k_b_hsmac_faces(const BoxRec *tab, const 
     __shared__ double lds[16];
     const WorkItem w = work[blockIdx.x];
     const BoxRec &B = tab[w.box];
+    if (b_halted(B)) return;
     const DevScalars *S = B.S;
     if (S->hs_done) return;
     const bool alt = S->hs_par != 0;
@@ -2695,6 +2706,7 @@ __global__ void __launch_bounds__(XfPatch<T>::N *XfPatch<T>::N)
     __shared__ double red_lds[16];
     const WorkItem w = work[blockIdx.x];
     const BoxRec &B = tab[w.box];
+    if (b_halted(B)) return;
     const DevScalars *S = B.S;
     if (S->hs_done) return;
     const bool alt = S->hs_par != 0;
@@ -2728,6 +2740,7 @@ __global__ void __launch_bounds__(256)
 {
     const WorkItem w = work[blockIdx.x];
     const BoxRec &B = tab[w.box];
+    if (b_halted(B)) return;
     const DevScalars *S = B.S;
     if (S->hs_done) return;
     const bool alt = S->hs_par != 0;
@@ -2787,6 +2800,7 @@ __global__ void k_b_hsmac_check(const BoxRec *tab, BCheckKinds kinds, int npass,
 {
     __shared__ double lds[16];
     const BoxRec &B = tab[blockIdx.x];
+    if (b_halted(B)) return;
     DevScalars *S = B.S;
     if (S->hs_done) return;
     double acc = 0.0;
@@ -2822,13 +2836,23 @@ int launch_b_hsmac_check(hipStream_t s, const BoxRec *tab, int nbox, const int *
     return LCS_OK;
 }
 
-// begin / flip / end(slot) / commit(set) / poll, one thread per box
+// begin / flip / end(slot) / commit(set) / poll / gate(stage) / unhalt, one
+// thread per box
 __global__ void k_b_hsmac_ctl(BoxRec *tab, int nbox, int op, int arg, HsPoll *poll)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbox) return;
     BoxRec &B = tab[b];
+    if (op == kHsUnhalt) {
+        if (b == 0) *B.halt = 0;
+        return;
+    }
+    if (b_halted(B)) return;
     DevScalars *S = B.S;
+    if (op == kHsGate) {  // every thread that stores, stores the same value
+        if (!S->hs_done) *B.halt = arg;
+        return;
+    }
     if (op == kHsBegin) {
         S->hs_done = 0;
         S->hs_itr = 0;
@@ -2868,6 +2892,7 @@ __global__ void k_b_state(const BoxRec *tab, int nbox, DevScalars *gather)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbox) return;
+    if (b_halted(tab[b])) return;
     DevScalars *S = tab[b].S;
     S->step += 1;
     S->t += S->dt;

@@ -283,8 +283,16 @@ struct BoxRec {
     int row_blocks;      // blocks of the row-walking reductions (CFL, forcing)
     int lf_force;        // linear-power forcing: applied by the first stage-2 sweep
     int nprof;
+    // the batch's stop flag (one int for all records): while it is nonzero
+    // every batched kernel returns at once (b_halted), so the rest of a
+    // captured step graph is skipped once a stage needs more HSMAC groups
+    // than the graph holds
+    int *halt;
     DevProfile prof[kMaxProfiles];
 };
+#if defined(__HIPCC__)
+__device__ inline bool b_halted(const BoxRec &B) { return *(volatile const int *)B.halt != 0; }
+#endif
 struct WorkItem {
     int box, blk;
 };
@@ -339,7 +347,9 @@ int launch_b_particle_dt(hipStream_t s, int esize, const BoxRec *tab, const Work
                          double dtmax);
 int launch_b_state(hipStream_t s, const BoxRec *tab, int nbox, DevScalars *gather);
 // HSMAC control ops for launch_b_hsmac_ctl
-enum { kHsBegin = 0, kHsFlip = 1, kHsEnd = 2, kHsCommit = 3, kHsPoll = 4 };
+// Gate(arg): sets the stop flag to arg if any box has not converged;
+// Unhalt clears it (the only op that runs while it is set).
+enum { kHsBegin = 0, kHsFlip = 1, kHsEnd = 2, kHsCommit = 3, kHsPoll = 4, kHsGate = 5, kHsUnhalt = 6 };
 
 // a box leaving its batch (lcs_box_destroy); a change of the box's buffers
 // that the batch's device table must pick up

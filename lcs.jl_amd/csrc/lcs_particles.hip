@@ -898,6 +898,7 @@ __global__ void __launch_bounds__(256) k_b_p_stage(const BoxRec *tab, const Work
 {
     const WorkItem w = work[blockIdx.x];
     const BoxRec &B = tab[w.box];
+    if (b_halted(B)) return;
     const int q = b_profile(B, w.blk, false);
     if (q < 0) return;
     const DevProfile &P = B.prof[q];
@@ -939,6 +940,7 @@ __global__ void __launch_bounds__(256) k_b_p_maxdiam(const BoxRec *tab, const Wo
     __shared__ double lds[16];
     const WorkItem w = work[blockIdx.x];
     const BoxRec &B = tab[w.box];
+    if (b_halted(B)) return;
     const int q = b_profile(B, w.blk, true);
     if (q < 0) return;
     const DevProfile &P = B.prof[q];
@@ -952,6 +954,7 @@ __global__ void k_b_p_dt(const BoxRec *tab, double dtmax)
 {
     __shared__ double lds[16];
     const BoxRec &B = tab[blockIdx.x];
+    if (b_halted(B)) return;
     DevScalars *S = B.S;
     for (int q = 0; q < B.nprof; q++) {
         const DevProfile &P = B.prof[q];

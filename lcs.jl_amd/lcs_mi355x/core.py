@@ -410,8 +410,29 @@ class Batch:
         return [log_dict(logs[i]) for i in range(n)]
 
     def set_graphs(self, on):
-        """Replay each step's launch runs as captured hipGraphs (default on)."""
-        check(self.L.lcs_batch_set_graphs(self.handle, int(bool(on))))
+        """hipGraph mode of the batched step (lcs_batch_set_graphs): True or 1
+        one graph per step once warm (default), 2 the runs between polls only,
+        3 one graph per step with the fewest HSMAC groups, False or 0 eager."""
+        check(self.L.lcs_batch_set_graphs(self.handle, int(on)))
+
+    def autotune(self):
+        """Picks the batched sweep's tile shape and z-chunk per size class by
+        timing (lcs_batch_autotune); no box changes.  Returns tiles()."""
+        check(self.L.lcs_batch_autotune(self.handle))
+        return self.tiles()
+
+    def set_tiles(self, i, tx, ty, zchunk):
+        """Tiling of the size class of box i: tile tx x ty, z-chunks of about zchunk planes."""
+        check(self.L.lcs_batch_set_tiles(self.handle, int(i), int(tx), int(ty), int(zchunk)))
+
+    def tiles(self):
+        """[(tx, ty, z-chunk planes)] of every box's sweep tiling, in order of addition."""
+        out = []
+        for i in range(len(self)):
+            v = [C.c_int() for _ in range(3)]
+            check(self.L.lcs_batch_get_tiles(self.handle, i, *[C.byref(x) for x in v]))
+            out.append(tuple(x.value for x in v))
+        return out
 
     def sync(self):
         check(self.L.lcs_batch_sync(self.handle))

@@ -105,20 +105,26 @@ def test_batch_membership_changes():
     batch.close()
 
 
+@pytest.mark.parametrize("mode", [1, 2, 3])
 @pytest.mark.parametrize("dtype,tb,itrmin", [("f32", True, 6), ("f64", True, 6), ("f64", True, 2), ("f32", True, 0),
                                              ("f64", False, 0), ("f64", False, 3)])
-def test_graph_replays_match_eager_steps(dtype, tb, itrmin):
+def test_graph_replays_match_eager_steps(dtype, tb, itrmin, mode):
     """Every replayed step equals the eager step bit for bit, with eager work
     (downloads, a membership change that drops and re-captures the graphs)
     between replays.  Small itrmin puts the group that applies the deferred
     linear forcing right before the first convergence poll; that group must
-    not be the one the later polled groups replay."""
+    not be the one the later polled groups replay.  Mode 1 runs whole-step
+    graphs from the third step after a membership change, mode 2 the
+    segments between polls only, mode 3 whole-step graphs that stop at their
+    gate whenever a stage needs a group past itrmin, so the step goes on from
+    the gate on the host."""
     import lcs_mi355x as L
     from lcs_mi355x import config as C
     cfgs = mixed_configs(dtype, C.HSMAC(tb, True, "red-black-fast", 1.7, 1e-3, itrmin, 300))
     ga = [L.build(c) for c in cfgs]
     ea = [L.build(c) for c in cfgs]
     bg, be = L.Batch(), L.Batch()
+    bg.set_graphs(mode)
     be.set_graphs(False)
     for c, b in zip(cfgs, ga):
         bg.add(b, L.step_config(c, localize_interval=2))
@@ -139,3 +145,54 @@ def test_graph_replays_match_eager_steps(dtype, tb, itrmin):
                 np.testing.assert_array_equal(a.download_prop(0, "x"), b.download_prop(0, "x"))
     bg.close()
     be.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_batch_tiling_changes_no_result(dtype):
+    """The autotuner's timing sweeps leave every box as it was, and the
+    per-class tilings it (or set_tiles) picks give the default tiling's
+    fields and sweep counts bit for bit.  The logged divergence and forcing
+    power are sums of per-tile partials: their last bits follow the tile
+    count, so they are compared to rounding."""
+    import lcs_mi355x as L
+    cfgs = mixed_configs(dtype)
+    ta = [L.build(c) for c in cfgs]
+    da = [L.build(c) for c in cfgs]
+    bt, bd = L.Batch(), L.Batch()
+    for c, b in zip(cfgs, ta):
+        bt.add(b, L.step_config(c, localize_interval=2))
+    for c, b in zip(cfgs, da):
+        bd.add(b, L.step_config(c, localize_interval=2))
+    rtol = 1e-12 if dtype == "f64" else 1e-5
+
+    def same_logs(la, lb, s):
+        for a, b in zip(la, lb):
+            for c in ("couple1", "couple2"):
+                assert a["flow"][c]["itr"] == b["flow"][c]["itr"], s
+                assert np.isclose(a["flow"][c]["div"], b["flow"][c]["div"], rtol=rtol, atol=0), s
+            assert np.isclose(a["flow"]["forcing"]["p"], b["flow"]["forcing"]["p"], rtol=rtol, atol=0), s
+            assert (a["step"], a["t"], a["dt"]) == (b["step"], b["t"], b["dt"]), s
+
+    assert bt.tiles() == bd.tiles() and all(t[:2] == (32, 8) for t in bt.tiles())
+    same_logs(bt.step(), bd.step(), -1)
+    before = [[b.download(n) for n in ("U", "V", "W", "P")] for b in ta]
+    tiles = bt.autotune()
+    print("autotuned tiles:", tiles, flush=True)
+    for b, ref in zip(ta, before):
+        for n, a in zip(("U", "V", "W", "P"), ref):
+            np.testing.assert_array_equal(b.download(n), a, err_msg=n)
+    # boxes of one size class share a tiling
+    assert tiles[0] == tiles[3]
+    for s in range(3):
+        if s == 1:  # an explicit tiling of the 32 x 16 x 24 class
+            bt.set_tiles(1, 64, 16, 8)
+            assert bt.tiles()[1][:2] == (64, 16)
+        same_logs(bt.step(), bd.step(), s)
+        for q, (a, b) in enumerate(zip(ta, da)):
+            for n in ("U", "V", "W", "P"):
+                np.testing.assert_array_equal(a.download(n), b.download(n), err_msg=f"step {s} box {q} {n}")
+    from lcs_mi355x._lib import LCSError
+    with pytest.raises(LCSError):
+        bt.set_tiles(0, 128, 8, 28)
+    bt.close()
+    bd.close()
