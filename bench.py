@@ -137,7 +137,9 @@ def bench_ensemble(args):
     batch.sync()
     el = time.perf_counter() - t0
     value = len(boxes) * args.steps / el
-    print(f"# batched: {value:.1f} box-steps/s ({value / serial:.2f}x serial)", file=sys.stderr, flush=True)
+    gstats = batch.graph_stats()
+    print(f"# batched: {value:.1f} box-steps/s ({value / serial:.2f}x serial); steps by path {gstats}",
+          file=sys.stderr, flush=True)
     ns = sorted({c.grid.dims[0] for c in cfgs})
     out = {
         "metric": METRIC, "value": value, "unit": "box-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -148,7 +150,8 @@ def bench_ensemble(args):
                    "boxes": len(boxes), "cells": cells,
                    "particles": sum(c.particles[0].population.valid for c in cfgs),
                    "particle_localize_interval": args.localize_interval, "graphs": not args.no_graphs,
-                   "sweep_tiling": tiling or "32x8, z-chunks of about 28 planes (default)"},
+                   "sweep_tiling": tiling or "32x8, z-chunks of about 28 planes (default)",
+                   "steps_by_path": gstats},
         "cell_steps_per_s": cells * args.steps / el,
         "serial": {"value": serial, "unit": "box-steps/s", "steps_per_box": args.serial_steps,
                    "how": "the same boxes, one lcs_step call per box at a time, same GPU"},

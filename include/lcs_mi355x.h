@@ -393,6 +393,10 @@ int lcs_batch_sync(lcs_batch *batch);
  * those before itrmin (every step that needs more goes on from the gate; for
  * tests).  on = 0: eager launches. */
 int lcs_batch_set_graphs(lcs_batch *batch, int on);
+/* Steps taken since the batch was created: whole = one graph that ran to its
+ * end, gated = one graph stopped at a gate and finished with polled groups,
+ * segmented = eager or segment launches.  Any pointer may be NULL. */
+int lcs_batch_graph_stats(lcs_batch *batch, int64_t *whole, int64_t *gated, int64_t *segmented);
 /* No counterpart in the reference (its kernels have fixed workgroups).  The
  * batched HSMAC sweep tiles each size class of boxes (equal nx, ny, nz) with
  * one shape, 64x8, 32x8, 64x16 or 32x16, in z-chunks of a target length; by

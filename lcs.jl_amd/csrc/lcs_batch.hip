@@ -75,6 +75,7 @@ struct Batch {
     std::array<int, 2> post_seen = {0, 0};        // post groups the last call of each stage needed
     std::deque<std::array<int, 2>> post_hist;     // ... over the last kPostWindow steps
     int *halt_d = nullptr, *halt_h = nullptr;     // stop flag of the step graphs (BoxRec::halt)
+    int64_t n_whole = 0, n_gated = 0, n_segmented = 0;  // steps by path (lcs_batch_graph_stats)
     // the next fused sweep applies the deferred linear forcing of the
     // linear-power boxes (set around stage 2's HSMAC only)
     bool force_next = false;
@@ -790,6 +791,7 @@ static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
         int halted = 0;
         if ((rc = step_graph(B, G, halted))) return rc;
         if (!halted) {
+            B.n_whole++;
             mirror_commit(B, 1, B.poll_h);
             mirror_commit(B, 0, B.poll_h + n);
             B.post_seen = {hs_post_needed(B, B.poll_h), hs_post_needed(B, B.poll_h + n)};
@@ -799,6 +801,7 @@ static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
         }
         // a stage needs more groups than the graph held: every kernel after
         // its gate returned at once; go on from there with polled groups
+        B.n_gated++;
         if ((rc = launch_b_hsmac_ctl(B.stream, B.tab.d, n, kHsUnhalt, 0, nullptr))) return rc;
         if (halted == 1) {
             if ((rc = b_hsmac_from(B, 1, 0.5, 0, 10, pre + G[0])) || (rc = step_from_stage2(B, logs, nlogs)))
@@ -820,6 +823,7 @@ static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
         record_post(B);
         return LCS_OK;
     }
+    B.n_segmented++;
     rc = segment(B, 1, [&]() { return enqueue_stage1_prefix(B); });
     if (rc || (rc = b_hsmac(B, 1, 0.5, 0, 10))) return rc;
     if ((rc = step_from_stage2(B, logs, nlogs))) return rc;
@@ -1082,6 +1086,16 @@ int lcs_batch_set_graphs(lcs_batch *batch, int on)
     drop_graphs(B);
     B.graphs = on != 0;
     if (on) B.graph_mode = on;
+    return LCS_OK;
+}
+
+int lcs_batch_graph_stats(lcs_batch *batch, int64_t *whole, int64_t *gated, int64_t *segmented)
+{
+    if (!batch) return fail(LCS_ERR_INVALID, "null argument");
+    const Batch &B = *BT(batch);
+    if (whole) *whole = B.n_whole;
+    if (gated) *gated = B.n_gated;
+    if (segmented) *segmented = B.n_segmented;
     return LCS_OK;
 }
 

@@ -415,6 +415,12 @@ class Batch:
         3 one graph per step with the fewest HSMAC groups, False or 0 eager."""
         check(self.L.lcs_batch_set_graphs(self.handle, int(on)))
 
+    def graph_stats(self):
+        """{"whole", "gated", "segmented"}: steps by path (lcs_batch_graph_stats)."""
+        v = [C.c_int64() for _ in range(3)]
+        check(self.L.lcs_batch_graph_stats(self.handle, *[C.byref(x) for x in v]))
+        return dict(zip(("whole", "gated", "segmented"), (x.value for x in v)))
+
     def autotune(self):
         """Picks the batched sweep's tile shape and z-chunk per size class by
         timing (lcs_batch_autotune); no box changes.  Returns tiles()."""

@@ -18,15 +18,17 @@ from test_gpu_step import c1_config, oracle_from_device
 pytestmark = pytest.mark.gpu
 
 
-def mixed_configs(dtype="f64", hsmac=None):
+def mixed_configs(dtype="f64", hsmac=None, rcf=True):
+    """Four boxes; the last has rc-energy forcing (linear-power with rcf=False)."""
     import dataclasses
 
     from lcs_mi355x import config as C
     cfgs = []
+    last = C.EnergyPreserveRCF(0.649, 1, 2, 6) if rcf else C.ConstantPowerLF(0.4)
     for dims, npart, forcing, profiles in [((24, 24, 24), 500, C.ConstantPowerLF(0.5), 1),
                                            ((32, 16, 24), 1200, C.ConstantPowerLF(0.3), 2),
                                            ((16, 16, 16), 0, C.ConstantPowerLF(0.7), 0),
-                                           ((24, 24, 24), 400, C.EnergyPreserveRCF(0.649, 1, 2, 6), 1)]:
+                                           ((24, 24, 24), 400, last, 1)]:
         cfg = c1_config(dims=dims, nparticles=max(npart, 1), forcing=forcing, dtype=dtype, profiles=profiles)
         if hsmac is not None:
             cfg = dataclasses.replace(cfg, flow=dataclasses.replace(cfg.flow, couple=hsmac))
@@ -105,10 +107,10 @@ def test_batch_membership_changes():
     batch.close()
 
 
-@pytest.mark.parametrize("mode", [1, 2, 3])
+@pytest.mark.parametrize("mode,rcf", [(1, False), (2, False), (3, False), (1, True)])
 @pytest.mark.parametrize("dtype,tb,itrmin", [("f32", True, 6), ("f64", True, 6), ("f64", True, 2), ("f32", True, 0),
                                              ("f64", False, 0), ("f64", False, 3)])
-def test_graph_replays_match_eager_steps(dtype, tb, itrmin, mode):
+def test_graph_replays_match_eager_steps(dtype, tb, itrmin, mode, rcf):
     """Every replayed step equals the eager step bit for bit, with eager work
     (downloads, a membership change that drops and re-captures the graphs)
     between replays.  Small itrmin puts the group that applies the deferred
@@ -117,10 +119,11 @@ def test_graph_replays_match_eager_steps(dtype, tb, itrmin, mode):
     graphs from the third step after a membership change, mode 2 the
     segments between polls only, mode 3 whole-step graphs that stop at their
     gate whenever a stage needs a group past itrmin, so the step goes on from
-    the gate on the host."""
+    the gate on the host.  A batch with an rc-energy box keeps the segments
+    (its forcing takes host pointers)."""
     import lcs_mi355x as L
     from lcs_mi355x import config as C
-    cfgs = mixed_configs(dtype, C.HSMAC(tb, True, "red-black-fast", 1.7, 1e-3, itrmin, 300))
+    cfgs = mixed_configs(dtype, C.HSMAC(tb, True, "red-black-fast", 1.7, 1e-3, itrmin, 300), rcf=rcf)
     ga = [L.build(c) for c in cfgs]
     ea = [L.build(c) for c in cfgs]
     bg, be = L.Batch(), L.Batch()
@@ -143,6 +146,13 @@ def test_graph_replays_match_eager_steps(dtype, tb, itrmin, mode):
                 np.testing.assert_array_equal(a.download(name), b.download(name), err_msg=f"step {s} box {q} {name}")
             if a.profiles:
                 np.testing.assert_array_equal(a.download_prop(0, "x"), b.download_prop(0, "x"))
+    st = bg.graph_stats()
+    print("steps by path:", st, flush=True)
+    assert sum(st.values()) == 7
+    if mode == 2 or rcf:
+        assert st["whole"] == st["gated"] == 0
+    else:  # steps 2, 3 and 6 (the two after each membership change are eager and segmented)
+        assert st["whole"] + st["gated"] == 3
     bg.close()
     be.close()
 
