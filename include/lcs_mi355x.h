@@ -256,7 +256,8 @@ int lcs_particle_density(lcs_box *box, int profile, double *density);
 /* No counterpart in the reference (its kernels are KernelAbstractions
  * workgroups of a fixed size).  The fused RHS and HSMAC sweep kernels march
  * tx x ty columns of cells up in z; the shape changes speed, never results.
- * Sweep shapes: (64|32) x (16|8).  RHS shapes: FP32 (64|32) x (16|8), FP64
+ * Sweep shapes: (64|32) x (16|8), and in FP32 also 128x8 and 128x16 (two
+ * cells per thread).  RHS shapes: FP32 (64|32) x (16|8), FP64
  * 64x8, 32x8, 32x4.  A zero sweep shape means the size rule for all of them. */
 typedef struct {
     int sweep_tx, sweep_ty;

@@ -375,8 +375,9 @@ static int rhs_tile_launch(Box &b, const RhsArgs<T> &a0, DtRef dt)
 
 // Tile shapes (tx, ty) the fused kernels are built for (the LDS of one CU
 // bounds them).  Sweep: 64 or 32 wide and 16 or 8 tall, and in FP32 also
-// 128 x 8.  RHS: FP32 up to 64 x 16; FP64 up to 64 x 8.
-static const int kSweepShapesF[][2] = {{64, 16}, {64, 8}, {32, 16}, {32, 8}, {128, 8}};
+// 128 x 8 and 128 x 16 (two cells per thread).  RHS: FP32 up to 64 x 16;
+// FP64 up to 64 x 8.
+static const int kSweepShapesF[][2] = {{64, 16}, {64, 8}, {32, 16}, {32, 8}, {128, 8}, {128, 16}};
 static const int kSweepShapesD[][2] = {{64, 16}, {64, 8}, {32, 16}, {32, 8}};
 static const int kRhsShapesF[][2] = {{64, 16}, {64, 8}, {32, 16}, {32, 8}};
 static const int kRhsShapesD[][2] = {{64, 8}, {32, 8}, {32, 4}};
@@ -387,7 +388,7 @@ int tile_candidates(int esize, int kind, int out[][2], int cap)
 {
     const int(*src)[2] = kind == 0 ? (esize == 8 ? kSweepShapesD : kSweepShapesF)
                                    : kind == 2 ? kSweep2Shapes : (esize == 8 ? kRhsShapesD : kRhsShapesF);
-    const int n = kind == 0 ? (esize == 8 ? 4 : 5) : kind == 2 ? (esize == 8 ? 0 : 4) : (esize == 8 ? 3 : 4);
+    const int n = kind == 0 ? (esize == 8 ? 4 : 6) : kind == 2 ? (esize == 8 ? 0 : 4) : (esize == 8 ? 3 : 4);
     for (int i = 0; i < n && i < cap; i++) {
         out[i][0] = src[i][0];
         out[i][1] = src[i][1];
@@ -702,8 +703,9 @@ __device__ inline void hs_bind(HsArgs<T> &a, const BufSets<T> &bs, bool par)
 // FORCE: the linear forcing of S->lf is applied to U, V, W as they are
 // loaded, and each workgroup's forcing power goes to ppart.
 template <typename T, int COLORING, int TY, int TX, bool FORCE = false>
-__global__ void __launch_bounds__(TX *TY) k_hsmac_sweep(Geom g, BufSets<T> bs, Box3 R, HsCoef<T> c, DtRef dt,
-                                                       DevScalars *S, double *part, int zc, double *ppart, int wrap)
+__global__ void __launch_bounds__((HsTile<T, COLORING, TY, TX>::NT))
+    k_hsmac_sweep(Geom g, BufSets<T> bs, Box3 R, HsCoef<T> c, DtRef dt, DevScalars *S, double *part, int zc,
+                  double *ppart, int wrap)
 {
     extern __shared__ __align__(16) unsigned char hs_lds_raw[];
     __shared__ double red_lds[16];
@@ -1023,8 +1025,10 @@ struct HsmacRun {
         if (tx == 64 && ty == 8) return fused_launch<COLORING, 8, 64>(r);
         if (tx == 32 && ty == 16) return fused_launch<COLORING, 16, 32>(r);
         if (tx == 32 && ty == 8) return fused_launch<COLORING, 8, 32>(r);
-        if constexpr (sizeof(T) == 4)
+        if constexpr (sizeof(T) == 4) {
             if (tx == 128 && ty == 8) return fused_launch<COLORING, 8, 128>(r);
+            if (tx == 128 && ty == 16) return fused_launch<COLORING, 16, 128>(r);
+        }
         return fail(LCS_ERR_INVALID, "unsupported sweep tile shape");
     }
     // one fused sweep with the box's tile shape
@@ -2520,8 +2524,8 @@ __device__ inline void hs_coef(const BoxRec &B, HsCoef<T> &c)
 }
 
 template <typename T, int COLORING, int TY, int TX, bool FORCE = false>
-__global__ void __launch_bounds__(TX *TY) k_b_hsmac_sweep(const BoxRec *tab, const TileItem *tiles, int set, int kind,
-                                                         int slot, double dtfac, int wrap)
+__global__ void __launch_bounds__((HsTile<T, COLORING, TY, TX>::NT))
+    k_b_hsmac_sweep(const BoxRec *tab, const TileItem *tiles, int set, int kind, int slot, double dtfac, int wrap)
 {
     extern __shared__ __align__(16) unsigned char hs_lds_raw[];
     __shared__ double red_lds[16];

@@ -140,9 +140,17 @@ struct HsArgs {
 // FORCE: the sweep applies the linear forcing of HsArgs::xf on load (the
 // first sweep of couple! after forcing!); a separate instantiation, so the
 // other sweeps carry none of its registers.
+//
+// Tiles of more than 1024 cells (128 x 16) give each thread CX = cells / 1024
+// cells of one row, WX = TX / CX columns apart, so each load and store of a
+// wave still covers 64 consecutive elements of a row; the red and black
+// slots then take several passes per thread.
 template <typename T, int COLORING, int TY_, int TX_ = 64, bool FORCE = false>
 struct HsTile {
-    static constexpr int TX = TX_, TY = TY_, NT = TX * TY;
+    static constexpr int TX = TX_, TY = TY_, NC = TX * TY;
+    static constexpr int CX = NC > 1024 ? NC / 1024 : 1;  // owned cells per thread
+    static constexpr int NT = NC / CX, WX = TX / CX;        // threads; threads per tile row
+    static_assert(CX * NT == NC && WX * CX == TX, "cells split evenly over threads");
     static constexpr int LX = TX + 4, LY = TY + 4, PLANE = LX * LY;  // fields: x in [X0-2, X1+2]
     static constexpr int DXN = TX + 3, DYN = TY + 3, DPL = DXN * DYN;  // dp: x in [X0-1, X1+2]
     static constexpr int OFF_V = 3 * PLANE, OFF_W = 6 * PLANE, OFF_D = 10 * PLANE;
@@ -151,30 +159,33 @@ struct HsTile {
     static constexpr int NHS = (NHALO + NT - 1) / NT;              // halo elements per thread
     static constexpr int RPR = (DXN + 1) / 2, BPR = (TX + 2) / 2;  // cells of one colour per row
     static constexpr int BYN = TY + 1;
+    static constexpr int RS = (RPR * DYN + NT - 1) / NT, BS = (BPR * BYN + NT - 1) / NT;  // slots per thread
     // plane accesses (lcs_bufio.h) for 1024-thread tiles
     static constexpr bool PLANES = NT >= 1024;
     // register slots of fetched planes: plane p lives in slot (p - z0) mod
     // NSLOT from the step that fetches it (p - NSLOT) to the step that stores
     // it (p - 1)
     static constexpr int NSLOT = PLANES ? 3 : 2;
-    static_assert(RPR * DYN <= NT && BPR * BYN <= NT, "one red and one black slot per thread");
 
     struct Thread {
         int tid, X0, Y0, z0, z1;
         int h_lds[NHS];   // LDS offsets of this thread's halo elements, -1 if none
         long h_off[NHS];  // their in-plane offsets, -1 outside the array
-        long o_off;     // in-plane offset of the owned cell, -1 outside the array
-        long u_off;     // where its U, V, W are loaded from (o_off, or its image when wrapping)
-        uint32_t o_vo, u_vo;     // byte offsets of o_off and u_off (kOff0 outside the array)
-        uint32_t h_vo[NHS];      // byte offsets of h_off (kOff0: none)
-        int r_row, r_k, b_row, b_k;
-        T pf[NSLOT][3][1 + NHS];  // fetched planes in flight: own cell, halo elements
-        T own[3][4];    // own cell's U, V, W at planes q-3..q
-        T pq[4];        // P of the owned cell at planes q-3..q (the last just fetched)
+        long o_off[CX];   // in-plane offsets of the owned cells, -1 outside the array
+        long u_off[CX];   // where their U, V, W are loaded from (o_off, or the image when wrapping)
+        uint32_t o_vo[CX], u_vo[CX];  // byte offsets of o_off and u_off (kOff0 outside the array)
+        uint32_t h_vo[NHS];           // byte offsets of h_off (kOff0: none)
+        int r_row[RS], r_k[RS], b_row[BS], b_k[BS];
+        T pf[NSLOT][3][CX + NHS];  // fetched planes in flight: owned cells, halo elements
+        T own[CX][3][4];  // owned cells' U, V, W at planes q-3..q
+        T pq[CX][4];      // P of the owned cells at planes q-3..q (the last just fetched)
         double acc;
-        bool own_xy;    // the owned column is interior in x and y
-        double pacc;    // forcing power of the owned interior cells
+        bool own_xy[CX];  // the owned column is interior in x and y
+        double pacc;      // forcing power of the owned interior cells
     };
+    // column of owned cell k of thread tid within the tile, and its row
+    __host__ __device__ static int cell_x(int tid, int k) { return tid % WX + k * WX; }
+    __host__ __device__ static int cell_y(int tid) { return tid / WX; }
 
     __host__ __device__ static int m3(int q) { return ((q % 3) + 3) % 3; }
     __host__ __device__ static int m4(int q) { return q & 3; }
@@ -222,8 +233,12 @@ struct HsTile {
         th.Y0 = 1 - g.h + TY * by;
         th.z0 = 1 - g.h + a.zc * bz;
         th.z1 = th.z0 + a.zc - 1 < g.nz + g.h ? th.z0 + a.zc - 1 : g.nz + g.h;
-        th.o_off = in_plane(g, th.X0 + tid % TX, th.Y0 + tid / TX);
-        th.u_off = a.wrap ? in_plane_img(g, th.X0 + tid % TX, th.Y0 + tid / TX) : th.o_off;
+        for (int k = 0; k < CX; k++) {
+            const int ox = th.X0 + cell_x(tid, k), oy = th.Y0 + cell_y(tid);
+            th.o_off[k] = in_plane(g, ox, oy);
+            th.u_off[k] = a.wrap ? in_plane_img(g, ox, oy) : th.o_off[k];
+            th.own_xy[k] = ox >= 1 && ox <= g.nx && oy >= 1 && oy <= g.ny;
+        }
         for (int s = 0; s < NHS; s++) {
             const int e = tid + s * NT;
             th.h_lds[s] = -1;
@@ -243,22 +258,26 @@ struct HsTile {
             const int hx = th.X0 - 2 + lx, hy = th.Y0 - 2 + ly;
             th.h_off[s] = a.wrap ? in_plane_img(g, hx, hy) : in_plane(g, hx, hy);
         }
-        th.o_vo = vo(th.o_off);
-        th.u_vo = vo(th.u_off);
+        for (int k = 0; k < CX; k++) {
+            th.o_vo[k] = vo(th.o_off[k]);
+            th.u_vo[k] = vo(th.u_off[k]);
+        }
         for (int s = 0; s < NHS; s++) th.h_vo[s] = vo(th.h_off[s]);
-        th.r_row = tid / RPR;
-        th.r_k = tid % RPR;
-        th.b_row = tid / BPR;
-        th.b_k = tid % BPR;
-        for (int f = 0; f < 3; f++)
-            for (int s = 0; s < 4; s++) th.own[f][s] = T(0);
-        for (int k = 0; k < 4; k++) th.pq[k] = T(0);
+        for (int r = 0; r < RS; r++) {
+            th.r_row[r] = (tid + r * NT) / RPR;
+            th.r_k[r] = (tid + r * NT) % RPR;
+        }
+        for (int r = 0; r < BS; r++) {
+            th.b_row[r] = (tid + r * NT) / BPR;
+            th.b_k[r] = (tid + r * NT) % BPR;
+        }
+        for (int k = 0; k < CX; k++) {
+            for (int f = 0; f < 3; f++)
+                for (int s = 0; s < 4; s++) th.own[k][f][s] = T(0);
+            for (int s = 0; s < 4; s++) th.pq[k][s] = T(0);
+        }
         th.acc = 0.0;
         th.pacc = 0.0;
-        {
-            const int ox = th.X0 + tid % TX, oy = th.Y0 + tid / TX;
-            th.own_xy = ox >= 1 && ox <= g.nx && oy >= 1 && oy <= g.ny;
-        }
     }
 
     // loads of plane q into register slot s (planes past z1 + 2 are not
@@ -270,15 +289,15 @@ struct HsTile {
         if constexpr (PLANES) {
             const Plane pu = plane_of(a.Us, a.g, base, zin), pv = plane_of(a.Vs, a.g, base, zin),
                         pw = plane_of(a.Ws, a.g, base, zin);
-            for (int e = 0; e <= NHS; e++) {
-                const uint32_t off = e == 0 ? th.u_vo : th.h_vo[e - 1];
+            for (int e = 0; e < CX + NHS; e++) {
+                const uint32_t off = e < CX ? th.u_vo[e] : th.h_vo[e - CX];
                 th.pf[s][0][e] = pload<T>(pu, off);
                 th.pf[s][1][e] = pload<T>(pv, off);
                 th.pf[s][2][e] = pload<T>(pw, off);
             }
         } else {
-            for (int e = 0; e <= NHS; e++) {
-                const long off = e == 0 ? th.u_off : th.h_off[e - 1];
+            for (int e = 0; e < CX + NHS; e++) {
+                const long off = e < CX ? th.u_off[e] : th.h_off[e - CX];
                 const bool ok = zin && off >= 0;
                 const long p = ok ? base + off : 0;
                 th.pf[s][0][e] = ok ? a.Us[p] : T(0);
@@ -293,11 +312,11 @@ struct HsTile {
     __host__ __device__ static void force(const HsArgs<T> &a, int q, Thread &th, int s)
     {
         if (!plane_in(a.g, q)) return;
-        for (int e = 0; e <= NHS; e++) {
-            if ((e == 0 ? th.o_off : th.h_off[e - 1]) < 0) continue;
+        for (int e = 0; e < CX + NHS; e++) {
+            if ((e < CX ? th.o_off[e] : th.h_off[e - CX]) < 0) continue;
             const T u = th.pf[s][0][e], v = th.pf[s][1][e], w = th.pf[s][2][e];
             const T usrc = a.xA * (u - a.xm[0]), vsrc = a.xA * (v - a.xm[1]), wsrc = a.xA * (w - a.xm[2]);
-            if (e == 0 && th.own_xy && q >= th.z0 && q <= th.z1 && q >= 1 && q <= a.g.nz)
+            if (e < CX && th.own_xy[e] && q >= th.z0 && q <= th.z1 && q >= 1 && q <= a.g.nz)
                 th.pacc += (double)((u * usrc + v * vsrc) + w * wsrc);
             th.pf[s][0][e] = u + usrc * a.xdt;
             th.pf[s][1][e] = v + vsrc * a.xdt;
@@ -309,16 +328,18 @@ struct HsTile {
     __host__ __device__ static void store(const HsArgs<T> &a, T *lds, int q, Thread &th, int s)
     {
         if (FORCE && a.xf) force(a, q, th, s);
-        const int e = (th.tid / TX + 2) * LX + th.tid % TX + 2;
-        Uf(lds, q, 0, e) = th.pf[s][0][0];
-        Vf(lds, q, 0, e) = th.pf[s][1][0];
-        Wf(lds, q, 0, e) = th.pf[s][2][0];
+        for (int k = 0; k < CX; k++) {
+            const int e = (cell_y(th.tid) + 2) * LX + cell_x(th.tid, k) + 2;
+            Uf(lds, q, 0, e) = th.pf[s][0][k];
+            Vf(lds, q, 0, e) = th.pf[s][1][k];
+            Wf(lds, q, 0, e) = th.pf[s][2][k];
+        }
         for (int k = 0; k < NHS; k++) {
             const int h = th.h_lds[k];
             if (h < 0) continue;
-            Uf(lds, q, 0, h) = th.pf[s][0][k + 1];
-            Vf(lds, q, 0, h) = th.pf[s][1][k + 1];
-            Wf(lds, q, 0, h) = th.pf[s][2][k + 1];
+            Uf(lds, q, 0, h) = th.pf[s][0][k + CX];
+            Vf(lds, q, 0, h) = th.pf[s][1][k + CX];
+            Wf(lds, q, 0, h) = th.pf[s][2][k + CX];
         }
         shift(th, s);
     }
@@ -326,22 +347,26 @@ struct HsTile {
     // advance the own-cell queue by one plane (the new top comes from slot s)
     __host__ __device__ static void shift(Thread &th, int s)
     {
-        for (int f = 0; f < 3; f++) {
-            th.own[f][0] = th.own[f][1];
-            th.own[f][1] = th.own[f][2];
-            th.own[f][2] = th.own[f][3];
-            th.own[f][3] = th.pf[s][f][0];
-        }
+        for (int k = 0; k < CX; k++)
+            for (int f = 0; f < 3; f++) {
+                th.own[k][f][0] = th.own[k][f][1];
+                th.own[k][f][1] = th.own[k][f][2];
+                th.own[k][f][2] = th.own[k][f][3];
+                th.own[k][f][3] = th.pf[s][f][k];
+            }
     }
 
     // P of the owned cell at plane o into the top of the P queue (planes
     // outside [z0, z1] read as 0)
     __host__ __device__ static void fetch_p(const HsArgs<T> &a, int o, Thread &th)
     {
-        if constexpr (PLANES)
-            th.pq[3] = pload<T>(plane_of(a.Ps, a.g, plane_base(a.g, o), o >= th.z0 && o <= th.z1), th.o_vo);
-        else if (th.o_off >= 0)
-            th.pq[3] = a.Ps[plane_base(a.g, o) + th.o_off];
+        if constexpr (PLANES) {
+            const Plane pp = plane_of(a.Ps, a.g, plane_base(a.g, o), o >= th.z0 && o <= th.z1);
+            for (int k = 0; k < CX; k++) th.pq[k][3] = pload<T>(pp, th.o_vo[k]);
+        } else {
+            for (int k = 0; k < CX; k++)
+                if (th.o_off[k] >= 0) th.pq[k][3] = a.Ps[plane_base(a.g, o) + th.o_off[k]];
+        }
     }
 
     __host__ __device__ static bool owned(const Thread &th, int x, int y, int z)
@@ -352,11 +377,15 @@ struct HsTile {
     // phase A: red dp on plane q
     __host__ __device__ static void red(const HsArgs<T> &a, T *lds, int q, Thread &th)
     {
-        if (th.r_row >= DYN) return;
-        const int xs = th.X0 - 1, y = th.Y0 - 1 + th.r_row;
-        const int dx = ((xs + y + q) & 1) + 2 * th.r_k;
+        for (int r = 0; r < RS; r++) red_slot(a, lds, q, th, th.r_row[r], th.r_k[r]);
+    }
+    __host__ __device__ static void red_slot(const HsArgs<T> &a, T *lds, int q, Thread &th, int r_row, int r_k)
+    {
+        if (r_row >= DYN) return;
+        const int xs = th.X0 - 1, y = th.Y0 - 1 + r_row;
+        const int dx = ((xs + y + q) & 1) + 2 * r_k;
         if (dx >= DXN) return;
-        const int x = xs + dx, dy = th.r_row, lx = dx + 1, ly = dy + 1;
+        const int x = xs + dx, dy = r_row, lx = dx + 1, ly = dy + 1;
         const HsCoef<T> &c = a.c;
         T dp = T(0);
         if (visited<COLORING>(a.R, x, y, q)) {
@@ -374,11 +403,15 @@ struct HsTile {
     // their red neighbours
     __host__ __device__ static void black(const HsArgs<T> &a, T *lds, int p, Thread &th)
     {
-        if (th.b_row >= BYN) return;
-        const int y = th.Y0 + th.b_row;
-        const int bx = ((th.X0 + y + p + 1) & 1) + 2 * th.b_k;
+        for (int r = 0; r < BS; r++) black_slot(a, lds, p, th, th.b_row[r], th.b_k[r]);
+    }
+    __host__ __device__ static void black_slot(const HsArgs<T> &a, T *lds, int p, Thread &th, int b_row, int b_k)
+    {
+        if (b_row >= BYN) return;
+        const int y = th.Y0 + b_row;
+        const int bx = ((th.X0 + y + p + 1) & 1) + 2 * b_k;
         if (bx > TX) return;
-        const int x = th.X0 + bx, dx = bx + 1, dy = th.b_row + 1, lx = bx + 2, ly = th.b_row + 2;
+        const int x = th.X0 + bx, dx = bx + 1, dy = b_row + 1, lx = bx + 2, ly = b_row + 2;
         const HsCoef<T> &c = a.c;
         T dp = T(0);
         if (visited<COLORING>(a.R, x, y, p)) {
@@ -403,17 +436,21 @@ struct HsTile {
     // outside the array compute values that are not stored.
     __host__ __device__ static void output(const HsArgs<T> &a, T *lds, int o, Thread &th)
     {
-        if (!PLANES && th.o_off < 0) return;
-        const int tx = th.tid % TX, ty = th.tid / TX;
+        for (int k = 0; k < CX; k++) output_cell(a, lds, o, th, k);
+    }
+    __host__ __device__ static void output_cell(const HsArgs<T> &a, T *lds, int o, Thread &th, int k)
+    {
+        if (!PLANES && th.o_off[k] < 0) return;
+        const int tx = cell_x(th.tid, k), ty = cell_y(th.tid);
         const int x = th.X0 + tx, y = th.Y0 + ty, dx = tx + 1, dy = ty + 1;
         const HsCoef<T> &c = a.c;
         const bool isred = ((x + y + o) & 1) == 0;
         // correction of this cell and of its +x, +y, +z neighbours (0 outside R)
         const T d0 = D(lds, o, dy, dx), dx1 = D(lds, o, dy, dx + 1), dy1 = D(lds, o, dy + 1, dx);
         const T dz1 = D(lds, o + 1, dy, dx);
-        T pv = th.pq[0];
+        T pv = th.pq[k][0];
         if (visited<COLORING>(a.R, x, y, o)) pv += d0 * a.dti;
-        T u = th.own[0][0], v = th.own[1][0], w = th.own[2][0];
+        T u = th.own[k][0][0], v = th.own[k][1][0], w = th.own[k][2][0];
         if (isred) {
             u += d0 * c.dxi;
             u -= dx1 * c.dxi;
@@ -432,12 +469,12 @@ struct HsTile {
         const long base = plane_base(a.g, o);
         if constexpr (PLANES) {
             const bool zin = o >= th.z0;
-            pstore<T>(plane_of(a.Pd, a.g, base, zin), th.o_vo, pv);
-            pstore<T>(plane_of(a.Ud, a.g, base, zin), th.o_vo, u);
-            pstore<T>(plane_of(a.Vd, a.g, base, zin), th.o_vo, v);
-            pstore<T>(plane_of(a.Wd, a.g, base, zin), th.o_vo, w);
+            pstore<T>(plane_of(a.Pd, a.g, base, zin), th.o_vo[k], pv);
+            pstore<T>(plane_of(a.Ud, a.g, base, zin), th.o_vo[k], u);
+            pstore<T>(plane_of(a.Vd, a.g, base, zin), th.o_vo[k], v);
+            pstore<T>(plane_of(a.Wd, a.g, base, zin), th.o_vo[k], w);
         } else {
-            const long gi = base + th.o_off;
+            const long gi = base + th.o_off[k];
             a.Pd[gi] = pv;
             a.Ud[gi] = u;
             a.Vd[gi] = v;
@@ -471,12 +508,14 @@ struct HsTile {
             store(a, lds, q + 1, th, ss);
         else
             shift(th, ss);
-        if constexpr (PLANES) {
-            th.pq[0] = th.pq[1];
-            th.pq[1] = th.pq[2];
-            th.pq[2] = th.pq[3];
-        } else {
-            th.pq[0] = th.pq[3];
+        for (int k = 0; k < CX; k++) {
+            if constexpr (PLANES) {
+                th.pq[k][0] = th.pq[k][1];
+                th.pq[k][1] = th.pq[k][2];
+                th.pq[k][2] = th.pq[k][3];
+            } else {
+                th.pq[k][0] = th.pq[k][3];
+            }
         }
     }
     // the first and last steps of a chunk's march

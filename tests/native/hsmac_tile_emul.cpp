@@ -80,6 +80,8 @@ extern "C" int hsmac_sweep_emul(int ty, int tx, int esize, int coloring, int nx,
     else if (ty == 8 && tx == 64) RUN(T, C, 8, 64); \
     else if (ty == 8 && tx == 32) RUN(T, C, 8, 32); \
     else if (ty == 4 && tx == 32) RUN(T, C, 4, 32); \
+    else if (ty == 8 && tx == 128) RUN(T, C, 8, 128); \
+    else if (ty == 16 && tx == 128) RUN(T, C, 16, 128); \
     else return -1;
     if (esize == 4) {
         if (coloring == LCS_COLORING_RED_BLACK) { BY_SHAPE(float, LCS_COLORING_RED_BLACK) }

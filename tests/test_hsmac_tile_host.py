@@ -8,7 +8,8 @@ colour-0 pass followed by its colour-1 pass over R (hsmac.jl:1-35,
 lib/parallel/src/foraxes.jl:73-107) bit for bit -- every element of U, V, W,
 P, ghosts included -- and its div^2 sum over (1:n+1)^3 must equal the
 oracle's.  Boxes: the three time-blocking domains d1, d2, d3 and the overlap
-core (2:n)^3; tiles 64 and 32 wide, ragged tiles in x and y, several
+core (2:n)^3; tiles 128, 64 and 32 wide (128 x 16: two cells per thread),
+ragged tiles in x and y, several
 z-chunkings, both colourings, both element types.
 """
 import ctypes as C
@@ -46,7 +47,8 @@ def _box(name, n):
 CASES = [((8, 8, 8), 4, 64, 14, "d1"), ((16, 12, 10), 4, 64, 5, "d2"), ((70, 9, 7), 8, 64, 3, "d3"),
          ((17, 5, 9), 8, 64, 4, "core"), ((20, 13, 11), 4, 64, 1, "d1"), ((66, 20, 6), 8, 64, 12, "d3"),
          ((5, 3, 4), 4, 64, 2, "d1"), ((40, 10, 9), 8, 32, 4, "d1"), ((27, 9, 6), 8, 32, 2, "core"),
-         ((33, 7, 5), 4, 32, 3, "d2"), ((70, 20, 6), 16, 64, 3, "d1"), ((12, 35, 5), 16, 64, 2, "core")]
+         ((33, 7, 5), 4, 32, 3, "d2"), ((70, 20, 6), 16, 64, 3, "d1"), ((12, 35, 5), 16, 64, 2, "core"),
+         ((126, 9, 7), 8, 128, 3, "d3"), ((130, 20, 6), 16, 128, 3, "d1"), ((20, 35, 5), 16, 128, 2, "core")]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -79,7 +81,7 @@ def test_fused_sweep_equals_two_colour_passes(emul, dims, ty, tx, zc, box, color
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("dims,ty,tx,zc,box", [((20, 13, 11), 8, 64, 4, "d1"), ((40, 10, 9), 8, 32, 3, "core"),
-                                               ((70, 20, 6), 16, 64, 2, "d3")])
+                                               ((70, 20, 6), 16, 64, 2, "d3"), ((130, 20, 6), 16, 128, 2, "d3")])
 def test_fused_sweep_with_linear_forcing_on_load(emul, dims, ty, tx, zc, box, dtype):
     """The first sweep of couple! after the linear forcing applies it as it
     loads U, V, W: the result equals x <- x + (A (x - mean)) dt on every
@@ -126,7 +128,8 @@ def test_fused_sweep_with_linear_forcing_on_load(emul, dims, ty, tx, zc, box, dt
                                                         ((16, 12, 10), 4, 64, 5, "core", 1),
                                                         ((5, 3, 4), 4, 64, 2, "d1", 0),
                                                         ((40, 10, 9), 8, 32, 3, "d1", 1),
-                                                        ((70, 20, 6), 16, 64, 2, "core", 0)])
+                                                        ((70, 20, 6), 16, 64, 2, "core", 0),
+                                                        ((130, 20, 6), 16, 128, 2, "d1", 1)])
 def test_fused_sweep_with_fill_on_load(emul, dims, ty, tx, zc, box, coloring, forced, dtype):
     """The first sweep of a time-blocked group loads U, V, W of ghost cells
     from their periodic images: equal bit for bit, ghosts included, to the
