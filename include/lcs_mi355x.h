@@ -367,8 +367,8 @@ int lcs_set_state(lcs_box *box, int64_t step, double t);
  * HSMAC convergence, and advances exactly as lcs_step would advance it alone.
  * Boxes of one batch share the float type, halo size, cfl, dtmax and HSMAC
  * parameters; sizes, particle profiles and forcing (kind and parameters) may
- * differ.  Linear-power forcing runs batched; rc-energy boxes are forced one
- * by one within the batched step. */
+ * differ.  Both forcings run batched: one launch per phase over every box with
+ * that forcing. */
 typedef struct lcs_batch lcs_batch;
 int lcs_batch_create(lcs_ctx *ctx, lcs_batch **out);
 int lcs_batch_destroy(lcs_batch *batch); /* detaches every box first */
@@ -388,8 +388,7 @@ int lcs_batch_sync(lcs_batch *batch);
  * stage as any of the last 8 steps needed, a device-side gate after them that
  * stops the rest of the graph (every kernel returns at once) if a box still
  * needs groups, and the copies of both stages' convergence state to the host.
- * A stopped step goes on from the gate with polled groups.  Batches with
- * rc-energy boxes stay on the segments (their forcing takes host pointers).
+ * A stopped step goes on from the gate with polled groups.
  * on = 2: segments only.  on = 3: one graph per step holding no groups beyond
  * those before itrmin (every step that needs more goes on from the gate; for
  * tests).  on = 0: eager launches. */

@@ -282,6 +282,7 @@ int lcs_box_destroy(lcs_box *box)
     if (b->part) (void)hipFree(b->part);
     if (b->xf) (void)hipFree(b->xf);
     if (b->rcf) (void)hipFree(b->rcf);
+    if (b->rcf_modes) (void)hipFree(b->rcf_modes);
     if (b->S_host) (void)hipHostFree(b->S_host);
     prof_collect(*b);
     for (hipEvent_t e : b->prof.pool) (void)hipEventDestroy(e);

@@ -59,7 +59,9 @@ struct Batch {
     DevVec<WorkItem> xf_work, xf_commit;  // x-slab pair: patches, then 256-row blocks
     bool xface = false;                   // every box runs its x slabs as the pair
     DevVec<int> lf_boxes;            // boxes with linear-power forcing
-    std::vector<int> rcf_boxes;      // boxes with rc-energy forcing (forced one by one)
+    std::vector<int> rcf_boxes;      // boxes with rc-energy forcing
+    DevVec<int> rcf_boxes_d;
+    DevVec<WorkItem> rcf_mean, rcf_dft, rcf_apply;  // work of the batched rc-energy forcing
     DevVec<TileItem> rhs_tiles, hs_tiles[4];  // sweep tiles by shape: kBatchSweepShapes
     HsPoll *poll_d = nullptr, *poll_h = nullptr;
     DevScalars *sc_d = nullptr, *sc_h = nullptr;
@@ -245,7 +247,7 @@ static int rebuild(Batch &B)
     B.wrap_ok = true;
     for (const Box *b : B.boxes)
         if (b->g.nx < b->g.h || b->g.ny < b->g.h || b->g.nz < b->g.h) B.wrap_ok = false;
-    std::vector<WorkItem> gfull, gone, rows, lf_rows, parts, diams, faces[3], xfw, xfc;
+    std::vector<WorkItem> gfull, gone, rows, lf_rows, parts, diams, faces[3], xfw, xfc, rmean, rdft, rapply;
     std::vector<int> lf_boxes;
     B.rcf_boxes.clear();
     std::vector<TileItem> rtiles, htiles[4];
@@ -352,7 +354,21 @@ static int rebuild(Batch &B)
             lf_boxes.push_back(i);
             for (int k = 0; k < r.row_blocks; k++) lf_rows.push_back({i, k});
         } else if (B.cfgs[i].forcing.kind == LCS_FORCING_RC_ENERGY) {
+            const lcs_forcing &f = B.cfgs[i].forcing;
+            int rc = rcf_prepare(b, f.kmin, f.kmax, f.coarse_size);
+            if (rc) return rc;
             B.rcf_boxes.push_back(i);
+            r.rcf_buf = b.rcf;
+            r.rcf_modes = b.rcf_modes;
+            r.rcf_energy = f.energy;
+            r.rcf_c = f.coarse_size;
+            r.rcf_kmin = f.kmin;
+            r.rcf_kmax = f.kmax;
+            r.rcf_blocks = rcf_apply_blocks(b);
+            const int c = f.coarse_size;
+            for (int k = 0; k < 3 * c * c * c; k++) rmean.push_back({i, k});
+            for (int k = 0; k < rcf_crop_slots(f.kmax); k++) rdft.push_back({i, k});
+            for (int k = 0; k < r.rcf_blocks; k++) rapply.push_back({i, k});
         }
         // particles
         r.nprof = (int)b.profiles.size();
@@ -393,6 +409,9 @@ static int rebuild(Batch &B)
     for (int a = 0; a < 3; a++)
         if ((rc = B.faces[a].assign(xcd_order(faces[a]), B.stream))) return rc;
     if ((rc = B.xf_work.assign(xcd_order(xfw), B.stream)) || (rc = B.xf_commit.assign(xcd_order(xfc), B.stream)))
+        return rc;
+    if ((rc = B.rcf_boxes_d.assign(B.rcf_boxes, B.stream)) || (rc = B.rcf_mean.assign(xcd_order(rmean), B.stream)) ||
+        (rc = B.rcf_dft.assign(xcd_order(rdft), B.stream)) || (rc = B.rcf_apply.assign(xcd_order(rapply), B.stream)))
         return rc;
     LCS_HIP(hipStreamSynchronize(B.stream));  // host vectors above are temporaries
     B.dirty = false;
@@ -613,8 +632,8 @@ static int enqueue_stage1_prefix(Batch &B)
     return LCS_OK;
 }
 
-// stage 2 up to forcing!: linear-power boxes batched (rc-energy boxes are
-// forced one by one after it, with the host's current array pointers)
+// stage 2 up to forcing!: the linear-power boxes' means (their forcing is
+// applied by the first sweep of couple!), then the rc-energy boxes' forcing
 static bool lf_deferred(const Batch &B) { return B.lf_boxes.n > 0 && B.hs.itrmax >= 1; }
 static int enqueue_stage2_prefix(Batch &B)
 {
@@ -635,7 +654,8 @@ static int enqueue_stage2_prefix(Batch &B)
                              !lf_defer)))
             return r;
     }
-    return LCS_OK;
+    return launch_b_rcf(B.stream, B.esize, B.tab.d, B.rcf_mean.d, B.rcf_mean.n, B.rcf_dft.d, B.rcf_dft.n,
+                        B.rcf_boxes_d.d, B.rcf_boxes_d.n, B.rcf_apply.d, B.rcf_apply.n);
 }
 
 static void fill_logs(Batch &B, lcs_step_log *logs, int nlogs)
@@ -661,11 +681,6 @@ static int step_from_stage2(Batch &B, lcs_step_log *logs, int nlogs)
     const int n = (int)B.boxes.size();
     int rc = segment(B, 2, [&]() { return enqueue_stage2_prefix(B); });
     if (rc) return rc;
-    for (int i : B.rcf_boxes) {
-        Box &b = *B.boxes[i];
-        const lcs_forcing &f = B.cfgs[i].forcing;
-        if ((rc = launch_force_rcf(b, f.energy, f.kmin, f.kmax, f.coarse_size, DtRef{&b.S->dt, 1.0}))) return rc;
-    }
     // the first sweep of stage 2's couple! applies the linear forcing as it
     // loads U, V, W (its coefficients were set above)
     B.force_next = lf_deferred(B);
@@ -784,9 +799,8 @@ static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
     if (moved && (rc = upload_table(B))) return rc;
     const int pre = hs_pre_groups(B);
     // the whole step as one graph once the batch is warm and the post-group
-    // counts of a step are known (not with rc-energy boxes, which are forced
-    // through the host's pointers)
-    if (B.graphs && B.warm && B.graph_mode != 2 && B.rcf_boxes.empty() && !B.post_hist.empty()) {
+    // counts of a step are known
+    if (B.graphs && B.warm && B.graph_mode != 2 && !B.post_hist.empty()) {
         const std::array<int, 2> G = step_graph_groups(B);
         int halted = 0;
         if ((rc = step_graph(B, G, halted))) return rc;
@@ -1004,6 +1018,10 @@ int lcs_batch_destroy(lcs_batch *batch)
                              &B->xf_commit};
     for (auto *v : w) v->release();
     B->lf_boxes.release();
+    B->rcf_boxes_d.release();
+    B->rcf_mean.release();
+    B->rcf_dft.release();
+    B->rcf_apply.release();
     B->rhs_tiles.release();
     for (auto &t : B->hs_tiles) t.release();
     B->tab.release();

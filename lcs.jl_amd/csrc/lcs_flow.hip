@@ -1786,13 +1786,11 @@ int launch_force_linear(Box &b, double power, DtRef dt)
 // ---------------------------------------------------------------------------
 // Box mean over (n/c)^3 blocks (ext/AMDGPUExt.jl:18-41): one block per coarse
 // cell and component, deterministic tree sum, times the reciprocal weight.
+// (the body of one block, shared with the batched kernel)
 template <typename T>
-__global__ void __launch_bounds__(256) k_boxmean(Geom g, const T *U, const T *V, const T *W, int c,
-                                                double *coarse)
+__device__ inline void boxmean_block(const Geom &g, const T *A, int c, int cell, int comp, double *coarse,
+                                     double *lds)
 {
-    __shared__ double lds[16];
-    int cell = blockIdx.x, comp = blockIdx.y;
-    const T *A = comp == 0 ? U : (comp == 1 ? V : W);
     int ic = cell % c, jc = (cell / c) % c, kc = cell / (c * c);
     int kx = g.nx / c, ky = g.ny / c, kz = g.nz / c;
     long nb = (long)kx * ky * kz;
@@ -1810,16 +1808,24 @@ __global__ void __launch_bounds__(256) k_boxmean(Geom g, const T *U, const T *V,
     }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) k_boxmean(Geom g, const T *U, const T *V, const T *W, int c,
+                                                double *coarse)
+{
+    __shared__ double lds[16];
+    const int comp = blockIdx.y;
+    boxmean_block<T>(g, comp == 0 ? U : (comp == 1 ? V : W), c, blockIdx.x, comp, coarse, lds);
+}
+
 __host__ __device__ inline int fftidx(int k, int n) { return k >= 0 ? k : n + k; }
+__host__ __device__ inline int rcf_crop_slots_dev(int kmax) { return (kmax + 1) * (2 * kmax + 1) * (2 * kmax + 1); }
 
 // Real DFT (FFTW sign, unnormalised) of each coarse grid at the wavevector
 // whose rfft-array index is (kx, fftindex(ky,c), fftindex(kz,c)); one block
 // per crop slot (kx in 0:kmax, ky,kz in -kmax:kmax).
-__global__ void __launch_bounds__(256) k_rcf_dft(const double *coarse, int c, int kmax, double *hat)
+__device__ inline void rcf_dft_block(const double *coarse, int c, int kmax, int slot, double *hat, double *lds)
 {
-    __shared__ double lds[16];
     int nk = 2 * kmax + 1;
-    int slot = blockIdx.x;
     int kx = slot % (kmax + 1);
     int ky = (slot / (kmax + 1)) % nk - kmax;
     int kz = slot / ((kmax + 1) * nk) - kmax;
@@ -1845,6 +1851,12 @@ __global__ void __launch_bounds__(256) k_rcf_dft(const double *coarse, int c, in
     }
 }
 
+__global__ void __launch_bounds__(256) k_rcf_dft(const double *coarse, int c, int kmax, double *hat)
+{
+    __shared__ double lds[16];
+    rcf_dft_block(coarse, c, kmax, blockIdx.x, hat, lds);
+}
+
 __device__ inline int kcount_dev(int k0)
 {
     int c = 0;
@@ -1861,9 +1873,9 @@ __device__ inline int kcount_dev(int k0)
 // enespe + shell rescaling + crop (rcf.jl:49-67, FourierTools.jl:103-157) on
 // one thread: a few hundred operations.  Aliased wavevectors (c < 2kmax+1)
 // share one rfft entry, as in the reference, via rep[].
-__global__ void k_rcf_spectrum(double *hat, int c, int kmin, int kmax, double e10, double *crop, DevScalars *S)
+__device__ inline void rcf_spectrum_one(double *hat, int c, int kmin, int kmax, double e10, double *crop,
+                                        DevScalars *S)
 {
-    if (threadIdx.x != 0) return;
     int nk = 2 * kmax + 1;
     int ncrop = (kmax + 1) * nk * nk;
     auto slot_of = [&](int kx, int ky, int kz) { return kx + (kmax + 1) * ((ky + kmax) + nk * (kz + kmax)); };
@@ -1922,6 +1934,11 @@ __global__ void k_rcf_spectrum(double *hat, int c, int kmin, int kmax, double e1
     S->e10 = e10;
 }
 
+__global__ void k_rcf_spectrum(double *hat, int c, int kmin, int kmax, double e10, double *crop, DevScalars *S)
+{
+    if (threadIdx.x == 0) rcf_spectrum_one(hat, c, kmin, kmax, e10, crop, S);
+}
+
 struct Cx {
     double re, im;
 };
@@ -1949,17 +1966,17 @@ struct RcfModes {
 // Sparse inverse DFT + apply (rcf.jl:74-98; sirdft FourierTools.jl:165-208):
 // per cell, cispi phases from the global index, 2*Re(A_hat(k) e^{ik.x}) summed
 // per shell in kset order, U += src, power = u.src/dt.
+// block blk of nblk (a grid-stride loop over the cells); returns the
+// thread's part of the forcing power
 template <typename T>
-__global__ void __launch_bounds__(256) k_rcf_apply(Geom g, T *U, T *V, T *W, int c, int kmax, RcfModes md,
-                                                  const double *crop, DtRef dt, double *part)
+__device__ inline double rcf_apply_part(const Geom &g, T *U, T *V, T *W, int c, int kmax, const RcfModes &md,
+                                        const double *crop, double dti, int blk, int nblk)
 {
-    __shared__ double lds[16];
     const int nk = 2 * kmax + 1, ncrop = (kmax + 1) * nk * nk;
     const double phi = (double)(g.nx / c) / 2.0;
     const double nxi = 1.0 / g.nx, nyi = 1.0 / g.ny, nzi = 1.0 / g.nz;
-    const double dti = 1.0 / (dt.factor * (*dt.src));
     double sp = 0.0;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < g.cells(); t += (long)gridDim.x * blockDim.x) {
+    for (long t = (long)blk * blockDim.x + threadIdx.x; t < g.cells(); t += (long)nblk * blockDim.x) {
         int i = (int)(t % g.nx) + 1;
         long q = t / g.nx;
         int j = (int)(q % g.ny) + 1;
@@ -1991,7 +2008,16 @@ __global__ void __launch_bounds__(256) k_rcf_apply(Geom g, T *U, T *V, T *W, int
         W[p] = w + wsrc;
         sp += (double)pw;
     }
-    double r = block_sum(sp, lds);
+    return sp;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_rcf_apply(Geom g, T *U, T *V, T *W, int c, int kmax, RcfModes md,
+                                                  const double *crop, DtRef dt, double *part)
+{
+    __shared__ double lds[16];
+    const double dti = 1.0 / (dt.factor * (*dt.src));
+    double r = block_sum(rcf_apply_part(g, U, V, W, c, kmax, md, crop, dti, blockIdx.x, gridDim.x), lds);
     if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
@@ -2055,14 +2081,17 @@ int launch_boxmean(Box &b, int c, double *coarse_dev)
     return LCS_OK;
 }
 
-int launch_force_rcf(Box &b, double energy, int kmin, int kmax, int c, DtRef dt, double *scratch)
+static int rcf_check(int kmin, int kmax, int c)
 {
     if (kmin < 1 || kmax < kmin) return fail(LCS_ERR_INVALID, "rc-energy needs 1 <= kmin <= kmax");
     if (kmax > c / 2) return fail(LCS_ERR_INVALID, "rc-energy needs kmax <= coarse_size/2");
     if (kmax - kmin + 1 > 8) return fail(LCS_ERR_INVALID, "at most 8 forced shells");
-    int rc = scratch ? LCS_OK : rcf_buffers(b, c, kmax);
-    if (rc) return rc;
-    RcfModes md;
+    return LCS_OK;
+}
+
+// the forced wavevectors of shells kmin..kmax, shell by shell in kset order
+static int rcf_modes_host(int kmin, int kmax, RcfModes &md)
+{
     memset(&md, 0, sizeof(md));
     md.kmin = kmin;
     for (int K = kmin; K <= kmax; K++) {
@@ -2072,6 +2101,125 @@ int launch_force_rcf(Box &b, double energy, int kmin, int kmax, int c, DtRef dt,
         md.kn[K - kmin] = n;
         md.n += n;
     }
+    return LCS_OK;
+}
+
+int rcf_apply_blocks(const Box &b) { return grid_for(b.g.cells(), 256); }
+int rcf_crop_slots(int kmax) { return (kmax + 1) * (2 * kmax + 1) * (2 * kmax + 1); }
+
+int rcf_prepare(Box &b, int kmin, int kmax, int c)
+{
+    int rc = rcf_check(kmin, kmax, c);
+    if (rc) return rc;
+    const Geom &g = b.g;
+    if (c < 1 || g.nx % c || g.ny % c || g.nz % c)
+        return fail(LCS_ERR_INVALID, "coarse_size must divide the grid dims");
+    if ((rc = rcf_buffers(b, c, kmax))) return rc;
+    if (b.rcf_modes && b.rcf_key[0] == kmin && b.rcf_key[1] == kmax && b.rcf_key[2] == c) return LCS_OK;
+    RcfModes md;
+    if ((rc = rcf_modes_host(kmin, kmax, md))) return rc;
+    if (!b.rcf_modes) {
+        LCS_HIP(hipMalloc(&b.rcf_modes, sizeof(RcfModes)));
+        b.bytes += sizeof(RcfModes);
+    }
+    LCS_HIP(hipMemcpy(b.rcf_modes, &md, sizeof(RcfModes), hipMemcpyHostToDevice));
+    b.rcf_key[0] = kmin;
+    b.rcf_key[1] = kmax;
+    b.rcf_key[2] = c;
+    return LCS_OK;
+}
+
+// ---- batched rc-energy forcing: the per-box work of launch_force_rcf with the
+// box, its parameters and buffers taken from the batch's records ----
+template <typename T>
+__global__ void __launch_bounds__(256) k_b_boxmean(const BoxRec *tab, const WorkItem *work)
+{
+    __shared__ double lds[16];
+    const WorkItem w = work[blockIdx.x];
+    const BoxRec &B = tab[w.box];
+    if (b_halted(B)) return;
+    const int c = B.rcf_c, n3 = c * c * c, comp = w.blk / n3;
+    boxmean_block<T>(B.g, (const T *)B.f[comp], c, w.blk % n3, comp, B.rcf_buf, lds);
+}
+
+__global__ void __launch_bounds__(256) k_b_rcf_dft(const BoxRec *tab, const WorkItem *work)
+{
+    __shared__ double lds[16];
+    const WorkItem w = work[blockIdx.x];
+    const BoxRec &B = tab[w.box];
+    if (b_halted(B)) return;
+    const int c = B.rcf_c;
+    rcf_dft_block(B.rcf_buf, c, B.rcf_kmax, w.blk, B.rcf_buf + 3L * c * c * c, lds);
+}
+
+__global__ void k_b_rcf_spectrum(const BoxRec *tab, const int *boxes)
+{
+    const BoxRec &B = tab[boxes[blockIdx.x]];
+    if (b_halted(B)) return;
+    const int c = B.rcf_c;
+    double *hat = B.rcf_buf + 3L * c * c * c;
+    if (threadIdx.x == 0)
+        rcf_spectrum_one(hat, c, B.rcf_kmin, B.rcf_kmax, B.rcf_energy, hat + 6L * rcf_crop_slots_dev(B.rcf_kmax), B.S);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_b_rcf_apply(const BoxRec *tab, const WorkItem *work)
+{
+    __shared__ double lds[16];
+    const WorkItem w = work[blockIdx.x];
+    const BoxRec &B = tab[w.box];
+    if (b_halted(B)) return;
+    const int c = B.rcf_c, kmax = B.rcf_kmax;
+    const double *crop = B.rcf_buf + 3L * c * c * c + 6L * rcf_crop_slots_dev(kmax);
+    const double dti = 1.0 / (1.0 * B.S->dt);
+    const double r = block_sum(rcf_apply_part(B.g, (T *)B.f[0], (T *)B.f[1], (T *)B.f[2], c, kmax,
+                                              *(const RcfModes *)B.rcf_modes, crop, dti, w.blk, B.rcf_blocks),
+                               lds);
+    if (threadIdx.x == 0) B.part[w.blk] = r;
+}
+
+// p = mean over the cells of the power partials (k_mean_to per box)
+__global__ void k_b_rcf_power(const BoxRec *tab, const int *boxes)
+{
+    __shared__ double lds[16];
+    const BoxRec &B = tab[boxes[blockIdx.x]];
+    if (b_halted(B)) return;
+    double s = 0.0;
+    for (int q = threadIdx.x; q < B.rcf_blocks; q += blockDim.x) s += B.part[q];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) B.S->p = s / (double)B.g.cells();
+}
+
+template <typename T>
+static void b_rcf_t(hipStream_t s, const BoxRec *tab, const WorkItem *mean, int nmean, const WorkItem *dft, int ndft,
+                    const int *boxes, int nbox, const WorkItem *apply, int napply)
+{
+    hipLaunchKernelGGL(k_b_boxmean<T>, dim3(nmean), dim3(256), 0, s, tab, mean);
+    hipLaunchKernelGGL(k_b_rcf_dft, dim3(ndft), dim3(256), 0, s, tab, dft);
+    hipLaunchKernelGGL(k_b_rcf_spectrum, dim3(nbox), dim3(64), 0, s, tab, boxes);
+    hipLaunchKernelGGL(k_b_rcf_apply<T>, dim3(napply), dim3(256), 0, s, tab, apply);
+    hipLaunchKernelGGL(k_b_rcf_power, dim3(nbox), dim3(256), 0, s, tab, boxes);
+}
+
+int launch_b_rcf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *mean, int nmean, const WorkItem *dft,
+                 int ndft, const int *boxes, int nbox, const WorkItem *apply, int napply)
+{
+    if (nbox == 0) return LCS_OK;
+    if (esize == 8)
+        b_rcf_t<double>(s, tab, mean, nmean, dft, ndft, boxes, nbox, apply, napply);
+    else
+        b_rcf_t<float>(s, tab, mean, nmean, dft, ndft, boxes, nbox, apply, napply);
+    LCS_HIP(hipGetLastError());
+    return LCS_OK;
+}
+
+int launch_force_rcf(Box &b, double energy, int kmin, int kmax, int c, DtRef dt, double *scratch)
+{
+    int rc = rcf_check(kmin, kmax, c);
+    if (rc) return rc;
+    if (!scratch && (rc = rcf_buffers(b, c, kmax))) return rc;
+    RcfModes md;
+    if ((rc = rcf_modes_host(kmin, kmax, md))) return rc;
     int nk = 2 * kmax + 1, ncrop = (kmax + 1) * nk * nk;
     double *coarse = scratch ? scratch : b.rcf;
     double *hat = coarse + 3L * c * c * c;

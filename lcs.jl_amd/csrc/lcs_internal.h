@@ -158,6 +158,8 @@ struct Box {
     size_t xf_bytes = 0;
     double *rcf = nullptr;   // coarse grids + cropped spectra
     size_t rcf_bytes = 0;
+    void *rcf_modes = nullptr;       // forced-mode table on the device (batched forcing)
+    int rcf_key[3] = {0, 0, 0};      // its kmin, kmax, coarse size
     std::vector<ParticleProfile> profiles;
     Profiler prof;
     size_t bytes = 0;
@@ -283,6 +285,13 @@ struct BoxRec {
     int row_blocks;      // blocks of the row-walking reductions (CFL, forcing)
     int lf_force;        // linear-power forcing: applied by the first stage-2 sweep
     int nprof;
+    // rc-energy forcing of the batch (rcf_c = 0: none): parameters, the
+    // box's coarse-grid and spectrum buffer, its forced-mode table, and the
+    // blocks of the apply pass (the single-box launch's grid)
+    double *rcf_buf;
+    const void *rcf_modes;
+    double rcf_energy;
+    int rcf_c, rcf_kmin, rcf_kmax, rcf_blocks;
     // the batch's stop flag (one int for all records): while it is nonzero
     // every batched kernel returns at once (b_halted), so the rest of a
     // captured step graph is skipped once a stage needs more HSMAC groups
@@ -320,6 +329,17 @@ int launch_b_cfl(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *ro
 int launch_b_lf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *rows, int nrows, const int *boxes,
                 int nbox, bool apply = true);
 int launch_b_lf_power(hipStream_t s, const BoxRec *tab, const int *boxes, int nbox);
+// rc-energy forcing of the listed boxes (src/flows/forcings/rcf.jl:21-191),
+// the single-box kernels' arithmetic in one launch per phase: box means
+// (work items: box, 3 c^3 blocks), forward DFT (box, crop slot), spectrum
+// rescaling (one block per box), apply (box, rcf_blocks blocks), power
+int launch_b_rcf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *mean, int nmean, const WorkItem *dft,
+                 int ndft, const int *boxes, int nbox, const WorkItem *apply, int napply);
+// checks rc-energy parameters for box b and sets up its buffers and its
+// forced-mode table on the device (synchronous; not in captured work)
+int rcf_prepare(Box &b, int kmin, int kmax, int c);
+int rcf_apply_blocks(const Box &b);
+int rcf_crop_slots(int kmax);
 int launch_b_rhs(hipStream_t s, int esize, const BoxRec *tab, const TileItem *tiles, int ntiles, int stage);
 // tx: tile width of the fused sweep (32 or 64); the tiles listed must all be of that width
 // force: boxes with lf_force apply the linear forcing of S->lf on load and

@@ -107,10 +107,12 @@ def test_batch_membership_changes():
     batch.close()
 
 
-@pytest.mark.parametrize("mode,rcf", [(1, False), (2, False), (3, False), (1, True)])
-@pytest.mark.parametrize("dtype,tb,itrmin", [("f32", True, 6), ("f64", True, 6), ("f64", True, 2), ("f32", True, 0),
-                                             ("f64", False, 0), ("f64", False, 3)])
-def test_graph_replays_match_eager_steps(dtype, tb, itrmin, mode, rcf):
+@pytest.mark.parametrize("mode,rcf", [(1, False), (2, False), (3, False), (1, True), (3, True)])
+@pytest.mark.parametrize("dtype,tb,itrmin,itrmax", [("f32", True, 6, 300), ("f64", True, 6, 300),
+                                                    ("f64", True, 2, 300), ("f32", True, 0, 300),
+                                                    ("f64", False, 0, 300), ("f64", False, 3, 300),
+                                                    ("f64", True, 6, 6), ("f32", False, 1, 4)])
+def test_graph_replays_match_eager_steps(dtype, tb, itrmin, itrmax, mode, rcf):
     """Every replayed step equals the eager step bit for bit, with eager work
     (downloads, a membership change that drops and re-captures the graphs)
     between replays.  Small itrmin puts the group that applies the deferred
@@ -119,11 +121,12 @@ def test_graph_replays_match_eager_steps(dtype, tb, itrmin, mode, rcf):
     graphs from the third step after a membership change, mode 2 the
     segments between polls only, mode 3 whole-step graphs that stop at their
     gate whenever a stage needs a group past itrmin, so the step goes on from
-    the gate on the host.  A batch with an rc-energy box keeps the segments
-    (its forcing takes host pointers)."""
+    the gate on the host, with and without an rc-energy box (its forcing is
+    batched from the device table like every other phase).  With itrmax =
+    itrmin every group is in the graph and there is no gate."""
     import lcs_mi355x as L
     from lcs_mi355x import config as C
-    cfgs = mixed_configs(dtype, C.HSMAC(tb, True, "red-black-fast", 1.7, 1e-3, itrmin, 300), rcf=rcf)
+    cfgs = mixed_configs(dtype, C.HSMAC(tb, True, "red-black-fast", 1.7, 1e-3, itrmin, itrmax), rcf=rcf)
     ga = [L.build(c) for c in cfgs]
     ea = [L.build(c) for c in cfgs]
     bg, be = L.Batch(), L.Batch()
@@ -149,7 +152,7 @@ def test_graph_replays_match_eager_steps(dtype, tb, itrmin, mode, rcf):
     st = bg.graph_stats()
     print("steps by path:", st, flush=True)
     assert sum(st.values()) == 7
-    if mode == 2 or rcf:
+    if mode == 2:
         assert st["whole"] == st["gated"] == 0
     else:  # steps 2, 3 and 6 (the two after each membership change are eager and segmented)
         assert st["whole"] + st["gated"] == 3
