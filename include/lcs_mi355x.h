@@ -380,16 +380,17 @@ int lcs_batch_size(lcs_batch *batch, int *nboxes);
 int lcs_batch_step(lcs_batch *batch, lcs_step_log *logs, int nlogs);
 int lcs_batch_sync(lcs_batch *batch);
 /* hipGraphs of the batched step; results are the same in every mode.
- * on = 1 (default): the first step after a membership change runs eagerly;
- * the second captures its runs of launches between convergence polls (stage
- * prefix up to couple!, the HSMAC groups before itrmin, one HSMAC group,
- * stage 2 up to the linear-power forcing) and replays them; from the third
- * step on, each step is ONE graph: both stages with as many HSMAC groups per
- * stage as any of the last 8 steps needed, a device-side gate after them that
+ * on = 1 (default): the first two steps after a membership change run
+ * eagerly (a server's membership often changes again within a step or two);
+ * from the third step on, each step is ONE graph: both stages with as many
+ * HSMAC groups per stage as any of the last 8 steps needed, a device-side gate
+ * after them that
  * stops the rest of the graph (every kernel returns at once) if a box still
  * needs groups, and the copies of both stages' convergence state to the host.
- * A stopped step goes on from the gate with polled groups.
- * on = 2: segments only.  on = 3: one graph per step holding no groups beyond
+ * A stopped step goes on from the gate with polled groups, whose runs of
+ * launches between convergence polls (one HSMAC group; stage 2 up to the
+ * forcing; ...) are captured and replayed.  on = 2: those runs only, from
+ * the second step on.  on = 3: one graph per step holding no groups beyond
  * those before itrmin (every step that needs more goes on from the gate; for
  * tests).  on = 0: eager launches. */
 int lcs_batch_set_graphs(lcs_batch *batch, int on);

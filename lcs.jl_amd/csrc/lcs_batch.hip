@@ -66,12 +66,16 @@ struct Batch {
     HsPoll *poll_d = nullptr, *poll_h = nullptr;
     DevScalars *sc_d = nullptr, *sc_h = nullptr;
     int cap_boxes = 0;
-    // hipGraphs of the runs of launches between host polls, keyed by segment
-    // id; captured on the second step after a membership change (the first
-    // runs eagerly and sets the kernels' attributes) and replayed afterwards.
-    // From the third step on, each step is one graph (step_graphs, keyed by
-    // the post groups it holds per stage) unless graph_mode is 2.
-    bool graphs = true, warm = false;
+    // hipGraphs.  The first two steps after a membership change run eagerly
+    // (the first sets the kernels' attributes; a server's membership often
+    // changes again within a step or two, and a capture that is not replayed
+    // costs more than it saves).  From the third step on each step is one
+    // graph (step_graphs, keyed by the post groups it holds per stage), and
+    // the runs of launches between host polls (gexec, keyed by segment id)
+    // are captured as a stopped step graph needs them; with graph_mode 2
+    // the segments alone, from the second step on.
+    bool graphs = true;
+    int stable = 0;  // steps since the tables were last rebuilt
     int graph_mode = 1;  // lcs_batch_set_graphs: 1 whole steps, 2 segments only, 3 whole steps, fewest groups
     std::map<std::array<int, 2>, hipGraphExec_t> step_graphs;
     std::array<int, 2> post_seen = {0, 0};        // post groups the last call of each stage needed
@@ -98,7 +102,7 @@ static void drop_graphs(Batch &B)
     for (auto &kv : B.step_graphs) (void)hipGraphExecDestroy(kv.second);
     B.step_graphs.clear();
     B.post_hist.clear();
-    B.warm = false;
+    B.stable = 0;
 }
 
 // Enqueues a run of launches with no host synchronisation inside, through its
@@ -108,7 +112,7 @@ static void drop_graphs(Batch &B)
 template <typename F>
 static int segment(Batch &B, int id, F &&enqueue)
 {
-    if (!B.graphs || !B.warm) return enqueue();
+    if (!B.graphs || B.stable < (B.graph_mode == 2 ? 1 : 2)) return enqueue();
     auto it = B.gexec.find(id);
     if (it == B.gexec.end()) {
         LCS_HIP(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
@@ -798,9 +802,9 @@ static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
     }
     if (moved && (rc = upload_table(B))) return rc;
     const int pre = hs_pre_groups(B);
-    // the whole step as one graph once the batch is warm and the post-group
-    // counts of a step are known
-    if (B.graphs && B.warm && B.graph_mode != 2 && !B.post_hist.empty()) {
+    // the whole step as one graph from the third step after a membership
+    // change, once the post-group counts of a step are known
+    if (B.graphs && B.graph_mode != 2 && B.stable >= 2 && !B.post_hist.empty()) {
         const std::array<int, 2> G = step_graph_groups(B);
         int halted = 0;
         if ((rc = step_graph(B, G, halted))) return rc;
@@ -841,8 +845,8 @@ static int batch_step(Batch &B, lcs_step_log *logs, int nlogs)
     rc = segment(B, 1, [&]() { return enqueue_stage1_prefix(B); });
     if (rc || (rc = b_hsmac(B, 1, 0.5, 0, 10))) return rc;
     if ((rc = step_from_stage2(B, logs, nlogs))) return rc;
-    if (B.warm) record_post(B);
-    B.warm = true;
+    record_post(B);
+    B.stable++;
     return LCS_OK;
 }
 
