@@ -333,7 +333,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "c6"])
     ap.add_argument("--requests", type=int, default=2000, help="served requests (workload c5)")
     ap.add_argument("--rate", type=float, default=0.0,
                     help="Poisson arrival rate, requests/s (c5); 0: --load times the measured capacity")

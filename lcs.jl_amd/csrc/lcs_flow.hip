@@ -1956,57 +1956,139 @@ __device__ inline Cx cx_pow(Cx x, int n)
 }
 
 constexpr int kMaxRcfModes = 128;
+constexpr int kMaxRcfKx = 8;  // kx in 0..kmax; the mode cap keeps kmax <= 4
 struct RcfModes {
     int n;
     int kmin;
     int kof[8], kn[8];
     signed char k[kMaxRcfModes][3];
+    // the modes grouped by kx (kset order within a group): group kx is
+    // byx[xof[kx] .. xof[kx + 1])
+    unsigned char byx[kMaxRcfModes];
+    int xof[kMaxRcfKx + 1];
 };
 
+// per-block scratch of the apply: each (row, mode)'s A_hat(k) e^{i(ky y + kz z)}
+// for the rows in hand, then their sums over the modes of one kx
+constexpr int kRcfCellsPerPass = 1024;  // 4 cells per thread of a 256-thread block
+constexpr int kRcfRowSlots = 256;       // rows x modes at once
+constexpr int kRcfMaxRows = 16;
+struct RcfRowLds {
+    double a[kMaxRcfModes][6];  // A_hat(k) of each mode, three components
+    double t[kRcfRowSlots][6];
+    double b[kRcfMaxRows][3][kMaxRcfKx][2];
+};
+
+// rows a block takes at once: up to 1024 cells, within the LDS slots
+__device__ inline int rcf_rows_per_group(const Geom &g, int nmodes)
+{
+    int R = kRcfCellsPerPass / g.nx;
+    R = min(R, kRcfRowSlots / max(nmodes, 1));
+    return max(1, min(R, kRcfMaxRows));
+}
+
 // Sparse inverse DFT + apply (rcf.jl:74-98; sirdft FourierTools.jl:165-208):
-// per cell, cispi phases from the global index, 2*Re(A_hat(k) e^{ik.x}) summed
-// per shell in kset order, U += src, power = u.src/dt.
-// block blk of nblk (a grid-stride loop over the cells); returns the
-// thread's part of the forcing power
+// src(x) = sum over the forced modes of 2 Re(A_hat(k) e^{ik.x}), with the
+// phases cispi from the global index raised to integer powers, U += src,
+// power = u.src/dt.  The reference evaluates every mode at every cell; here
+// a block takes R x-rows (j, k) at a time, folds the y and z phases into the
+// amplitudes once per row, B(kx) = sum_{ky,kz} A_hat(k) e^{i(ky y + kz z)},
+// and a cell sums kmax + 1 terms 2 Re(B(kx) e^{i kx x}): the same sum, its
+// terms grouped by kx.  The rows' velocities are loaded before the row sums
+// are formed, so their latency hides behind that work.  Row groups blk,
+// blk + nblk, ... of the box; returns the thread's part of the forcing power.
 template <typename T>
 __device__ inline double rcf_apply_part(const Geom &g, T *U, T *V, T *W, int c, int kmax, const RcfModes &md,
-                                        const double *crop, double dti, int blk, int nblk)
+                                        const double *crop, double dti, int blk, int nblk, RcfRowLds &L)
 {
-    const int nk = 2 * kmax + 1, ncrop = (kmax + 1) * nk * nk;
+    constexpr int Q = kRcfCellsPerPass / 256;
+    const int nk = 2 * kmax + 1, ncrop = (kmax + 1) * nk * nk, nkx = kmax + 1;
     const double phi = (double)(g.nx / c) / 2.0;
     const double nxi = 1.0 / g.nx, nyi = 1.0 / g.ny, nzi = 1.0 / g.nz;
+    const long rows = (long)g.ny * g.nz;
+    const int R = rcf_rows_per_group(g, md.n);
+    const long groups = (rows + R - 1) / R;
+    for (int m = threadIdx.x; m < md.n; m += blockDim.x) {
+        const int ky = md.k[m][1], kz = md.k[m][2];
+        const int o = md.k[m][0] + (kmax + 1) * (fftidx(ky, nk) + nk * fftidx(kz, nk));
+        for (int a = 0; a < 3; a++) {
+            L.a[m][2 * a] = crop[2 * ((long)a * ncrop + o)];
+            L.a[m][2 * a + 1] = crop[2 * ((long)a * ncrop + o) + 1];
+        }
+    }
+    __syncthreads();
     double sp = 0.0;
-    for (long t = (long)blk * blockDim.x + threadIdx.x; t < g.cells(); t += (long)nblk * blockDim.x) {
-        int i = (int)(t % g.nx) + 1;
-        long q = t / g.nx;
-        int j = (int)(q % g.ny) + 1;
-        int k = (int)(q / g.ny) + 1;
-        Cx ek, ej, ei;
-        sincospi(2 * ((double)k - phi) * nzi, &ek.im, &ek.re);
-        sincospi(2 * ((double)j - phi) * nyi, &ej.im, &ej.re);
-        sincospi(2 * ((double)i - phi) * nxi, &ei.im, &ei.re);
-        double src[3] = {0, 0, 0};
-        for (int K = 0; K <= kmax - md.kmin; K++) {
-            double acc[3] = {0, 0, 0};
-            for (int m = md.kof[K]; m < md.kof[K] + md.kn[K]; m++) {
-                int kx = md.k[m][0], ky = md.k[m][1], kz = md.k[m][2];
-                Cx xyz = cx_mul(cx_mul(cx_pow(ei, kx), cx_pow(ej, ky)), cx_pow(ek, kz));
-                int o = kx + (kmax + 1) * (fftidx(ky, nk) + nk * fftidx(kz, nk));
-                for (int a = 0; a < 3; a++) {
-                    double are = crop[2 * ((long)a * ncrop + o)], aim = crop[2 * ((long)a * ncrop + o) + 1];
-                    acc[a] += 2 * (are * xyz.re - aim * xyz.im);
+    for (long gi = blk; gi < groups; gi += nblk) {  // uniform over the block: every barrier is reached by all
+        const long r0 = gi * R;
+        const int nr = (int)min((long)R, rows - r0);
+        const int ncell = nr * g.nx;  // more than one pass only when R = 1
+        for (int x0 = 0; x0 < ncell; x0 += kRcfCellsPerPass) {
+            long p[Q];
+            int rr[Q], ii[Q];
+            T u[Q], v[Q], w[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const int e = x0 + threadIdx.x + 256 * q;
+                rr[q] = e / g.nx;
+                ii[q] = e - rr[q] * g.nx + 1;
+                if (e < ncell) {
+                    const long r = r0 + rr[q];
+                    p[q] = g.at(ii[q], (int)(r % g.ny) + 1, (int)(r / g.ny) + 1);
+                    u[q] = U[p[q]];
+                    v[q] = V[p[q]];
+                    w[q] = W[p[q]];
                 }
             }
-            for (int a = 0; a < 3; a++) src[a] += acc[a];
+            if (x0 == 0) {
+                for (int it = threadIdx.x; it < nr * md.n; it += blockDim.x) {
+                    const int q = it / md.n, m = it - q * md.n;
+                    const long r = r0 + q;
+                    const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
+                    Cx ej, ek;
+                    sincospi(2 * ((double)k - phi) * nzi, &ek.im, &ek.re);
+                    sincospi(2 * ((double)j - phi) * nyi, &ej.im, &ej.re);
+                    const Cx yz = cx_mul(cx_pow(ej, md.k[m][1]), cx_pow(ek, md.k[m][2]));
+                    for (int a = 0; a < 3; a++) {
+                        const Cx A = {L.a[m][2 * a], L.a[m][2 * a + 1]};
+                        const Cx t = cx_mul(A, yz);
+                        L.t[it][2 * a] = t.re;
+                        L.t[it][2 * a + 1] = t.im;
+                    }
+                }
+                __syncthreads();  // (also: every thread is past the previous group's reads of L.b)
+                for (int it = threadIdx.x; it < nr * 3 * nkx; it += blockDim.x) {
+                    const int q = it / (3 * nkx), a = (it / nkx) % 3, kx = it % nkx;
+                    const double(*t)[6] = L.t + q * md.n;
+                    double re = 0.0, im = 0.0;
+                    for (int s = md.xof[kx]; s < md.xof[kx + 1]; s++) {
+                        re += t[md.byx[s]][2 * a];
+                        im += t[md.byx[s]][2 * a + 1];
+                    }
+                    L.b[q][a][kx][0] = re;
+                    L.b[q][a][kx][1] = im;
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                if (x0 + (int)threadIdx.x + 256 * q >= ncell) continue;
+                Cx ei;
+                sincospi(2 * ((double)ii[q] - phi) * nxi, &ei.im, &ei.re);
+                const double(*bq)[kMaxRcfKx][2] = L.b[rr[q]];
+                double src[3] = {0, 0, 0};
+                Cx e = {1.0, 0.0};
+                for (int kx = 0; kx <= kmax; kx++) {
+                    if (kx) e = cx_mul(ei, e);
+                    for (int a = 0; a < 3; a++) src[a] += 2 * (bq[a][kx][0] * e.re - bq[a][kx][1] * e.im);
+                }
+                const T usrc = T(src[0]), vsrc = T(src[1]), wsrc = T(src[2]);
+                const T pw = ((u[q] * usrc + v[q] * vsrc) + w[q] * wsrc) * T(dti);
+                U[p[q]] = u[q] + usrc;
+                V[p[q]] = v[q] + vsrc;
+                W[p[q]] = w[q] + wsrc;
+                sp += (double)pw;
+            }
         }
-        long p = g.at(i, j, k);
-        T usrc = T(src[0]), vsrc = T(src[1]), wsrc = T(src[2]);
-        T u = U[p], v = V[p], w = W[p];
-        T pw = ((u * usrc + v * vsrc) + w * wsrc) * T(dti);
-        U[p] = u + usrc;
-        V[p] = v + vsrc;
-        W[p] = w + wsrc;
-        sp += (double)pw;
     }
     return sp;
 }
@@ -2016,8 +2098,13 @@ __global__ void __launch_bounds__(256) k_rcf_apply(Geom g, T *U, T *V, T *W, int
                                                   const double *crop, DtRef dt, double *part)
 {
     __shared__ double lds[16];
+    __shared__ RcfRowLds rl;
+    __shared__ RcfModes mds;  // the mode table in LDS, off the vector-memory queue of the field loads
+    for (int q = threadIdx.x; q < (int)(sizeof(RcfModes) / sizeof(int)); q += blockDim.x)
+        ((int *)&mds)[q] = ((const int *)&md)[q];
+    __syncthreads();
     const double dti = 1.0 / (dt.factor * (*dt.src));
-    double r = block_sum(rcf_apply_part(g, U, V, W, c, kmax, md, crop, dti, blockIdx.x, gridDim.x), lds);
+    double r = block_sum(rcf_apply_part(g, U, V, W, c, kmax, mds, crop, dti, blockIdx.x, gridDim.x, rl), lds);
     if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
@@ -2086,6 +2173,7 @@ static int rcf_check(int kmin, int kmax, int c)
     if (kmin < 1 || kmax < kmin) return fail(LCS_ERR_INVALID, "rc-energy needs 1 <= kmin <= kmax");
     if (kmax > c / 2) return fail(LCS_ERR_INVALID, "rc-energy needs kmax <= coarse_size/2");
     if (kmax - kmin + 1 > 8) return fail(LCS_ERR_INVALID, "at most 8 forced shells");
+    if (kmax >= kMaxRcfKx) return fail(LCS_ERR_INVALID, "rc-energy needs kmax <= 7");
     return LCS_OK;
 }
 
@@ -2101,10 +2189,18 @@ static int rcf_modes_host(int kmin, int kmax, RcfModes &md)
         md.kn[K - kmin] = n;
         md.n += n;
     }
+    int s = 0;
+    for (int kx = 0; kx < kMaxRcfKx; kx++) {
+        md.xof[kx] = s;
+        for (int m = 0; m < md.n; m++)
+            if (md.k[m][0] == kx) md.byx[s++] = (unsigned char)m;
+    }
+    md.xof[kMaxRcfKx] = s;
+    if (s != md.n) return fail(LCS_ERR_INVALID, "forced wavevector outside kx in 0..7");
     return LCS_OK;
 }
 
-int rcf_apply_blocks(const Box &b) { return grid_for(b.g.cells(), 256); }
+int rcf_apply_blocks(const Box &b) { return grid_rows(b.g); }
 int rcf_crop_slots(int kmax) { return (kmax + 1) * (2 * kmax + 1) * (2 * kmax + 1); }
 
 int rcf_prepare(Box &b, int kmin, int kmax, int c)
@@ -2166,16 +2262,23 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_b_rcf_apply(const BoxRec *tab, const WorkItem *work)
 {
     __shared__ double lds[16];
+    __shared__ RcfRowLds rl;
+    __shared__ RcfModes md;  // a copy the field stores cannot alias, off the vector-memory queue
     const WorkItem w = work[blockIdx.x];
     const BoxRec &B = tab[w.box];
     if (b_halted(B)) return;
-    const int c = B.rcf_c, kmax = B.rcf_kmax;
+    static_assert(sizeof(RcfModes) % sizeof(int) == 0, "RcfModes copied as ints");
+    for (int q = threadIdx.x; q < (int)(sizeof(RcfModes) / sizeof(int)); q += blockDim.x)
+        ((int *)&md)[q] = ((const int *)B.rcf_modes)[q];
+    __syncthreads();
+    const Geom g = B.g;
+    const int c = B.rcf_c, kmax = B.rcf_kmax, nblk = B.rcf_blocks;
     const double *crop = B.rcf_buf + 3L * c * c * c + 6L * rcf_crop_slots_dev(kmax);
     const double dti = 1.0 / (1.0 * B.S->dt);
-    const double r = block_sum(rcf_apply_part(B.g, (T *)B.f[0], (T *)B.f[1], (T *)B.f[2], c, kmax,
-                                              *(const RcfModes *)B.rcf_modes, crop, dti, w.blk, B.rcf_blocks),
-                               lds);
-    if (threadIdx.x == 0) B.part[w.blk] = r;
+    T *U = (T *)B.f[0], *V = (T *)B.f[1], *W = (T *)B.f[2];
+    double *part = B.part;
+    const double r = block_sum(rcf_apply_part(g, U, V, W, c, kmax, md, crop, dti, w.blk, nblk, rl), lds);
+    if (threadIdx.x == 0) part[w.blk] = r;
 }
 
 // p = mean over the cells of the power partials (k_mean_to per box)
@@ -2229,7 +2332,7 @@ int launch_force_rcf(Box &b, double energy, int kmin, int kmax, int c, DtRef dt,
     LCS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_rcf_spectrum, dim3(1), dim3(64), 0, b.stream, hat, c, kmin, kmax, energy, crop, b.S);
     LCS_HIP(hipGetLastError());
-    int blocks = grid_for(b.g.cells(), 256);
+    const int blocks = rcf_apply_blocks(b);
     ProfScope ps(b, "rcf_apply", 6.0 * b.esize * b.g.cells());
     if (b.esize == 8)
         hipLaunchKernelGGL(k_rcf_apply<double>, dim3(blocks), dim3(256), 0, b.stream, b.g, (double *)b.f[0],

@@ -4,7 +4,7 @@ Every input is generated from a seed by the package's own initialisers
 (seeded spectrum, uniform particle positions, Stokes numbers, zero slip), so
 the benchmark, the tests and the oracle start from identical states.
 
-* ``single_box("c1" | "c2" | "c3")``: one periodic box as a ``Config``;
+* ``single_box("c1" | "c2" | "c3" | "c6")``: one periodic box as a ``Config``;
 * ``ensemble_configs(nbox)``: the batched ensemble of mixed 32^3-128^3 boxes;
 * ``serve_requests(nreq)``: the request stream of the served workload, as
   keyword arguments of ``Server.submit``, and ``request_config`` giving the
@@ -18,10 +18,10 @@ from . import config as C
 HSMAC_DEFAULT = C.HSMAC(True, True, "red-black-fast", 1.7, 1e-3, 6, 300)
 
 
-def _particles(npart, seed, st):
+def _particles(npart, seed, st, drag="linear", invalid_ratio=0.0):
     if not npart:
         return ()
-    return (C.ParticleConfig(C.Population(npart), C.ParticleParams(0.0, 843.0), 4, "linear",
+    return (C.ParticleConfig(C.Population(npart, invalid_ratio), C.ParticleParams(0.0, 843.0), 4, drag,
                              C.ParticleInit(seed=seed, velocity="zero-slip", size="stokes", st_min=st[0],
                                             st_max=st[1], eps=0.5)),)
 
@@ -33,8 +33,19 @@ def single_box(name, n_override=None, particles_override=None, dtype=None):
     c2: 256^3 FP32, rc-energy (energy 0.649, kmin 1, kmax 2, coarse 8),
         Re = 360, 2e6 particles St log-uniform [0.1, 10].
     c3: 1024^3 FP32, linear-power P = 0.5, Re = 1333, 6.4e7 particles St = 1.
+    c6: 750^3 FP64, rc-energy (energy 0.66, kmin 1, kmax 2, coarse 10),
+        Re = 2220, 325^3 particles St = 1 with nonlinear drag and half the
+        slots spare: the single-device case of the reference's weak-scaling
+        runs (data/gpu-weak.csv:2 with the settings of cfgs/N1500.lcs-yaml;
+        coarse 12 there does not divide 750, 10 is the nearest divisor).
     All start from the seeded spectrum E(k) ~ k^4 exp(-2 (k/4)^2), u' = 1."""
-    if name == "c3":
+    drag, spare = "linear", 0.0
+    if name == "c6":
+        n, npart, dt, Re, forcing, st = 750, 325**3, "f64", 2220.0, C.EnergyPreserveRCF(0.66, 1, 2, 10), (1.0, 1.0)
+        drag, spare = "nonlinear", 0.5
+        desc = ("single periodic box 750^3 FP64, rc-energy (coarse 10), Re=2220, 325^3 particles St=1, "
+                "nonlinear drag")
+    elif name == "c3":
         n, npart, dt, Re, forcing, st = 1024, 64_000_000, "f32", 1333.3, C.ConstantPowerLF(0.5), (1.0, 1.0)
         desc = "single periodic box 1024^3 FP32, linear-power P=0.5, Re=1333, 6.4e7 particles St=1"
     elif name == "c2":
@@ -48,7 +59,16 @@ def single_box(name, n_override=None, particles_override=None, dtype=None):
     if n_override:
         npart = int(round(npart * (n_override / n) ** 3))
         n = n_override
-        desc += f" (sampled at {n}^3 with {npart} particles)"
+        if isinstance(forcing, C.EnergyPreserveRCF) and n % forcing.coarse_size:
+            # the coarse size must divide the grid: the largest divisor not
+            # above the workload's own that still holds the forced shells
+            fit = [c for c in range(2 * forcing.kmax, n + 1) if n % c == 0]
+            below = [c for c in fit if c <= forcing.coarse_size]
+            c = below[-1] if below else fit[0]
+            forcing = C.EnergyPreserveRCF(forcing.energy, forcing.kmin, forcing.kmax, c)
+            desc += f" (sampled at {n}^3 with {npart} particles, coarse {c})"
+        else:
+            desc += f" (sampled at {n}^3 with {npart} particles)"
     if particles_override is not None:
         npart = particles_override
     if dtype is not None:
@@ -56,7 +76,7 @@ def single_box(name, n_override=None, particles_override=None, dtype=None):
     cfg = C.Config(grid=C.Grid((n, n, n), 3), timestep=C.TimeStep(0.3, 0.01), simulate=C.SimulateConfig(0),
                    flow=C.FlowConfig(C.FlowParams(1.0, 1.0, 1.0 / Re), C.SpectrumInit(1, 4.0, 1.0), HSMAC_DEFAULT,
                                      forcing),
-                   particles=_particles(npart, 7, st), dtype=dt)
+                   particles=_particles(npart, 7, st, drag, spare), dtype=dt)
     return cfg, desc
 
 
