@@ -193,7 +193,11 @@ int lcs_hsmac(lcs_box *box, int stage, double dt_stage, const lcs_hsmac_params *
 /* forcing!(::ConstantPowerLF) (src/flows/forcings/lf.jl:6-106) on Us */
 int lcs_force_linear(lcs_box *box, double power, double dt, double *p);
 /* forcing!(::EnergyPreserveRCF) (src/flows/forcings/rcf.jl:1-191) on Us;
- * box mean, spectrum rescaling and sparse inverse DFT all run on the device */
+ * box mean, spectrum rescaling and sparse inverse DFT all run on the device.
+ * Needs 1 <= kmin <= kmax <= coarse_size/2, coarse_size dividing every grid
+ * dimension, and at most 128 forced wavevectors (shells kmin..kmax counted
+ * once per +-k pair: 9 for shell 1, 31 for shell 2, 105 for shell 4 alone,
+ * so kmax <= 4); otherwise LCS_ERR_INVALID. */
 int lcs_force_rcf(lcs_box *box, double energy, int kmin, int kmax, int coarse_size, double dt,
                   double *p);
 /* lcs_force_rcf without any device allocation.  It has no counterpart in the
@@ -201,7 +205,7 @@ int lcs_force_rcf(lcs_box *box, double energy, int kmin, int kmax, int coarse_si
  * capture, tight serving loops).  Same arguments, kernels and results as
  * lcs_force_rcf, but the coarse grids, spectra and crop live in one fixed
  * 1 MiB device array per GPU (131072 doubles: 3*c^3 + 12*(kmax+1)*(2kmax+1)^2
- * must fit, e.g. coarse_size <= 32 with kmax <= 8) instead of the box's own
+ * must fit, e.g. coarse_size <= 32 with kmax <= 4) instead of the box's own
  * buffer; larger requests fail with LCS_ERR_CAPACITY.  Every call on a GPU
  * shares that array, so calls that overlap on one GPU (boxes on different
  * streams) race; the array keeps the last call's contents, and every element
