@@ -117,6 +117,8 @@ def bench_ensemble(args):
     batch = L.Batch()
     if args.no_graphs:
         batch.set_graphs(False)
+    if args.persistent:
+        batch.set_persistent(True)
     for b, s in zip(boxes, scfgs):
         batch.add(b, s)
     for _ in range(args.warmup):
@@ -150,6 +152,7 @@ def bench_ensemble(args):
                    "boxes": len(boxes), "cells": cells,
                    "particles": sum(c.particles[0].population.valid for c in cfgs),
                    "particle_localize_interval": args.localize_interval, "graphs": not args.no_graphs,
+                   "persistent_sweeps": args.persistent,
                    "sweep_tiling": tiling or "32x8, z-chunks of about 28 planes (default)",
                    "steps_by_path": gstats},
         "cell_steps_per_s": cells * args.steps / el,
@@ -354,6 +357,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="default run: skip the served-stream sub-run")
     ap.add_argument("--serve-requests", type=int, default=200, help="requests of the served-stream sub-run")
     ap.add_argument("--no-autotune", action="store_true", help="use the size rule for tile shapes")
+    ap.add_argument("--persistent", action="store_true",
+                    help="c4: batched HSMAC sweeps as persistent grids walking the tile table")
     ap.add_argument("--no-graphs", action="store_true",
                     help="c4: launch the batched step eagerly instead of replaying hipGraphs (kernel work is the "
                          "same; used for profiling, where rocprofv3 ended graph replays with a host fault)")

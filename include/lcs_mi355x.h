@@ -402,6 +402,14 @@ int lcs_batch_set_graphs(lcs_batch *batch, int on);
  * end, gated = one graph stopped at a gate and finished with polled groups,
  * segmented = eager or segment launches.  Any pointer may be NULL. */
 int lcs_batch_graph_stats(lcs_batch *batch, int64_t *whole, int64_t *gated, int64_t *segmented);
+/* No counterpart in the reference (its kernels launch one workgroup per
+ * block).  cus < 0: each batched HSMAC sweep runs as a persistent grid, as
+ * many workgroups as the device holds at once, each walking the tile table
+ * (tiles b, b + G, b + 2G, ...); cus > 0: the grid that many CUs hold (at most
+ * the device's); cus = 0 (default): one workgroup per tile.  Results are the
+ * same in every case (a tile's arithmetic does not depend on the workgroup
+ * that runs it).  Drops the batch's captured graphs. */
+int lcs_batch_set_persistent(lcs_batch *batch, int cus);
 /* No counterpart in the reference (its kernels have fixed workgroups).  The
  * batched HSMAC sweep tiles each size class of boxes (equal nx, ny, nz) with
  * one shape, 64x8, 32x8, 64x16 or 32x16, in z-chunks of a target length; by

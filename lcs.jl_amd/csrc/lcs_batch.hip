@@ -91,6 +91,12 @@ struct Batch {
     // sweep tiling per size class (nx, ny, nz): index into kBatchSweepShapes
     // and z-chunk target in planes (lcs_batch_autotune, lcs_batch_set_tiles)
     std::map<std::array<int, 3>, std::array<int, 2>> sweep_tune;
+    // lcs_batch_set_persistent: persist_cus > 0 runs the sweeps as persistent
+    // grids over the tile table, as many workgroups as that many CUs hold at
+    // once; 0: one workgroup per tile.  cus: the device's CU count.
+    int persist_cus = 0;
+    int cus = 256;
+    int *queue_d = nullptr;  // the persistent sweep's per-XCD tile queues (9 ints, zero between launches)
 };
 
 constexpr int kPostWindow = 8;
@@ -437,7 +443,7 @@ static int b_sweep(Batch &B, int set, int kind, int slot, double dtfac)
     for (int q = 0; q < 4; q++)
         if ((rc = launch_b_hsmac_sweep(B.stream, B.esize, B.hs.coloring, kBatchSweepShapes[q][0],
                                        kBatchSweepShapes[q][1], B.tab.d, B.hs_tiles[q].d, B.hs_tiles[q].n, set, kind,
-                                       slot, dtfac, force, wrap)))
+                                       slot, dtfac, force, wrap, B.persist_cus, B.queue_d)))
             return rc;
     if (force && (rc = launch_b_lf_power(B.stream, B.tab.d, B.lf_boxes.d, B.lf_boxes.n))) return rc;
     return launch_b_hsmac_ctl(B.stream, B.tab.d, (int)B.boxes.size(), kHsFlip, 0, nullptr);
@@ -872,7 +878,7 @@ static int time_group_sweeps(Batch &B, int reps, hipEvent_t e0, hipEvent_t e1, f
             for (int q = 0; q < 4 && r == LCS_OK; q++)
                 r = launch_b_hsmac_sweep(B.stream, B.esize, B.hs.coloring, kBatchSweepShapes[q][0],
                                          kBatchSweepShapes[q][1], B.tab.d, B.hs_tiles[q].d, B.hs_tiles[q].n, 0, k, 0,
-                                         1.0, false, false);
+                                         1.0, false, false, B.persist_cus, B.queue_d);
         return r;
     };
     if ((rc = run())) return rc;
@@ -992,15 +998,19 @@ int lcs_batch_create(lcs_ctx *ctx, lcs_batch **out)
     LCS_HIP(hipSetDevice(c->device));
     Batch *B = new Batch;
     B->ctx = c;
+    if (hipDeviceGetAttribute(&B->cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || B->cus < 1)
+        B->cus = 256;
     if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) {
         delete B;
         return fail(LCS_ERR_HIP, "hipStreamCreate failed");
     }
     if (hipMalloc(&B->halt_d, sizeof(int)) != hipSuccess ||
         hipHostMalloc((void **)&B->halt_h, sizeof(int), hipHostMallocDefault) != hipSuccess ||
-        hipMemset(B->halt_d, 0, sizeof(int)) != hipSuccess) {
+        hipMemset(B->halt_d, 0, sizeof(int)) != hipSuccess || hipMalloc(&B->queue_d, 9 * sizeof(int)) != hipSuccess ||
+        hipMemset(B->queue_d, 0, 9 * sizeof(int)) != hipSuccess) {
         if (B->halt_d) (void)hipFree(B->halt_d);
         if (B->halt_h) (void)hipHostFree(B->halt_h);
+        if (B->queue_d) (void)hipFree(B->queue_d);
         (void)hipStreamDestroy(B->stream);
         delete B;
         return fail(LCS_ERR_HIP, "stop flag allocation failed");
@@ -1035,6 +1045,7 @@ int lcs_batch_destroy(lcs_batch *batch)
     if (B->sc_h) (void)hipHostFree(B->sc_h);
     if (B->halt_d) (void)hipFree(B->halt_d);
     if (B->halt_h) (void)hipHostFree(B->halt_h);
+    if (B->queue_d) (void)hipFree(B->queue_d);
     if (B->stream) (void)hipStreamDestroy(B->stream);
     delete B;
     return LCS_OK;
@@ -1108,6 +1119,16 @@ int lcs_batch_set_graphs(lcs_batch *batch, int on)
     drop_graphs(B);
     B.graphs = on != 0;
     if (on) B.graph_mode = on;
+    return LCS_OK;
+}
+
+int lcs_batch_set_persistent(lcs_batch *batch, int cus)
+{
+    if (!batch) return fail(LCS_ERR_INVALID, "null argument");
+    Batch &B = *BT(batch);
+    LCS_HIP(hipStreamSynchronize(B.stream));
+    drop_graphs(B);  // captured launches keep the grid they were captured with
+    B.persist_cus = cus < 0 ? B.cus : std::min(cus, B.cus);
     return LCS_OK;
 }
 

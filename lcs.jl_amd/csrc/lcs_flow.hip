@@ -2774,13 +2774,11 @@ __device__ inline void hs_coef(const BoxRec &B, HsCoef<T> &c)
     c.omega = T(B.hs_omega);
 }
 
-template <typename T, int COLORING, int TY, int TX, bool FORCE = false>
-__global__ void __launch_bounds__((HsTile<T, COLORING, TY, TX>::NT))
-    k_b_hsmac_sweep(const BoxRec *tab, const TileItem *tiles, int set, int kind, int slot, double dtfac, int wrap)
+// one tile of the batched sweep (a no-op for a halted or converged box)
+template <typename T, int COLORING, int TY, int TX, bool FORCE>
+__device__ __forceinline__ void b_sweep_tile(const BoxRec *tab, const TileItem &t, int set, int kind, int slot, double dtfac,
+                                    int wrap, T *lds, double *red_lds)
 {
-    extern __shared__ __align__(16) unsigned char hs_lds_raw[];
-    __shared__ double red_lds[16];
-    const TileItem t = tiles[blockIdx.x];
     const BoxRec &B = tab[t.box];
     if (b_halted(B)) return;
     if (B.S->hs_done) return;
@@ -2804,8 +2802,7 @@ __global__ void __launch_bounds__((HsTile<T, COLORING, TY, TX>::NT))
         a.xdt = T(dtfac * B.S->dt);
     }
     double pw = 0.0;
-    const double s = hsmac_sweep_block<T, COLORING, TY, TX, FORCE>(a, t.bx, t.by, t.bz,
-                                                                   reinterpret_cast<T *>(hs_lds_raw), red_lds, &pw);
+    const double s = hsmac_sweep_block<T, COLORING, TY, TX, FORCE>(a, t.bx, t.by, t.bz, lds, red_lds, &pw);
     const int gx = (B.g.nx + 2 * B.g.h + TX - 1) / TX, gy = (B.g.ny + 2 * B.g.h + TY - 1) / TY;
     const long tix = t.bx + gx * (t.by + gy * t.bz);
     if (threadIdx.x == 0) {
@@ -2814,45 +2811,108 @@ __global__ void __launch_bounds__((HsTile<T, COLORING, TY, TX>::NT))
     }
 }
 
+// One workgroup per tile of the list.
+template <typename T, int COLORING, int TY, int TX, bool FORCE = false>
+__global__ void __launch_bounds__((HsTile<T, COLORING, TY, TX>::NT))
+    k_b_hsmac_sweep(const BoxRec *tab, const TileItem *tiles, int set, int kind, int slot, double dtfac, int wrap)
+{
+    extern __shared__ __align__(16) unsigned char hs_lds_raw[];
+    __shared__ double red_lds[16];
+    b_sweep_tile<T, COLORING, TY, TX, FORCE>(tab, tiles[blockIdx.x], set, kind, slot, dtfac, wrap,
+                                             reinterpret_cast<T *>(hs_lds_raw), red_lds);
+}
+
+// The same sweep as a persistent grid that walks the tile list.  queue =
+// null: workgroup b takes tiles b, b + G, b + 2G, ... (G = the grid).
+// Otherwise (G a multiple of 8) tiles come from eight queues, one per XCD:
+// the list puts item L on XCD L % 8, and workgroup b, which the hardware runs
+// on XCD b % 8, takes items x, x + 8, x + 16, ... (x = b % 8) in the order
+// its queue hands them out.  queue[0..7] count the items handed out,
+// queue[8] the workgroups done; the last one done sets all nine back to zero
+// for the next launch.  A tile's result does not depend on the workgroup that
+// runs it.
+template <typename T, int COLORING, int TY, int TX, bool FORCE = false>
+__global__ void __launch_bounds__((HsTile<T, COLORING, TY, TX>::NT))
+    k_b_hsmac_sweep_walk(const BoxRec *tab, const TileItem *tiles, int ntiles, int set, int kind, int slot,
+                         double dtfac, int wrap, int *queue)
+{
+    extern __shared__ __align__(16) unsigned char hs_lds_raw[];
+    __shared__ double red_lds[16];
+    __shared__ int item;
+    const int x = blockIdx.x % 8;
+    for (int k = 0;; k++) {
+        if (k) __syncthreads();  // the previous tile is done with the LDS, and every thread has read `item`
+        if (threadIdx.x == 0) item = queue ? x + 8 * atomicAdd(&queue[x], 1) : (int)(blockIdx.x + k * gridDim.x);
+        __syncthreads();
+        const int i = item;
+        if (i >= ntiles) break;  // uniform over the workgroup
+        b_sweep_tile<T, COLORING, TY, TX, FORCE>(tab, tiles[i], set, kind, slot, dtfac, wrap,
+                                                 reinterpret_cast<T *>(hs_lds_raw), red_lds);
+    }
+    if (queue && threadIdx.x == 0 && atomicAdd(&queue[8], 1) == (int)gridDim.x - 1)
+        for (int q = 0; q < 9; q++) atomicExch(&queue[q], 0);
+}
+
 template <typename T, int COLORING, int TX, int TY, bool FORCE>
 static int b_sweep_f(hipStream_t s, const BoxRec *tab, const TileItem *tiles, int ntiles, int set, int kind, int slot,
-                     double dtfac, int wrap)
+                     double dtfac, int wrap, int cus, int *queue)
 {
     using Tile = HsTile<T, COLORING, TY, TX, FORCE>;
     const size_t lds = Tile::LDS_ELEMS * sizeof(T);
+    if (cus > 0) {
+        int per_cu = 1;
+        const void *k = (const void *)k_b_hsmac_sweep_walk<T, COLORING, TY, TX, FORCE>;
+        int rc = kernel_setup(k, Tile::NT, lds, &per_cu);
+        if (rc) return rc;
+        long grid = (long)cus * per_cu;
+        if (grid < ntiles) {
+            int *q = nullptr;
+            if (queue && grid >= 8) {  // per-XCD queues need every XCD's workgroups present
+                grid -= grid % 8;
+                q = queue;
+            }
+            hipLaunchKernelGGL((k_b_hsmac_sweep_walk<T, COLORING, TY, TX, FORCE>), dim3((int)grid), dim3(Tile::NT),
+                               lds, s, tab, tiles, ntiles, set, kind, slot, dtfac, wrap, q);
+            LCS_HIP(hipGetLastError());
+            return LCS_OK;
+        }
+    }
     int rc = kernel_setup((const void *)k_b_hsmac_sweep<T, COLORING, TY, TX, FORCE>, Tile::NT, lds, nullptr);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_b_hsmac_sweep<T, COLORING, TY, TX, FORCE>), dim3(ntiles), dim3(Tile::NT), lds, s, tab, tiles,
-                       set, kind, slot, dtfac, wrap);
+    hipLaunchKernelGGL((k_b_hsmac_sweep<T, COLORING, TY, TX, FORCE>), dim3(ntiles), dim3(Tile::NT), lds, s, tab,
+                       tiles, set, kind, slot, dtfac, wrap);
     LCS_HIP(hipGetLastError());
     return LCS_OK;
 }
 
 template <typename T, int COLORING, int TX, int TY>
 static int b_sweep_t(hipStream_t s, const BoxRec *tab, const TileItem *tiles, int ntiles, int set, int kind, int slot,
-                     double dtfac, bool force, int wrap)
+                     double dtfac, bool force, int wrap, int cus, int *queue)
 {
-    return force ? b_sweep_f<T, COLORING, TX, TY, true>(s, tab, tiles, ntiles, set, kind, slot, dtfac, wrap)
-                 : b_sweep_f<T, COLORING, TX, TY, false>(s, tab, tiles, ntiles, set, kind, slot, dtfac, wrap);
+    return force ? b_sweep_f<T, COLORING, TX, TY, true>(s, tab, tiles, ntiles, set, kind, slot, dtfac, wrap, cus, queue)
+                 : b_sweep_f<T, COLORING, TX, TY, false>(s, tab, tiles, ntiles, set, kind, slot, dtfac, wrap, cus,
+                                                          queue);
 }
 
 template <typename T, int TX, int TY>
 static int b_sweep_c(hipStream_t s, int coloring, const BoxRec *tab, const TileItem *tiles, int ntiles, int set,
-                     int kind, int slot, double dtfac, bool force, int wrap)
+                     int kind, int slot, double dtfac, bool force, int wrap, int cus, int *queue)
 {
     return coloring == LCS_COLORING_RED_BLACK
                ? b_sweep_t<T, LCS_COLORING_RED_BLACK, TX, TY>(s, tab, tiles, ntiles, set, kind, slot, dtfac, force,
-                                                              wrap)
+                                                              wrap, cus, queue)
                : b_sweep_t<T, LCS_COLORING_RED_BLACK_FAST, TX, TY>(s, tab, tiles, ntiles, set, kind, slot, dtfac,
-                                                                    force, wrap);
+                                                                    force, wrap, cus, queue);
 }
 
 template <typename T>
 static int b_sweep_shape(hipStream_t s, int coloring, int tx, int ty, const BoxRec *tab, const TileItem *tiles,
-                         int ntiles, int set, int kind, int slot, double dtfac, bool force, int wrap)
+                         int ntiles, int set, int kind, int slot, double dtfac, bool force, int wrap, int cus,
+                         int *queue)
 {
 #define B_SHAPE(X, Y) \
-    if (tx == X && ty == Y) return b_sweep_c<T, X, Y>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac, force, wrap)
+    if (tx == X && ty == Y)                                                                                         \
+        return b_sweep_c<T, X, Y>(s, coloring, tab, tiles, ntiles, set, kind, slot, dtfac, force, wrap, cus, queue)
     B_SHAPE(64, 8);
     B_SHAPE(32, 8);
     B_SHAPE(64, 16);
@@ -2863,13 +2923,14 @@ static int b_sweep_shape(hipStream_t s, int coloring, int tx, int ty, const BoxR
 
 int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, int tx, int ty, const BoxRec *tab,
                          const TileItem *tiles, int ntiles, int set, int kind, int slot, double dtfac, bool force,
-                         bool wrap)
+                         bool wrap, int persist_cus, int *queue)
 {
     if (ntiles == 0) return LCS_OK;
-    const int w = wrap ? 1 : 0;
-    return esize == 8
-               ? b_sweep_shape<double>(s, coloring, tx, ty, tab, tiles, ntiles, set, kind, slot, dtfac, force, w)
-               : b_sweep_shape<float>(s, coloring, tx, ty, tab, tiles, ntiles, set, kind, slot, dtfac, force, w);
+    const int w = wrap ? 1 : 0, c = persist_cus;
+    return esize == 8 ? b_sweep_shape<double>(s, coloring, tx, ty, tab, tiles, ntiles, set, kind, slot, dtfac, force,
+                                              w, c, queue)
+                      : b_sweep_shape<float>(s, coloring, tx, ty, tab, tiles, ntiles, set, kind, slot, dtfac, force,
+                                             w, c, queue);
 }
 
 // one colour over both face slabs of one axis, in place on the current set

@@ -346,9 +346,13 @@ int launch_b_rhs(hipStream_t s, int esize, const BoxRec *tab, const TileItem *ti
 // leave their per-tile power in part[kPowerSlot]
 // wrap: U, V, W of ghost cells are loaded from their periodic images (the
 // sweep stands in for a FullHalo fill of U, V, W of width h before it)
+// persist_cus > 0: a persistent grid of as many workgroups as that many CUs
+// hold at once walks the tile list, drawing from per-XCD queues (queue: 9
+// zeroed ints on the device, left zeroed) where the grid has 8 or more
+// workgroups and queue is given, else by stride; 0: one workgroup per tile
 int launch_b_hsmac_sweep(hipStream_t s, int esize, int coloring, int tx, int ty, const BoxRec *tab,
                          const TileItem *tiles, int ntiles, int set, int kind, int slot, double dtfac,
-                         bool force = false, bool wrap = false);
+                         bool force = false, bool wrap = false, int persist_cus = 0, int *queue = nullptr);
 // both colours over the x face slabs of every box (lcs_hsmac_xface.h): one
 // work item per patch, partials of a box in part[slot][patch]; then one work
 // item per 256 slab rows for the commit

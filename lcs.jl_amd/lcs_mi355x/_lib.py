@@ -168,6 +168,7 @@ SIGNATURES = {
     "lcs_batch_step": [P, P, I],
     "lcs_batch_sync": [P],
     "lcs_batch_set_graphs": [P, I],
+    "lcs_batch_set_persistent": [P, I],
     "lcs_batch_autotune": [P],
     "lcs_batch_graph_stats": [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "lcs_batch_set_tiles": [P, I, I, I, I],

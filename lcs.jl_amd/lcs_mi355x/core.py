@@ -415,6 +415,14 @@ class Batch:
         3 one graph per step with the fewest HSMAC groups, False or 0 eager."""
         check(self.L.lcs_batch_set_graphs(self.handle, int(on)))
 
+    def set_persistent(self, on, cus=None):
+        """Persistent sweep grids over the tile table (lcs_batch_set_persistent):
+        True walks the tiles with as many workgroups as the device holds at
+        once (or as `cus` CUs hold), False (default) launches one workgroup
+        per tile."""
+        n = 0 if not on else (-1 if cus is None else int(cus))
+        check(self.L.lcs_batch_set_persistent(self.handle, n))
+
     def graph_stats(self):
         """{"whole", "gated", "segmented"}: steps by path (lcs_batch_graph_stats)."""
         v = [C.c_int64() for _ in range(3)]

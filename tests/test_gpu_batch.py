@@ -160,6 +160,38 @@ def test_graph_replays_match_eager_steps(dtype, tb, itrmin, itrmax, mode, rcf):
     be.close()
 
 
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_persistent_sweeps_change_no_result(dtype):
+    """Persistent sweep grids give the one-workgroup-per-tile results bit for
+    bit, logs included: a tile's arithmetic and its partial-sum slot do not
+    depend on the workgroup that runs it.  First the grid one CU holds (fewer
+    than 8 workgroups: each walks the table by stride), then the grid of two
+    CUs (8 or more: workgroups draw tiles from the per-XCD queues, many tiles
+    each).  Eight steps; steps 2-4 and 7 replay step graphs, so the queues'
+    reset to zero at the end of every launch is exercised by replays too."""
+    import lcs_mi355x as L
+    cfgs = mixed_configs(dtype)
+    ta = [L.build(c) for c in cfgs]
+    da = [L.build(c) for c in cfgs]
+    bt, bd = L.Batch(), L.Batch()
+    for c, b in zip(cfgs, ta):
+        bt.add(b, L.step_config(c, localize_interval=2))
+    for c, b in zip(cfgs, da):
+        bd.add(b, L.step_config(c, localize_interval=2))
+    bt.set_persistent(True, cus=1)
+    for s in range(8):
+        if s == 5:  # (drops the captured graphs: steps 5 and 6 run eagerly)
+            bt.set_persistent(True, cus=2)
+        la, lb = bt.step(), bd.step()
+        for a, b in zip(la, lb):
+            assert a == b, (s, a, b)
+        for q, (a, b) in enumerate(zip(ta, da)):
+            for n in ("U", "V", "W", "P"):
+                np.testing.assert_array_equal(a.download(n), b.download(n), err_msg=f"step {s} box {q} {n}")
+    st = bt.graph_stats()
+    assert sum(st.values()) == 8 and st["whole"] + st["gated"] == 4, st
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_batch_tiling_changes_no_result(dtype):
     """The autotuner's timing sweeps leave every box as it was, and the

@@ -236,9 +236,12 @@ def test_boxmean_kat_on_device():
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-@pytest.mark.parametrize("dims,c", [((16, 16, 16), 4), ((24, 24, 24), 6), ((32, 32, 32), 8)])
+@pytest.mark.parametrize("dims,c", [((16, 16, 16), 4), ((24, 24, 24), 6), ((32, 32, 32), 8),
+                                    ((1040, 10, 20), 10), ((30, 20, 10), 10)])
 def test_rc_energy_forcing(rng, dtype, dims, c):
-    """test/test_flows.jl:150-164 uses 16^3 with coarse 4 (aliased Nyquist shell)."""
+    """test/test_flows.jl:150-164 uses 16^3 with coarse 4 (aliased Nyquist shell).
+    The apply takes x-rows in groups of up to 1024 cells: 1040-cell rows take
+    two passes each, 30-cell rows go six to a group with a ragged last group."""
     h, dt = 3, 0.01
     Us = [rand_halo(rng, dims, h, dtype) for _ in range(3)]
     box = make_box(dims, h, dtype)
