@@ -212,35 +212,49 @@ struct RhsTile {
         }
     }
 
-    // phase B: in-plane fluxes at plane z (reads U, V at z-1..z+2, W at z)
+    // phase B: in-plane fluxes at plane z (reads U, V at z-1..z+2, W at z).
+    // Each item reads every ring value it needs before it writes a flux: the
+    // ring and the flux arrays never overlap, and with the reads first the
+    // compiler issues them together instead of waiting on LDS after each
+    // write.  The arithmetic is unchanged, operand for operand.
     __host__ __device__ static void phase_fluxes(T *lds, int z, int tid)
     {
         // x-fluxes: A = U interpolated along c, Q = component c along x
         for (int e = tid; e < FX * TY; e += NT) {
             const int fx = e % FX, ty = e / FX;
             const int lx = fx + 1, ly = ty + 3;
-            const T ac[3] = {
-                i4(R(lds, 0, z, ly, lx - 1), R(lds, 0, z, ly, lx), R(lds, 0, z, ly, lx + 1), R(lds, 0, z, ly, lx + 2)),
-                i4(R(lds, 0, z, ly - 1, lx), R(lds, 0, z, ly, lx), R(lds, 0, z, ly + 1, lx), R(lds, 0, z, ly + 2, lx)),
-                i4(R(lds, 0, z - 1, ly, lx), R(lds, 0, z, ly, lx), R(lds, 0, z + 1, ly, lx), R(lds, 0, z + 2, ly, lx))};
+            T q[3][4];  // component c at x = lx-1 .. lx+2
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) q[c][k] = R(lds, c, z, ly, lx - 1 + k);
+            const T uy[3] = {R(lds, 0, z, ly - 1, lx), R(lds, 0, z, ly + 1, lx), R(lds, 0, z, ly + 2, lx)};
+            const T uz[3] = {R(lds, 0, z - 1, ly, lx), R(lds, 0, z + 1, ly, lx), R(lds, 0, z + 2, ly, lx)};
+            const T ac[3] = {i4(q[0][0], q[0][1], q[0][2], q[0][3]), i4(uy[0], q[0][1], uy[1], uy[2]),
+                             i4(uz[0], q[0][1], uz[1], uz[2])};
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                JXr(lds, 2 * c, ty, fx) = ac[c] * i2(R(lds, c, z, ly, lx), R(lds, c, z, ly, lx + 1));
-                JXr(lds, 2 * c + 1, ty, fx) = ac[c] * i2(R(lds, c, z, ly, lx - 1), R(lds, c, z, ly, lx + 2));
+                JXr(lds, 2 * c, ty, fx) = ac[c] * i2(q[c][1], q[c][2]);
+                JXr(lds, 2 * c + 1, ty, fx) = ac[c] * i2(q[c][0], q[c][3]);
             }
         }
         // y-fluxes: A = V interpolated along c, Q = component c along y
         for (int e = tid; e < TX * FY; e += NT) {
             const int tx = e % TX, fy = e / TX;
             const int lx = tx + 3, ly = fy + 1;
-            const T ac[3] = {
-                i4(R(lds, 1, z, ly, lx - 1), R(lds, 1, z, ly, lx), R(lds, 1, z, ly, lx + 1), R(lds, 1, z, ly, lx + 2)),
-                i4(R(lds, 1, z, ly - 1, lx), R(lds, 1, z, ly, lx), R(lds, 1, z, ly + 1, lx), R(lds, 1, z, ly + 2, lx)),
-                i4(R(lds, 1, z - 1, ly, lx), R(lds, 1, z, ly, lx), R(lds, 1, z + 1, ly, lx), R(lds, 1, z + 2, ly, lx))};
+            T q[3][4];  // component c at y = ly-1 .. ly+2
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) q[c][k] = R(lds, c, z, ly - 1 + k, lx);
+            const T vx[3] = {R(lds, 1, z, ly, lx - 1), R(lds, 1, z, ly, lx + 1), R(lds, 1, z, ly, lx + 2)};
+            const T vz[3] = {R(lds, 1, z - 1, ly, lx), R(lds, 1, z + 1, ly, lx), R(lds, 1, z + 2, ly, lx)};
+            const T ac[3] = {i4(vx[0], q[1][1], vx[1], vx[2]), i4(q[1][0], q[1][1], q[1][2], q[1][3]),
+                             i4(vz[0], q[1][1], vz[1], vz[2])};
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                JYr(lds, 2 * c, fy, tx) = ac[c] * i2(R(lds, c, z, ly, lx), R(lds, c, z, ly + 1, lx));
-                JYr(lds, 2 * c + 1, fy, tx) = ac[c] * i2(R(lds, c, z, ly - 1, lx), R(lds, c, z, ly + 2, lx));
+                JYr(lds, 2 * c, fy, tx) = ac[c] * i2(q[c][1], q[c][2]);
+                JYr(lds, 2 * c + 1, fy, tx) = ac[c] * i2(q[c][0], q[c][3]);
             }
         }
     }
