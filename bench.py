@@ -6,6 +6,11 @@ whole box, with all state resident in HBM.  Default workload: one 1024^3 FP32
 box, linear-power P = 0.5, Re = 1333 (Re_lambda ~ 200 at u' = 1, eps = P),
 6.4e7 uniform particles with St = 1, 5 warm-up + 20 timed steps.
 
+The default run adds three sub-objects to its line: ``ensemble`` (the batched
+96-box ensemble against the same boxes stepped one at a time), ``served`` (the
+served request stream) and ``reference_case`` (750^3 FP64 with 325^3 particles,
+the one-device case of the reference's published weak-scaling runs).
+
 Prints one JSON line.  ``roofline`` describes the kernel with the largest total
 time in the timed region, timed live with HIP events on the box stream;
 ``cpu_baseline`` times the CPU oracle (the C restatement of the reference
@@ -164,6 +169,39 @@ def bench_ensemble(args):
     for b in boxes:
         b.close()
     return out
+
+
+def bench_reference_case(args, steps=6, warmup=3):
+    """The one-device case of the reference's weak-scaling runs (workload c6:
+    750^3 FP64, rc-energy, 325^3 particles with nonlinear drag), timed beside
+    the published figure it corresponds to (data/gpu-weak.csv:2: 0.9237 s on
+    an H100 in FP64; the csv does not say whether that is one step)."""
+    import lcs_mi355x as L
+    cfg, desc = workload("c6")
+    box = L.build(cfg)
+    tiles = L.get_tiles(box) if args.no_autotune else L.autotune(box)
+    scfg = L.step_config(cfg, localize_interval=args.localize_interval)
+    for _ in range(warmup):
+        L.step(box, scfg)
+    box.sync()
+    sweeps = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        log = L.step(box, scfg)
+        sweeps += log["flow"]["couple1"]["itr"] + log["flow"]["couple2"]["itr"] + 2
+    box.sync()
+    el = time.perf_counter() - t0
+    box.close()
+    n3 = math.prod(cfg.grid.dims)
+    ms = el / steps * 1e3
+    print(f"# reference case: {desc}: {ms:.1f} ms/step", file=sys.stderr, flush=True)
+    return {"workload": desc, "dtype": cfg.dtype, "steps": steps, "warmup": warmup, "ms_per_step": ms,
+            "cell_steps_per_s": n3 * steps / el, "hsmac_sweeps_per_step": sweeps / steps,
+            "tiles": {k: list(v) for k, v in tiles.items()},
+            "reference": {"seconds": 0.9237, "hardware": "H100 SXM5, FP64", "source": "data/gpu-weak.csv:2",
+                          "note": "unit not stated in the csv; the reference starts from a restart file of "
+                                  "developed turbulence, this run from the seeded spectrum"},
+            "ratio_if_per_step": 923.7 / ms}
 
 
 def serve_requests(nreq, seed=4242):
@@ -356,6 +394,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=30, help="timed batch steps of the ensemble sub-run")
     ap.add_argument("--no-serve", action="store_true", help="default run: skip the served-stream sub-run")
     ap.add_argument("--serve-requests", type=int, default=200, help="requests of the served-stream sub-run")
+    ap.add_argument("--no-reference-case", action="store_true",
+                    help="default run: skip the 750^3 FP64 sub-run of the reference's one-device case")
     ap.add_argument("--no-autotune", action="store_true", help="use the size rule for tile shapes")
     ap.add_argument("--persistent", action="store_true",
                     help="c4: batched HSMAC sweeps as persistent grids walking the tile table")
@@ -478,6 +518,8 @@ def main():
         sa = argparse.Namespace(**vars(args))
         sa.requests = args.serve_requests
         out["served"] = bench_serve(sa)
+    if args.workload == "c3" and args.size is None and not args.no_reference_case:
+        out["reference_case"] = bench_reference_case(args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_size, args.cpu_steps, args.cpu1_size,
                                            args.cpu1_steps)
