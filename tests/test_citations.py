@@ -19,7 +19,7 @@ REF = "/root/reference"
 CITE = re.compile(r"(?<![\w/.])((?:[\w.-]+/)*[\w.-]+\.(?:jl|toml|yaml|lcs-yaml|csv|md)):(\d+(?:-\d+)?(?:,\d+(?:-\d+)?)*)")
 SKIP_DIRS = {".git", "_work", "gpurun_out", "__pycache__", "_build", ".pytest_cache", "oracle/_ref"}
 SCAN_EXT = (".py", ".hip", ".h", ".c", ".cpp", ".md", ".jl", ".toml", ".txt")
-# files written by the survey and review steps, not part of the build
+# top-level documents that this repository does not write (their citations are not checked here)
 NOT_OURS = {"SURVEY.md", "VERDICT.md", "ADVICE.md", "BASELINE.md"}
 
 pytestmark = pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "src")), reason="reference checkout absent")
