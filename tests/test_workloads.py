@@ -33,3 +33,15 @@ def test_sampled_grid_coarse_size_divides(n, want):
 def test_sampled_linear_forcing_unchanged():
     cfg, _ = single_box("c3", n_override=64)
     assert isinstance(cfg.flow.forcing, C.ConstantPowerLF) and cfg.grid.dims == (64, 64, 64)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU; asked for more it says so and exits 2
+    before touching a device."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU only" in r.stderr and r.stdout == ""
