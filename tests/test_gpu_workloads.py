@@ -19,6 +19,9 @@
 * C5 (served stream): 60 requests from the workload's own distribution,
   arriving over time; every run completes with its step count, and three
   sampled runs equal the same runs served alone, bit for bit.
+* C6 (750^3 FP64, the reference's one-device weak-scaling case) sampled at
+  100^3: three steps against the FP64 oracle within 1e-9, particles by id,
+  every cell index.
 * A served FP64 run, admitted while other runs are on the GPU, follows the
   oracle stepped from the same seeded initial state (fields within relative
   L2 1e-8), for both forcing kinds.
@@ -149,6 +152,28 @@ def test_c2_256cubed_rc_energy_against_oracle_then_fp64():
     assert abs(f32["eps"] / f64["eps"] - 1) < 5e-3
     box.close()
     b64.close()
+
+
+def test_c6_reference_case_sampled_against_oracle():
+    """The 750^3 FP64 reference case (cfgs/N1500.lcs-yaml settings at one
+    device's share: rc-energy, Re 2220, nonlinear drag, half the particle
+    slots spare) sampled at 100^3 with the same particle density: three steps
+    follow the FP64 oracle within 1e-9 (fields, forcing power, sweep counts),
+    particles by id, and every cell index equals the oracle's."""
+    import lcs_mi355x as L
+    from lcs_mi355x.workloads import single_box
+    cfg, desc = single_box("c6", n_override=100)
+    say("C6 sampled:", desc)
+    p = cfg.particles[0]
+    assert p.drag == "nonlinear" and p.population.ntotal > p.population.valid
+    box = L.build(cfg)
+    scfg = L.step_config(cfg, localize_interval=30)
+    sim = oracle_from_device(cfg, box)
+    _step_against(box, sim, scfg, 3, 1e-9, 1e-9, 1e-9, "C6 sampled")
+    compare_particles_by_id(box, sim, 0, 1e-9)
+    nv = check_cellindex_against_oracle(box, 0, cfg.grid.dims, 4)
+    say("C6 sampled cell indices equal the oracle's for", nv, "particles")
+    box.close()
 
 
 # ---------------------------------------------------------------------------
