@@ -133,7 +133,9 @@ def bench_ensemble(args):
     if not args.no_autotune:  # the batch's sweep tiling per size class, timed on the boxes as they are
         t0 = time.perf_counter()
         tiles = batch.autotune()
-        tiling = {"x".join(map(str, c.grid.dims)): f"{t[0]}x{t[1]}, z-chunks of {t[2]}" for c, t in zip(cfgs, tiles)}
+        rtiles = batch.rhs_tiles()
+        tiling = {"x".join(map(str, c.grid.dims)): f"{t[0]}x{t[1]}, z-chunks of {t[2]}; RHS {r[0]}x{r[1]}"
+                  for c, t, r in zip(cfgs, tiles, rtiles)}
         print(f"# batch tiling ({time.perf_counter() - t0:.1f} s): {tiling}", file=sys.stderr, flush=True)
         for _ in range(max(1, args.warmup)):  # tables rebuilt, graphs captured again
             batch.step(want_log=False)
@@ -158,7 +160,7 @@ def bench_ensemble(args):
                    "particles": sum(c.particles[0].population.valid for c in cfgs),
                    "particle_localize_interval": args.localize_interval, "graphs": not args.no_graphs,
                    "persistent_sweeps": args.persistent,
-                   "sweep_tiling": tiling or "32x8, z-chunks of about 28 planes (default)",
+                   "sweep_tiling": tiling or "32x8, z-chunks of about 28 planes; RHS 32x8 (default)",
                    "steps_by_path": gstats},
         "cell_steps_per_s": cells * args.steps / el,
         "serial": {"value": serial, "unit": "box-steps/s", "steps_per_box": args.serial_steps,

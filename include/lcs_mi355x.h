@@ -428,6 +428,13 @@ int lcs_batch_autotune(lcs_batch *batch);
 int lcs_batch_set_tiles(lcs_batch *batch, int index, int tx, int ty, int zchunk);
 /* The sweep tile of box `index` and its z-chunk length in planes. */
 int lcs_batch_get_tiles(lcs_batch *batch, int index, int *tx, int *ty, int *zchunk);
+/* No counterpart in the reference.  The fused RHS tile of the size class of
+ * box `index`: FP32 32x8 (default), 64x8, 64x16 or 32x16; FP64 32x4
+ * (default), 32x8 or 64x8.  lcs_batch_autotune also picks it per class, by
+ * timing stage 1's RHS into the boxes' spare arrays (no box changes).  Fields
+ * do not depend on it. */
+int lcs_batch_set_rhs_tiles(lcs_batch *batch, int index, int tx, int ty);
+int lcs_batch_get_rhs_tiles(lcs_batch *batch, int index, int *tx, int *ty);
 
 /* ---- serving ---------------------------------------------------------------- */
 /* No counterpart in the reference.  A request loop over one batch: runs are

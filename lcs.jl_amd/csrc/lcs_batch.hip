@@ -62,7 +62,8 @@ struct Batch {
     std::vector<int> rcf_boxes;      // boxes with rc-energy forcing
     DevVec<int> rcf_boxes_d;
     DevVec<WorkItem> rcf_mean, rcf_dft, rcf_apply;  // work of the batched rc-energy forcing
-    DevVec<TileItem> rhs_tiles, hs_tiles[4];  // sweep tiles by shape: kBatchSweepShapes
+    DevVec<TileItem> rhs_tiles[kBatchRhsShapesMax];  // RHS tiles by shape: batch_rhs_shape
+    DevVec<TileItem> hs_tiles[4];                    // sweep tiles by shape: kBatchSweepShapes
     HsPoll *poll_d = nullptr, *poll_h = nullptr;
     DevScalars *sc_d = nullptr, *sc_h = nullptr;
     int cap_boxes = 0;
@@ -91,6 +92,8 @@ struct Batch {
     // sweep tiling per size class (nx, ny, nz): index into kBatchSweepShapes
     // and z-chunk target in planes (lcs_batch_autotune, lcs_batch_set_tiles)
     std::map<std::array<int, 3>, std::array<int, 2>> sweep_tune;
+    // RHS tile shape per size class: index into batch_rhs_shape (default 0)
+    std::map<std::array<int, 3>, int> rhs_tune;
     // lcs_batch_set_persistent: persist_cus > 0 runs the sweeps as persistent
     // grids over the tile table, as many workgroups as that many CUs hold at
     // once; 0: one workgroup per tile.  cus: the device's CU count.
@@ -177,6 +180,25 @@ static std::array<int, 2> sweep_tiling(const Batch &B, const Geom &g)
     return it == B.sweep_tune.end() ? std::array<int, 2>{1, kBatchSweepChunk} : it->second;
 }
 
+static int rhs_shape(const Batch &B, const Geom &g)
+{
+    const auto it = B.rhs_tune.find({g.nx, g.ny, g.nz});
+    // (a choice made for the other float type falls back to the default)
+    return it == B.rhs_tune.end() || it->second >= batch_rhs_shapes(B.esize) ? 0 : it->second;
+}
+
+// the fused RHS of a stage over every box, one launch per tile shape in use
+static int b_rhs(Batch &B, int stage, bool spare_out = false)
+{
+    for (int q = 0; q < batch_rhs_shapes(B.esize); q++) {
+        const int *sh = batch_rhs_shape(B.esize, q);
+        int rc = launch_b_rhs(B.stream, B.esize, sh[0], sh[1], B.tab.d, B.rhs_tiles[q].d, B.rhs_tiles[q].n, stage,
+                              spare_out);
+        if (rc) return rc;
+    }
+    return LCS_OK;
+}
+
 template <typename T>
 static void fill_coefs(const Box &b, const lcs_hsmac_params &hs, BoxRec &r)
 {
@@ -260,7 +282,7 @@ static int rebuild(Batch &B)
     std::vector<WorkItem> gfull, gone, rows, lf_rows, parts, diams, faces[3], xfw, xfc, rmean, rdft, rapply;
     std::vector<int> lf_boxes;
     B.rcf_boxes.clear();
-    std::vector<TileItem> rtiles, htiles[4];
+    std::vector<TileItem> rtiles[kBatchRhsShapesMax], htiles[4];
     const int domain = B.hs.time_blocking ? 1 : 3;
     B.xface = n > 0;
     for (int i = 0; i < n; i++) {
@@ -290,15 +312,16 @@ static int rebuild(Batch &B)
             fill_coefs<double>(b, B.hs, r);
         else
             fill_coefs<float>(b, B.hs, r);
-        // fused RHS tiles (kBatchRhsTX x batch_rhs_ty), z-chunks of about 24 planes
+        // fused RHS tiles of the class's shape, z-chunks of about 24 planes
         // (each re-reads six halo planes; 64-plane chunks were 0.2 ms a step slower)
-        const int rtx = kBatchRhsTX, rty = batch_rhs_ty(b.esize);
+        const int rq = rhs_shape(B, g);
+        const int rtx = batch_rhs_shape(b.esize, rq)[0], rty = batch_rhs_shape(b.esize, rq)[1];
         const int rgx = (g.nx + rtx - 1) / rtx, rgy = (g.ny + rty - 1) / rty;
         const int rgz = std::max(1, g.nz / 24);
         r.rhs_zc = (g.nz + rgz - 1) / rgz;
         for (int z = 0; z * r.rhs_zc < g.nz; z++)
             for (int y = 0; y < rgy; y++)
-                for (int x = 0; x < rgx; x++) rtiles.push_back({i, x, y, z});
+                for (int x = 0; x < rgx; x++) rtiles[rq].push_back({i, x, y, z});
         // fused HSMAC sweep tiles over the whole array, in z-chunks (by default
         // of about 28 planes: short chunks balance the launch across boxes of
         // different sizes despite their five re-read halo planes; 64-plane
@@ -411,11 +434,12 @@ static int rebuild(Batch &B)
     if ((rc = B.ghost_full.assign(xcd_order(gfull), B.stream)) ||
         (rc = B.ghost_one.assign(xcd_order(gone), B.stream)) || (rc = B.rows.assign(xcd_order(rows), B.stream)) ||
         (rc = B.lf_rows.assign(xcd_order(lf_rows), B.stream)) || (rc = B.lf_boxes.assign(lf_boxes, B.stream)) ||
-        (rc = B.parts.assign(xcd_order(parts), B.stream)) || (rc = B.diams.assign(xcd_order(diams), B.stream)) ||
-        (rc = B.rhs_tiles.assign(xcd_order(rtiles), B.stream)))
+        (rc = B.parts.assign(xcd_order(parts), B.stream)) || (rc = B.diams.assign(xcd_order(diams), B.stream)))
         return rc;
     for (int q = 0; q < 4; q++)
         if ((rc = B.hs_tiles[q].assign(xcd_order(htiles[q]), B.stream))) return rc;
+    for (int q = 0; q < kBatchRhsShapesMax; q++)
+        if ((rc = B.rhs_tiles[q].assign(xcd_order(rtiles[q]), B.stream))) return rc;
     for (int a = 0; a < 3; a++)
         if ((rc = B.faces[a].assign(xcd_order(faces[a]), B.stream))) return rc;
     if ((rc = B.xf_work.assign(xcd_order(xfw), B.stream)) || (rc = B.xf_commit.assign(xcd_order(xfc), B.stream)))
@@ -637,7 +661,7 @@ static int enqueue_stage1_prefix(Batch &B)
         (r = launch_b_cfl(B.stream, B.esize, B.tab.d, B.rows.d, B.rows.n, n, c0.cfl, c0.dtmax)) ||
         (r = launch_b_particle_dt(B.stream, B.esize, B.tab.d, B.diams.d, B.diams.n, n, c0.dtmax)) ||
         (r = launch_b_particles(B.stream, B.esize, B.tab.d, B.parts.d, B.parts.n, 1)) ||
-        (r = launch_b_rhs(B.stream, B.esize, B.tab.d, B.rhs_tiles.d, B.rhs_tiles.n, 1)))
+        (r = b_rhs(B, 1)))
         return r;
     return LCS_OK;
 }
@@ -652,7 +676,7 @@ static int enqueue_stage2_prefix(Batch &B)
     int r;
     if ((r = launch_b_ghost_fill(B.stream, B.esize, B.tab.d, B.ghost_full.d, B.ghost_full.n, 1, 4, h, 1, 0)) ||
         (r = launch_b_particles(B.stream, B.esize, B.tab.d, B.parts.d, B.parts.n, 2)) ||
-        (r = launch_b_rhs(B.stream, B.esize, B.tab.d, B.rhs_tiles.d, B.rhs_tiles.n, 2)))
+        (r = b_rhs(B, 2)))
         return r;
     if (B.lf_boxes.n > 0) {
         // the means read the x = 0, y = 0, z = 0 ghosts alone; where the
@@ -895,9 +919,30 @@ static int time_group_sweeps(Batch &B, int reps, hipEvent_t e0, hipEvent_t e1, f
     return LCS_OK;
 }
 
+// Fastest of `reps` runs of stage 1's fused RHS over the whole batch with the
+// current RHS tiling, written to the boxes' spare arrays: no box changes.
+static int time_rhs(Batch &B, int reps, hipEvent_t e0, hipEvent_t e1, float &best)
+{
+    int rc;
+    if ((rc = rebuild(B))) return rc;
+    if ((rc = b_rhs(B, 1, true))) return rc;
+    best = 1e30f;
+    for (int r = 0; r < reps; r++) {
+        LCS_HIP(hipEventRecord(e0, B.stream));
+        if ((rc = b_rhs(B, 1, true))) return rc;
+        LCS_HIP(hipEventRecord(e1, B.stream));
+        LCS_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        LCS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, ms);
+    }
+    return LCS_OK;
+}
+
 // Coordinate descent over size classes: each class in turn tries every shape
 // and chunk target with the other classes at their current choice, and keeps
-// a candidate only if it beats the current one by more than 1 %.
+// a candidate only if it beats the current one by more than 1 %.  Then the
+// same for the fused RHS's tile shape.
 static int batch_autotune(Batch &B)
 {
     if (B.boxes.empty()) return LCS_OK;
@@ -942,6 +987,26 @@ static int batch_autotune(Batch &B)
                     }
                 }
             B.sweep_tune[c] = best;
+        }
+    for (int round = 0; round < 2 && rc == LCS_OK; round++)
+        for (const auto &c : classes) {
+            if (!B.rhs_tune.count(c)) B.rhs_tune[c] = 0;
+            int best = B.rhs_tune[c];
+            float best_ms = 0;
+            B.dirty = true;
+            if ((rc = time_rhs(B, reps, e0, e1, best_ms))) break;
+            for (int q = 0; q < batch_rhs_shapes(B.esize) && rc == LCS_OK; q++) {
+                if (q == best) continue;
+                B.rhs_tune[c] = q;
+                B.dirty = true;
+                float ms = 0;
+                if ((rc = time_rhs(B, reps, e0, e1, ms))) break;
+                if (ms < 0.99f * best_ms) {
+                    best = q;
+                    best_ms = ms;
+                }
+            }
+            B.rhs_tune[c] = best;
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1036,7 +1101,7 @@ int lcs_batch_destroy(lcs_batch *batch)
     B->rcf_mean.release();
     B->rcf_dft.release();
     B->rcf_apply.release();
-    B->rhs_tiles.release();
+    for (auto &t : B->rhs_tiles) t.release();
     for (auto &t : B->hs_tiles) t.release();
     B->tab.release();
     if (B->poll_d) (void)hipFree(B->poll_d);
@@ -1179,6 +1244,35 @@ int lcs_batch_get_tiles(lcs_batch *batch, int index, int *tx, int *ty, int *zchu
     *tx = r.hs_tx;
     *ty = r.hs_ty;
     *zchunk = r.hs_zc;
+    return LCS_OK;
+}
+
+int lcs_batch_set_rhs_tiles(lcs_batch *batch, int index, int tx, int ty)
+{
+    if (!batch) return fail(LCS_ERR_INVALID, "null argument");
+    Batch &B = *BT(batch);
+    if (index < 0 || index >= (int)B.boxes.size()) return fail(LCS_ERR_INVALID, "box index out of range");
+    int q = -1;
+    for (int k = 0; k < batch_rhs_shapes(B.esize); k++)
+        if (batch_rhs_shape(B.esize, k)[0] == tx && batch_rhs_shape(B.esize, k)[1] == ty) q = k;
+    if (q < 0)
+        return fail(LCS_ERR_INVALID, B.esize == 8 ? "FP64 batched RHS tile shape must be 32x4, 32x8 or 64x8"
+                                                  : "FP32 batched RHS tile shape must be 32x8, 64x8, 64x16 or 32x16");
+    const Geom &g = B.boxes[index]->g;
+    LCS_HIP(hipStreamSynchronize(B.stream));
+    B.rhs_tune[{g.nx, g.ny, g.nz}] = q;
+    B.dirty = true;
+    return LCS_OK;
+}
+
+int lcs_batch_get_rhs_tiles(lcs_batch *batch, int index, int *tx, int *ty)
+{
+    if (!batch || !tx || !ty) return fail(LCS_ERR_INVALID, "null argument");
+    Batch &B = *BT(batch);
+    if (index < 0 || index >= (int)B.boxes.size()) return fail(LCS_ERR_INVALID, "box index out of range");
+    const int *sh = batch_rhs_shape(B.esize, rhs_shape(B, B.boxes[index]->g));
+    *tx = sh[0];
+    *ty = sh[1];
     return LCS_OK;
 }
 

@@ -2720,7 +2720,8 @@ int launch_b_lf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *row
 
 // fused RHS + update of one stage for every box
 template <typename T, int TX, int TY>
-__global__ void __launch_bounds__(TX *TY) k_b_rhs_tile(const BoxRec *tab, const TileItem *tiles, int stage)
+__global__ void __launch_bounds__(TX *TY) k_b_rhs_tile(const BoxRec *tab, const TileItem *tiles, int stage,
+                                                        int spare_out)
 {
     extern __shared__ __align__(16) unsigned char rhs_lds_raw[];
     const TileItem t = tiles[blockIdx.x];
@@ -2734,7 +2735,7 @@ __global__ void __launch_bounds__(TX *TY) k_b_rhs_tile(const BoxRec *tab, const 
     a.W = (const T *)B.f[in + 2];
     a.P = (const T *)B.f[6];
     for (int q = 0; q < 3; q++) {
-        a.out[q] = (T *)B.f[(stage == 1 ? 3 : 0) + q];
+        a.out[q] = spare_out ? (T *)B.spare[q] : (T *)B.f[(stage == 1 ? 3 : 0) + q];
         a.base[q] = stage == 1 ? nullptr : (const T *)B.f[q];  // stage 1: base == input
         a.di[q] = T(B.rhs_di[q]);
         a.di2[q] = T(B.rhs_di2[q]);
@@ -2745,23 +2746,35 @@ __global__ void __launch_bounds__(TX *TY) k_b_rhs_tile(const BoxRec *tab, const 
     rhs_tile_block<T, TX, TY, true>(a, t.bx, t.by, t.bz, rdt, reinterpret_cast<T *>(rhs_lds_raw));
 }
 
-template <typename T>
-static int b_rhs_t(hipStream_t s, const BoxRec *tab, const TileItem *tiles, int ntiles, int stage)
+template <typename T, int TX, int TY>
+static int b_rhs_t(hipStream_t s, const BoxRec *tab, const TileItem *tiles, int ntiles, int stage, int spare_out)
 {
-    constexpr int TX = kBatchRhsTX, TY = batch_rhs_ty(sizeof(T));
     using Tile = RhsTile<T, TX, TY>;
     const size_t lds = Tile::LDS_ELEMS * sizeof(T);
+    static_assert(Tile::LDS_ELEMS * sizeof(T) <= 160 * 1024, "RHS tile exceeds the LDS of a CU");
     int rc = kernel_setup((const void *)k_b_rhs_tile<T, TX, TY>, TX * TY, lds, nullptr);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_b_rhs_tile<T, TX, TY>), dim3(ntiles), dim3(TX * TY), lds, s, tab, tiles, stage);
+    hipLaunchKernelGGL((k_b_rhs_tile<T, TX, TY>), dim3(ntiles), dim3(TX * TY), lds, s, tab, tiles, stage, spare_out);
     LCS_HIP(hipGetLastError());
     return LCS_OK;
 }
 
-int launch_b_rhs(hipStream_t s, int esize, const BoxRec *tab, const TileItem *tiles, int ntiles, int stage)
+int launch_b_rhs(hipStream_t s, int esize, int tx, int ty, const BoxRec *tab, const TileItem *tiles, int ntiles,
+                 int stage, bool spare_out)
 {
     if (ntiles == 0) return LCS_OK;
-    return esize == 8 ? b_rhs_t<double>(s, tab, tiles, ntiles, stage) : b_rhs_t<float>(s, tab, tiles, ntiles, stage);
+    const int so = spare_out ? 1 : 0;
+    if (esize == 8) {
+        if (tx == 32 && ty == 4) return b_rhs_t<double, 32, 4>(s, tab, tiles, ntiles, stage, so);
+        if (tx == 32 && ty == 8) return b_rhs_t<double, 32, 8>(s, tab, tiles, ntiles, stage, so);
+        if (tx == 64 && ty == 8) return b_rhs_t<double, 64, 8>(s, tab, tiles, ntiles, stage, so);
+    } else {
+        if (tx == 32 && ty == 8) return b_rhs_t<float, 32, 8>(s, tab, tiles, ntiles, stage, so);
+        if (tx == 64 && ty == 8) return b_rhs_t<float, 64, 8>(s, tab, tiles, ntiles, stage, so);
+        if (tx == 64 && ty == 16) return b_rhs_t<float, 64, 16>(s, tab, tiles, ntiles, stage, so);
+        if (tx == 32 && ty == 16) return b_rhs_t<float, 32, 16>(s, tab, tiles, ntiles, stage, so);
+    }
+    return fail(LCS_ERR_INVALID, "unsupported batched RHS tile shape");
 }
 
 template <typename T>

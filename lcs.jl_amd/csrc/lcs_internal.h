@@ -314,10 +314,15 @@ struct HsPoll {
     double div;
 };
 constexpr int kBatchRowBlocks = 64;
-// fused RHS tile of the batched kernels: narrower than the single-box tile so
-// 32^3 boxes do not leave most of a workgroup idle
-constexpr int kBatchRhsTX = 32;
-__host__ __device__ constexpr int batch_rhs_ty(int esize) { return esize == 8 ? 4 : 8; }  // row-walking reduction blocks per batched box
+// fused RHS tile shapes of the batched kernels, chosen per size class
+// (lcs_batch_autotune, lcs_batch_set_rhs_tiles); shape 0, the default, is
+// narrower than the single-box tile so 32^3 boxes do not leave most of a
+// workgroup idle
+constexpr int kBatchRhsShapesMax = 4;
+static const int kBatchRhsShapesF[kBatchRhsShapesMax][2] = {{32, 8}, {64, 8}, {64, 16}, {32, 16}};
+static const int kBatchRhsShapesD[kBatchRhsShapesMax][2] = {{32, 4}, {32, 8}, {64, 8}, {0, 0}};
+inline int batch_rhs_shapes(int esize) { return esize == 8 ? 3 : 4; }
+inline const int *batch_rhs_shape(int esize, int q) { return esize == 8 ? kBatchRhsShapesD[q] : kBatchRhsShapesF[q]; }
 
 // batched launchers (lcs_flow.hip / lcs_particles.hip); all enqueue on `s`
 int launch_b_ghost_fill(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *work, int nwork, int set,
@@ -340,7 +345,10 @@ int launch_b_rcf(hipStream_t s, int esize, const BoxRec *tab, const WorkItem *me
 int rcf_prepare(Box &b, int kmin, int kmax, int c);
 int rcf_apply_blocks(const Box &b);
 int rcf_crop_slots(int kmax);
-int launch_b_rhs(hipStream_t s, int esize, const BoxRec *tab, const TileItem *tiles, int ntiles, int stage);
+// tiles of one shape (tx x ty, batch_rhs_shape); spare_out: write the result
+// to the boxes' spare arrays instead of the stage's output (for timing)
+int launch_b_rhs(hipStream_t s, int esize, int tx, int ty, const BoxRec *tab, const TileItem *tiles, int ntiles,
+                 int stage, bool spare_out = false);
 // tx: tile width of the fused sweep (32 or 64); the tiles listed must all be of that width
 // force: boxes with lf_force apply the linear forcing of S->lf on load and
 // leave their per-tile power in part[kPowerSlot]

@@ -173,6 +173,8 @@ SIGNATURES = {
     "lcs_batch_graph_stats": [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "lcs_batch_set_tiles": [P, I, I, I, I],
     "lcs_batch_get_tiles": [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)],
+    "lcs_batch_set_rhs_tiles": [P, I, I, I],
+    "lcs_batch_get_rhs_tiles": [P, I, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "lcs_server_create": [P, C.POINTER(ServerConfig), C.POINTER(P)],
     "lcs_server_destroy": [P],
     "lcs_server_submit": [P, C.POINTER(Request), C.POINTER(C.c_int64)],

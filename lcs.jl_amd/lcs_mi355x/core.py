@@ -430,8 +430,9 @@ class Batch:
         return dict(zip(("whole", "gated", "segmented"), (x.value for x in v)))
 
     def autotune(self):
-        """Picks the batched sweep's tile shape and z-chunk per size class by
-        timing (lcs_batch_autotune); no box changes.  Returns tiles()."""
+        """Picks the batched sweep's tile shape and z-chunk, and the fused RHS's
+        tile shape, per size class by timing (lcs_batch_autotune); no box
+        changes.  Returns tiles()."""
         check(self.L.lcs_batch_autotune(self.handle))
         return self.tiles()
 
@@ -445,6 +446,19 @@ class Batch:
         for i in range(len(self)):
             v = [C.c_int() for _ in range(3)]
             check(self.L.lcs_batch_get_tiles(self.handle, i, *[C.byref(x) for x in v]))
+            out.append(tuple(x.value for x in v))
+        return out
+
+    def set_rhs_tiles(self, i, tx, ty):
+        """Fused RHS tile of the size class of box i (lcs_batch_set_rhs_tiles)."""
+        check(self.L.lcs_batch_set_rhs_tiles(self.handle, int(i), int(tx), int(ty)))
+
+    def rhs_tiles(self):
+        """[(tx, ty)] of every box's fused RHS tile, in order of addition."""
+        out = []
+        for i in range(len(self)):
+            v = [C.c_int() for _ in range(2)]
+            check(self.L.lcs_batch_get_rhs_tiles(self.handle, i, *[C.byref(x) for x in v]))
             out.append(tuple(x.value for x in v))
         return out
 

@@ -194,9 +194,10 @@ def test_persistent_sweeps_change_no_result(dtype):
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_batch_tiling_changes_no_result(dtype):
-    """The autotuner's timing sweeps leave every box as it was, and the
-    per-class tilings it (or set_tiles) picks give the default tiling's
-    fields and sweep counts bit for bit.  The logged divergence and forcing
+    """The autotuner's timing runs (sweeps, and stage-1 RHS into the spare
+    arrays) leave every box as it was, and the per-class sweep and RHS tilings
+    it (or set_tiles, set_rhs_tiles) picks give the default tiling's fields
+    and sweep counts bit for bit.  The logged divergence and forcing
     power are sums of per-tile partials: their last bits follow the tile
     count, so they are compared to rounding."""
     import lcs_mi355x as L
@@ -219,19 +220,25 @@ def test_batch_tiling_changes_no_result(dtype):
             assert (a["step"], a["t"], a["dt"]) == (b["step"], b["t"], b["dt"]), s
 
     assert bt.tiles() == bd.tiles() and all(t[:2] == (32, 8) for t in bt.tiles())
+    rdef = (32, 4) if dtype == "f64" else (32, 8)
+    assert bt.rhs_tiles() == bd.rhs_tiles() == [rdef] * len(ta)
     same_logs(bt.step(), bd.step(), -1)
-    before = [[b.download(n) for n in ("U", "V", "W", "P")] for b in ta]
+    names = ("U", "V", "W", "P", "U2", "V2", "W2")
+    before = [[b.download(n) for n in names] for b in ta]
     tiles = bt.autotune()
-    print("autotuned tiles:", tiles, flush=True)
-    for b, ref in zip(ta, before):
-        for n, a in zip(("U", "V", "W", "P"), ref):
+    print("autotuned tiles:", tiles, "RHS tiles:", bt.rhs_tiles(), flush=True)
+    for b, ref in zip(ta, before):  # the timing runs changed no box, stage arrays included
+        for n, a in zip(names, ref):
             np.testing.assert_array_equal(b.download(n), a, err_msg=n)
     # boxes of one size class share a tiling
-    assert tiles[0] == tiles[3]
+    assert tiles[0] == tiles[3] and bt.rhs_tiles()[0] == bt.rhs_tiles()[3]
     for s in range(3):
         if s == 1:  # an explicit tiling of the 32 x 16 x 24 class
             bt.set_tiles(1, 64, 16, 8)
             assert bt.tiles()[1][:2] == (64, 16)
+            rset = (64, 8) if dtype == "f64" else (64, 16)
+            bt.set_rhs_tiles(1, *rset)
+            assert bt.rhs_tiles()[1] == rset
         same_logs(bt.step(), bd.step(), s)
         for q, (a, b) in enumerate(zip(ta, da)):
             for n in ("U", "V", "W", "P"):
@@ -239,5 +246,7 @@ def test_batch_tiling_changes_no_result(dtype):
     from lcs_mi355x._lib import LCSError
     with pytest.raises(LCSError):
         bt.set_tiles(0, 128, 8, 28)
+    with pytest.raises(LCSError):
+        bt.set_rhs_tiles(0, 128, 8)
     bt.close()
     bd.close()
