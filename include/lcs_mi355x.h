@@ -420,7 +420,9 @@ int lcs_batch_set_persistent(lcs_batch *batch, int cus);
  * The timed sweeps read the boxes' current fields and write only their
  * ping-pong partners, so no box changes.  Fields and sweep counts do not
  * depend on the tiling; the logged divergence and forcing power are sums of
- * per-tile partials, so their last bits do.  Synchronises; the next step
+ * per-tile partials, so their last bits do.  It then picks the fused RHS's
+ * tile shape per class the same way (lcs_batch_set_rhs_tiles), timing
+ * stage 1's RHS written into the spare arrays.  Synchronises; the next step
  * rebuilds the batch's tables and graphs. */
 int lcs_batch_autotune(lcs_batch *batch);
 /* Sets the tiling of the size class of box `index` (order of addition);
