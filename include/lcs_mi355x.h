@@ -262,7 +262,7 @@ int lcs_particle_density(lcs_box *box, int profile, double *density);
  * tx x ty columns of cells up in z; the shape changes speed, never results.
  * Sweep shapes: (64|32) x (16|8), and in FP32 also 128x8 and 128x16 (two
  * cells per thread).  RHS shapes: FP32 (64|32) x (16|8), FP64
- * 64x8, 32x8, 32x4.  A zero sweep shape means the size rule for all of them. */
+ * 64x8, 32x8, 32x4, 32x16.  A zero sweep shape means the size rule for all of them. */
 typedef struct {
     int sweep_tx, sweep_ty;
     int rhs_tx, rhs_ty;

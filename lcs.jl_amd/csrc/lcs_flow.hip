@@ -376,11 +376,11 @@ static int rhs_tile_launch(Box &b, const RhsArgs<T> &a0, DtRef dt)
 // Tile shapes (tx, ty) the fused kernels are built for (the LDS of one CU
 // bounds them).  Sweep: 64 or 32 wide and 16 or 8 tall, and in FP32 also
 // 128 x 8 and 128 x 16 (two cells per thread).  RHS: FP32 up to 64 x 16;
-// FP64 up to 64 x 8.
+// FP64 64 x 8, 32 x 16 and smaller.
 static const int kSweepShapesF[][2] = {{64, 16}, {64, 8}, {32, 16}, {32, 8}, {128, 8}, {128, 16}};
 static const int kSweepShapesD[][2] = {{64, 16}, {64, 8}, {32, 16}, {32, 8}};
 static const int kRhsShapesF[][2] = {{64, 16}, {64, 8}, {32, 16}, {32, 8}};
-static const int kRhsShapesD[][2] = {{64, 8}, {32, 8}, {32, 4}};
+static const int kRhsShapesD[][2] = {{64, 8}, {32, 8}, {32, 4}, {32, 16}};
 // two-sweep pass (FP32 only; (0, 0) = two single sweeps)
 static const int kSweep2Shapes[][2] = {{32, 24}, {64, 12}, {32, 16}, {0, 0}};
 
@@ -388,7 +388,7 @@ int tile_candidates(int esize, int kind, int out[][2], int cap)
 {
     const int(*src)[2] = kind == 0 ? (esize == 8 ? kSweepShapesD : kSweepShapesF)
                                    : kind == 2 ? kSweep2Shapes : (esize == 8 ? kRhsShapesD : kRhsShapesF);
-    const int n = kind == 0 ? (esize == 8 ? 4 : 6) : kind == 2 ? (esize == 8 ? 0 : 4) : (esize == 8 ? 3 : 4);
+    const int n = kind == 0 ? (esize == 8 ? 4 : 6) : kind == 2 ? (esize == 8 ? 0 : 4) : 4;
     for (int i = 0; i < n && i < cap; i++) {
         out[i][0] = src[i][0];
         out[i][1] = src[i][1];
@@ -431,6 +431,7 @@ static int rhs_shape(Box &b, const RhsArgs<T> &a, DtRef dt)
         if (t.rhs_tx == 64 && t.rhs_ty == 8) return rhs_tile_launch<T, FUSED, 64, 8>(b, a, dt);
         if (t.rhs_tx == 32 && t.rhs_ty == 8) return rhs_tile_launch<T, FUSED, 32, 8>(b, a, dt);
         if (t.rhs_tx == 32 && t.rhs_ty == 4) return rhs_tile_launch<T, FUSED, 32, 4>(b, a, dt);
+        if (t.rhs_tx == 32 && t.rhs_ty == 16) return rhs_tile_launch<T, FUSED, 32, 16>(b, a, dt);
     }
     return fail(LCS_ERR_INVALID, "unsupported RHS tile shape");
 }

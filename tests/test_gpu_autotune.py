@@ -15,7 +15,7 @@ from test_gpu_step import c1_config
 pytestmark = pytest.mark.gpu
 
 SWEEP = {"f32": [(64, 16), (64, 8), (32, 16), (32, 8), (128, 8), (128, 16)], "f64": [(64, 16), (64, 8), (32, 16), (32, 8)]}
-RHS = {"f32": [(64, 16), (64, 8), (32, 16), (32, 8)], "f64": [(64, 8), (32, 8), (32, 4)]}
+RHS = {"f32": [(64, 16), (64, 8), (32, 16), (32, 8)], "f64": [(64, 8), (32, 8), (32, 16), (32, 4)]}
 SWEEP2 = [(32, 24), (64, 12), (32, 16)]
 
 
