@@ -1796,11 +1796,20 @@ __device__ inline void boxmean_block(const Geom &g, const T *A, int c, int cell,
     int kx = g.nx / c, ky = g.ny / c, kz = g.nz / c;
     long nb = (long)kx * ky * kz;
     double s = 0.0;
-    for (long t = threadIdx.x; t < nb; t += blockDim.x) {
-        int di = (int)(t % kx);
-        long q = t / kx;
-        int dj = (int)(q % ky), dk = (int)(q / ky);
+    // element t = di + kx (dj + ky dk), t = tid, tid + B, ...: the indices
+    // advance by B in mixed radix (kx, ky) with carries, not divisions
+    const int B = blockDim.x;
+    int di = threadIdx.x % kx, dj = (threadIdx.x / kx) % ky, dk = threadIdx.x / (kx * ky);
+    const int bx = B % kx, by = (B / kx) % ky, bz = B / (kx * ky);
+    for (long t = threadIdx.x; t < nb; t += B) {
         s += (double)A[g.at(ic * kx + di + 1, jc * ky + dj + 1, kc * kz + dk + 1)];
+        di += bx;
+        int carry = di >= kx ? 1 : 0;
+        di -= carry * kx;
+        dj += by + carry;
+        carry = dj >= ky ? 1 : 0;
+        dj -= carry * ky;
+        dk += bz + carry;
     }
     s = block_sum(s, lds);
     if (threadIdx.x == 0) {
