@@ -2032,6 +2032,7 @@ __device__ inline double rcf_apply_part(const Geom &g, T *U, T *V, T *W, int c, 
         const long r0 = gi * R;
         const int nr = (int)min((long)R, rows - r0);
         const int ncell = nr * g.nx;  // more than one pass only when R = 1
+        const int j0 = (int)(r0 % g.ny), k0 = (int)(r0 / g.ny);  // row r0 at (j0 + 1, k0 + 1)
         for (int x0 = 0; x0 < ncell; x0 += kRcfCellsPerPass) {
             long p[Q];
             int rr[Q], ii[Q];
@@ -2039,11 +2040,12 @@ __device__ inline double rcf_apply_part(const Geom &g, T *U, T *V, T *W, int c, 
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 const int e = x0 + threadIdx.x + 256 * q;
-                rr[q] = e / g.nx;
+                rr[q] = R == 1 ? 0 : e / g.nx;
                 ii[q] = e - rr[q] * g.nx + 1;
                 if (e < ncell) {
-                    const long r = r0 + rr[q];
-                    p[q] = g.at(ii[q], (int)(r % g.ny) + 1, (int)(r / g.ny) + 1);
+                    const int jr = j0 + rr[q];  // 32-bit: row r0 + rr in (j, k)
+                    const int j = R == 1 ? j0 : jr % g.ny, k = R == 1 ? k0 : k0 + jr / g.ny;
+                    p[q] = g.at(ii[q], j + 1, k + 1);
                     u[q] = U[p[q]];
                     v[q] = V[p[q]];
                     w[q] = W[p[q]];
@@ -2052,8 +2054,7 @@ __device__ inline double rcf_apply_part(const Geom &g, T *U, T *V, T *W, int c, 
             if (x0 == 0) {
                 for (int it = threadIdx.x; it < nr * md.n; it += blockDim.x) {
                     const int q = it / md.n, m = it - q * md.n;
-                    const long r = r0 + q;
-                    const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
+                    const int jr = j0 + q, j = jr % g.ny + 1, k = k0 + jr / g.ny + 1;
                     Cx ej, ek;
                     sincospi(2 * ((double)k - phi) * nzi, &ek.im, &ek.re);
                     sincospi(2 * ((double)j - phi) * nyi, &ej.im, &ej.re);
