@@ -371,7 +371,12 @@ static int rebuild(Batch &B)
             const long cb = (SlabAxes<0>::rows(xs[0]) + 255) / 256;
             for (long k = 0; k < cb; k++) xfc.push_back({i, (int)k});
         }
-        // ghost fills
+        // ghost fills (their cell counts must fit the kernel's 32-bit indices)
+        {
+            const long X = g.nx + 2L * g.h, Y = g.ny + 2L * g.h;
+            if (2L * g.h * (X * Y + g.nz * X + (long)g.ny * g.nz) >= (1L << 31))
+                return fail(LCS_ERR_INVALID, "box too large for the batched ghost fill");
+        }
         auto ghost = [&](int rr, int full, std::vector<WorkItem> &w) {
             const long X = full ? g.nx + 2 * rr : g.nx, Y = full ? g.ny + 2 * rr : g.ny;
             const long tot = 2L * rr * X * Y + 2L * rr * g.nz * X + 2L * rr * g.ny * g.nz;

@@ -198,8 +198,9 @@ struct Ptr4 {
 // converged and acts on the spare buffer set B when the ping-pong parity says
 // the current state lives there.
 // Ghost cell t of the fill (z-slabs, then y-slabs, then x-slabs): its
-// offset d and the offset s of its periodic source.
-__device__ inline void ghost_cell(const Geom &g, int r, int full, long cz, long cy, long t, long &d, long &s)
+// offset d and the offset s of its periodic source.  The counts of a fill fit
+// 32 bits (the launchers check), so the index arithmetic is 32-bit.
+__device__ inline void ghost_cell(const Geom &g, int r, int full, int cz, int cy, int t, long &d, long &s)
 {
     const int X = full ? g.nx + 2 * r : g.nx;
     const int Y = full ? g.ny + 2 * r : g.ny;
@@ -207,28 +208,28 @@ __device__ inline void ghost_cell(const Geom &g, int r, int full, long cz, long 
     const int r2 = 2 * r;
     int i, j, k;
     if (t < cz) {
-        int ii = (int)(t % X);
-        long q = t / X;
-        int jj = (int)(q % Y);
-        int kk = (int)(q / Y);
+        int ii = t % X;
+        int q = t / X;
+        int jj = q % Y;
+        int kk = q / Y;
         i = x0 + ii;
         j = y0 + jj;
         k = kk < r ? 1 - r + kk : g.nz + 1 + kk - r;
     } else if (t < cz + cy) {
         t -= cz;
-        int ii = (int)(t % X);
-        long q = t / X;
-        int jj = (int)(q % r2);
-        int kk = (int)(q / r2);
+        int ii = t % X;
+        int q = t / X;
+        int jj = q % r2;
+        int kk = q / r2;
         i = x0 + ii;
         j = jj < r ? 1 - r + jj : g.ny + 1 + jj - r;
         k = 1 + kk;
     } else {
         t -= cz + cy;
-        int ii = (int)(t % r2);
-        long q = t / r2;
-        int jj = (int)(q % g.ny);
-        int kk = (int)(q / g.ny);
+        int ii = t % r2;
+        int q = t / r2;
+        int jj = q % g.ny;
+        int kk = q / g.ny;
         i = ii < r ? 1 - r + ii : g.nx + 1 + ii - r;
         j = 1 + jj;
         k = 1 + kk;
@@ -255,7 +256,7 @@ __global__ void k_ghost_fill(Geom g, Ptr4<T> A, Ptr4<T> Bs, int narr, int r, int
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
     long d, s;
-    ghost_cell(g, r, full, cz, cy, t, d, s);
+    ghost_cell(g, r, full, (int)cz, (int)cy, (int)t, d, s);
     for (int a = 0; a < narr; a++) {
         T *q = alt ? Bs.p[a] : A.p[a];
         q[d] = q[s];
@@ -272,6 +273,7 @@ static int ghost_fill_t(Box &b, void *const *arrays, void *const *alt, int narra
     long cz, cy, cx;
     ghost_counts(g, r, full, cz, cy, cx);
     long total = cz + cy + cx;
+    if (total >= (1L << 31)) return fail(LCS_ERR_INVALID, "grid too large for the ghost fill");
     Ptr4<T> A = {{nullptr, nullptr, nullptr, nullptr}}, Bs = A;
     for (int q = 0; q < narrays; q++) {
         A.p[q] = (T *)arrays[q];
@@ -2573,7 +2575,7 @@ __global__ void __launch_bounds__(256) k_b_ghost_fill(const BoxRec *tab, const W
     const long t = (long)w.blk * blockDim.x + threadIdx.x;
     if (t >= cz + cy + cx) return;
     long d, s;
-    ghost_cell(g, r, full, cz, cy, t, d, s);
+    ghost_cell(g, r, full, (int)cz, (int)cy, (int)t, d, s);
     for (int q = 0; q < narr; q++) {
         T *A = (T *)(alt ? B.spare[q] : set_array(B, set, q));
         A[d] = A[s];
